@@ -1,0 +1,210 @@
+// Shared definitions of the MI355X PMPIC hot-path library (libpmpic_hip.so).
+//
+// Device data layout (all device-resident between calls; see the C-ABI in include/pmpic_hip.h):
+//  * parcels: structure of arrays, FP64, capacity `cap`, field f at parcels + f*cap, fields in
+//    writebinary order x..tag then the 12 RK arrays (model_core/src/parcels/parcels.F90:19-34);
+//  * physical grids: interior nodes only, [nzn][ny][nx] (x fastest, z slowest); the reference's
+//    x/y halos (decomposition.F90:449-451) become periodic index arithmetic, so the halo fold of
+//    par2grid (MPIC_Haloswap.F90:188-290) and the fill of grid2par (:95-184) are implicit;
+//  * spectral arrays: [nzn][sy][sx] with sy = (ny/2+1)*2, sx = (nx/2+1)*2 holding the
+//    reference's (Re,Im)-interleaved semi-spectral layout (pencilfft.F90:838-899), z slowest so
+//    that one thread per (kx,ky) column walks z with unit-stride, coalesced across threads;
+//  * cells: c = (k*ny + j)*nx + i over ncz*ny*nx cells.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#include "../../include/pmpic_status.h"
+
+namespace pmpic {
+
+enum Field {
+  FX = 0, FY, FZ, FP, FQ, FR, FDXDT, FDYDT, FDZDT, FDPDT, FDQDT, FDRDT, FH, FB, FVOL,
+  FSTRETCH, FTAG, FXO, FYO, FZO, FXF, FYF, FZF, FPO, FQO, FRO, FPF, FQF, FRF, NFIELDS
+};
+
+// grid ids exported through the C-ABI (writebinary.F90:287-296 order, then the tendencies)
+enum GridId { GU = 0, GV, GW, GP, GQ, GR, GB, GHG, GHGLIQ, GVOL, GDP, GDQ, GDR, NGRIDS };
+
+struct Geometry {
+  int nx, ny, ncz, nzn;          // cells x, y, z; z nodes
+  int sx, sy;                    // spectral extents
+  double xbot, ybot, zbot, xtop, ytop, ztop;  // config bounds
+  double dx, dy, dz;
+  double xmin_h, ymin_h, zmin_h, zmax_h;      // parcel_interpolation.F90:104-115
+  double hs_xmin, hs_xmax, hs_ymin, hs_ymax;  // haloswap.F90:314-318
+  double xshift_e, xshift_w, yshift_n, yshift_s;  // haloswap.F90:529-552
+  double vmin;                   // parcel_mixing.F90:54
+  double lx_cfl, ly_cfl, lz_cfl; // vort2vel.F90:346-348
+};
+
+// Per-launch scalars living in device memory (no host round trip inside a stage).
+struct DevScalars {
+  double dtm;        // current timestep
+  double dtmax;      // configured dtmax
+  double prefactor;  // timestep_prefactor
+  double umax2, vmax2, wmax2, omax2;  // reductions (stage 1)
+  int64_t n;         // parcel count
+  int64_t cross;     // parcels crossing x/y this haloswap
+  int64_t nsplit;    // parcels split this step
+  int64_t nremove, nadd, nzero, nkeep1, nbackfill;
+  int status;        // first error code raised by a kernel
+  int pad;
+};
+
+struct Ctx {
+  Geometry g;
+  int64_t cap;
+  int64_t n;             // host copy of the parcel count (valid between calls)
+  double time, dtm, dtmax, prefactor;
+  int it_v2v, it_vt, rkstep, it_split, it_mix;  // per-component iteration counters
+  int precision;
+  int splitting_enabled, mixing_enabled;
+  int64_t last_split, last_removed, last_added;
+  hipStream_t stream;
+  // parcels
+  double *par;           // NFIELDS * cap
+  double *tmp;           // NFIELDS * cap staging for permutations (allocated lazily)
+  int64_t tmp_cap;
+  int32_t *cell, *rank, *sorted;  // cap
+  uint8_t *flag;                  // cap (haloswap direction / keep code)
+  int32_t *list_a, *list_b;       // cap compaction outputs
+  // cells
+  int64_t ncell;
+  int32_t *cell_count, *cell_start;   // ncell + 1
+  double *partial;                    // 48 * ncell corner partial sums
+  double *partial2;                   // 48 * ncell (mixing residual pass)
+  int32_t *npercell;                  // ncell
+  uint8_t *cellflag;                  // ncell
+  // grids (interior [nzn][ny][nx])
+  int64_t gsz, ssz, csz;
+  double *grid[NGRIDS];
+  double *gtmp[8];                    // vorticity tendency derivative grids
+  double *kres[6];                    // mixing residual grids
+  // spectral [nzn][sy][sx]
+  double *sa, *sb, *sc, *sd, *se, *sf, *su, *sv, *sw, *sbs;
+  double *sder[8];
+  double *atop, *abot, *btop, *bbot;  // [sy][sx]
+  double *ubar, *vbar;                // [nzn]
+  double *kx, *ky, *k2, *filt;        // [sy][sx]
+  double *coords;                     // x_coords (nx+4), y_coords (ny+4), z_coords (nzn)
+  double *fbuf;                       // host-array staging for standalone ops (max(gsz, ssz, cap))
+  int32_t *ibuf;                      // cap int32 staging
+  double2 *cbuf;                      // complex FFT intermediates, 8 fields
+  double2 *tw_x, *tw_y;               // twiddles W_n^e, e in [0,n)
+  // scan scratch
+  void *scan_tmp;
+  size_t scan_tmp_bytes;
+  int32_t *blk;                       // per-block counters (10 * nblk_max + 16)
+  int64_t nblk_max;
+  DevScalars *dsc;                    // device scalars
+  DevScalars hsc;                     // host mirror
+  // HIP-event timing of the hot kernels (0 = par2grid cell accumulation, 1 = gather + RK)
+  int timing;
+  hipEvent_t ev[2][64][2];
+  int nev[2];
+  double t_ms[2];
+  int64_t t_calls[2];
+};
+
+#define PMPIC_CHECK(call)                                   \
+  do {                                                      \
+    hipError_t e_ = (call);                                 \
+    if (e_ != hipSuccess) return PMPIC_ERR_HIP;             \
+  } while (0)
+
+#define PMPIC_TRY(expr)                                     \
+  do {                                                      \
+    int s_ = (expr);                                        \
+    if (s_ != PMPIC_OK) return s_;                          \
+  } while (0)
+
+static inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+__device__ __forceinline__ void raise_status(DevScalars *s, int code) {
+  atomicCAS(&s->status, 0, code);
+}
+
+// cache_parcel_interp_weights for one parcel (parcel_interpolation.F90:206-246), returning the
+// periodic interior cell (i0, j0, k0) and the offsets.  Arithmetic is the reference's, operand
+// for operand.  Returns a status code.
+__device__ __forceinline__ int locate(const Geometry &g, double xp, double yp, double zp,
+                                      int &i0, int &j0, int &k0, double &delx, double &dely,
+                                      double &delz) {
+  const double ax = xp - g.xmin_h, ay = yp - g.ymin_h, az = zp - g.zmin_h;
+  const int i = (int)floor(ax / g.dx) + 1;
+  const int j = (int)floor(ay / g.dy) + 1;
+  const int k = (int)floor(az / g.dz) + 1;
+  // the reference accepts 1 <= i < nx+4; halo columns 1,2 only arise for a parcel the
+  // haloswap failed to wrap, which is rejected here instead of silently mis-deposited
+  if (i < 3 || i > g.nx + 3) return PMPIC_ERR_OUT_OF_BOX_X;
+  if (j < 3 || j > g.ny + 3) return PMPIC_ERR_OUT_OF_BOX_Y;
+  if (k < 1 || k >= g.nzn) return PMPIC_ERR_OUT_OF_BOX_Z;
+  delx = (ax - (double)(i - 1) * g.dx) / g.dx;
+  dely = (ay - (double)(j - 1) * g.dy) / g.dy;
+  delz = (az - (double)(k - 1) * g.dz) / g.dz;
+  if (delx > 1. || delx < 0) return PMPIC_ERR_DELTA;
+  if (dely > 1. || dely < 0) return PMPIC_ERR_DELTA;
+  if (delz > 1. || delz < 0) return PMPIC_ERR_DELTA;
+  i0 = i - 3;
+  if (i0 >= g.nx) i0 -= g.nx;
+  j0 = j - 3;
+  if (j0 >= g.ny) j0 -= g.ny;
+  k0 = k - 1;
+  return PMPIC_OK;
+}
+
+// the 8 trilinear corner weights times volume, in the reference's order and association
+// (parcel_interpolation.F90:390-434): corner bits z<<2 | y<<1 | x
+__device__ __forceinline__ void corner_weights(double delx, double dely, double delz, double v,
+                                               double w[8]) {
+  w[0] = (1 - delz) * (1 - dely) * (1 - delx) * v;
+  w[1] = (1 - delz) * (1 - dely) * (delx) * v;
+  w[2] = (1 - delz) * (dely) * (1 - delx) * v;
+  w[3] = (1 - delz) * (dely) * (delx) * v;
+  w[4] = (delz) * (1 - dely) * (1 - delx) * v;
+  w[5] = (delz) * (1 - dely) * (delx) * v;
+  w[6] = (delz) * (dely) * (1 - delx) * v;
+  w[7] = (delz) * (dely) * (delx) * v;
+}
+
+__device__ __forceinline__ int64_t gidx(const Geometry &g, int k, int j, int i) {
+  return ((int64_t)k * g.ny + j) * g.nx + i;
+}
+
+// ---- launchers implemented in the translation units ----
+// interp.hip
+int bin_parcels(Ctx *c, int64_t n);
+int par2grid_fused(Ctx *c, int64_t n);  // P,Q,R,B grids from p,q,r,btot
+int par2grid_single(Ctx *c, int64_t n, const double *d_var, double *d_grid);
+int grid2par_single(Ctx *c, int64_t n, const double *d_grid, double *d_var);
+int mixing_grids(Ctx *c, int64_t n);
+// fft.hip
+int fft_setup(Ctx *c);
+int fft_forward(Ctx *c, int nf, const double *const *src, double *const *dst);
+int fft_backward(Ctx *c, int nf, const double *const *src, double *const *dst);
+// spectral.hip
+int spectral_setup(Ctx *c);
+int vort2vel_spectral(Ctx *c, int stage);
+int vorticity_tendency_spectral(Ctx *c, int stage);
+int op_diffx(Ctx *c, const double *in, double *out);
+int op_diffy(Ctx *c, const double *in, double *out);
+int op_filter(Ctx *c, const double *in, double *out);
+int op_diffz(Ctx *c, const double *in, double *out, const double *bot, const double *top);
+int op_laplinv(Ctx *c, double *f, int df0);
+int grid_max_reduce(Ctx *c);
+// parcels.hip
+int rk_stage_update(Ctx *c, int stage, int64_t n, int stretch);
+int timing_begin(Ctx *c, int which);
+int timing_end(Ctx *c, int which);
+int timing_harvest(Ctx *c);
+int parcel_haloswap(Ctx *c, int64_t n, int reflect);
+int parcel_split(Ctx *c);
+int parcel_mix(Ctx *c);
+int compact_flags(Ctx *c, const uint8_t *flags, uint8_t want, int64_t n, int32_t *out,
+                  int64_t *d_count);
+int scan_exclusive_i32(Ctx *c, const int32_t *in, int32_t *out, int64_t n);
+int sync_scalars(Ctx *c);
+
+}  // namespace pmpic

@@ -1,0 +1,314 @@
+// Semi-spectral 2-D transforms on MI355X (periodic x, y; z stays physical).
+//
+// Reference: model_core/src/fft/pencilfft.F90:151-212 (forward: r2c along y, (Re,Im) pairs
+// interleaved, then r2c along x of each real y-row; backward: the reverse), over FFTE's 1-D
+// transforms wrapped by model_core/src/fft/ffte_mod.F90:66-124 (forward sum x_j e^{-2 pi i jk/N},
+// unnormalised; inverse = conj . forward . conj, scaled by 1/N, real part kept, so the imaginary
+// parts of the DC and Nyquist bins are dropped).
+//
+// The spectral element S[k][2m+a][2l+b] is component b (Re/Im) at x-wavenumber l of the x-DFT
+// of component a of the y-DFT at y-wavenumber m.  With F(m,l) the full 2-D DFT,
+//   x-DFT of Re Y_m = (F(m,l) + F(-m,l))/2,   x-DFT of Im Y_m = (F(m,l) - F(-m,l))/(2i),
+// so the layout follows from one real-to-complex pass along the contiguous x rows and one
+// complex pass along y.  Each pass is one kernel: a workgroup stages a batch of rows (or of
+// adjacent columns, for coalesced strided access) in LDS and runs a mixed radix-4/2/3/5
+// Stockham FFT there, so each pass reads and writes HBM once.  The inverse builds, per (z, m),
+// the complex row H = A_m + i B_m of the two real x-spectra, transforms it along x, and finishes
+// with a complex-to-real pass along y.
+#include "common.h"
+
+#include <math.h>
+#include <vector>
+
+namespace pmpic {
+
+namespace {
+
+struct Plan {
+  int n;
+  int nrad;
+  int rad[16];
+};
+
+constexpr int FFT_TPB = 256;
+constexpr int LDS_CPLX = 2048;  // complex elements per LDS buffer (32 KiB in FP64)
+constexpr int MAXF = 8;
+
+struct FieldPtrs {
+  const double *in[MAXF];
+  double *out[MAXF];
+};
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) {
+  return make_double2(a.x + b.x, a.y + b.y);
+}
+
+// nb transforms of length p.n stored t-major in `a`; result in the returned buffer.
+// tw[e] = exp(sign 2 pi i e / n) staged in LDS.
+__device__ double2 *lds_fft(double2 *a, double2 *b, const double2 *tw, const Plan &p, int nb) {
+  const int n = p.n;
+  int Ns = 1;
+  for (int s = 0; s < p.nrad; s++) {
+    const int r = p.rad[s];
+    const int nq = n / r;
+    const int tot = nb * nq;
+    for (int bi = threadIdx.x; bi < tot; bi += blockDim.x) {
+      const int t = bi / nq, i = bi - t * nq;
+      const int k = i % Ns;
+      const double2 *x = a + t * n;
+      double2 *y = b + t * n;
+      double2 v[5];
+      const int tstep = n / (Ns * r);
+      for (int q = 0; q < r; q++) {
+        double2 e = x[i + q * nq];
+        int te = (q * k * tstep) % n;
+        v[q] = (te == 0) ? e : cmul(e, tw[te]);
+      }
+      const int base = (i / Ns) * Ns * r + k;
+      const int rstep = n / r;
+      for (int o = 0; o < r; o++) {
+        double2 acc = v[0];
+        for (int q = 1; q < r; q++) {
+          int te = (o * q * rstep) % n;
+          acc = cadd(acc, (te == 0) ? v[q] : cmul(v[q], tw[te]));
+        }
+        y[base + o * Ns] = acc;
+      }
+    }
+    __syncthreads();
+    double2 *sw = a;
+    a = b;
+    b = sw;
+    Ns *= r;
+  }
+  return a;
+}
+
+__device__ void stage_twiddles(double2 *tws, const double2 *tw, int n, bool conj_) {
+  for (int e = threadIdx.x; e < n; e += blockDim.x) {
+    double2 w = tw[e];
+    if (conj_) w.y = -w.y;
+    tws[e] = w;
+  }
+}
+
+// forward pass 1: r2c along x of every row (k, y) -> C1[k][y][l], l <= nx/2
+__global__ void __launch_bounds__(FFT_TPB) k_rows_r2c(Plan p, const double2 *__restrict__ tw,
+                                                      FieldPtrs fp, double2 *__restrict__ c1,
+                                                      int64_t c1_field, int nrows, int nb) {
+  __shared__ double2 A[LDS_CPLX], B[LDS_CPLX], T[512];
+  const int f = blockIdx.y;
+  const double *src = fp.in[f];
+  double2 *dst = c1 + (int64_t)f * c1_field;
+  const int n = p.n, nl = n / 2 + 1;
+  const int r0 = blockIdx.x * nb;
+  const int nbb = min(nb, nrows - r0);
+  stage_twiddles(T, tw, n, false);
+  for (int e = threadIdx.x; e < nbb * n; e += blockDim.x) A[e] = make_double2(src[(int64_t)r0 * n + e], 0.0);
+  __syncthreads();
+  double2 *R = lds_fft(A, B, T, p, nbb);
+  for (int e = threadIdx.x; e < nbb * nl; e += blockDim.x) {
+    int t = e / nl, l = e - t * nl;
+    dst[(int64_t)(r0 + t) * nl + l] = R[t * n + l];
+  }
+}
+
+// forward pass 2: complex FFT along y of nb adjacent columns l of one z level, then the
+// (Re,Im)-pair split into the semi-spectral layout
+__global__ void __launch_bounds__(FFT_TPB) k_cols_fwd(Plan p, const double2 *__restrict__ tw,
+                                                      FieldPtrs fp, const double2 *__restrict__ c1,
+                                                      int64_t c1_field, int nl, int nb, int sx,
+                                                      int sy) {
+  __shared__ double2 A[LDS_CPLX], B[LDS_CPLX], T[512];
+  const int f = blockIdx.z;
+  const double2 *src = c1 + (int64_t)f * c1_field;
+  double *dst = fp.out[f];
+  const int ny = p.n;
+  const int k = blockIdx.y;
+  const int l0 = blockIdx.x * nb;
+  const int nbb = min(nb, nl - l0);
+  stage_twiddles(T, tw, ny, false);
+  const double2 *lev = src + (int64_t)k * ny * nl;
+  for (int e = threadIdx.x; e < nbb * ny; e += blockDim.x) {
+    int y = e / nbb, t = e - y * nbb;
+    A[t * ny + y] = lev[(int64_t)y * nl + l0 + t];
+  }
+  __syncthreads();
+  double2 *R = lds_fft(A, B, T, p, nbb);
+  const int nm = ny / 2 + 1;
+  double *out = dst + (int64_t)k * sy * sx;
+  for (int e = threadIdx.x; e < nbb * nm; e += blockDim.x) {
+    int m = e / nbb, t = e - m * nbb;
+    int l = l0 + t;
+    double2 g = R[t * ny + m];
+    double2 h = R[t * ny + ((ny - m) % ny)];
+    double2 s = make_double2(g.x + h.x, g.y + h.y);
+    double2 d = make_double2(g.x - h.x, g.y - h.y);
+    out[(int64_t)(2 * m) * sx + 2 * l] = s.x / 2;
+    out[(int64_t)(2 * m) * sx + 2 * l + 1] = s.y / 2;
+    out[(int64_t)(2 * m + 1) * sx + 2 * l] = d.y / 2;
+    out[(int64_t)(2 * m + 1) * sx + 2 * l + 1] = -d.x / 2;
+  }
+}
+
+// inverse pass 1: per (k, m) build H = A_m + i B_m over l in [0, nx) from the two real
+// x-spectra (imaginary parts of the DC and Nyquist bins dropped as c2r drops them), inverse
+// FFT along x, scale 1/nx -> C2[k][m][x]
+__global__ void __launch_bounds__(FFT_TPB) k_rows_inv(Plan p, const double2 *__restrict__ tw,
+                                                      FieldPtrs fp, double2 *__restrict__ c2,
+                                                      int64_t c2_field, int nrows, int nm, int nb,
+                                                      int sx) {
+  __shared__ double2 A[LDS_CPLX], B[LDS_CPLX], T[512];
+  const int f = blockIdx.y;
+  const double *src = fp.in[f];
+  double2 *dst = c2 + (int64_t)f * c2_field;
+  const int nx = p.n;
+  const int r0 = blockIdx.x * nb;
+  const int nbb = min(nb, nrows - r0);
+  stage_twiddles(T, tw, nx, true);
+  const int half = nx / 2;
+  const bool even = (nx % 2) == 0;
+  for (int e = threadIdx.x; e < nbb * nx; e += blockDim.x) {
+    int t = e / nx, l = e - t * nx;
+    int row = r0 + t;          // row = k * nm + m
+    int k = row / nm, m = row - k * nm;
+    const double *sa = src + ((int64_t)k * (2 * nm) + 2 * m) * sx;
+    const double *sb = sa + sx;
+    int ll = (l <= half) ? l : nx - l;
+    double ar = sa[2 * ll], ai = sa[2 * ll + 1], br = sb[2 * ll], bi = sb[2 * ll + 1];
+    if (ll == 0 || (even && ll == half)) { ai = 0.0; bi = 0.0; }
+    double2 h;
+    if (l <= half) h = make_double2(ar - bi, ai + br);
+    else h = make_double2(ar + bi, -ai + br);
+    A[t * nx + l] = h;
+  }
+  __syncthreads();
+  double2 *R = lds_fft(A, B, T, p, nbb);
+  const double dn = 1.0 / (double)nx;
+  for (int e = threadIdx.x; e < nbb * nx; e += blockDim.x) {
+    int t = e / nx, x = e - t * nx;
+    double2 v = R[t * nx + x];
+    dst[(int64_t)(r0 + t) * nx + x] = make_double2(v.x * dn, v.y * dn);
+  }
+}
+
+// inverse pass 2: c2r along y for nb adjacent x columns of one z level
+__global__ void __launch_bounds__(FFT_TPB) k_cols_c2r(Plan p, const double2 *__restrict__ tw,
+                                                      FieldPtrs fp, const double2 *__restrict__ c2,
+                                                      int64_t c2_field, int nx, int nb) {
+  __shared__ double2 A[LDS_CPLX], B[LDS_CPLX], T[512];
+  const int f = blockIdx.z;
+  const double2 *src = c2 + (int64_t)f * c2_field;
+  double *dst = fp.out[f];
+  const int ny = p.n, nm = ny / 2 + 1;
+  const int k = blockIdx.y;
+  const int x0 = blockIdx.x * nb;
+  const int nbb = min(nb, nx - x0);
+  stage_twiddles(T, tw, ny, true);
+  const double2 *lev = src + (int64_t)k * nm * nx;
+  const bool even = (ny % 2) == 0;
+  for (int e = threadIdx.x; e < nbb * ny; e += blockDim.x) {
+    int y = e / nbb, t = e - y * nbb;
+    int m = (y < nm) ? y : ny - y;
+    double2 w = lev[(int64_t)m * nx + x0 + t];
+    if (m == 0 || (even && m == ny / 2)) w.y = 0.0;
+    if (y >= nm) w.y = -w.y;
+    A[t * ny + y] = w;
+  }
+  __syncthreads();
+  double2 *R = lds_fft(A, B, T, p, nbb);
+  const double dn = 1.0 / (double)ny;
+  double *out = dst + (int64_t)k * ny * nx;
+  for (int e = threadIdx.x; e < nbb * ny; e += blockDim.x) {
+    int y = e / nbb, t = e - y * nbb;
+    out[(int64_t)y * nx + x0 + t] = R[t * ny + y].x * dn;
+  }
+}
+
+Plan make_plan(int n) {
+  Plan p{};
+  p.n = n;
+  int m = n;
+  while (m % 4 == 0) { p.rad[p.nrad++] = 4; m /= 4; }
+  while (m % 2 == 0) { p.rad[p.nrad++] = 2; m /= 2; }
+  while (m % 3 == 0) { p.rad[p.nrad++] = 3; m /= 3; }
+  while (m % 5 == 0) { p.rad[p.nrad++] = 5; m /= 5; }
+  if (m != 1) p.nrad = -1;
+  return p;
+}
+
+int upload_twiddles(int n, double2 **dptr) {
+  std::vector<double2> h(n);
+  for (int e = 0; e < n; e++) {
+    // quarter turns are exact so that purely real or imaginary inputs stay so
+    if ((4L * e) % n == 0) {
+      const int q = (int)((4L * e) / n);
+      const double re[4] = {1.0, 0.0, -1.0, 0.0}, im[4] = {0.0, -1.0, 0.0, 1.0};
+      h[e] = make_double2(re[q], im[q]);
+      continue;
+    }
+    long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)e / (long double)n;
+    h[e] = make_double2((double)cosl(ang), (double)sinl(ang));
+  }
+  PMPIC_CHECK(hipMalloc(dptr, sizeof(double2) * n));
+  PMPIC_CHECK(hipMemcpy(*dptr, h.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
+  return PMPIC_OK;
+}
+
+}  // namespace
+
+int fft_setup(Ctx *c) {
+  if (c->g.nx > 512 || c->g.ny > 512) return PMPIC_ERR_FFT_SIZE;
+  if (make_plan(c->g.nx).nrad < 0 || make_plan(c->g.ny).nrad < 0) return PMPIC_ERR_FFT_SIZE;
+  PMPIC_TRY(upload_twiddles(c->g.nx, &c->tw_x));
+  PMPIC_TRY(upload_twiddles(c->g.ny, &c->tw_y));
+  return PMPIC_OK;
+}
+
+static int64_t c1_field_size(const Geometry &g) { return (int64_t)g.nzn * g.ny * (g.nx / 2 + 1); }
+static int64_t c2_field_size(const Geometry &g) { return (int64_t)g.nzn * (g.ny / 2 + 1) * g.nx; }
+
+int fft_forward(Ctx *c, int nf, const double *const *src, double *const *dst) {
+  const Geometry &g = c->g;
+  if (nf < 1 || nf > MAXF) return PMPIC_ERR_ARG;
+  FieldPtrs fp{};
+  for (int f = 0; f < nf; f++) { fp.in[f] = src[f]; fp.out[f] = dst[f]; }
+  Plan px = make_plan(g.nx), py = make_plan(g.ny);
+  const int64_t c1f = c1_field_size(g);
+  const int nrows = g.nzn * g.ny;
+  const int nbx = LDS_CPLX / g.nx;
+  hipLaunchKernelGGL(k_rows_r2c, dim3(cdiv(nrows, nbx), nf), dim3(FFT_TPB), 0, c->stream, px,
+                     c->tw_x, fp, c->cbuf, c1f, nrows, nbx);
+  const int nl = g.nx / 2 + 1;
+  int nby = LDS_CPLX / g.ny;
+  if (nby > 32) nby = 32;
+  hipLaunchKernelGGL(k_cols_fwd, dim3(cdiv(nl, nby), g.nzn, nf), dim3(FFT_TPB), 0, c->stream, py,
+                     c->tw_y, fp, c->cbuf, c1f, nl, nby, g.sx, g.sy);
+  PMPIC_CHECK(hipGetLastError());
+  return PMPIC_OK;
+}
+
+int fft_backward(Ctx *c, int nf, const double *const *src, double *const *dst) {
+  const Geometry &g = c->g;
+  if (nf < 1 || nf > MAXF) return PMPIC_ERR_ARG;
+  FieldPtrs fp{};
+  for (int f = 0; f < nf; f++) { fp.in[f] = src[f]; fp.out[f] = dst[f]; }
+  Plan px = make_plan(g.nx), py = make_plan(g.ny);
+  const int64_t c2f = c2_field_size(g);
+  const int nm = g.ny / 2 + 1;
+  const int nrows = g.nzn * nm;
+  const int nbx = LDS_CPLX / g.nx;
+  hipLaunchKernelGGL(k_rows_inv, dim3(cdiv(nrows, nbx), nf), dim3(FFT_TPB), 0, c->stream, px,
+                     c->tw_x, fp, c->cbuf, c2f, nrows, nm, nbx, g.sx);
+  int nby = LDS_CPLX / g.ny;
+  if (nby > 32) nby = 32;
+  hipLaunchKernelGGL(k_cols_c2r, dim3(cdiv(g.nx, nby), g.nzn, nf), dim3(FFT_TPB), 0, c->stream,
+                     py, c->tw_y, fp, c->cbuf, c2f, g.nx, nby);
+  PMPIC_CHECK(hipGetLastError());
+  return PMPIC_OK;
+}
+
+}  // namespace pmpic

@@ -1,0 +1,562 @@
+// Semi-spectral operators and the vort2vel / vorticity_tendency compositions on MI355X.
+//
+// Reference: model_core/src/fft/fft_ops.F90 (fftops_init :48-168, spectral_filter :176-197,
+// diffx :204-302, diffy :305-401, diffz :419-483, laplinv :505-601, tridiagonal :620-656),
+// components/vort2vel/src/vort2vel.F90:98-400 and
+// components/vorticity_tendency/src/vorticity_tendency.F90:107-334.
+//
+// Every operator is memory bound: element-wise kernels over the [nzn][sy][sx] spectral arrays
+// (fused where the reference's sequence allows it without changing any operation) and column
+// kernels with one thread per (kx,ky) column walking z; with z slowest in memory the column
+// walk is coalesced across the threads of a wavefront.  The tridiagonal solves keep the
+// reference's Thomas recurrence operation for operation (FP32-rounded literal coefficients,
+// skipped k^2 = 0 columns).
+#include "common.h"
+
+#include <math.h>
+#include <vector>
+
+namespace pmpic {
+
+namespace {
+
+constexpr int TPB = 256;
+
+struct SpecDims {
+  int sx, sy, nzn;
+  int64_t ssz;
+};
+
+__device__ __forceinline__ SpecDims sdims(const Geometry &g) {
+  SpecDims d;
+  d.sx = g.sx; d.sy = g.sy; d.nzn = g.nzn;
+  d.ssz = (int64_t)g.nzn * g.sy * g.sx;
+  return d;
+}
+
+// diffx of element q (fft_ops.F90:266-269, :295): pair swap along x, times kx
+__device__ __forceinline__ double dx_at(const double *in, const double *kx, int64_t q, int ii,
+                                        int64_t col) {
+  return ((ii & 1) == 0 ? (-1. * in[q + 1]) : in[q - 1]) * kx[col];
+}
+// diffy of element q (fft_ops.F90:366-369, :394)
+__device__ __forceinline__ double dy_at(const double *in, const double *ky, int64_t q, int jj,
+                                        int sx, int64_t col) {
+  return ((jj & 1) == 0 ? (-1. * in[q + sx]) : in[q - sx]) * ky[col];
+}
+
+__global__ void k_diffx(Geometry g, const double *__restrict__ in, double *__restrict__ out,
+                        const double *__restrict__ kx) {
+  const int64_t ssz = (int64_t)g.nzn * g.sy * g.sx;
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ssz) return;
+  const int64_t col = q % ((int64_t)g.sy * g.sx);
+  const int ii = (int)(q % g.sx);
+  out[q] = dx_at(in, kx, q, ii, col);
+}
+
+__global__ void k_diffy(Geometry g, const double *__restrict__ in, double *__restrict__ out,
+                        const double *__restrict__ ky) {
+  const int64_t ssz = (int64_t)g.nzn * g.sy * g.sx;
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ssz) return;
+  const int64_t col = q % ((int64_t)g.sy * g.sx);
+  const int jj = (int)(col / g.sx);
+  out[q] = dy_at(in, ky, q, jj, g.sx, col);
+}
+
+__global__ void k_filter(Geometry g, const double *__restrict__ in, double *__restrict__ out,
+                         const double *__restrict__ filt) {
+  const int64_t ssz = (int64_t)g.nzn * g.sy * g.sx;
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ssz) return;
+  out[q] = in[q] * filt[q % ((int64_t)g.sy * g.sx)];
+}
+
+// vort2vel.F90:158-174: f = (diffx(a) + diffy(b)) + dc/dz (explicit 2nd order, one-sided at
+// the ends), then the constant column zeroed
+__global__ void k_divergence(Geometry g, const double *__restrict__ a, const double *__restrict__ b,
+                             const double *__restrict__ c, double *__restrict__ f,
+                             const double *__restrict__ kx, const double *__restrict__ ky,
+                             double hdzi) {
+  const int64_t plane = (int64_t)g.sy * g.sx;
+  const int64_t ssz = (int64_t)g.nzn * plane;
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ssz) return;
+  const int64_t col = q % plane;
+  const int k = (int)(q / plane);
+  const int ii = (int)(q % g.sx), jj = (int)(col / g.sx);
+  const int nz = g.nzn;
+  const double d = dx_at(a, kx, q, ii, col);
+  const double e = dy_at(b, ky, q, jj, g.sx, col);
+  double fz;
+  if (k == 0)
+    fz = hdzi * (4. * c[q + plane] - c[q + 2 * plane] - 3. * c[q]);
+  else if (k == nz - 1)
+    fz = hdzi * (c[q - 2 * plane] + 3. * c[q] - 4. * c[q - plane]);
+  else
+    fz = (c[q + plane] - c[q - plane]) * hdzi;
+  double r = d + e + fz;
+  if (col == 0) r = 0.0;
+  f[q] = r;
+}
+
+// vort2vel.F90:184-194: a -= diffx(f); b -= diffy(f)
+__global__ void k_correct_ab(Geometry g, double *__restrict__ a, double *__restrict__ b,
+                             const double *__restrict__ f, const double *__restrict__ kx,
+                             const double *__restrict__ ky) {
+  const int64_t plane = (int64_t)g.sy * g.sx;
+  const int64_t ssz = (int64_t)g.nzn * plane;
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ssz) return;
+  const int64_t col = q % plane;
+  const int ii = (int)(q % g.sx), jj = (int)(col / g.sx);
+  a[q] = a[q] - dx_at(f, kx, q, ii, col);
+  b[q] = b[q] - dy_at(f, ky, q, jj, g.sx, col);
+}
+
+// vort2vel.F90:197-218: c -= dz(f); c(level 1) = 0; d,e,f = filtered a,b,c; cache the top and
+// bottom levels of the unfiltered a, b
+__global__ void k_correct_c_filter(Geometry g, const double *__restrict__ a,
+                                   const double *__restrict__ b, double *__restrict__ c,
+                                   const double *__restrict__ dzf, double *__restrict__ d,
+                                   double *__restrict__ e, double *__restrict__ f,
+                                   const double *__restrict__ filt, double *__restrict__ atop,
+                                   double *__restrict__ abot, double *__restrict__ btop,
+                                   double *__restrict__ bbot) {
+  const int64_t plane = (int64_t)g.sy * g.sx;
+  const int64_t ssz = (int64_t)g.nzn * plane;
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ssz) return;
+  const int64_t col = q % plane;
+  const int k = (int)(q / plane);
+  double cv = c[q] - dzf[q];
+  if (k == 0) cv = 0.;
+  c[q] = cv;
+  const double fl = filt[col];
+  d[q] = a[q] * fl;
+  e[q] = b[q] * fl;
+  f[q] = cv * fl;
+  if (k == g.nzn - 1) { atop[col] = a[q]; btop[col] = b[q]; }
+  if (k == 0) { abot[col] = a[q]; bbot[col] = b[q]; }
+}
+
+// mean flow, vort2vel.F90:230-241 (sequential over z; 1./nz is a default-real quotient)
+__global__ void k_mean_flow(Geometry g, const double *__restrict__ d, const double *__restrict__ e,
+                            double *__restrict__ ubar, double *__restrict__ vbar) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int64_t plane = (int64_t)g.sy * g.sx;
+  const int nz = g.nzn;
+  const double dz = g.dz;
+  ubar[0] = 0;
+  vbar[0] = 0;
+  for (int k = 1; k <= nz - 1; k++) {
+    ubar[k] = ubar[k - 1] + dz / 2. * (e[(int64_t)(k - 1) * plane] + e[(int64_t)k * plane]);
+    vbar[k] = vbar[k - 1] - dz / 2. * (d[(int64_t)(k - 1) * plane] + d[(int64_t)k * plane]);
+  }
+  const double rnz = (double)(1.f / (float)nz);
+  double su = 0, sv = 0;
+  for (int k = 1; k <= nz - 1; k++) {
+    su += ubar[k - 1] + 0.5 * ubar[nz - 1];
+    sv += vbar[k - 1] + 0.5 * vbar[nz - 1];
+  }
+  const double uavg = rnz * su, vavg = rnz * sv;
+  for (int k = 0; k < nz; k++) {
+    ubar[k] = ubar[k] - uavg;
+    vbar[k] = vbar[k] - vavg;
+  }
+}
+
+// laplinv (fft_ops.F90:505-601), in place, one thread per column.  The right-hand side of row
+// k reads the original f(k-1), f(k), f(k+1); a sliding window keeps the originals while the
+// forward sweep overwrites f with d'.  cp is staged in scratch (same layout).
+__global__ void k_laplinv(Geometry g, double *__restrict__ f, double *__restrict__ cp,
+                          const double *__restrict__ k2a, int df0, DevScalars *sc) {
+  const int64_t plane = (int64_t)g.sy * g.sx;
+  int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= plane) return;
+  const double k2 = k2a[col];
+  if (k2 == 0) return;
+  const int nz = g.nzn;
+  const double f16 = (double)(1.f / 6.f), f13 = (double)(1.f / 3.f);
+  const double f112 = (double)(1.f / 12.f), f56 = (double)(5.f / 6.f);
+  const double dz = g.dz;
+  const double i2dz2 = 2. / dz / dz, idz2 = 1. / dz / dz;
+  // interior coefficients (constant along the column)
+  const double ai = idz2 - f112 * k2, bi = -i2dz2 - f56 * k2, ci = idz2 - f112 * k2;
+  double *F = f + col;
+  double *CP = cp + col;
+  double fm = 0.0, f0 = F[0], fp = F[plane];  // originals f(k-1), f(k), f(k+1)
+  // row 1
+  double b1, c1, d1;
+  if (!df0) { b1 = 1.; c1 = 0.; d1 = 0.; }
+  else { b1 = -idz2 - f13 * k2; c1 = idz2 - f16 * k2; d1 = f13 * f0 + f16 * fp; }
+  double denominator = b1;
+  double fprev = d1 / denominator;
+  F[0] = fprev;
+  double cprev = c1;  // c(i-1)
+  for (int k = 2; k <= nz; k++) {
+    fm = f0;
+    f0 = fp;
+    fp = (k < nz) ? F[(int64_t)k * plane] : 0.0;
+    double a, b, c, d;
+    if (k < nz) {
+      a = ai; b = bi; c = ci;
+      d = f112 * fm + f56 * f0 + f112 * fp;
+    } else if (!df0) {
+      a = 0.; b = 1.; c = 0.; d = 0.;
+    } else {
+      a = idz2 - f16 * k2; b = -idz2 - f13 * k2; c = 0.;
+      d = f13 * f0 + f16 * fm;
+    }
+    const double cpk = cprev / denominator;
+    CP[(int64_t)(k - 1) * plane] = cpk;
+    denominator = b - a * cpk;
+    if (denominator == 0) { raise_status(sc, PMPIC_ERR_ZERO_PIVOT); return; }
+    fprev = (d - a * fprev) / (denominator);
+    F[(int64_t)(k - 1) * plane] = fprev;
+    cprev = c;
+  }
+  double fnext = fprev;
+  for (int k = nz - 1; k >= 1; k--) {
+    const double v = F[(int64_t)(k - 1) * plane] - CP[(int64_t)k * plane] * fnext;
+    F[(int64_t)(k - 1) * plane] = v;
+    fnext = v;
+  }
+}
+
+// diffz (fft_ops.F90:419-483), out of place, one thread per column
+__global__ void k_diffz(Geometry g, const double *__restrict__ f, double *__restrict__ out,
+                        double *__restrict__ cp, const double *__restrict__ bot,
+                        const double *__restrict__ top, DevScalars *sc) {
+  const int64_t plane = (int64_t)g.sy * g.sx;
+  int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= plane) return;
+  const int nz = g.nzn;
+  const bool bt = (bot != nullptr && top != nullptr);
+  const double a1 = (double)(1.f / 6.f), b1 = (double)(2.f / 3.f), c1 = (double)(1.f / 6.f);
+  const double al = (double)(1.f / 3.f), cl = (double)(1.f / 3.f);
+  const double dz = g.dz;
+  const double *F = f + col;
+  double *O = out + col;
+  double *CP = cp + col;
+  // row 1
+  double d = bt ? F[plane] / dz - a1 * bot[col] * dz : 0.0;
+  double denominator = 1.;  // b(1) = b0
+  double fprev = d / denominator;
+  O[0] = fprev;
+  double cprev = bt ? cl : 0.;  // c(1)
+  for (int k = 2; k <= nz; k++) {
+    double a, b, c;
+    if (k < nz) {
+      a = a1; b = b1; c = c1;
+      d = (F[(int64_t)k * plane] - F[(int64_t)(k - 2) * plane]) / 2. / dz;
+    } else {
+      a = bt ? al : 0.; b = 1.; c = 0.;
+      d = bt ? -F[(int64_t)(nz - 2) * plane] / dz + a1 * top[col] * dz : 0.0;
+    }
+    const double cpk = cprev / denominator;
+    CP[(int64_t)(k - 1) * plane] = cpk;
+    denominator = b - a * cpk;
+    if (denominator == 0) { raise_status(sc, PMPIC_ERR_ZERO_PIVOT); return; }
+    fprev = (d - a * fprev) / (denominator);
+    O[(int64_t)(k - 1) * plane] = fprev;
+    cprev = c;
+  }
+  double fnext = fprev;
+  for (int k = nz - 1; k >= 1; k--) {
+    const double v = O[(int64_t)(k - 1) * plane] - CP[(int64_t)k * plane] * fnext;
+    O[(int64_t)(k - 1) * plane] = v;
+    fnext = v;
+  }
+}
+
+// u, v, w spectra (vort2vel.F90:251-320): fu = dzb - diffy(c), column (1,1) = ubar;
+// fv = diffx(c) - dza, column (1,1) = vbar; fw = diffy(a) - diffx(b); u_s.. = filtered
+__global__ void k_velocity_spectra(Geometry g, const double *__restrict__ a,
+                                   const double *__restrict__ b, const double *__restrict__ c,
+                                   const double *__restrict__ dzb, const double *__restrict__ dza,
+                                   const double *__restrict__ ubar, const double *__restrict__ vbar,
+                                   double *__restrict__ fu, double *__restrict__ fv,
+                                   double *__restrict__ fw, double *__restrict__ us,
+                                   double *__restrict__ vs, double *__restrict__ ws,
+                                   const double *__restrict__ kx, const double *__restrict__ ky,
+                                   const double *__restrict__ filt) {
+  const int64_t plane = (int64_t)g.sy * g.sx;
+  const int64_t ssz = (int64_t)g.nzn * plane;
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ssz) return;
+  const int64_t col = q % plane;
+  const int k = (int)(q / plane);
+  const int ii = (int)(q % g.sx), jj = (int)(col / g.sx);
+  const double fl = filt[col];
+  double u = dzb[q] - dy_at(c, ky, q, jj, g.sx, col);
+  if (col == 0) u = ubar[k];
+  double v = dx_at(c, kx, q, ii, col) - dza[q];
+  if (col == 0) v = vbar[k];
+  const double w = dy_at(a, ky, q, jj, g.sx, col) - dx_at(b, kx, q, ii, col);
+  fu[q] = u; fv[q] = v; fw[q] = w;
+  us[q] = u * fl; vs[q] = v * fl; ws[q] = w * fl;
+}
+
+// the eight spectral derivatives of vorticity_tendency.F90:155-232
+__global__ void k_tendency_spectra(Geometry g, const double *__restrict__ bs,
+                                   const double *__restrict__ us, const double *__restrict__ vs,
+                                   const double *__restrict__ ws, double *__restrict__ o0,
+                                   double *__restrict__ o1, double *__restrict__ o2,
+                                   double *__restrict__ o3, double *__restrict__ o4,
+                                   double *__restrict__ o5, double *__restrict__ o6,
+                                   double *__restrict__ o7, const double *__restrict__ kx,
+                                   const double *__restrict__ ky) {
+  const int64_t plane = (int64_t)g.sy * g.sx;
+  const int64_t ssz = (int64_t)g.nzn * plane;
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ssz) return;
+  const int64_t col = q % plane;
+  const int ii = (int)(q % g.sx), jj = (int)(col / g.sx);
+  o0[q] = dy_at(bs, ky, q, jj, g.sx, col);   // db/dy
+  o1[q] = dx_at(us, kx, q, ii, col);         // du/dx
+  o2[q] = dy_at(us, ky, q, jj, g.sx, col);   // du/dy
+  o3[q] = dx_at(ws, kx, q, ii, col);         // dw/dx
+  o4[q] = dx_at(bs, kx, q, ii, col);         // db/dx
+  o5[q] = dx_at(vs, kx, q, ii, col);         // dv/dx
+  o6[q] = dy_at(vs, ky, q, jj, g.sx, col);   // dv/dy
+  o7[q] = dy_at(ws, ky, q, jj, g.sx, col);   // dw/dy
+}
+
+// vorticity_tendency.F90:169-265, physical interior, accumulation order as written
+__global__ void k_tendency(int64_t nnode, const double *__restrict__ P,
+                           const double *__restrict__ Q, const double *__restrict__ R,
+                           const double *__restrict__ dbdy, const double *__restrict__ dudx,
+                           const double *__restrict__ dudy, const double *__restrict__ dwdx,
+                           const double *__restrict__ dbdx, const double *__restrict__ dvdx,
+                           const double *__restrict__ dvdy, const double *__restrict__ dwdy,
+                           double *__restrict__ DP, double *__restrict__ DQ,
+                           double *__restrict__ DR) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nnode) return;
+  const double p = P[q], qq = Q[q], r = R[q];
+  double dp = dbdy[q];
+  dp = dp + p * dudx[q];
+  dp = dp + qq * dudy[q];
+  dp = dp + r * (qq + dwdx[q]);
+  double dq = dbdx[q];
+  dq = -dq + p * dvdx[q];
+  dq = dq + qq * dvdy[q];
+  dq = dq + r * (-p + dwdy[q]);
+  double dr = p * dwdx[q];
+  dr = dr + qq * dwdy[q];
+  dr = dr + r * (-dudx[q]);
+  dr = dr + r * (-dvdy[q]);
+  DP[q] = dp; DQ[q] = dq; DR[q] = dr;
+}
+
+// maxima of u^2, v^2, w^2 and p^2+q^2+r^2 over the interior (vort2vel.F90:331-337,
+// vorticity_tendency.F90:279-282).  All values are non-negative, so their bit patterns order
+// like the values and an integer atomicMax gives a deterministic result.
+__global__ void k_max4(int64_t nnode, const double *__restrict__ U, const double *__restrict__ V,
+                       const double *__restrict__ W, const double *__restrict__ P,
+                       const double *__restrict__ Q, const double *__restrict__ R,
+                       DevScalars *sc) {
+  __shared__ double sm[4][TPB];
+  double m[4] = {0, 0, 0, 0};
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nnode;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const double u = U[q], v = V[q], w = W[q];
+    const double o = P[q] * P[q] + Q[q] * Q[q] + R[q] * R[q];
+    m[0] = fmax(m[0], u * u);
+    m[1] = fmax(m[1], v * v);
+    m[2] = fmax(m[2], w * w);
+    m[3] = fmax(m[3], o);
+  }
+  for (int f = 0; f < 4; f++) sm[f][threadIdx.x] = m[f];
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int f = 0; f < 4; f++) sm[f][threadIdx.x] = fmax(sm[f][threadIdx.x], sm[f][threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    unsigned long long *t = (unsigned long long *)&sc->umax2;
+    for (int f = 0; f < 4; f++)
+      atomicMax(t + f, (unsigned long long)__double_as_longlong(sm[f][0]));
+  }
+}
+
+__global__ void k_zero_max(DevScalars *sc) {
+  sc->umax2 = 0; sc->vmax2 = 0; sc->wmax2 = 0; sc->omax2 = 0;
+}
+
+// timestep selection at RK stage 1: vort2vel.F90:329-381 then vorticity_tendency.F90:277-321
+__global__ void k_select_dt(Geometry g, DevScalars *sc) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double umax = sqrt(sc->umax2), vmax = sqrt(sc->vmax2), wmax = sqrt(sc->wmax2);
+  const double tiny = (double)1.E-15f;
+  if (umax == 0) umax = tiny;
+  if (vmax == 0) vmax = tiny;
+  if (wmax == 0) wmax = tiny;
+  const double t1 = g.lx_cfl / umax, t2 = g.ly_cfl / vmax, t3 = g.lz_cfl / wmax;
+  double dtmaxg = t1;
+  if (t2 < dtmaxg) dtmaxg = t2;
+  if (t3 < dtmaxg) dtmaxg = t3;
+  double dtm = (sc->dtmax > dtmaxg) ? dtmaxg : sc->dtmax;
+  const double omax = sqrt(sc->omax2);
+  double dtv;
+  if (omax > 0.) dtv = sc->prefactor / omax;
+  else dtv = sc->dtmax;
+  if (dtm > dtv) dtm = dtv;
+  sc->dtm = dtm;
+}
+
+inline int64_t ssz_of(const Geometry &g) { return (int64_t)g.nzn * g.sy * g.sx; }
+inline int64_t plane_of(const Geometry &g) { return (int64_t)g.sy * g.sx; }
+
+}  // namespace
+
+// wavenumbers and 2/3-rule filter, fft_ops.F90:111-139 on one rank (x_start = y_start = 1)
+int spectral_setup(Ctx *c) {
+  const Geometry &g = c->g;
+  const int sx = g.sx, sy = g.sy;
+  std::vector<double> kx(sx * sy), ky(sx * sy), k2(sx * sy), fl(sx * sy);
+  const double twopi = 8.0 * atan(1.0);
+  const double lx = g.dx * (double)g.nx, ly = g.dy * (double)g.ny;
+  const double hx = (double)((float)g.nx / 2.0f), hy = (double)((float)g.ny / 2.0f);
+  double ksqmax = (hx / lx) * (hx / lx) + (hy / ly) * (hy / ly);
+  ksqmax = ksqmax * 2. / 9.;
+  for (int jj = 0; jj < sy; jj++)
+    for (int ii = 0; ii < sx; ii++) {
+      const double xval = (double)(ii / 2) / lx;
+      const double yval = (double)(jj / 2) / ly;
+      const double kk = xval * xval + yval * yval;
+      const int q = jj * sx + ii;
+      fl[q] = (kk > ksqmax) ? 0. : 1.;
+      kx[q] = xval * twopi;
+      ky[q] = yval * twopi;
+      k2[q] = kk * twopi * twopi;
+    }
+  const size_t b = sizeof(double) * sx * sy;
+  PMPIC_CHECK(hipMemcpy(c->kx, kx.data(), b, hipMemcpyHostToDevice));
+  PMPIC_CHECK(hipMemcpy(c->ky, ky.data(), b, hipMemcpyHostToDevice));
+  PMPIC_CHECK(hipMemcpy(c->k2, k2.data(), b, hipMemcpyHostToDevice));
+  PMPIC_CHECK(hipMemcpy(c->filt, fl.data(), b, hipMemcpyHostToDevice));
+  return PMPIC_OK;
+}
+
+int op_diffx(Ctx *c, const double *in, double *out) {
+  hipLaunchKernelGGL(k_diffx, dim3(cdiv(ssz_of(c->g), TPB)), dim3(TPB), 0, c->stream, c->g, in,
+                     out, c->kx);
+  PMPIC_CHECK(hipGetLastError());
+  return PMPIC_OK;
+}
+int op_diffy(Ctx *c, const double *in, double *out) {
+  hipLaunchKernelGGL(k_diffy, dim3(cdiv(ssz_of(c->g), TPB)), dim3(TPB), 0, c->stream, c->g, in,
+                     out, c->ky);
+  PMPIC_CHECK(hipGetLastError());
+  return PMPIC_OK;
+}
+int op_filter(Ctx *c, const double *in, double *out) {
+  hipLaunchKernelGGL(k_filter, dim3(cdiv(ssz_of(c->g), TPB)), dim3(TPB), 0, c->stream, c->g, in,
+                     out, c->filt);
+  PMPIC_CHECK(hipGetLastError());
+  return PMPIC_OK;
+}
+int op_diffz(Ctx *c, const double *in, double *out, const double *bot, const double *top) {
+  hipLaunchKernelGGL(k_diffz, dim3(cdiv(plane_of(c->g), 64)), dim3(64), 0, c->stream, c->g, in,
+                     out, c->sder[7], bot, top, c->dsc);
+  PMPIC_CHECK(hipGetLastError());
+  return PMPIC_OK;
+}
+int op_laplinv(Ctx *c, double *f, int df0) {
+  hipLaunchKernelGGL(k_laplinv, dim3(cdiv(plane_of(c->g), 64)), dim3(64), 0, c->stream, c->g, f,
+                     c->sder[7], c->k2, df0, c->dsc);
+  PMPIC_CHECK(hipGetLastError());
+  return PMPIC_OK;
+}
+
+// vort2vel timestep_callback (vort2vel.F90:134-320) after par2grid of p, q, r into the grids:
+// corrected and filtered vorticity back into grid[GP..GR], velocities into grid[GU..GW], the
+// filtered velocity spectra into su, sv, sw.
+int vort2vel_spectral(Ctx *c, int stage) {
+  (void)stage;
+  const Geometry &g = c->g;
+  const int64_t ssz = ssz_of(g);
+  const dim3 eb(cdiv(ssz, TPB)), et(TPB);
+  {
+    const double *src[3] = {c->grid[GP], c->grid[GQ], c->grid[GR]};
+    double *dst[3] = {c->sa, c->sb, c->sc};
+    PMPIC_TRY(fft_forward(c, 3, src, dst));
+  }
+  const double hdzi = 1. / 2. / g.dz;
+  hipLaunchKernelGGL(k_divergence, eb, et, 0, c->stream, g, c->sa, c->sb, c->sc, c->sf, c->kx,
+                     c->ky, hdzi);
+  PMPIC_TRY(op_laplinv(c, c->sf, 1));
+  PMPIC_TRY(op_filter(c, c->sf, c->sf));
+  hipLaunchKernelGGL(k_correct_ab, eb, et, 0, c->stream, g, c->sa, c->sb, c->sf, c->kx, c->ky);
+  PMPIC_TRY(op_diffz(c, c->sf, c->sd, nullptr, nullptr));
+  hipLaunchKernelGGL(k_correct_c_filter, eb, et, 0, c->stream, g, c->sa, c->sb, c->sc, c->sd,
+                     c->sd, c->se, c->sf, c->filt, c->atop, c->abot, c->btop, c->bbot);
+  hipLaunchKernelGGL(k_mean_flow, dim3(1), dim3(64), 0, c->stream, g, c->sd, c->se, c->ubar,
+                     c->vbar);
+  PMPIC_TRY(op_laplinv(c, c->sa, 0));
+  PMPIC_TRY(op_laplinv(c, c->sb, 0));
+  PMPIC_TRY(op_laplinv(c, c->sc, 1));
+  // dz(b) with b's cached top/bottom into sder[0], dz(a) into sder[1]
+  PMPIC_TRY(op_diffz(c, c->sb, c->sder[0], c->bbot, c->btop));
+  PMPIC_TRY(op_diffz(c, c->sa, c->sder[1], c->abot, c->atop));
+  hipLaunchKernelGGL(k_velocity_spectra, eb, et, 0, c->stream, g, c->sa, c->sb, c->sc, c->sder[0],
+                     c->sder[1], c->ubar, c->vbar, c->sder[2], c->sder[3], c->sder[4], c->su,
+                     c->sv, c->sw, c->kx, c->ky, c->filt);
+  {
+    const double *src[6] = {c->sd, c->se, c->sf, c->sder[2], c->sder[3], c->sder[4]};
+    double *dst[6] = {c->grid[GP], c->grid[GQ], c->grid[GR], c->grid[GU], c->grid[GV],
+                      c->grid[GW]};
+    PMPIC_TRY(fft_backward(c, 6, src, dst));
+  }
+  PMPIC_CHECK(hipGetLastError());
+  return PMPIC_OK;
+}
+
+// vorticity_tendency timestep_callback (vorticity_tendency.F90:147-265) after par2grid of btot
+// into grid[GB]; tendencies into grid[GDP..GDR]
+int vorticity_tendency_spectral(Ctx *c, int stage) {
+  (void)stage;
+  const Geometry &g = c->g;
+  const int64_t ssz = ssz_of(g);
+  const dim3 eb(cdiv(ssz, TPB)), et(TPB);
+  {
+    const double *src[1] = {c->grid[GB]};
+    double *dst[1] = {c->sbs};
+    PMPIC_TRY(fft_forward(c, 1, src, dst));
+  }
+  hipLaunchKernelGGL(k_tendency_spectra, eb, et, 0, c->stream, g, c->sbs, c->su, c->sv, c->sw,
+                     c->sder[0], c->sder[1], c->sder[2], c->sder[3], c->sder[4], c->sder[5],
+                     c->sder[6], c->sder[7], c->kx, c->ky);
+  {
+    const double *src[8];
+    double *dst[8];
+    for (int f = 0; f < 8; f++) { src[f] = c->sder[f]; dst[f] = c->gtmp[f]; }
+    PMPIC_TRY(fft_backward(c, 8, src, dst));
+  }
+  const int64_t nnode = (int64_t)g.nzn * g.ny * g.nx;
+  hipLaunchKernelGGL(k_tendency, dim3(cdiv(nnode, TPB)), dim3(TPB), 0, c->stream, nnode,
+                     c->grid[GP], c->grid[GQ], c->grid[GR], c->gtmp[0], c->gtmp[1], c->gtmp[2],
+                     c->gtmp[3], c->gtmp[4], c->gtmp[5], c->gtmp[6], c->gtmp[7], c->grid[GDP],
+                     c->grid[GDQ], c->grid[GDR]);
+  PMPIC_CHECK(hipGetLastError());
+  return PMPIC_OK;
+}
+
+int grid_max_reduce(Ctx *c) {
+  const Geometry &g = c->g;
+  const int64_t nnode = (int64_t)g.nzn * g.ny * g.nx;
+  hipLaunchKernelGGL(k_zero_max, dim3(1), dim3(1), 0, c->stream, c->dsc);
+  int64_t nb = cdiv(nnode, TPB);
+  if (nb > 1024) nb = 1024;
+  hipLaunchKernelGGL(k_max4, dim3(nb), dim3(TPB), 0, c->stream, nnode, c->grid[GU], c->grid[GV],
+                     c->grid[GW], c->grid[GP], c->grid[GQ], c->grid[GR], c->dsc);
+  hipLaunchKernelGGL(k_select_dt, dim3(1), dim3(1), 0, c->stream, g, c->dsc);
+  PMPIC_CHECK(hipGetLastError());
+  return PMPIC_OK;
+}
+
+}  // namespace pmpic
