@@ -7,11 +7,12 @@
 //
 // par2grid is a scatter; here it is a gather over a cell binning so that no floating-point
 // atomics are used and every node sum has a fixed order:
-//   1. bin:   each parcel finds its cell and takes a slot in the cell's bucket (integer atomics
-//             only), an exclusive scan turns bucket sizes into offsets, parcel indices are
-//             scattered into buckets and each bucket is sorted by parcel index;
-//   2. cell:  one thread per cell walks its parcels in index order and accumulates the 8 corner
-//             weights (and weight * value) into per-cell corner partial sums;
+//   1. bin:   each parcel finds its cell and takes a slot in the cell's bucket (integer atomics,
+//             one per run of equal cells within a wavefront), an exclusive scan turns bucket
+//             sizes into offsets and parcel indices are scattered into the buckets;
+//   2. cell:  a subgroup of lanes per cell loads the cell's parcels (coalesced), sorts their
+//             indices across the lanes, forms the 8 corner weights (and weight * value) per
+//             parcel and reduces them over the subgroup in a fixed butterfly;
 //   3. node:  one thread per node adds the matching corner partials of its 8 adjacent cells
 //             (periodic in x, y) in a fixed order and normalises by the summed weights.
 // The result differs from the reference's sequential sum only by the order of additions.
@@ -35,23 +36,37 @@ __global__ void k_fill_f64(double *a, int64_t n, double v) {
   if (i < n) a[i] = v;
 }
 
+// cell of every parcel and its slot in the cell's bucket.  Parcels that are neighbours in slot
+// order usually share a cell, so lanes are grouped into runs of equal cell and one lane per run
+// takes the whole run's slots with a single integer atomic.
 __global__ void k_bin_count(Geometry g, const double *__restrict__ X, const double *__restrict__ Y,
                             const double *__restrict__ Z, int64_t n, int32_t *__restrict__ cell,
                             int32_t *__restrict__ rank, int32_t *__restrict__ count,
                             DevScalars *sc) {
-  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  int i0, j0, k0;
-  double dx_, dy_, dz_;
-  int st = locate(g, X[p], Y[p], Z[p], i0, j0, k0, dx_, dy_, dz_);
-  if (st != PMPIC_OK) {
-    raise_status(sc, st);
-    cell[p] = -1;
-    return;
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  int32_t c = -1;
+  if (p < n) {
+    int i0, j0, k0;
+    double dx_, dy_, dz_;
+    const int st = locate(g, X[p], Y[p], Z[p], i0, j0, k0, dx_, dy_, dz_);
+    if (st != PMPIC_OK) raise_status(sc, st);
+    else c = (int32_t)(((int64_t)k0 * g.ny + j0) * g.nx + i0);
   }
-  int32_t c = (int32_t)(((int64_t)k0 * g.ny + j0) * g.nx + i0);
-  cell[p] = c;
-  rank[p] = atomicAdd(&count[c], 1);
+  const int32_t cprev = __shfl_up(c, 1);
+  const bool head = (lane == 0) || (c != cprev);
+  const unsigned long long hm = __ballot(head);
+  const unsigned long long upto = hm & ((2ull << lane) - 1ull);
+  const int hl = 63 - __clzll(upto);
+  const unsigned long long above = hm & ~((2ull << hl) - 1ull);
+  const int next = above ? (__ffsll(above) - 1) : 64;
+  int base = 0;
+  if (head && c >= 0) base = atomicAdd(&count[c], next - hl);
+  base = __shfl(base, hl);
+  if (p < n) {
+    cell[p] = c;
+    rank[p] = base + (lane - hl);
+  }
 }
 
 __global__ void k_bin_scatter(const int32_t *__restrict__ cell, const int32_t *__restrict__ rank,
@@ -64,12 +79,14 @@ __global__ void k_bin_scatter(const int32_t *__restrict__ cell, const int32_t *_
   sorted[start[c] + rank[p]] = (int32_t)p;
 }
 
-// insertion sort of each bucket by parcel index (buckets hold a few tens of parcels)
-__global__ void k_bin_sort(const int32_t *__restrict__ start, int64_t ncell,
-                           int32_t *__restrict__ sorted) {
+// buckets larger than one subgroup are sorted by parcel index here (insertion sort; such cells
+// are few); smaller buckets are sorted inside the accumulation kernel
+__global__ void k_bin_sort_big(const int32_t *__restrict__ start, int64_t ncell, int sgsize,
+                               int32_t *__restrict__ sorted) {
   int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
   const int32_t b = start[c], e = start[c + 1];
+  if (e - b <= sgsize) return;
   for (int32_t a = b + 1; a < e; a++) {
     int32_t v = sorted[a];
     int32_t q = a - 1;
@@ -84,62 +101,114 @@ __global__ void k_bin_sort(const int32_t *__restrict__ start, int64_t ncell,
 // value sources for the per-cell accumulation
 enum AccMode { ACC_P2G = 0, ACC_SINGLE = 1, ACC_MIX_KEEP = 2, ACC_MIX_RES = 3 };
 
-template <int MODE, int NV>
-__global__ void k_cell_accum(Geometry g, const double *__restrict__ par, int64_t cap,
-                             const double *__restrict__ var, const int32_t *__restrict__ start,
-                             const int32_t *__restrict__ sorted, int64_t ncell,
-                             double *__restrict__ partial, int32_t *__restrict__ npercell) {
-  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncell) return;
+// Per-cell corner partial sums.  A subgroup of S lanes owns one cell: each lane takes one of the
+// cell's parcels (consecutive bucket entries, which are mostly consecutive parcel slots, so the
+// loads coalesce), the subgroup sorts its parcel indices (bitonic network over the lanes) so the
+// result does not depend on the order the bucket was filled in, every lane forms the 8 corner
+// weights times (1, values), and a butterfly over the subgroup adds them in a fixed tree.
+// Cells with more than S parcels are processed in chunks of S (bucket pre-sorted) whose sums are
+// added in chunk order.
+template <int MODE, int NV, int S>
+__global__ void __launch_bounds__(256) k_cell_accum(Geometry g, const double *__restrict__ par,
+                                                    int64_t cap, const double *__restrict__ var,
+                                                    const int32_t *__restrict__ start,
+                                                    const int32_t *__restrict__ sorted,
+                                                    int64_t ncell, double *__restrict__ partial,
+                                                    int32_t *__restrict__ npercell) {
+  const int lane = threadIdx.x & 63;
+  const int sl = lane & (S - 1);
+  const int64_t c = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / S;
+  const bool valid = c < ncell;
+  const int32_t b = valid ? start[c] : 0;
+  const int32_t e = valid ? start[c + 1] : 0;
+  const int nch = (e - b + S - 1) / S;
+  int wmax = nch;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) wmax = max(wmax, __shfl_xor(wmax, o));
   const double *X = par + FX * cap, *Y = par + FY * cap, *Z = par + FZ * cap;
   const double *VOL = par + FVOL * cap;
-  double acc[8][NV + 1];
+  int kept = 0;
+  for (int t_ = 0; t_ < wmax; t_++) {
+    const int32_t s = b + t_ * S + sl;
+    int32_t p = (s < e) ? sorted[s] : INT32_MAX;
 #pragma unroll
-  for (int q = 0; q < 8; q++)
+    for (int k = 2; k <= S; k <<= 1)
 #pragma unroll
-    for (int f = 0; f <= NV; f++) acc[q][f] = 0.0;
-  int count = 0;
-  const int32_t b = start[c], e = start[c + 1];
-  for (int32_t s = b; s < e; s++) {
-    const int64_t p = sorted[s];
-    const double v = VOL[p];
-    if (MODE == ACC_MIX_KEEP && v < g.vmin) continue;
-    if (MODE == ACC_MIX_RES && !(v < g.vmin)) continue;
-    int i0, j0, k0;
-    double delx, dely, delz;
-    locate(g, X[p], Y[p], Z[p], i0, j0, k0, delx, dely, delz);
-    double val[NV > 0 ? NV : 1];
-    if (MODE == ACC_P2G) {
-      val[0] = par[FP * cap + p];
-      val[1] = par[FQ * cap + p];
-      val[2] = par[FR * cap + p];
-      // total buoyancy, vorticity_tendency.F90:121-126
-      const double hl = par[FH * cap + p] - exp(-Z[p]);
-      val[3] = par[FB * cap + p] + 12.5 * (hl > 0.0 ? hl : 0.0);
-    } else if (MODE == ACC_SINGLE) {
-      val[0] = var[p];
-    } else {
-      val[0] = par[FB * cap + p];
-      val[1] = par[FH * cap + p];
-      val[2] = par[FP * cap + p];
-      val[3] = par[FQ * cap + p];
-      val[4] = par[FR * cap + p];
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        const int32_t q = __shfl_xor(p, j);
+        const bool up = (sl & k) == 0, lower = (sl & j) == 0;
+        p = (lower == up) ? min(p, q) : max(p, q);
+      }
+    // per-lane factors; lanes without a (selected) parcel contribute exact zeros
+    double val[NV];
+    double fx[2] = {0.0, 0.0}, fy[2] = {0.0, 0.0}, fz[2] = {0.0, 0.0}, vol = 0.0;
+#pragma unroll
+    for (int f = 0; f < NV; f++) val[f] = 0.0;
+    if (p != INT32_MAX) {
+      const double pv = VOL[p];
+      bool use = true;
+      if (MODE == ACC_MIX_KEEP) use = !(pv < g.vmin);
+      if (MODE == ACC_MIX_RES) use = (pv < g.vmin);
+      if (use) {
+        int i0, j0, k0;
+        double delx, dely, delz;
+        locate(g, X[p], Y[p], Z[p], i0, j0, k0, delx, dely, delz);
+        fx[0] = 1 - delx; fx[1] = delx;
+        fy[0] = 1 - dely; fy[1] = dely;
+        fz[0] = 1 - delz; fz[1] = delz;
+        vol = pv;
+        if (MODE == ACC_P2G) {
+          val[0] = par[FP * cap + p];
+          val[1] = par[FQ * cap + p];
+          val[2] = par[FR * cap + p];
+          // total buoyancy, vorticity_tendency.F90:121-126
+          const double hl = par[FH * cap + p] - exp(-Z[p]);
+          val[3] = par[FB * cap + p] + 12.5 * (hl > 0.0 ? hl : 0.0);
+        } else if (MODE == ACC_SINGLE) {
+          val[0] = var[p];
+        } else {
+          val[0] = par[FB * cap + p];
+          val[1] = par[FH * cap + p];
+          val[2] = par[FP * cap + p];
+          val[3] = par[FQ * cap + p];
+          val[4] = par[FR * cap + p];
+        }
+        kept++;
+      }
     }
-    double w[8];
-    corner_weights(delx, dely, delz, v, w);
-#pragma unroll
+    // one corner at a time: weight (parcel_interpolation.F90:390-434 association) and
+    // weight * value, reduced over the subgroup
+#pragma unroll 1
     for (int q = 0; q < 8; q++) {
-      acc[q][0] += w[q];
+      const double w = fz[(q >> 2) & 1] * fy[(q >> 1) & 1] * fx[q & 1] * vol;
+      double t[NV + 1];
+      t[0] = w;
 #pragma unroll
-      for (int f = 0; f < NV; f++) acc[q][f + 1] += w[q] * val[f];
+      for (int f = 0; f < NV; f++) t[f + 1] = w * val[f];
+#pragma unroll
+      for (int o = S / 2; o > 0; o >>= 1)
+#pragma unroll
+        for (int f = 0; f <= NV; f++) t[f] += __shfl_xor(t[f], o);
+      if (valid && sl == 0 && t_ < nch) {
+#pragma unroll
+        for (int f = 0; f <= NV; f++) {
+          double *d = partial + (int64_t)(q * (NV + 1) + f) * ncell + c;
+          *d = (t_ == 0) ? t[f] : *d + t[f];
+        }
+      }
     }
-    count++;
   }
+  if (valid && sl == 0 && nch == 0) {
 #pragma unroll
-  for (int q = 0; q < 8; q++)
+    for (int q = 0; q < 8; q++)
 #pragma unroll
-    for (int f = 0; f <= NV; f++) partial[(int64_t)(q * (NV + 1) + f) * ncell + c] = acc[q][f];
-  if (MODE == ACC_MIX_KEEP) npercell[c] = count;
+      for (int f = 0; f <= NV; f++) partial[(int64_t)(q * (NV + 1) + f) * ncell + c] = 0.0;
+  }
+  if (MODE == ACC_MIX_KEEP) {
+#pragma unroll
+    for (int o = S / 2; o > 0; o >>= 1) kept += __shfl_xor(kept, o);
+    if (valid && sl == 0) npercell[c] = kept;
+  }
 }
 
 // node sums: NORM = divide by summed weights (par2grid, parcel_interpolation.F90:463-469);
@@ -236,6 +305,12 @@ int scan_exclusive_i32(Ctx *c, const int32_t *in, int32_t *out, int64_t n) {
   return PMPIC_OK;
 }
 
+// subgroup size for the cell accumulation: the smallest of 16, 32, 64 holding a typical cell
+static int subgroup_size(const Ctx *c, int64_t n) {
+  const double ppc = (double)n / (double)(c->ncell > 0 ? c->ncell : 1);
+  return ppc <= 12.0 ? 16 : (ppc <= 28.0 ? 32 : 64);
+}
+
 int bin_parcels(Ctx *c, int64_t n) {
   const Geometry &g = c->g;
   const int64_t nc1 = c->ncell + 1;
@@ -249,19 +324,32 @@ int bin_parcels(Ctx *c, int64_t n) {
   if (n > 0)
     hipLaunchKernelGGL(k_bin_scatter, dim3(cdiv(n, TPB)), dim3(TPB), 0, c->stream, c->cell,
                        c->rank, c->cell_start, n, c->sorted);
-  hipLaunchKernelGGL(k_bin_sort, dim3(cdiv(c->ncell, TPB)), dim3(TPB), 0, c->stream,
-                     c->cell_start, c->ncell, c->sorted);
+  hipLaunchKernelGGL(k_bin_sort_big, dim3(cdiv(c->ncell, TPB)), dim3(TPB), 0, c->stream,
+                     c->cell_start, c->ncell, subgroup_size(c, n), c->sorted);
   PMPIC_CHECK(hipGetLastError());
   return PMPIC_OK;
+}
+
+template <int MODE, int NV>
+static void launch_accum(Ctx *c, int S, const double *var, double *partial, int32_t *npc) {
+  const int64_t threads = c->ncell * S;
+  const dim3 grid(cdiv(threads, 256)), block(256);
+  if (S == 16)
+    hipLaunchKernelGGL((k_cell_accum<MODE, NV, 16>), grid, block, 0, c->stream, c->g, c->par,
+                       c->cap, var, c->cell_start, c->sorted, c->ncell, partial, npc);
+  else if (S == 32)
+    hipLaunchKernelGGL((k_cell_accum<MODE, NV, 32>), grid, block, 0, c->stream, c->g, c->par,
+                       c->cap, var, c->cell_start, c->sorted, c->ncell, partial, npc);
+  else
+    hipLaunchKernelGGL((k_cell_accum<MODE, NV, 64>), grid, block, 0, c->stream, c->g, c->par,
+                       c->cap, var, c->cell_start, c->sorted, c->ncell, partial, npc);
 }
 
 int par2grid_fused(Ctx *c, int64_t n) {
   PMPIC_TRY(bin_parcels(c, n));
   const int64_t ncell = c->ncell;
   PMPIC_TRY(timing_begin(c, 0));
-  hipLaunchKernelGGL((k_cell_accum<ACC_P2G, 4>), dim3(cdiv(ncell, TPB)), dim3(TPB), 0, c->stream,
-                     c->g, c->par, c->cap, (const double *)nullptr, c->cell_start, c->sorted,
-                     ncell, c->partial, (int32_t *)nullptr);
+  launch_accum<ACC_P2G, 4>(c, subgroup_size(c, n), nullptr, c->partial, nullptr);
   PMPIC_TRY(timing_end(c, 0));
   OutTable table = {{c->grid[GP], c->grid[GQ], c->grid[GR], c->grid[GB], nullptr, nullptr}};
   const int64_t nnode = (int64_t)c->g.nzn * c->g.ny * c->g.nx;
@@ -274,9 +362,7 @@ int par2grid_fused(Ctx *c, int64_t n) {
 int par2grid_single(Ctx *c, int64_t n, const double *d_var, double *d_grid) {
   PMPIC_TRY(bin_parcels(c, n));
   const int64_t ncell = c->ncell;
-  hipLaunchKernelGGL((k_cell_accum<ACC_SINGLE, 1>), dim3(cdiv(ncell, TPB)), dim3(TPB), 0,
-                     c->stream, c->g, c->par, c->cap, d_var, c->cell_start, c->sorted, ncell,
-                     c->partial, (int32_t *)nullptr);
+  launch_accum<ACC_SINGLE, 1>(c, subgroup_size(c, n), d_var, c->partial, nullptr);
   OutTable table = {{d_grid, nullptr, nullptr, nullptr, nullptr, nullptr}};
   const int64_t nnode = (int64_t)c->g.nzn * c->g.ny * c->g.nx;
   hipLaunchKernelGGL((k_node_sum<1, true>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0, c->stream, c->g,
@@ -300,12 +386,9 @@ int grid2par_single(Ctx *c, int64_t n, const double *d_grid, double *d_var) {
 int mixing_grids(Ctx *c, int64_t n) {
   PMPIC_TRY(bin_parcels(c, n));
   const int64_t ncell = c->ncell;
-  hipLaunchKernelGGL((k_cell_accum<ACC_MIX_KEEP, 5>), dim3(cdiv(ncell, TPB)), dim3(TPB), 0,
-                     c->stream, c->g, c->par, c->cap, (const double *)nullptr, c->cell_start,
-                     c->sorted, ncell, c->partial, c->npercell);
-  hipLaunchKernelGGL((k_cell_accum<ACC_MIX_RES, 5>), dim3(cdiv(ncell, TPB)), dim3(TPB), 0,
-                     c->stream, c->g, c->par, c->cap, (const double *)nullptr, c->cell_start,
-                     c->sorted, ncell, c->partial2, (int32_t *)nullptr);
+  const int S = subgroup_size(c, n);
+  launch_accum<ACC_MIX_KEEP, 5>(c, S, nullptr, c->partial, c->npercell);
+  launch_accum<ACC_MIX_RES, 5>(c, S, nullptr, c->partial2, nullptr);
   OutTable kept = {{c->grid[GVOL], c->grid[GB], c->grid[GHG], c->grid[GP], c->grid[GQ],
                     c->grid[GR]}};
   OutTable res = {{c->kres[0], c->kres[1], c->kres[2], c->kres[3], c->kres[4], c->kres[5]}};
