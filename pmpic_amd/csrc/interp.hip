@@ -10,9 +10,9 @@
 //   1. bin:   each parcel finds its cell and takes a slot in the cell's bucket (integer atomics,
 //             one per run of equal cells within a wavefront), an exclusive scan turns bucket
 //             sizes into offsets and parcel indices are scattered into the buckets;
-//   2. cell:  a subgroup of lanes per cell loads the cell's parcels (coalesced), sorts their
-//             indices across the lanes, forms the 8 corner weights (and weight * value) per
-//             parcel and reduces them over the subgroup in a fixed butterfly;
+//   2. cell:  a workgroup stages a tile of cells' parcels in LDS (coalesced loads, one
+//             location per parcel), then one thread per cell adds the 8 corner weights (and
+//             weight * value) of its parcels in ascending parcel order;
 //   3. node:  one thread per node adds the matching corner partials of its 8 adjacent cells
 //             (periodic in x, y) in a fixed order and normalises by the summed weights.
 // The result differs from the reference's sequential sum only by the order of additions.
@@ -79,8 +79,8 @@ __global__ void k_bin_scatter(const int32_t *__restrict__ cell, const int32_t *_
   sorted[start[c] + rank[p]] = (int32_t)p;
 }
 
-// buckets larger than one subgroup are sorted by parcel index here (insertion sort; such cells
-// are few); smaller buckets are sorted inside the accumulation kernel
+// buckets larger than the LDS tile are sorted by parcel index here (insertion sort; such cells
+// do not occur at the target workloads); smaller buckets are ordered inside the accumulation
 __global__ void k_bin_sort_big(const int32_t *__restrict__ start, int64_t ncell, int sgsize,
                                int32_t *__restrict__ sorted) {
   int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -101,70 +101,188 @@ __global__ void k_bin_sort_big(const int32_t *__restrict__ start, int64_t ncell,
 // value sources for the per-cell accumulation
 enum AccMode { ACC_P2G = 0, ACC_SINGLE = 1, ACC_MIX_KEEP = 2, ACC_MIX_RES = 3 };
 
-// Per-cell corner partial sums.  A subgroup of S lanes owns one cell: each lane takes one of the
-// cell's parcels (consecutive bucket entries, which are mostly consecutive parcel slots, so the
-// loads coalesce), the subgroup sorts its parcel indices (bitonic network over the lanes) so the
-// result does not depend on the order the bucket was filled in, every lane forms the 8 corner
-// weights times (1, values), and a butterfly over the subgroup adds them in a fixed tree.
-// Cells with more than S parcels are processed in chunks of S (bucket pre-sorted) whose sums are
-// added in chunk order.
-template <int MODE, int NV, int S>
-__global__ void __launch_bounds__(256) k_cell_accum(Geometry g, const double *__restrict__ par,
-                                                    int64_t cap, const double *__restrict__ var,
-                                                    const int32_t *__restrict__ start,
-                                                    const int32_t *__restrict__ sorted,
-                                                    int64_t ncell, double *__restrict__ partial,
-                                                    int32_t *__restrict__ npercell) {
-  const int lane = threadIdx.x & 63;
-  const int sl = lane & (S - 1);
-  const int64_t c = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / S;
-  const bool valid = c < ncell;
-  const int32_t b = valid ? start[c] : 0;
-  const int32_t e = valid ? start[c + 1] : 0;
-  const int nch = (e - b + S - 1) / S;
-  int wmax = nch;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) wmax = max(wmax, __shfl_xor(wmax, o));
+// Per-cell corner partial sums.
+// A workgroup owns TILE consecutive cells.  Their buckets are consecutive in the bucket array, so
+// the workgroup loads them with coalesced reads (bucket entries are mostly consecutive parcel
+// slots), locates each parcel once and stages (offsets, volume, values, parcel index) in LDS.
+// Then one thread per cell orders its rows by parcel index (insertion sort of row numbers; a no-op
+// pass for buckets filled in slot order) and accumulates the 8 corner weights and weight * value
+// sequentially, so within a cell the additions run in the reference's parcel order.
+// MIX mode accumulates kept parcels into `partial` (and counts them) and residual parcels into
+// `partial2`.  Cells with more than ROWS parcels (never at the target workloads) are summed
+// straight from global memory after k_bin_sort_big ordered their buckets.
+constexpr int TILE = 32;
+constexpr int ROWS_P2G = 1024, ROWS_MIX = 896;
+
+template <int MODE, int NV, int ROWS>
+__global__ void __launch_bounds__(256) k_cell_accum_lds(Geometry g, const double *__restrict__ par,
+                                                        int64_t cap, const double *__restrict__ var,
+                                                        const int32_t *__restrict__ start,
+                                                        const int32_t *__restrict__ sorted,
+                                                        int64_t ncell, double *__restrict__ partial,
+                                                        double *__restrict__ partial2,
+                                                        int32_t *__restrict__ npercell) {
+  constexpr int NF = NV + 1;
+  constexpr bool MIX = (MODE == ACC_MIX_KEEP);
+  __shared__ double sdat[4 + NV][ROWS];   // delx, dely, delz, vol, values
+  __shared__ int32_t sidx[ROWS];
+  __shared__ int16_t sperm[ROWS];
+  __shared__ uint8_t skeep[MIX ? ROWS : 1];
+  __shared__ int32_t sstart[TILE + 1];
+  const int tid = threadIdx.x;
+  const int64_t c0 = (int64_t)blockIdx.x * TILE;
+  const int ntile = (int)min((int64_t)TILE, ncell - c0);
+  for (int t = tid; t <= ntile; t += blockDim.x) sstart[t] = start[c0 + t];
+  __syncthreads();
   const double *X = par + FX * cap, *Y = par + FY * cap, *Z = par + FZ * cap;
   const double *VOL = par + FVOL * cap;
+  double acc[8][NF];
+#pragma unroll
+  for (int q = 0; q < 8; q++)
+#pragma unroll
+    for (int f = 0; f < NF; f++) acc[q][f] = 0.0;
   int kept = 0;
-  for (int t_ = 0; t_ < wmax; t_++) {
-    const int32_t s = b + t_ * S + sl;
-    int32_t p = (s < e) ? sorted[s] : INT32_MAX;
-#pragma unroll
-    for (int k = 2; k <= S; k <<= 1)
-#pragma unroll
-      for (int j = k >> 1; j > 0; j >>= 1) {
-        const int32_t q = __shfl_xor(p, j);
-        const bool up = (sl & k) == 0, lower = (sl & j) == 0;
-        p = (lower == up) ? min(p, q) : max(p, q);
-      }
-    // per-lane factors; lanes without a (selected) parcel contribute exact zeros
-    double val[NV];
-    double fx[2] = {0.0, 0.0}, fy[2] = {0.0, 0.0}, fz[2] = {0.0, 0.0}, vol = 0.0;
-#pragma unroll
-    for (int f = 0; f < NV; f++) val[f] = 0.0;
-    if (p != INT32_MAX) {
-      const double pv = VOL[p];
-      bool use = true;
-      if (MODE == ACC_MIX_KEEP) use = !(pv < g.vmin);
-      if (MODE == ACC_MIX_RES) use = (pv < g.vmin);
-      if (use) {
+  bool anyres = false;
+  bool big = false;
+  int first = 0;
+  while (first < ntile) {
+    // the group of cells [first, last) whose rows fit in LDS (at least one cell)
+    int last = first + 1;
+    while (last < ntile && sstart[last + 1] - sstart[first] <= ROWS) last++;
+    const int32_t r0 = sstart[first];
+    const int nrows = sstart[last] - r0;
+    const bool fits = nrows <= ROWS;
+    if (fits) {
+      for (int r = tid; r < nrows; r += blockDim.x) {
+        const int32_t p = sorted[r0 + r];
         int i0, j0, k0;
         double delx, dely, delz;
         locate(g, X[p], Y[p], Z[p], i0, j0, k0, delx, dely, delz);
-        fx[0] = 1 - delx; fx[1] = delx;
-        fy[0] = 1 - dely; fy[1] = dely;
-        fz[0] = 1 - delz; fz[1] = delz;
-        vol = pv;
-        if (MODE == ACC_P2G) {
+        const double v = VOL[p];
+        sdat[0][r] = delx;
+        sdat[1][r] = dely;
+        sdat[2][r] = delz;
+        sdat[3][r] = v;
+        if constexpr (MODE == ACC_P2G) {
+          sdat[4][r] = par[FP * cap + p];
+          sdat[5][r] = par[FQ * cap + p];
+          sdat[6][r] = par[FR * cap + p];
+          // total buoyancy, vorticity_tendency.F90:121-126
+          const double hl = par[FH * cap + p] - exp(-Z[p]);
+          sdat[7][r] = par[FB * cap + p] + 12.5 * (hl > 0.0 ? hl : 0.0);
+        } else if constexpr (MODE == ACC_SINGLE) {
+          sdat[4][r] = var[p];
+        } else {
+          sdat[4][r] = par[FB * cap + p];
+          sdat[5][r] = par[FH * cap + p];
+          sdat[6][r] = par[FP * cap + p];
+          sdat[7][r] = par[FQ * cap + p];
+          sdat[8][r] = par[FR * cap + p];
+          skeep[r] = (v < g.vmin) ? 0 : 1;
+        }
+        sidx[r] = p;
+        sperm[r] = (int16_t)r;
+      }
+    }
+    __syncthreads();
+    if (tid >= first && tid < last) {
+      if (!fits) {
+        big = true;
+      } else {
+        const int lo = sstart[tid] - r0, hi = sstart[tid + 1] - r0;
+        for (int a = lo + 1; a < hi; a++) {
+          const int16_t v = sperm[a];
+          const int32_t key = sidx[v];
+          int b = a - 1;
+          while (b >= lo && sidx[sperm[b]] > key) {
+            sperm[b + 1] = sperm[b];
+            b--;
+          }
+          sperm[b + 1] = v;
+        }
+        for (int a = lo; a < hi; a++) {
+          const int r = sperm[a];
+          if (MIX && !skeep[r]) {
+            anyres = true;
+            continue;
+          }
+          const double fx[2] = {1 - sdat[0][r], sdat[0][r]};
+          const double fy[2] = {1 - sdat[1][r], sdat[1][r]};
+          const double fz[2] = {1 - sdat[2][r], sdat[2][r]};
+          const double v = sdat[3][r];
+#pragma unroll
+          for (int q = 0; q < 8; q++) {
+            const double w = fz[(q >> 2) & 1] * fy[(q >> 1) & 1] * fx[q & 1] * v;
+            acc[q][0] += w;
+#pragma unroll
+            for (int f = 0; f < NV; f++) acc[q][f + 1] += w * sdat[4 + f][r];
+          }
+          kept++;
+        }
+        if (MIX) {
+          // kept sums are final for this cell: store, then sum the residual parcels
+          const int64_t c = c0 + tid;
+#pragma unroll
+          for (int q = 0; q < 8; q++)
+#pragma unroll
+            for (int f = 0; f < NF; f++) {
+              partial[(int64_t)(q * NF + f) * ncell + c] = acc[q][f];
+              acc[q][f] = 0.0;
+            }
+          npercell[c] = kept;
+          if (anyres) {
+            for (int a = lo; a < hi; a++) {
+              const int r = sperm[a];
+              if (skeep[r]) continue;
+              const double fx[2] = {1 - sdat[0][r], sdat[0][r]};
+              const double fy[2] = {1 - sdat[1][r], sdat[1][r]};
+              const double fz[2] = {1 - sdat[2][r], sdat[2][r]};
+              const double v = sdat[3][r];
+#pragma unroll
+              for (int q = 0; q < 8; q++) {
+                const double w = fz[(q >> 2) & 1] * fy[(q >> 1) & 1] * fx[q & 1] * v;
+                acc[q][0] += w;
+#pragma unroll
+                for (int f = 0; f < NV; f++) acc[q][f + 1] += w * sdat[4 + f][r];
+              }
+            }
+          }
+#pragma unroll
+          for (int q = 0; q < 8; q++)
+#pragma unroll
+            for (int f = 0; f < NF; f++) partial2[(int64_t)(q * NF + f) * ncell + c] = acc[q][f];
+        }
+      }
+    }
+    __syncthreads();
+    first = last;
+  }
+  if (tid >= ntile) return;
+  const int64_t c = c0 + tid;
+  if (big) {
+    // bucket already sorted by k_bin_sort_big; same sums straight from global memory
+    const int32_t b = sstart[tid], e = sstart[tid + 1];
+    for (int pass = 0; pass < (MIX ? 2 : 1); pass++) {
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+#pragma unroll
+        for (int f = 0; f < NF; f++) acc[q][f] = 0.0;
+      kept = 0;
+      for (int32_t sidx_ = b; sidx_ < e; sidx_++) {
+        const int32_t p = sorted[sidx_];
+        const double v = VOL[p];
+        if (MIX && ((pass == 0) == (v < g.vmin))) continue;
+        int i0, j0, k0;
+        double delx, dely, delz;
+        locate(g, X[p], Y[p], Z[p], i0, j0, k0, delx, dely, delz);
+        double val[NV];
+        if constexpr (MODE == ACC_P2G) {
           val[0] = par[FP * cap + p];
           val[1] = par[FQ * cap + p];
           val[2] = par[FR * cap + p];
-          // total buoyancy, vorticity_tendency.F90:121-126
           const double hl = par[FH * cap + p] - exp(-Z[p]);
           val[3] = par[FB * cap + p] + 12.5 * (hl > 0.0 ? hl : 0.0);
-        } else if (MODE == ACC_SINGLE) {
+        } else if constexpr (MODE == ACC_SINGLE) {
           val[0] = var[p];
         } else {
           val[0] = par[FB * cap + p];
@@ -173,41 +291,28 @@ __global__ void __launch_bounds__(256) k_cell_accum(Geometry g, const double *__
           val[3] = par[FQ * cap + p];
           val[4] = par[FR * cap + p];
         }
+        const double fx[2] = {1 - delx, delx}, fy[2] = {1 - dely, dely}, fz[2] = {1 - delz, delz};
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+          const double w = fz[(q >> 2) & 1] * fy[(q >> 1) & 1] * fx[q & 1] * v;
+          acc[q][0] += w;
+#pragma unroll
+          for (int f = 0; f < NV; f++) acc[q][f + 1] += w * val[f];
+        }
         kept++;
       }
+      double *dst = (MIX && pass == 1) ? partial2 : partial;
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+#pragma unroll
+        for (int f = 0; f < NF; f++) dst[(int64_t)(q * NF + f) * ncell + c] = acc[q][f];
+      if (MIX && pass == 0) npercell[c] = kept;
     }
-    // one corner at a time: weight (parcel_interpolation.F90:390-434 association) and
-    // weight * value, reduced over the subgroup
-#pragma unroll 1
-    for (int q = 0; q < 8; q++) {
-      const double w = fz[(q >> 2) & 1] * fy[(q >> 1) & 1] * fx[q & 1] * vol;
-      double t[NV + 1];
-      t[0] = w;
-#pragma unroll
-      for (int f = 0; f < NV; f++) t[f + 1] = w * val[f];
-#pragma unroll
-      for (int o = S / 2; o > 0; o >>= 1)
-#pragma unroll
-        for (int f = 0; f <= NV; f++) t[f] += __shfl_xor(t[f], o);
-      if (valid && sl == 0 && t_ < nch) {
-#pragma unroll
-        for (int f = 0; f <= NV; f++) {
-          double *d = partial + (int64_t)(q * (NV + 1) + f) * ncell + c;
-          *d = (t_ == 0) ? t[f] : *d + t[f];
-        }
-      }
-    }
-  }
-  if (valid && sl == 0 && nch == 0) {
+  } else if (!MIX) {
 #pragma unroll
     for (int q = 0; q < 8; q++)
 #pragma unroll
-      for (int f = 0; f <= NV; f++) partial[(int64_t)(q * (NV + 1) + f) * ncell + c] = 0.0;
-  }
-  if (MODE == ACC_MIX_KEEP) {
-#pragma unroll
-    for (int o = S / 2; o > 0; o >>= 1) kept += __shfl_xor(kept, o);
-    if (valid && sl == 0) npercell[c] = kept;
+      for (int f = 0; f < NF; f++) partial[(int64_t)(q * NF + f) * ncell + c] = acc[q][f];
   }
 }
 
@@ -305,12 +410,6 @@ int scan_exclusive_i32(Ctx *c, const int32_t *in, int32_t *out, int64_t n) {
   return PMPIC_OK;
 }
 
-// subgroup size for the cell accumulation: the smallest of 16, 32, 64 holding a typical cell
-static int subgroup_size(const Ctx *c, int64_t n) {
-  const double ppc = (double)n / (double)(c->ncell > 0 ? c->ncell : 1);
-  return ppc <= 12.0 ? 16 : (ppc <= 28.0 ? 32 : 64);
-}
-
 int bin_parcels(Ctx *c, int64_t n) {
   const Geometry &g = c->g;
   const int64_t nc1 = c->ncell + 1;
@@ -325,31 +424,24 @@ int bin_parcels(Ctx *c, int64_t n) {
     hipLaunchKernelGGL(k_bin_scatter, dim3(cdiv(n, TPB)), dim3(TPB), 0, c->stream, c->cell,
                        c->rank, c->cell_start, n, c->sorted);
   hipLaunchKernelGGL(k_bin_sort_big, dim3(cdiv(c->ncell, TPB)), dim3(TPB), 0, c->stream,
-                     c->cell_start, c->ncell, subgroup_size(c, n), c->sorted);
+                     c->cell_start, c->ncell, ROWS_MIX, c->sorted);
   PMPIC_CHECK(hipGetLastError());
   return PMPIC_OK;
 }
 
-template <int MODE, int NV>
-static void launch_accum(Ctx *c, int S, const double *var, double *partial, int32_t *npc) {
-  const int64_t threads = c->ncell * S;
-  const dim3 grid(cdiv(threads, 256)), block(256);
-  if (S == 16)
-    hipLaunchKernelGGL((k_cell_accum<MODE, NV, 16>), grid, block, 0, c->stream, c->g, c->par,
-                       c->cap, var, c->cell_start, c->sorted, c->ncell, partial, npc);
-  else if (S == 32)
-    hipLaunchKernelGGL((k_cell_accum<MODE, NV, 32>), grid, block, 0, c->stream, c->g, c->par,
-                       c->cap, var, c->cell_start, c->sorted, c->ncell, partial, npc);
-  else
-    hipLaunchKernelGGL((k_cell_accum<MODE, NV, 64>), grid, block, 0, c->stream, c->g, c->par,
-                       c->cap, var, c->cell_start, c->sorted, c->ncell, partial, npc);
+template <int MODE, int NV, int ROWS>
+static void launch_accum_lds(Ctx *c, const double *var, double *partial, double *partial2,
+                             int32_t *npc) {
+  hipLaunchKernelGGL((k_cell_accum_lds<MODE, NV, ROWS>), dim3(cdiv(c->ncell, TILE)), dim3(256), 0,
+                     c->stream, c->g, c->par, c->cap, var, c->cell_start, c->sorted, c->ncell,
+                     partial, partial2, npc);
 }
 
 int par2grid_fused(Ctx *c, int64_t n) {
   PMPIC_TRY(bin_parcels(c, n));
   const int64_t ncell = c->ncell;
   PMPIC_TRY(timing_begin(c, 0));
-  launch_accum<ACC_P2G, 4>(c, subgroup_size(c, n), nullptr, c->partial, nullptr);
+  launch_accum_lds<ACC_P2G, 4, ROWS_P2G>(c, nullptr, c->partial, nullptr, nullptr);
   PMPIC_TRY(timing_end(c, 0));
   OutTable table = {{c->grid[GP], c->grid[GQ], c->grid[GR], c->grid[GB], nullptr, nullptr}};
   const int64_t nnode = (int64_t)c->g.nzn * c->g.ny * c->g.nx;
@@ -362,7 +454,7 @@ int par2grid_fused(Ctx *c, int64_t n) {
 int par2grid_single(Ctx *c, int64_t n, const double *d_var, double *d_grid) {
   PMPIC_TRY(bin_parcels(c, n));
   const int64_t ncell = c->ncell;
-  launch_accum<ACC_SINGLE, 1>(c, subgroup_size(c, n), d_var, c->partial, nullptr);
+  launch_accum_lds<ACC_SINGLE, 1, ROWS_P2G>(c, d_var, c->partial, nullptr, nullptr);
   OutTable table = {{d_grid, nullptr, nullptr, nullptr, nullptr, nullptr}};
   const int64_t nnode = (int64_t)c->g.nzn * c->g.ny * c->g.nx;
   hipLaunchKernelGGL((k_node_sum<1, true>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0, c->stream, c->g,
@@ -386,9 +478,7 @@ int grid2par_single(Ctx *c, int64_t n, const double *d_grid, double *d_var) {
 int mixing_grids(Ctx *c, int64_t n) {
   PMPIC_TRY(bin_parcels(c, n));
   const int64_t ncell = c->ncell;
-  const int S = subgroup_size(c, n);
-  launch_accum<ACC_MIX_KEEP, 5>(c, S, nullptr, c->partial, c->npercell);
-  launch_accum<ACC_MIX_RES, 5>(c, S, nullptr, c->partial2, nullptr);
+  launch_accum_lds<ACC_MIX_KEEP, 5, ROWS_MIX>(c, nullptr, c->partial, c->partial2, c->npercell);
   OutTable kept = {{c->grid[GVOL], c->grid[GB], c->grid[GHG], c->grid[GP], c->grid[GQ],
                     c->grid[GR]}};
   OutTable res = {{c->kres[0], c->kres[1], c->kres[2], c->kres[3], c->kres[4], c->kres[5]}};
