@@ -42,7 +42,28 @@ def parse():
     ap.add_argument("--ppc", type=int, default=3, help="parcels per cell per direction")
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (the reference kernel for "
+                         "scaling FETCH_SIZE / WRITE_SIZE counters) and exit")
     return ap.parse_args()
+
+
+def copy_calibrate(mib):
+    """hipMemcpy device-to-device of `mib` MiB: the runtime's copy kernel reads and writes
+    exactly that many bytes, which calibrates the HBM traffic counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": nbytes}))
 
 
 def cpu_baseline(steps):
@@ -68,6 +89,9 @@ def cpu_baseline(steps):
 
 def main():
     args = parse()
+    if args.copy_calibrate:
+        copy_calibrate(args.copy_calibrate)
+        return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
                           "n_gpus": args.gpus}))

@@ -105,14 +105,37 @@ enum AccMode { ACC_P2G = 0, ACC_SINGLE = 1, ACC_MIX_KEEP = 2, ACC_MIX_RES = 3 };
 // A workgroup owns TILE consecutive cells.  Their buckets are consecutive in the bucket array, so
 // the workgroup loads them with coalesced reads (bucket entries are mostly consecutive parcel
 // slots), locates each parcel once and stages (offsets, volume, values, parcel index) in LDS.
-// Then one thread per cell orders its rows by parcel index (insertion sort of row numbers; a no-op
-// pass for buckets filled in slot order) and accumulates the 8 corner weights and weight * value
-// sequentially, so within a cell the additions run in the reference's parcel order.
-// MIX mode accumulates kept parcels into `partial` (and counts them) and residual parcels into
+// One thread per cell then orders its rows by parcel index (insertion sort of row numbers; a
+// single comparison pass for buckets filled in slot order), and one thread per (cell, corner)
+// adds that corner's weight and weight * value over the cell's rows in ascending parcel order,
+// so within a cell the additions run in the reference's order.
+// MIX mode sums kept parcels into `partial` (and counts them) and residual parcels into
 // `partial2`.  Cells with more than ROWS parcels (never at the target workloads) are summed
 // straight from global memory after k_bin_sort_big ordered their buckets.
-constexpr int TILE = 32;
+constexpr int TILE = 32;  // cells per workgroup: 8 threads (corners) each
 constexpr int ROWS_P2G = 1024, ROWS_MIX = 896;
+
+template <int MODE, int NV>
+__device__ __forceinline__ void load_values(const double *__restrict__ par, int64_t cap,
+                                            const double *__restrict__ var, int64_t p, double z,
+                                            double *val) {
+  if constexpr (MODE == ACC_P2G) {
+    val[0] = par[FP * cap + p];
+    val[1] = par[FQ * cap + p];
+    val[2] = par[FR * cap + p];
+    // total buoyancy, vorticity_tendency.F90:121-126
+    const double hl = par[FH * cap + p] - exp(-z);
+    val[3] = par[FB * cap + p] + 12.5 * (hl > 0.0 ? hl : 0.0);
+  } else if constexpr (MODE == ACC_SINGLE) {
+    val[0] = var[p];
+  } else {
+    val[0] = par[FB * cap + p];
+    val[1] = par[FH * cap + p];
+    val[2] = par[FP * cap + p];
+    val[3] = par[FQ * cap + p];
+    val[4] = par[FR * cap + p];
+  }
+}
 
 template <int MODE, int NV, int ROWS>
 __global__ void __launch_bounds__(256) k_cell_accum_lds(Geometry g, const double *__restrict__ par,
@@ -130,19 +153,19 @@ __global__ void __launch_bounds__(256) k_cell_accum_lds(Geometry g, const double
   __shared__ uint8_t skeep[MIX ? ROWS : 1];
   __shared__ int32_t sstart[TILE + 1];
   const int tid = threadIdx.x;
+  const int cl = tid >> 3;          // cell within the tile
+  const int q = tid & 7;            // corner: bits z<<2 | y<<1 | x
+  const int bx = q & 1, by = (q >> 1) & 1, bz = (q >> 2) & 1;
   const int64_t c0 = (int64_t)blockIdx.x * TILE;
   const int ntile = (int)min((int64_t)TILE, ncell - c0);
   for (int t = tid; t <= ntile; t += blockDim.x) sstart[t] = start[c0 + t];
   __syncthreads();
   const double *X = par + FX * cap, *Y = par + FY * cap, *Z = par + FZ * cap;
   const double *VOL = par + FVOL * cap;
-  double acc[8][NF];
+  double acc[NF], res[NF];
 #pragma unroll
-  for (int q = 0; q < 8; q++)
-#pragma unroll
-    for (int f = 0; f < NF; f++) acc[q][f] = 0.0;
+  for (int f = 0; f < NF; f++) { acc[f] = 0.0; res[f] = 0.0; }
   int kept = 0;
-  bool anyres = false;
   bool big = false;
   int first = 0;
   while (first < ntile) {
@@ -153,166 +176,122 @@ __global__ void __launch_bounds__(256) k_cell_accum_lds(Geometry g, const double
     const int nrows = sstart[last] - r0;
     const bool fits = nrows <= ROWS;
     if (fits) {
-      for (int r = tid; r < nrows; r += blockDim.x) {
-        const int32_t p = sorted[r0 + r];
+      // all of this thread's rows are fetched before any is used, so the dependent
+      // bucket -> parcel loads of different rows overlap
+      constexpr int RPT = (ROWS + TILE * 8 - 1) / (TILE * 8);
+      int32_t pr[RPT];
+#pragma unroll
+      for (int k = 0; k < RPT; k++) {
+        const int r = tid + k * TILE * 8;
+        pr[k] = (r < nrows) ? sorted[r0 + r] : -1;
+      }
+      double px[RPT], py[RPT], pz[RPT], pv[RPT], pval[RPT][NV];
+#pragma unroll
+      for (int k = 0; k < RPT; k++) {
+        if (pr[k] >= 0) {
+          const int64_t pp = pr[k];
+          px[k] = X[pp];
+          py[k] = Y[pp];
+          pz[k] = Z[pp];
+          pv[k] = VOL[pp];
+          load_values<MODE, NV>(par, cap, var, pp, pz[k], pval[k]);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < RPT; k++) {
+        const int r = tid + k * TILE * 8;
+        if (pr[k] < 0) continue;
         int i0, j0, k0;
         double delx, dely, delz;
-        locate(g, X[p], Y[p], Z[p], i0, j0, k0, delx, dely, delz);
-        const double v = VOL[p];
+        locate(g, px[k], py[k], pz[k], i0, j0, k0, delx, dely, delz);
         sdat[0][r] = delx;
         sdat[1][r] = dely;
         sdat[2][r] = delz;
-        sdat[3][r] = v;
-        if constexpr (MODE == ACC_P2G) {
-          sdat[4][r] = par[FP * cap + p];
-          sdat[5][r] = par[FQ * cap + p];
-          sdat[6][r] = par[FR * cap + p];
-          // total buoyancy, vorticity_tendency.F90:121-126
-          const double hl = par[FH * cap + p] - exp(-Z[p]);
-          sdat[7][r] = par[FB * cap + p] + 12.5 * (hl > 0.0 ? hl : 0.0);
-        } else if constexpr (MODE == ACC_SINGLE) {
-          sdat[4][r] = var[p];
-        } else {
-          sdat[4][r] = par[FB * cap + p];
-          sdat[5][r] = par[FH * cap + p];
-          sdat[6][r] = par[FP * cap + p];
-          sdat[7][r] = par[FQ * cap + p];
-          sdat[8][r] = par[FR * cap + p];
-          skeep[r] = (v < g.vmin) ? 0 : 1;
-        }
-        sidx[r] = p;
+        sdat[3][r] = pv[k];
+#pragma unroll
+        for (int f = 0; f < NV; f++) sdat[4 + f][r] = pval[k][f];
+        if constexpr (MIX) skeep[r] = (pv[k] < g.vmin) ? 0 : 1;
+        sidx[r] = pr[k];
         sperm[r] = (int16_t)r;
       }
     }
     __syncthreads();
-    if (tid >= first && tid < last) {
-      if (!fits) {
-        big = true;
-      } else {
-        const int lo = sstart[tid] - r0, hi = sstart[tid + 1] - r0;
-        for (int a = lo + 1; a < hi; a++) {
-          const int16_t v = sperm[a];
-          const int32_t key = sidx[v];
-          int b = a - 1;
-          while (b >= lo && sidx[sperm[b]] > key) {
-            sperm[b + 1] = sperm[b];
-            b--;
-          }
-          sperm[b + 1] = v;
+    const bool mine = fits && cl >= first && cl < last;
+    const int lo = mine ? sstart[cl] - r0 : 0, hi = mine ? sstart[cl + 1] - r0 : 0;
+    bool ordered = true;
+    if (mine && q == 0)
+      for (int a = lo + 1; a < hi; a++) ordered = ordered && (sidx[a - 1] < sidx[a]);
+    if (mine && q == 0 && !ordered) {
+      for (int a = lo + 1; a < hi; a++) {
+        const int16_t v = sperm[a];
+        const int32_t key = sidx[v];
+        int b = a - 1;
+        while (b >= lo && sidx[sperm[b]] > key) {
+          sperm[b + 1] = sperm[b];
+          b--;
         }
-        for (int a = lo; a < hi; a++) {
-          const int r = sperm[a];
-          if (MIX && !skeep[r]) {
-            anyres = true;
-            continue;
-          }
-          const double fx[2] = {1 - sdat[0][r], sdat[0][r]};
-          const double fy[2] = {1 - sdat[1][r], sdat[1][r]};
-          const double fz[2] = {1 - sdat[2][r], sdat[2][r]};
-          const double v = sdat[3][r];
-#pragma unroll
-          for (int q = 0; q < 8; q++) {
-            const double w = fz[(q >> 2) & 1] * fy[(q >> 1) & 1] * fx[q & 1] * v;
-            acc[q][0] += w;
-#pragma unroll
-            for (int f = 0; f < NV; f++) acc[q][f + 1] += w * sdat[4 + f][r];
-          }
-          kept++;
-        }
-        if (MIX) {
-          // kept sums are final for this cell: store, then sum the residual parcels
-          const int64_t c = c0 + tid;
-#pragma unroll
-          for (int q = 0; q < 8; q++)
-#pragma unroll
-            for (int f = 0; f < NF; f++) {
-              partial[(int64_t)(q * NF + f) * ncell + c] = acc[q][f];
-              acc[q][f] = 0.0;
-            }
-          npercell[c] = kept;
-          if (anyres) {
-            for (int a = lo; a < hi; a++) {
-              const int r = sperm[a];
-              if (skeep[r]) continue;
-              const double fx[2] = {1 - sdat[0][r], sdat[0][r]};
-              const double fy[2] = {1 - sdat[1][r], sdat[1][r]};
-              const double fz[2] = {1 - sdat[2][r], sdat[2][r]};
-              const double v = sdat[3][r];
-#pragma unroll
-              for (int q = 0; q < 8; q++) {
-                const double w = fz[(q >> 2) & 1] * fy[(q >> 1) & 1] * fx[q & 1] * v;
-                acc[q][0] += w;
-#pragma unroll
-                for (int f = 0; f < NV; f++) acc[q][f + 1] += w * sdat[4 + f][r];
-              }
-            }
-          }
-#pragma unroll
-          for (int q = 0; q < 8; q++)
-#pragma unroll
-            for (int f = 0; f < NF; f++) partial2[(int64_t)(q * NF + f) * ncell + c] = acc[q][f];
-        }
+        sperm[b + 1] = v;
       }
     }
     __syncthreads();
+    if (mine) {
+      for (int a = lo; a < hi; a++) {
+        const int r = sperm[a];
+        const double fx = bx ? sdat[0][r] : 1 - sdat[0][r];
+        const double fy = by ? sdat[1][r] : 1 - sdat[1][r];
+        const double fz = bz ? sdat[2][r] : 1 - sdat[2][r];
+        const double w = fz * fy * fx * sdat[3][r];
+        if (!MIX || skeep[r]) {
+          acc[0] += w;
+#pragma unroll
+          for (int f = 0; f < NV; f++) acc[f + 1] += w * sdat[4 + f][r];
+          kept++;
+        } else {
+          res[0] += w;
+#pragma unroll
+          for (int f = 0; f < NV; f++) res[f + 1] += w * sdat[4 + f][r];
+        }
+      }
+    }
+    if (fits == false && cl == first) big = true;
+    __syncthreads();
     first = last;
   }
-  if (tid >= ntile) return;
-  const int64_t c = c0 + tid;
+  if (cl >= ntile) return;
+  const int64_t c = c0 + cl;
   if (big) {
-    // bucket already sorted by k_bin_sort_big; same sums straight from global memory
-    const int32_t b = sstart[tid], e = sstart[tid + 1];
-    for (int pass = 0; pass < (MIX ? 2 : 1); pass++) {
+    // bucket already sorted by k_bin_sort_big; the same sums straight from global memory
+    for (int32_t s = sstart[cl]; s < sstart[cl + 1]; s++) {
+      const int32_t p = sorted[s];
+      const double v = VOL[p];
+      int i0, j0, k0;
+      double delx, dely, delz;
+      const double zp = Z[p];
+      locate(g, X[p], Y[p], zp, i0, j0, k0, delx, dely, delz);
+      double val[NV];
+      load_values<MODE, NV>(par, cap, var, p, zp, val);
+      const double fx = bx ? delx : 1 - delx;
+      const double fy = by ? dely : 1 - dely;
+      const double fz = bz ? delz : 1 - delz;
+      const double w = fz * fy * fx * v;
+      if (!MIX || !(v < g.vmin)) {
+        acc[0] += w;
 #pragma unroll
-      for (int q = 0; q < 8; q++)
-#pragma unroll
-        for (int f = 0; f < NF; f++) acc[q][f] = 0.0;
-      kept = 0;
-      for (int32_t sidx_ = b; sidx_ < e; sidx_++) {
-        const int32_t p = sorted[sidx_];
-        const double v = VOL[p];
-        if (MIX && ((pass == 0) == (v < g.vmin))) continue;
-        int i0, j0, k0;
-        double delx, dely, delz;
-        locate(g, X[p], Y[p], Z[p], i0, j0, k0, delx, dely, delz);
-        double val[NV];
-        if constexpr (MODE == ACC_P2G) {
-          val[0] = par[FP * cap + p];
-          val[1] = par[FQ * cap + p];
-          val[2] = par[FR * cap + p];
-          const double hl = par[FH * cap + p] - exp(-Z[p]);
-          val[3] = par[FB * cap + p] + 12.5 * (hl > 0.0 ? hl : 0.0);
-        } else if constexpr (MODE == ACC_SINGLE) {
-          val[0] = var[p];
-        } else {
-          val[0] = par[FB * cap + p];
-          val[1] = par[FH * cap + p];
-          val[2] = par[FP * cap + p];
-          val[3] = par[FQ * cap + p];
-          val[4] = par[FR * cap + p];
-        }
-        const double fx[2] = {1 - delx, delx}, fy[2] = {1 - dely, dely}, fz[2] = {1 - delz, delz};
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-          const double w = fz[(q >> 2) & 1] * fy[(q >> 1) & 1] * fx[q & 1] * v;
-          acc[q][0] += w;
-#pragma unroll
-          for (int f = 0; f < NV; f++) acc[q][f + 1] += w * val[f];
-        }
+        for (int f = 0; f < NV; f++) acc[f + 1] += w * val[f];
         kept++;
+      } else {
+        res[0] += w;
+#pragma unroll
+        for (int f = 0; f < NV; f++) res[f + 1] += w * val[f];
       }
-      double *dst = (MIX && pass == 1) ? partial2 : partial;
-#pragma unroll
-      for (int q = 0; q < 8; q++)
-#pragma unroll
-        for (int f = 0; f < NF; f++) dst[(int64_t)(q * NF + f) * ncell + c] = acc[q][f];
-      if (MIX && pass == 0) npercell[c] = kept;
     }
-  } else if (!MIX) {
+  }
 #pragma unroll
-    for (int q = 0; q < 8; q++)
+  for (int f = 0; f < NF; f++) partial[(int64_t)(q * NF + f) * ncell + c] = acc[f];
+  if constexpr (MIX) {
 #pragma unroll
-      for (int f = 0; f < NF; f++) partial[(int64_t)(q * NF + f) * ncell + c] = acc[q][f];
+    for (int f = 0; f < NF; f++) partial2[(int64_t)(q * NF + f) * ncell + c] = res[f];
+    if (q == 0) npercell[c] = kept;
   }
 }
 
@@ -432,7 +411,7 @@ int bin_parcels(Ctx *c, int64_t n) {
 template <int MODE, int NV, int ROWS>
 static void launch_accum_lds(Ctx *c, const double *var, double *partial, double *partial2,
                              int32_t *npc) {
-  hipLaunchKernelGGL((k_cell_accum_lds<MODE, NV, ROWS>), dim3(cdiv(c->ncell, TILE)), dim3(256), 0,
+  hipLaunchKernelGGL((k_cell_accum_lds<MODE, NV, ROWS>), dim3(cdiv(c->ncell, TILE)), dim3(TILE * 8), 0,
                      c->stream, c->g, c->par, c->cap, var, c->cell_start, c->sorted, c->ncell,
                      partial, partial2, npc);
 }

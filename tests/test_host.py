@@ -81,3 +81,17 @@ def test_library_exports_every_declared_symbol():
     for name in declared:
         assert hasattr(L, name), name
     assert L.pmpic_status_message is not None
+
+
+def test_dump_roundtrip(tmp_path):
+    from pmpic_amd.dumpio import FIELDS, read_parcels, write_parcels
+    rng = np.random.default_rng(0)
+    fields = {k: rng.standard_normal(37) for k in FIELDS}
+    path = str(tmp_path / "parcels_00000_00001.dat")
+    write_parcels(path, 1.25, [0, 1, 2, 3, 4, 5], fields)
+    # 8 (time) + 48 (bounds) + 8 (count) + 17 arrays
+    assert os.path.getsize(path) == 8 + 48 + 8 + 17 * 37 * 8
+    t, b, back = read_parcels(path)
+    assert t == 1.25 and list(b) == [0, 1, 2, 3, 4, 5]
+    for k in FIELDS:
+        np.testing.assert_array_equal(back[k], fields[k])
