@@ -104,11 +104,11 @@ enum AccMode { ACC_P2G = 0, ACC_SINGLE = 1, ACC_MIX_KEEP = 2, ACC_MIX_RES = 3 };
 // Per-cell corner partial sums.
 // A workgroup owns TILE consecutive cells.  Their buckets are consecutive in the bucket array, so
 // the workgroup loads them with coalesced reads (bucket entries are mostly consecutive parcel
-// slots), locates each parcel once and stages (offsets, volume, values, parcel index) in LDS.
-// One thread per cell then orders its rows by parcel index (insertion sort of row numbers; a
-// single comparison pass for buckets filled in slot order), and one thread per (cell, corner)
-// adds that corner's weight and weight * value over the cell's rows in ascending parcel order,
-// so within a cell the additions run in the reference's order.
+// slots), locates each parcel once and stages (offsets, volume, values, parcel index) in LDS as
+// 16-byte pairs.  The 8 threads of a cell put its rows in ascending parcel order (a pass over
+// adjacent pairs, and a parallel ranking only when a pair is out of order); then one thread per
+// (cell, corner) adds that corner's weight and weight * value over the cell's rows in that
+// order, so within a cell the additions run in the reference's order.
 // MIX mode sums kept parcels into `partial` (and counts them) and residual parcels into
 // `partial2`.  Cells with more than ROWS parcels (never at the target workloads) are summed
 // straight from global memory after k_bin_sort_big ordered their buckets.
@@ -146,13 +146,16 @@ __global__ void __launch_bounds__(256) k_cell_accum_lds(Geometry g, const double
                                                         double *__restrict__ partial2,
                                                         int32_t *__restrict__ npercell) {
   constexpr int NF = NV + 1;
+  constexpr int NVS = (NV + 1) / 2;  // value pairs per row
   constexpr bool MIX = (MODE == ACC_MIX_KEEP);
-  __shared__ double sdat[4 + NV][ROWS];   // delx, dely, delz, vol, values
+  __shared__ double2 sgeo[2][ROWS];    // (delx, dely), (delz, vol)
+  __shared__ double2 sval[NVS][ROWS];  // values, two per slot
   __shared__ int32_t sidx[ROWS];
   __shared__ int16_t sperm[ROWS];
   __shared__ uint8_t skeep[MIX ? ROWS : 1];
   __shared__ int32_t sstart[TILE + 1];
   const int tid = threadIdx.x;
+  const int lane = tid & 63;
   const int cl = tid >> 3;          // cell within the tile
   const int q = tid & 7;            // corner: bits z<<2 | y<<1 | x
   const int bx = q & 1, by = (q >> 1) & 1, bz = (q >> 2) & 1;
@@ -185,9 +188,10 @@ __global__ void __launch_bounds__(256) k_cell_accum_lds(Geometry g, const double
         const int r = tid + k * TILE * 8;
         pr[k] = (r < nrows) ? sorted[r0 + r] : -1;
       }
-      double px[RPT], py[RPT], pz[RPT], pv[RPT], pval[RPT][NV];
+      double px[RPT], py[RPT], pz[RPT], pv[RPT], pval[RPT][2 * NVS];
 #pragma unroll
       for (int k = 0; k < RPT; k++) {
+        pval[k][2 * NVS - 1] = 0.0;
         if (pr[k] >= 0) {
           const int64_t pp = pr[k];
           px[k] = X[pp];
@@ -204,53 +208,60 @@ __global__ void __launch_bounds__(256) k_cell_accum_lds(Geometry g, const double
         int i0, j0, k0;
         double delx, dely, delz;
         locate(g, px[k], py[k], pz[k], i0, j0, k0, delx, dely, delz);
-        sdat[0][r] = delx;
-        sdat[1][r] = dely;
-        sdat[2][r] = delz;
-        sdat[3][r] = pv[k];
+        sgeo[0][r] = make_double2(delx, dely);
+        sgeo[1][r] = make_double2(delz, pv[k]);
 #pragma unroll
-        for (int f = 0; f < NV; f++) sdat[4 + f][r] = pval[k][f];
+        for (int f = 0; f < NVS; f++) sval[f][r] = make_double2(pval[k][2 * f], pval[k][2 * f + 1]);
         if constexpr (MIX) skeep[r] = (pv[k] < g.vmin) ? 0 : 1;
         sidx[r] = pr[k];
-        sperm[r] = (int16_t)r;
       }
     }
     __syncthreads();
     const bool mine = fits && cl >= first && cl < last;
     const int lo = mine ? sstart[cl] - r0 : 0, hi = mine ? sstart[cl + 1] - r0 : 0;
-    bool ordered = true;
-    if (mine && q == 0)
-      for (int a = lo + 1; a < hi; a++) ordered = ordered && (sidx[a - 1] < sidx[a]);
-    if (mine && q == 0 && !ordered) {
-      for (int a = lo + 1; a < hi; a++) {
-        const int16_t v = sperm[a];
-        const int32_t key = sidx[v];
-        int b = a - 1;
-        while (b >= lo && sidx[sperm[b]] > key) {
-          sperm[b + 1] = sperm[b];
-          b--;
-        }
-        sperm[b + 1] = v;
+    // Canonical order: a cell's rows by ascending parcel index.  Bucket slots come from atomics,
+    // so runs of a cell's parcels filled by different wavefronts may arrive out of order.  The 8
+    // corner lanes of a cell check their share of adjacent pairs; if any pair is out of order,
+    // each lane ranks its share of rows by counting smaller parcel indices in the cell.
+    bool ord = true;
+    for (int a = lo + 1 + q; a < hi; a += 8) ord &= (sidx[a - 1] < sidx[a]);
+    const unsigned long long bad = __ballot(!ord);
+    const bool sort_cell = ((bad >> (lane & ~7)) & 0xffull) != 0ull;
+    if (sort_cell) {
+      for (int a = lo + q; a < hi; a += 8) {
+        const int32_t key = sidx[a];
+        int rk = 0;
+        for (int b = lo; b < hi; b++) rk += (sidx[b] < key) ? 1 : 0;
+        sperm[lo + rk] = (int16_t)a;
       }
+    } else {
+      for (int a = lo + q; a < hi; a += 8) sperm[a] = (int16_t)a;
     }
     __syncthreads();
-    if (mine) {
-      for (int a = lo; a < hi; a++) {
-        const int r = sperm[a];
-        const double fx = bx ? sdat[0][r] : 1 - sdat[0][r];
-        const double fy = by ? sdat[1][r] : 1 - sdat[1][r];
-        const double fz = bz ? sdat[2][r] : 1 - sdat[2][r];
-        const double w = fz * fy * fx * sdat[3][r];
-        if (!MIX || skeep[r]) {
-          acc[0] += w;
+#pragma unroll 2
+    for (int a = lo; a < hi; a++) {
+      const int r = sperm[a];
+      const double2 g0 = sgeo[0][r], g1 = sgeo[1][r];
+      double v[2 * NVS];
 #pragma unroll
-          for (int f = 0; f < NV; f++) acc[f + 1] += w * sdat[4 + f][r];
-          kept++;
-        } else {
-          res[0] += w;
+      for (int f = 0; f < NVS; f++) {
+        const double2 t = sval[f][r];
+        v[2 * f] = t.x;
+        v[2 * f + 1] = t.y;
+      }
+      const double fx = bx ? g0.x : 1 - g0.x;
+      const double fy = by ? g0.y : 1 - g0.y;
+      const double fz = bz ? g1.x : 1 - g1.x;
+      const double w = fz * fy * fx * g1.y;
+      if (!MIX || skeep[r]) {
+        acc[0] += w;
 #pragma unroll
-          for (int f = 0; f < NV; f++) res[f + 1] += w * sdat[4 + f][r];
-        }
+        for (int f = 0; f < NV; f++) acc[f + 1] += w * v[f];
+        kept++;
+      } else {
+        res[0] += w;
+#pragma unroll
+        for (int f = 0; f < NV; f++) res[f + 1] += w * v[f];
       }
     }
     if (fits == false && cl == first) big = true;

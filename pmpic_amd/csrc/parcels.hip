@@ -96,10 +96,27 @@ __global__ void __launch_bounds__(TPB) k_rk_stage(Geometry g, double *__restrict
   const bool active = p < n;
   int dir = 0;
   if (active) {
-    double x = par[FX * cap + p], y = par[FY * cap + p], z = par[FZ * cap + p];
+    // every parcel value the stage reads is loaded up front, so the loads overlap each other
+    // and the grid gathers instead of waiting behind the stores
+    const double x = par[FX * cap + p], y = par[FY * cap + p], z0 = par[FZ * cap + p];
+    double yo[6], yf[6];
+    if (STAGE == 1) {
+      yo[0] = x;
+      yo[1] = y;
+      yo[2] = z0;
+#pragma unroll
+      for (int f = 3; f < 6; f++) yo[f] = par[(FX + f) * cap + p];
+    } else {
+#pragma unroll
+      for (int f = 0; f < 6; f++) {
+        if (STAGE != 4) yo[f] = par[(f < 3 ? FXO + f : FPO + f - 3) * cap + p];
+        yf[f] = par[(f < 3 ? FXF + f : FPF + f - 3) * cap + p];
+      }
+    }
+    const double s_old = (STAGE == 4 && stretch) ? par[FSTRETCH * cap + p] : 0.0;
     int i0, j0, k0;
     double delx, dely, delz;
-    const int st = locate(g, x, y, z, i0, j0, k0, delx, dely, delz);
+    const int st = locate(g, x, y, z0, i0, j0, k0, delx, dely, delz);
     if (st != PMPIC_OK) {
       raise_status(sc, st);
     } else {
@@ -123,47 +140,40 @@ __global__ void __launch_bounds__(TPB) k_rk_stage(Geometry g, double *__restrict
       }
       const double dt = sc->dtm;
       const double dt6 = dt / 6, dt2 = dt / 2, dt3 = dt / 3;
-      double yv[6];
+      double yv[6], yfn[6];
 #pragma unroll
       for (int f = 0; f < 6; f++) {
-        par[(FDXDT + f) * cap + p] = kv[f];
-        double *Y = par + (FX + f) * cap + p;
-        double *YO = par + (FXO + f) * cap + p;
-        double *YF = par + (FXF + f) * cap + p;
-        if (f >= 3) {  // vorticity: po/pf live after xo..zf
-          YO = par + (FPO + f - 3) * cap + p;
-          YF = par + (FPF + f - 3) * cap + p;
-        }
         if (STAGE == 1) {
-          const double yo = *Y;
-          *YO = yo;
-          *YF = yo + dt6 * kv[f];
-          yv[f] = yo + dt2 * kv[f];
+          yfn[f] = yo[f] + dt6 * kv[f];
+          yv[f] = yo[f] + dt2 * kv[f];
         } else if (STAGE == 2) {
-          *YF = *YF + dt3 * kv[f];
-          yv[f] = *YO + dt2 * kv[f];
+          yfn[f] = yf[f] + dt3 * kv[f];
+          yv[f] = yo[f] + dt2 * kv[f];
         } else if (STAGE == 3) {
-          *YF = *YF + dt3 * kv[f];
-          yv[f] = *YO + dt * kv[f];
+          yfn[f] = yf[f] + dt3 * kv[f];
+          yv[f] = yo[f] + dt * kv[f];
         } else {
-          const double yf = *YF + dt6 * kv[f];
-          *YF = yf;
-          yv[f] = yf;
+          yfn[f] = yf[f] + dt6 * kv[f];
+          yv[f] = yfn[f];
         }
       }
       // reflect the current z only (haloswap.F90:447-454)
-      z = yv[2];
+      double z = yv[2];
       if (z < g.zmin_h) z = g.zmin_h + (g.zmin_h - z);
       else if (z > g.zmax_h) z = g.zmax_h - (z - g.zmax_h);
       yv[2] = z;
 #pragma unroll
-      for (int f = 0; f < 6; f++) par[(FX + f) * cap + p] = yv[f];
+      for (int f = 0; f < 6; f++) {
+        par[(FDXDT + f) * cap + p] = kv[f];
+        if (STAGE == 1) par[(f < 3 ? FXO + f : FPO + f - 3) * cap + p] = yo[f];
+        par[(f < 3 ? FXF + f : FPF + f - 3) * cap + p] = yfn[f];
+        par[(FX + f) * cap + p] = yv[f];
+      }
       if (STAGE == 4 && stretch) {
         const double third = (double)(1.f / 3.f);
         double ds = yv[3] * kv[3] + yv[4] * kv[4] + yv[5] * kv[5];
         ds = pow(fabs(ds), third);
-        double *S = par + FSTRETCH * cap + p;
-        *S = *S + ds * dt;
+        par[FSTRETCH * cap + p] = s_old + ds * dt;
       }
       dir = direction(g, yv[0], yv[1]);
     }
@@ -439,30 +449,40 @@ __global__ void k_mix_absorb(Geometry g, double *__restrict__ par, int64_t cap, 
                              const uint8_t *__restrict__ keep, MixGrids mg) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= norig || keep[p] != 1) return;
+  // all loads first (parcel values, then the gathers), all stores last
+  const int fld[5] = {FB, FH, FP, FQ, FR};
+  const double x = par[FX * cap + p], y = par[FY * cap + p], z = par[FZ * cap + p];
+  const double v = par[FVOL * cap + p];
+  double fv[5];
+#pragma unroll
+  for (int f = 0; f < 5; f++) fv[f] = par[(int64_t)fld[f] * cap + p];
   int i0, j0, k0;
   double delx, dely, delz;
-  locate(g, par[FX * cap + p], par[FY * cap + p], par[FZ * cap + p], i0, j0, k0, delx, dely, delz);
+  locate(g, x, y, z, i0, j0, k0, delx, dely, delz);
   const int i1 = (i0 + 1 == g.nx) ? 0 : i0 + 1;
   const int j1 = (j0 + 1 == g.ny) ? 0 : j0 + 1;
   const int64_t q[8] = {gidx(g, k0, j0, i0),     gidx(g, k0, j0, i1),
                         gidx(g, k0, j1, i0),     gidx(g, k0, j1, i1),
                         gidx(g, k0 + 1, j0, i0), gidx(g, k0 + 1, j0, i1),
                         gidx(g, k0 + 1, j1, i0), gidx(g, k0 + 1, j1, i1)};
-  const double v = par[FVOL * cap + p];
-  double w[8];
+  double w[8], g0[8];
   corner_weights(delx, dely, delz, v, w);
+#pragma unroll
+  for (int c = 0; c < 8; c++) g0[c] = mg.G[0][q[c]];
   double vn = v;
 #pragma unroll
-  for (int c = 0; c < 8; c++) vn = vn + w[c] * mg.R[0][q[c]] / mg.G[0][q[c]];
-  par[FVOL * cap + p] = vn;
-  const int fld[5] = {FB, FH, FP, FQ, FR};
+  for (int c = 0; c < 8; c++) vn = vn + w[c] * mg.R[0][q[c]] / g0[c];
+  double out[5];
 #pragma unroll
   for (int f = 0; f < 5; f++) {
-    double acc = par[(int64_t)fld[f] * cap + p] * v;
+    double acc = fv[f] * v;
 #pragma unroll
-    for (int c = 0; c < 8; c++) acc = acc + w[c] / mg.G[0][q[c]] * mg.R[f + 1][q[c]];
-    par[(int64_t)fld[f] * cap + p] = acc / vn;
+    for (int c = 0; c < 8; c++) acc = acc + w[c] / g0[c] * mg.R[f + 1][q[c]];
+    out[f] = acc / vn;
   }
+  par[FVOL * cap + p] = vn;
+#pragma unroll
+  for (int f = 0; f < 5; f++) par[(int64_t)fld[f] * cap + p] = out[f];
 }
 
 // backfill (parcel_mixing.F90:347-389): j-th remaining removed slot <- j-th kept slot from the
