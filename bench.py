@@ -2,14 +2,17 @@
 """Throughput of the PMPIC per-timestep parcel-in-cell path on one MI355X.
 
 Workload: the moist thermal on a 128x128x128 grid (2pi x 2pi x pi box, periodic x/y, rigid z)
-with 27 parcels per cell on a jittered 3x3x3 lattice (56,623,104 parcels), plain FP64, RK4 with
-dt = 0.01, splitting and mixing on.  A step is one full timestep (the integrator group run for
-the four RK stages: par2grid, vort2vel, vorticity_tendency, grid2par + RK update, haloswap,
-and at stage 4 splitting and mixing) with the parcels resident in HBM.
+with 27 parcels per cell on a jittered 3x3x3 lattice (56,623,104 parcels), RK4 with dt = 0.01,
+splitting and mixing on.  A step is one full timestep (the integrator group run for the four RK
+stages: par2grid, vort2vel, vorticity_tendency, grid2par + RK update, haloswap, and at stage 4
+splitting and mixing) with the parcels resident in HBM.
 
-Prints one JSON line: parcel-steps/s over the timed steps, the roofline of the dominant kernel
-(HIP events on the library's stream around every launch in the timed region) and a CPU
-baseline (the C restatement of the same path, single-threaded, on a bounded sample).
+Both numeric forms run on the same initial condition: the FP32-bulk form (positions as cell
+index + FP32 offsets, FP32 parcel arithmetic; `value`) and the plain FP64 form.  The velocity
+error of the FP32-bulk form against the FP64 form is measured after --err-steps steps on the
+--err-n grid.  Prints one JSON line: parcel-steps/s of each form, the roofline of each form's
+dominant kernel (HIP events on the library's stream around every launch in the timed region)
+and a CPU baseline (the C restatement of the same path, single-threaded, on a bounded sample).
 """
 import argparse
 import json
@@ -30,6 +33,12 @@ HBM_PEAK_GBS = 8000.0
 #    tendencies written (48 B), direction flag (1 B); per node the six gathered grids (48 B)
 RK_STAGE_BYTES = {1: 24 + 24 + 18 * 8 + 48 + 1, 2: 24 + 12 * 8 + 12 * 8 + 48 + 1,
                   3: 24 + 12 * 8 + 12 * 8 + 48 + 1, 4: 24 + 7 * 8 + 13 * 8 + 48 + 1}
+# FP32-bulk form: cell accumulation reads per parcel the bucket entry, cell, offsets, vol,
+# p,q,r,b,h (44 B) and per cell two offsets and 40 FP32 partials (168 B); the RK stages move
+# cell (4 B) + offsets (12 B) for the position and the FP32 RK state (see bulk32.hip)
+ACC_BYTES = {0: (76, 328), 1: (44, 168)}
+RK32_STAGE_BYTES = {1: 28 + 80, 2: 68 + 52, 3: 68 + 52, 4: 60 + 56}
+GATHER_NODE_BYTES = {0: 48, 1: 24}
 
 
 def parse():
@@ -42,6 +51,11 @@ def parse():
     ap.add_argument("--ppc", type=int, default=3, help="parcels per cell per direction")
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--forms", default="fp32,fp64",
+                    help="numeric forms to time: fp32 (FP32-bulk), fp64 (plain), or both")
+    ap.add_argument("--err-steps", type=int, default=100,
+                    help="steps of the FP32-bulk vs FP64 velocity-error run (0 = skip)")
+    ap.add_argument("--err-n", type=int, default=64, help="grid of the error run (n x n x n)")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (the reference kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -87,6 +101,86 @@ def cpu_baseline(steps):
                       f"({len(ic['x'])} parcels), {steps} steps, single thread, {dt:.1f} s"}
 
 
+def time_form(precision, ic, args, n, nz):
+    """Warm up, then time args.steps whole steps of one numeric form; returns the figures."""
+    from pmpic_amd.lib import Context
+    lx, lz = ic["_lx"], ic["_lz"]
+    npar = len(ic["x"])
+    ctx = Context(n, n, nz, 0.0, lx, 0.0, lx, 0.0, lz, 2 * npar, dtm=0.01, dtmax=0.01,
+                  timestep_prefactor=0.5, precision=precision)
+    ctx.set_parcels(ic)
+    tag = "fp32-bulk" if precision else "fp64"
+    for s in range(args.warmup):
+        ctx.step(1)
+        print(f"bench[{tag}]: warmup step {s + 1} done, {ctx.n} parcels", file=sys.stderr,
+              flush=True)
+    ctx.timing_enable(True)
+    total = 0
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        ctx.step(1)  # returns after the library's stream has drained
+        total += ctx.n
+    elapsed = time.perf_counter() - t0
+    print(f"bench[{tag}]: {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+    p2g_ms, p2g_calls = ctx.timing_read(0)
+    rk_ms, rk_calls = ctx.timing_read(1)
+    ctx.close()
+    ncell = n * n * nz
+    nnode = n * n * (nz + 1)
+    avg_np = total / max(args.steps, 1)
+    if p2g_ms >= rk_ms:
+        name = "k_cell_accum (par2grid per-cell corner sums)" if not precision else \
+            "k32_cell_accum (FP32 par2grid per-cell corner sums)"
+        bp, bc = ACC_BYTES[precision]
+        per_launch = avg_np * bp + ncell * bc
+        avg_ms = p2g_ms / max(p2g_calls, 1)
+    else:
+        name = "k_rk_stage (grid2par of 6 fields + RK4 stage update)" if not precision else \
+            "k32_rk_stage (FP32 grid2par of 6 fields + RK4 stage update)"
+        table = RK32_STAGE_BYTES if precision else RK_STAGE_BYTES
+        per_launch = avg_np * sum(table.values()) / 4 + nnode * GATHER_NODE_BYTES[precision]
+        avg_ms = rk_ms / max(rk_calls, 1)
+    achieved = per_launch / (avg_ms * 1e-3) / 1e9
+    return {
+        "value": total / elapsed,
+        "ms_per_step": 1e3 * elapsed / max(args.steps, 1),
+        "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "avg_launch_ms": avg_ms, "bytes_per_launch": per_launch,
+                     "p2g_ms_total": p2g_ms, "p2g_launches": p2g_calls,
+                     "rk_ms_total": rk_ms, "rk_launches": rk_calls},
+    }
+
+
+def velocity_error(args):
+    """Both forms from the same state for args.err_steps steps on an err_n^3 grid; the FP32-bulk
+    form's maximum velocity error relative to the FP64 form's largest velocity component."""
+    import numpy as np
+    from pmpic_amd.config import fortran_real
+    from pmpic_amd.initial import thermal
+    from pmpic_amd.lib import Context
+    n = args.err_n
+    lx, lz = fortran_real("6.28318531"), fortran_real("3.14159265")
+    ic = thermal(n, n, n, lx, lx, lz, args.ppc, seed=0x504D504943000000 + 1)
+    grids = {}
+    counts = {}
+    for precision in (1, 0):
+        ctx = Context(n, n, n, 0.0, lx, 0.0, lx, 0.0, lz, 2 * len(ic["x"]), dtm=0.01,
+                      dtmax=0.01, timestep_prefactor=0.5, precision=precision)
+        ctx.set_parcels(ic)
+        ctx.step(args.err_steps)
+        grids[precision] = [ctx.grid(k) for k in ("u", "v", "w")]
+        counts[precision] = ctx.n
+        ctx.close()
+        print(f"bench: error run precision={precision} done ({args.err_steps} steps)",
+              file=sys.stderr, flush=True)
+    scale = max(float(np.max(np.abs(g))) for g in grids[0])
+    err = max(float(np.max(np.abs(a - b))) for a, b in zip(grids[1], grids[0])) / scale
+    return {"rel_velocity_error": err, "steps": args.err_steps, "grid": [n, n, n],
+            "parcels_fp32": counts[1], "parcels_fp64": counts[0],
+            "ok": bool(err <= 1e-5)}
+
+
 def main():
     args = parse()
     if args.copy_calibrate:
@@ -100,69 +194,47 @@ def main():
         return 2
     from pmpic_amd.config import fortran_real
     from pmpic_amd.initial import thermal
-    from pmpic_amd.lib import Context
 
     n, nz, ppcd = args.n, args.nz, args.ppc
     lx, lz = fortran_real("6.28318531"), fortran_real("3.14159265")
     t_ic = time.perf_counter()
     ic = thermal(n, n, nz, lx, lx, lz, ppcd, seed=0x504D504943000000 + 1)
+    ic["_lx"], ic["_lz"] = lx, lz
     npar = len(ic["x"])
     print(f"bench: initial condition {npar} parcels in {time.perf_counter() - t_ic:.1f} s",
           file=sys.stderr, flush=True)
-    ctx = Context(n, n, nz, 0.0, lx, 0.0, lx, 0.0, lz, 2 * npar, dtm=0.01, dtmax=0.01,
-                  timestep_prefactor=0.5)
-    ctx.set_parcels(ic)
+    forms = [f.strip() for f in args.forms.split(",") if f.strip()]
+    res = {}
+    for f in forms:
+        res[f] = time_form(1 if f == "fp32" else 0, {k: v for k, v in ic.items()}, args, n, nz)
     del ic
-    for s in range(args.warmup):
-        ctx.step(1)
-        print(f"bench: warmup step {s + 1} done, {ctx.n} parcels", file=sys.stderr, flush=True)
-    ctx.timing_enable(True)
-    total = 0
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        ctx.step(1)  # returns after the library's stream has drained
-        total += ctx.n
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    print(f"bench: {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
-    p2g_ms, p2g_calls = ctx.timing_read(0)
-    rk_ms, rk_calls = ctx.timing_read(1)
-    ncell = n * n * nz
-    nnode = n * n * (nz + 1)
-    avg_np = total / max(args.steps, 1)
-    if p2g_ms >= rk_ms:
-        name = "k_cell_accum (par2grid per-cell corner sums)"
-        per_launch = avg_np * 76 + ncell * 328
-        avg_ms = p2g_ms / max(p2g_calls, 1)
-    else:
-        name = "k_rk_stage (grid2par of 6 fields + RK4 stage update)"
-        per_launch = avg_np * sum(RK_STAGE_BYTES.values()) / 4 + nnode * 48
-        avg_ms = rk_ms / max(rk_calls, 1)
-    achieved = per_launch / (avg_ms * 1e-3) / 1e9
-    value = total / elapsed
+    head = res.get("fp32") or res["fp64"]
+    form = "FP32-bulk" if "fp32" in res else "plain FP64"
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": head["value"],
         "unit": "parcel-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / max(args.steps, 1),
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32" if "fp32" in res else "f64",
         "data": "synthetic (seeded jittered-lattice moist thermal)",
         "config": {"workload": f"moist thermal {n}x{n}x{nz} cells, {ppcd ** 3} parcels/cell "
-                               f"({npar} parcels), plain FP64, dt=0.01, split/mix on",
-                   "parcels": npar, "grid": [n, n, nz], "form": "plain FP64"},
-        "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "avg_launch_ms": avg_ms, "bytes_per_launch": per_launch,
-                     "p2g_ms_total": p2g_ms, "p2g_launches": p2g_calls,
-                     "rk_ms_total": rk_ms, "rk_launches": rk_calls},
+                               f"({npar} parcels), {form}, dt=0.01, split/mix on",
+                   "parcels": npar, "grid": [n, n, nz], "form": form},
+        "roofline": head["roofline"],
     }
-    ctx.close()
+    if "fp64" in res:
+        out["fp64"] = {"value": res["fp64"]["value"], "ms_per_step": res["fp64"]["ms_per_step"],
+                       "roofline": res["fp64"]["roofline"]}
+    if "fp32" in res and "fp64" in res:
+        out["fp32_speedup_vs_fp64"] = res["fp32"]["value"] / res["fp64"]["value"]
+    if args.err_steps > 0 and "fp32" in res:
+        out["fp32_error"] = velocity_error(args)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
     print(json.dumps(out))

@@ -1,0 +1,1097 @@
+// FP32-bulk form of the per-timestep parcel path on MI355X (pmpic_config.precision = 1).
+//
+// Reference path: components/rk4_integrator/src/rk4_integrator.F90:95-319,
+// model_core/src/parcels/parcel_interpolation.F90:186-481, model_core/src/parcels/haloswap.F90,
+// components/parcel_splitting/src/parcel_splitting.F90:47-168,
+// components/parcel_mixing/src/parcel_mixing.F90:68-624.
+//
+// The algorithm is the plain FP64 form's, step for step (same RK4 stage formulas, split and
+// mix rules, slot placement of children and new parcels, same deterministic cell-binned
+// par2grid); the representation is what changes:
+//  * a position is the index of the cell holding the parcel (int32, c = (k*ny + j)*nx + i) plus
+//    three FP32 offsets in [0,1) inside the cell, so it carries 24 bits relative to the cell
+//    rather than to the whole domain; vorticity, buoyancy, humidity, volume and stretch are
+//    FP32, and interpolation and the par2grid sums run in FP32;
+//  * the RK4 origin of the position is (cell_o, offset_o) and the RK4 accumulator
+//    dt/6 k1 + dt/3 k2 + dt/3 k3 + dt/6 k4 is kept in cells relative to it;
+//  * the haloswap becomes index arithmetic: leaving through x or y wraps the cell index and the
+//    z reflection mirrors (cell, offset) about the boundary, so parcels keep their slots (the
+//    plain form reproduces the reference's send-to-self slot permutation instead);
+//  * parcel tendencies dxdt..drdt are stored at stage 4 only (the values that survive a step).
+// The spectral inversion, the timestep selection and the conservation sums stay FP64: the
+// grids are a few percent of the parcel bytes.
+#include "common.h"
+
+#include <hipcub/hipcub.hpp>
+#include <math.h>
+#include <vector>
+
+namespace pmpic {
+
+namespace {
+
+constexpr int TPB = 256;
+constexpr int NWAVE = TPB / 64;
+
+__device__ __forceinline__ void cell_ijk(const Geometry &g, int32_t c, int &i, int &j, int &k) {
+  i = c % g.nx;
+  const int r = c / g.nx;
+  j = r % g.ny;
+  k = r / g.ny;
+}
+
+// fold an offset outside [0,1) into the cell index
+__device__ __forceinline__ void fold1(int &i, float &o) {
+  const float f = floorf(o);
+  i += (int)f;
+  o = o - f;
+  if (o >= 1.0f) {  // o was a tiny negative number
+    o -= 1.0f;
+    i += 1;
+  }
+}
+
+// cell (i, j, k) + offsets (tx, ty, tz) in cells, offsets anywhere -> cell index and offsets in
+// [0,1): periodic in x and y, z mirrored about 0 and nz (haloswap.F90:447-454 on the cell
+// scale; a position exactly on the top boundary stays in the top cell with offset 1).  Returns
+// false if the position is still outside in z.
+__device__ __forceinline__ bool place(const Geometry &g, int i, int j, int k, float tx, float ty,
+                                      float tz, int32_t &cell, float &ox, float &oy, float &oz) {
+  fold1(i, tx);
+  fold1(j, ty);
+  fold1(k, tz);
+  i %= g.nx;
+  if (i < 0) i += g.nx;
+  j %= g.ny;
+  if (j < 0) j += g.ny;
+  const int nz = g.ncz;
+  if (k < 0) {  // z' = -z
+    if (tz == 0.0f) {
+      k = -k;
+    } else {
+      k = -k - 1;
+      tz = 1.0f - tz;
+      if (tz >= 1.0f) { tz -= 1.0f; k += 1; }
+    }
+  } else if (k >= nz && !(k == nz && tz == 0.0f)) {  // z' = 2 nz - z
+    if (tz == 0.0f) {
+      k = 2 * nz - k;
+    } else {
+      k = 2 * nz - k - 1;
+      tz = 1.0f - tz;
+      if (tz >= 1.0f) { tz -= 1.0f; k += 1; }
+    }
+  }
+  if (k == nz && tz == 0.0f) {
+    k = nz - 1;
+    tz = 1.0f;
+  }
+  if (k < 0 || k >= nz) return false;
+  cell = (k * g.ny + j) * g.nx + i;
+  ox = tx;
+  oy = ty;
+  oz = tz;
+  return true;
+}
+
+// node offsets of the 8 corners of cell (i, j, k), corner bits z<<2 | y<<1 | x
+__device__ __forceinline__ void corner_nodes(const Geometry &g, int i, int j, int k, int32_t o[8]) {
+  const int i1 = (i + 1 == g.nx) ? 0 : i + 1;
+  const int j1 = (j + 1 == g.ny) ? 0 : j + 1;
+  const int32_t b00 = (k * g.ny + j) * g.nx, b01 = (k * g.ny + j1) * g.nx;
+  const int32_t b10 = ((k + 1) * g.ny + j) * g.nx, b11 = ((k + 1) * g.ny + j1) * g.nx;
+  o[0] = b00 + i; o[1] = b00 + i1; o[2] = b01 + i; o[3] = b01 + i1;
+  o[4] = b10 + i; o[5] = b10 + i1; o[6] = b11 + i; o[7] = b11 + i1;
+}
+
+// trilinear interpolation in the reference's order (parcel_interpolation.F90:309-321)
+__device__ __forceinline__ float interp(const float *__restrict__ G, const int32_t o[8], float ox,
+                                        float oy, float oz) {
+  const float c00 = G[o[0]] * (1 - oz) + G[o[4]] * oz;
+  const float c01 = G[o[1]] * (1 - oz) + G[o[5]] * oz;
+  const float c10 = G[o[2]] * (1 - oz) + G[o[6]] * oz;
+  const float c11 = G[o[3]] * (1 - oz) + G[o[7]] * oz;
+  const float c0 = c00 * (1 - oy) + c10 * oy;
+  const float c1 = c01 * (1 - oy) + c11 * oy;
+  return c0 * (1 - ox) + c1 * ox;
+}
+
+// ---- host <-> device conversion of the parcel state ----
+// axis a of the position: (x - bottom)/d -> cell index part and offset
+__global__ void k32_pos_in(Geometry g, int axis, const double *__restrict__ x, int64_t n,
+                           int32_t *__restrict__ cell, float *__restrict__ off, DevScalars *sc) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const double bot = axis == 0 ? g.xbot : (axis == 1 ? g.ybot : g.zbot);
+  const double d = axis == 0 ? g.dx : (axis == 1 ? g.dy : g.dz);
+  const int ncells = axis == 0 ? g.nx : (axis == 1 ? g.ny : g.ncz);
+  const double u = (x[p] - bot) / d;
+  const double fu = floor(u);
+  int i = (int)fu;
+  float o = (float)(u - fu);
+  if (o >= 1.0f) { o = 0.0f; i += 1; }
+  if (axis < 2) {
+    i %= ncells;
+    if (i < 0) i += ncells;
+  } else {
+    if (i == ncells && o == 0.0f) { i = ncells - 1; o = 1.0f; }
+    if (i < 0 || i >= ncells) {
+      raise_status(sc, PMPIC_ERR_OUT_OF_BOX_Z);
+      i = 0;
+    }
+  }
+  off[p] = o;
+  const int32_t stride = axis == 0 ? 1 : (axis == 1 ? g.nx : g.nx * g.ny);
+  cell[p] = (axis == 0 ? 0 : cell[p]) + i * stride;
+}
+
+__global__ void k32_pos_out(Geometry g, int axis, const int32_t *__restrict__ cell,
+                            const float *__restrict__ off, int64_t n, double *__restrict__ x) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  int i, j, k;
+  cell_ijk(g, cell[p], i, j, k);
+  const int idx = axis == 0 ? i : (axis == 1 ? j : k);
+  const double bot = axis == 0 ? g.xbot : (axis == 1 ? g.ybot : g.zbot);
+  const double d = axis == 0 ? g.dx : (axis == 1 ? g.dy : g.dz);
+  x[p] = bot + ((double)idx + (double)off[p]) * d;
+}
+
+__global__ void k32_from_f64(const double *__restrict__ a, int64_t n, float *__restrict__ b) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) b[p] = (float)a[p];
+}
+
+__global__ void k32_to_f64(const float *__restrict__ a, int64_t n, double *__restrict__ b) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) b[p] = (double)a[p];
+}
+
+// ---- cell binning (the plain form's, with the cell read instead of located) ----
+__global__ void k32_bin_count(const int32_t *__restrict__ cellv, int64_t n,
+                              int32_t *__restrict__ rank, int32_t *__restrict__ count) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int32_t c = (p < n) ? cellv[p] : -1;
+  const int32_t cprev = __shfl_up(c, 1);
+  const bool head = (lane == 0) || (c != cprev);
+  const unsigned long long hm = __ballot(head);
+  const unsigned long long upto = hm & ((2ull << lane) - 1ull);
+  const int hl = 63 - __clzll(upto);
+  const unsigned long long above = hm & ~((2ull << hl) - 1ull);
+  const int next = above ? (__ffsll(above) - 1) : 64;
+  int base = 0;
+  if (head && c >= 0) base = atomicAdd(&count[c], next - hl);
+  base = __shfl(base, hl);
+  if (p < n) rank[p] = base + (lane - hl);
+}
+
+__global__ void k32_bin_scatter(const int32_t *__restrict__ cellv, const int32_t *__restrict__ rank,
+                                const int32_t *__restrict__ start, int64_t n,
+                                int32_t *__restrict__ sorted) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  sorted[start[cellv[p]] + rank[p]] = (int32_t)p;
+}
+
+__global__ void k32_sort_big(const int32_t *__restrict__ start, int64_t ncell, int rows,
+                             int32_t *__restrict__ sorted) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const int32_t b = start[c], e = start[c + 1];
+  if (e - b <= rows) return;
+  for (int32_t a = b + 1; a < e; a++) {
+    const int32_t v = sorted[a];
+    int32_t q = a - 1;
+    while (q >= b && sorted[q] > v) {
+      sorted[q + 1] = sorted[q];
+      q--;
+    }
+    sorted[q + 1] = v;
+  }
+}
+
+__global__ void k32_fill_i32(int32_t *a, int64_t n, int32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+// ---- per-cell corner partial sums (the plain form's LDS kernel in FP32) ----
+enum Acc32 { A32_P2G = 0, A32_MIX = 1 };
+constexpr int TILE32 = 32;
+constexpr int ROWS32 = 1024;
+
+struct Zmap {
+  float zbot, dz;  // z of a parcel = zbot + (k + oz) dz
+};
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k32_cell_accum(Bulk32 b, int64_t cap, Geometry g, Zmap zm,
+                                                      float vmin, const int32_t *__restrict__ start,
+                                                      const int32_t *__restrict__ sorted,
+                                                      int64_t ncell, float *__restrict__ partial,
+                                                      float *__restrict__ partial2,
+                                                      int32_t *__restrict__ npercell) {
+  constexpr bool MIX = (MODE == A32_MIX);
+  constexpr int NV = MIX ? 5 : 4;
+  constexpr int NF = NV + 1;
+  __shared__ float4 sgeo[ROWS32];  // ox, oy, oz, vol
+  __shared__ float4 sva[ROWS32];   // values 0..3
+  __shared__ float svb[MIX ? ROWS32 : 1];
+  __shared__ int32_t sidx[ROWS32];
+  __shared__ int16_t sperm[ROWS32];
+  __shared__ uint8_t skeep[MIX ? ROWS32 : 1];
+  __shared__ int32_t sstart[TILE32 + 1];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int cl = tid >> 3;
+  const int q = tid & 7;
+  const int bx = q & 1, by = (q >> 1) & 1, bz = (q >> 2) & 1;
+  const int64_t c0 = (int64_t)blockIdx.x * TILE32;
+  const int ntile = (int)min((int64_t)TILE32, ncell - c0);
+  for (int t = tid; t <= ntile; t += blockDim.x) sstart[t] = start[c0 + t];
+  __syncthreads();
+  const float *F = b.f;
+  float acc[NF], res[NF];
+#pragma unroll
+  for (int f = 0; f < NF; f++) { acc[f] = 0.f; res[f] = 0.f; }
+  int kept = 0;
+  bool big = false;
+  int first = 0;
+  while (first < ntile) {
+    int last = first + 1;
+    while (last < ntile && sstart[last + 1] - sstart[first] <= ROWS32) last++;
+    const int32_t r0 = sstart[first];
+    const int nrows = sstart[last] - r0;
+    const bool fits = nrows <= ROWS32;
+    if (fits) {
+      constexpr int RPT = ROWS32 / (TILE32 * 8);
+      int32_t pr[RPT];
+#pragma unroll
+      for (int k = 0; k < RPT; k++) {
+        const int r = tid + k * TILE32 * 8;
+        pr[k] = (r < nrows) ? sorted[r0 + r] : -1;
+      }
+      float4 geo[RPT], va[RPT];
+      float vb[RPT];
+      int32_t pc[RPT];
+#pragma unroll
+      for (int k = 0; k < RPT; k++) {
+        vb[k] = 0.f;
+        if (pr[k] >= 0) {
+          const int64_t pp = pr[k];
+          geo[k] = make_float4(F[B_OX * cap + pp], F[B_OY * cap + pp], F[B_OZ * cap + pp],
+                               F[B_VOL * cap + pp]);
+          if constexpr (!MIX) {
+            pc[k] = b.cell[pp];
+            va[k] = make_float4(F[B_P * cap + pp], F[B_Q * cap + pp], F[B_R * cap + pp],
+                                F[B_B * cap + pp]);
+            vb[k] = F[B_H * cap + pp];  // humidity, for the total buoyancy below
+          } else {
+            va[k] = make_float4(F[B_B * cap + pp], F[B_H * cap + pp], F[B_P * cap + pp],
+                                F[B_Q * cap + pp]);
+            vb[k] = F[B_R * cap + pp];
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < RPT; k++) {
+        const int r = tid + k * TILE32 * 8;
+        if (pr[k] < 0) continue;
+        float4 v = va[k];
+        if constexpr (!MIX) {
+          // total buoyancy (vorticity_tendency.F90:121-126)
+          const int kc = pc[k] / (g.nx * g.ny);
+          const float z = zm.zbot + ((float)kc + geo[k].z) * zm.dz;
+          const float hl = vb[k] - expf(-z);
+          v.w = v.w + 12.5f * (hl > 0.f ? hl : 0.f);
+        } else {
+          svb[r] = vb[k];
+          skeep[r] = (geo[k].w < vmin) ? 0 : 1;
+        }
+        sgeo[r] = geo[k];
+        sva[r] = v;
+        sidx[r] = pr[k];
+      }
+    }
+    __syncthreads();
+    const bool mine = fits && cl >= first && cl < last;
+    const int lo = mine ? sstart[cl] - r0 : 0, hi = mine ? sstart[cl + 1] - r0 : 0;
+    bool ord = true;
+    for (int a = lo + 1 + q; a < hi; a += 8) ord &= (sidx[a - 1] < sidx[a]);
+    const unsigned long long bad = __ballot(!ord);
+    const bool sort_cell = ((bad >> (lane & ~7)) & 0xffull) != 0ull;
+    if (sort_cell) {
+      for (int a = lo + q; a < hi; a += 8) {
+        const int32_t key = sidx[a];
+        int rk = 0;
+        for (int bb = lo; bb < hi; bb++) rk += (sidx[bb] < key) ? 1 : 0;
+        sperm[lo + rk] = (int16_t)a;
+      }
+    } else {
+      for (int a = lo + q; a < hi; a += 8) sperm[a] = (int16_t)a;
+    }
+    __syncthreads();
+#pragma unroll 2
+    for (int a = lo; a < hi; a++) {
+      const int r = sperm[a];
+      const float4 gr = sgeo[r];
+      const float4 v = sva[r];
+      const float fx = bx ? gr.x : 1 - gr.x;
+      const float fy = by ? gr.y : 1 - gr.y;
+      const float fz = bz ? gr.z : 1 - gr.z;
+      const float w = fz * fy * fx * gr.w;
+      float vals[5] = {v.x, v.y, v.z, v.w, 0.f};
+      if constexpr (MIX) vals[4] = svb[r];
+      if (!MIX || skeep[r]) {
+        acc[0] += w;
+#pragma unroll
+        for (int f = 0; f < NV; f++) acc[f + 1] += w * vals[f];
+        kept++;
+      } else {
+        res[0] += w;
+#pragma unroll
+        for (int f = 0; f < NV; f++) res[f + 1] += w * vals[f];
+      }
+    }
+    if (fits == false && cl == first) big = true;
+    __syncthreads();
+    first = last;
+  }
+  if (cl >= ntile) return;
+  const int64_t c = c0 + cl;
+  if (big) {
+    for (int32_t s = sstart[cl]; s < sstart[cl + 1]; s++) {
+      const int64_t pp = sorted[s];
+      const float ox = F[B_OX * cap + pp], oy = F[B_OY * cap + pp], oz = F[B_OZ * cap + pp];
+      const float vol = F[B_VOL * cap + pp];
+      float vals[5];
+      if constexpr (!MIX) {
+        const int kc = b.cell[pp] / (g.nx * g.ny);
+        const float z = zm.zbot + ((float)kc + oz) * zm.dz;
+        const float hl = F[B_H * cap + pp] - expf(-z);
+        vals[0] = F[B_P * cap + pp];
+        vals[1] = F[B_Q * cap + pp];
+        vals[2] = F[B_R * cap + pp];
+        vals[3] = F[B_B * cap + pp] + 12.5f * (hl > 0.f ? hl : 0.f);
+        vals[4] = 0.f;
+      } else {
+        vals[0] = F[B_B * cap + pp];
+        vals[1] = F[B_H * cap + pp];
+        vals[2] = F[B_P * cap + pp];
+        vals[3] = F[B_Q * cap + pp];
+        vals[4] = F[B_R * cap + pp];
+      }
+      const float fx = bx ? ox : 1 - ox;
+      const float fy = by ? oy : 1 - oy;
+      const float fz = bz ? oz : 1 - oz;
+      const float w = fz * fy * fx * vol;
+      if (!MIX || !(vol < vmin)) {
+        acc[0] += w;
+#pragma unroll
+        for (int f = 0; f < NV; f++) acc[f + 1] += w * vals[f];
+        kept++;
+      } else {
+        res[0] += w;
+#pragma unroll
+        for (int f = 0; f < NV; f++) res[f + 1] += w * vals[f];
+      }
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < NF; f++) partial[(int64_t)(q * NF + f) * ncell + c] = acc[f];
+  if constexpr (MIX) {
+#pragma unroll
+    for (int f = 0; f < NF; f++) partial2[(int64_t)(q * NF + f) * ncell + c] = res[f];
+    if (q == 0) npercell[c] = kept;
+  }
+}
+
+// node sums of the FP32 corner partials (the plain form's k_node_sum): NORM divides by the
+// summed weight (par2grid) into FP64 grids; otherwise raw FP32 sums, doubled on the z
+// boundaries when DBL (mixing kept grids)
+template <typename T>
+struct Out32 {
+  T *p[6];
+};
+
+template <int NV, bool NORM, bool DBL, typename T>
+__global__ void k32_node_sum(Geometry g, const float *__restrict__ partial, int64_t ncell,
+                             Out32<T> out) {
+  const int64_t nnode = (int64_t)g.nzn * g.ny * g.nx;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nnode) return;
+  const int i = (int)(t % g.nx);
+  const int j = (int)((t / g.nx) % g.ny);
+  const int k = (int)(t / ((int64_t)g.nx * g.ny));
+  float s[NV + 1];
+#pragma unroll
+  for (int f = 0; f <= NV; f++) s[f] = 0.f;
+#pragma unroll
+  for (int bx = 0; bx < 2; bx++) {
+    int ci = i - bx;
+    if (ci < 0) ci += g.nx;
+#pragma unroll
+    for (int by = 0; by < 2; by++) {
+      int cj = j - by;
+      if (cj < 0) cj += g.ny;
+#pragma unroll
+      for (int bz = 0; bz < 2; bz++) {
+        const int ck = k - bz;
+        if (ck < 0 || ck >= g.ncz) continue;
+        const int64_t c = ((int64_t)ck * g.ny + cj) * g.nx + ci;
+        const int q = (bz << 2) | (by << 1) | bx;
+#pragma unroll
+        for (int f = 0; f <= NV; f++) s[f] += partial[(int64_t)(q * (NV + 1) + f) * ncell + c];
+      }
+    }
+  }
+  if (NORM) {
+    float w = s[0];
+    if (w == 0.f) w = 1.f;
+#pragma unroll
+    for (int f = 1; f <= NV; f++) out.p[f - 1][t] = (T)(s[f] / w);
+  } else {
+    const float m = (DBL && (k == 0 || k == g.nzn - 1)) ? 2.f : 1.f;
+#pragma unroll
+    for (int f = 0; f <= NV; f++) out.p[f][t] = (T)(s[f] * m);
+  }
+}
+
+__global__ void k32_grid_from_f64(const double *__restrict__ a, float *__restrict__ b, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) b[t] = (float)a[t];
+}
+
+// ---- RK4 stage: gather the six tendencies, update, place (wrap x, y; reflect z) ----
+struct Grid32Set {
+  const float *g[6];
+};
+
+struct Scale32 {
+  float idx, idy, idz;  // cells per unit length
+};
+
+template <int STAGE>
+__global__ void __launch_bounds__(TPB) k32_rk_stage(Geometry g, Bulk32 b, int64_t cap, int64_t n,
+                                                    Grid32Set gs, Scale32 sc3, int stretch,
+                                                    DevScalars *sc) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  float *F = b.f;
+  const int32_t cell = b.cell[p];
+  const float ox = F[B_OX * cap + p], oy = F[B_OY * cap + p], oz = F[B_OZ * cap + p];
+  int32_t cello;
+  float oo[3], df[3] = {0.f, 0.f, 0.f}, vo[3] = {0.f, 0.f, 0.f}, vf[3] = {0.f, 0.f, 0.f};
+  if (STAGE == 1) {
+    cello = cell;
+    oo[0] = ox;
+    oo[1] = oy;
+    oo[2] = oz;
+#pragma unroll
+    for (int a = 0; a < 3; a++) vo[a] = F[(B_P + a) * cap + p];
+  } else {
+    cello = b.cello[p];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      oo[a] = F[(B_OXO + a) * cap + p];
+      df[a] = F[(B_DFX + a) * cap + p];
+      if (STAGE != 4) vo[a] = F[(B_PO + a) * cap + p];
+      vf[a] = F[(B_PF + a) * cap + p];
+    }
+  }
+  const float s_old = (STAGE == 4 && stretch) ? F[B_STRETCH * cap + p] : 0.f;
+  int i, j, k;
+  cell_ijk(g, cell, i, j, k);
+  int32_t o[8];
+  corner_nodes(g, i, j, k, o);
+  float kv[6];
+#pragma unroll
+  for (int f = 0; f < 6; f++) kv[f] = interp(gs.g[f], o, ox, oy, oz);
+  const double dtd = sc->dtm;
+  const float dt = (float)dtd, dt6 = (float)(dtd / 6), dt2 = (float)(dtd / 2),
+              dt3 = (float)(dtd / 3);
+  const float kp[3] = {kv[0] * sc3.idx, kv[1] * sc3.idy, kv[2] * sc3.idz};
+  float tpos[3], dfn[3], vn[3], vfn[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    if (STAGE == 1) {
+      dfn[a] = dt6 * kp[a];
+      tpos[a] = oo[a] + dt2 * kp[a];
+      vfn[a] = vo[a] + dt6 * kv[3 + a];
+      vn[a] = vo[a] + dt2 * kv[3 + a];
+    } else if (STAGE == 2) {
+      dfn[a] = df[a] + dt3 * kp[a];
+      tpos[a] = oo[a] + dt2 * kp[a];
+      vfn[a] = vf[a] + dt3 * kv[3 + a];
+      vn[a] = vo[a] + dt2 * kv[3 + a];
+    } else if (STAGE == 3) {
+      dfn[a] = df[a] + dt3 * kp[a];
+      tpos[a] = oo[a] + dt * kp[a];
+      vfn[a] = vf[a] + dt3 * kv[3 + a];
+      vn[a] = vo[a] + dt * kv[3 + a];
+    } else {
+      dfn[a] = df[a] + dt6 * kp[a];
+      tpos[a] = oo[a] + dfn[a];
+      vfn[a] = vf[a] + dt6 * kv[3 + a];
+      vn[a] = vfn[a];
+    }
+  }
+  int io, jo, ko;
+  cell_ijk(g, cello, io, jo, ko);
+  int32_t ncell_;
+  float nox, noy, noz;
+  if (!place(g, io, jo, ko, tpos[0], tpos[1], tpos[2], ncell_, nox, noy, noz)) {
+    raise_status(sc, PMPIC_ERR_OUT_OF_BOX_Z);
+    return;
+  }
+  b.cell[p] = ncell_;
+  F[B_OX * cap + p] = nox;
+  F[B_OY * cap + p] = noy;
+  F[B_OZ * cap + p] = noz;
+#pragma unroll
+  for (int a = 0; a < 3; a++) F[(B_P + a) * cap + p] = vn[a];
+  if (STAGE == 1) {
+    b.cello[p] = cello;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      F[(B_OXO + a) * cap + p] = oo[a];
+      F[(B_PO + a) * cap + p] = vo[a];
+    }
+  }
+  if (STAGE != 4) {
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      F[(B_DFX + a) * cap + p] = dfn[a];
+      F[(B_PF + a) * cap + p] = vfn[a];
+    }
+  } else {
+#pragma unroll
+    for (int f = 0; f < 6; f++) F[(B_DXDT + f) * cap + p] = kv[f];
+    if (stretch) {
+      const float third = 1.f / 3.f;
+      float ds = vn[0] * kv[3] + vn[1] * kv[4] + vn[2] * kv[5];
+      ds = powf(fabsf(ds), third);
+      F[B_STRETCH * cap + p] = s_old + ds * dt;
+    }
+  }
+}
+
+// ---- splitting (parcel_splitting.F90:76-131, then the haloswap's wrap and reflection) ----
+__global__ void k32_split_flags(const float *__restrict__ F, int64_t cap, int64_t n,
+                                uint8_t *__restrict__ flag) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  flag[p] = (F[B_STRETCH * cap + p] > 4.f) ? 1 : 0;
+}
+
+__global__ void k32_split_apply(Geometry g, Bulk32 b, int64_t cap, int64_t n, Scale32 sc3,
+                                const int32_t *__restrict__ parents, int64_t nsplit,
+                                DevScalars *sc) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nsplit) return;
+  const int64_t i = parents[m];
+  const int64_t c = n + m;
+  if (c >= cap) {
+    raise_status(sc, PMPIC_ERR_MAXPARCELS);
+    return;
+  }
+  float *F = b.f;
+  const float third = 1.f / 3.f;
+  const float pi = 3.14159265358979f;
+  const float p = F[B_P * cap + i], q = F[B_Q * cap + i], r = F[B_R * cap + i];
+  const float vol = F[B_VOL * cap + i];
+  const float rr = powf(vol / 4.f / pi, third);
+  const float absw = sqrtf(p * p + q * q + r * r);
+  const float d[3] = {rr * p / absw * sc3.idx, rr * q / absw * sc3.idy, rr * r / absw * sc3.idz};
+  int ci, cj, ck;
+  cell_ijk(g, b.cell[i], ci, cj, ck);
+  const float ox = F[B_OX * cap + i], oy = F[B_OY * cap + i], oz = F[B_OZ * cap + i];
+  int32_t cc, cp;
+  float cx, cy, cz, px, py, pz;
+  const bool okc = place(g, ci, cj, ck, ox + d[0], oy + d[1], oz + d[2], cc, cx, cy, cz);
+  const bool okp = place(g, ci, cj, ck, ox - d[0], oy - d[1], oz - d[2], cp, px, py, pz);
+  if (!okc || !okp) {
+    raise_status(sc, PMPIC_ERR_OUT_OF_BOX_Z);
+    return;
+  }
+  b.cell[c] = cc;
+  F[B_OX * cap + c] = cx;
+  F[B_OY * cap + c] = cy;
+  F[B_OZ * cap + c] = cz;
+  F[B_P * cap + c] = p;
+  F[B_Q * cap + c] = q;
+  F[B_R * cap + c] = r;
+  for (int f = B_DXDT; f <= B_DRDT; f++) F[(int64_t)f * cap + c] = F[(int64_t)f * cap + i];
+  F[B_H * cap + c] = F[B_H * cap + i];
+  F[B_B * cap + c] = F[B_B * cap + i];
+  F[B_VOL * cap + c] = vol / 2;
+  F[B_STRETCH * cap + c] = 0.f;
+  b.tag[c] = b.tag[i];
+  b.cell[i] = cp;
+  F[B_OX * cap + i] = px;
+  F[B_OY * cap + i] = py;
+  F[B_OZ * cap + i] = pz;
+  F[B_VOL * cap + i] = vol / 2;
+  F[B_STRETCH * cap + i] = 0.f;
+}
+
+// ---- mixing (parcel_mixing.F90:157-389) ----
+__global__ void k32_keep_flags(const float *__restrict__ F, int64_t cap, int64_t n, float vmin,
+                               uint8_t *__restrict__ keep) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  keep[p] = (F[B_VOL * cap + p] < vmin) ? 0 : 1;
+}
+
+__global__ void k32_newcell_flags(Geometry g, const int32_t *__restrict__ npercell,
+                                  uint8_t *__restrict__ cellflag, int64_t ncell) {
+  const int64_t ord = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ord >= ncell) return;
+  const int k = (int)(ord % g.ncz);
+  const int j = (int)((ord / g.ncz) % g.ny);
+  const int i = (int)(ord / ((int64_t)g.ncz * g.ny));
+  const int64_t c = ((int64_t)k * g.ny + j) * g.nx + i;
+  cellflag[ord] = (npercell[c] < 3) ? 1 : 0;
+}
+
+struct Mix32 {
+  const float *G[6];  // kept vol, b, h, p, q, r (folded, doubled)
+  float *R[6];        // residual vol .. r
+};
+
+__global__ void k32_mix_new(Geometry g, Bulk32 b, int64_t cap, int64_t norig,
+                            const int32_t *__restrict__ newcells, int64_t nadd,
+                            const int32_t *__restrict__ zlist, int64_t nz_,
+                            uint8_t *__restrict__ keep, Mix32 mg, float vmin,
+                            float *__restrict__ portion_cell, DevScalars *sc) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nadd) return;
+  const int64_t ord = newcells[m];
+  const int k = (int)(ord % g.ncz);
+  const int j = (int)((ord / g.ncz) % g.ny);
+  const int i = (int)(ord / ((int64_t)g.ncz * g.ny));
+  const int64_t s = (m < nz_) ? (int64_t)zlist[m] : norig + (m - nz_);
+  if (s + 2 > cap) {
+    raise_status(sc, PMPIC_ERR_MAXPARCELS);
+    return;
+  }
+  const int i1 = (i + 1 == g.nx) ? 0 : i + 1;
+  const int j1 = (j + 1 == g.ny) ? 0 : j + 1;
+  // 2x2x2 block in Fortran array-element order (z fastest, then y, then x)
+  const int64_t q[8] = {gidx(g, k, j, i),      gidx(g, k + 1, j, i),  gidx(g, k, j1, i),
+                        gidx(g, k + 1, j1, i), gidx(g, k, j, i1),     gidx(g, k + 1, j, i1),
+                        gidx(g, k, j1, i1),    gidx(g, k + 1, j1, i1)};
+  float s6[6];
+#pragma unroll
+  for (int f = 0; f < 6; f++) {
+    float a = 0.f;
+#pragma unroll
+    for (int c = 0; c < 8; c++) a += mg.G[f][q[c]];
+    s6[f] = a;
+  }
+  const float v = s6[0];
+  float *F = b.f;
+  b.cell[s] = (k * g.ny + j) * g.nx + i;
+  F[B_OX * cap + s] = 0.5f;  // the cell centre
+  F[B_OY * cap + s] = 0.5f;
+  F[B_OZ * cap + s] = 0.5f;
+  F[B_VOL * cap + s] = 2 * vmin;
+  F[B_STRETCH * cap + s] = 0.f;
+  F[B_B * cap + s] = s6[1] / v;
+  F[B_H * cap + s] = s6[2] / v;
+  F[B_P * cap + s] = s6[3] / v;
+  F[B_Q * cap + s] = s6[4] / v;
+  F[B_R * cap + s] = s6[5] / v;
+  if (s + 1 < norig) keep[s] = 2;
+  portion_cell[((int64_t)k * g.ny + j) * g.nx + i] = 2 * vmin / v;
+}
+
+__global__ void k32_mix_residual(Geometry g, Mix32 mg, const uint8_t *__restrict__ cellflag,
+                                 const float *__restrict__ portion_cell) {
+  const int64_t nnode = (int64_t)g.nzn * g.ny * g.nx;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nnode) return;
+  const int i = (int)(t % g.nx);
+  const int j = (int)((t / g.nx) % g.ny);
+  const int k = (int)(t / ((int64_t)g.nx * g.ny));
+  int ci[2], cj[2];
+  if (i == 0) { ci[0] = 0; ci[1] = g.nx - 1; } else { ci[0] = i - 1; ci[1] = i; }
+  if (j == 0) { cj[0] = 0; cj[1] = g.ny - 1; } else { cj[0] = j - 1; cj[1] = j; }
+  float r[6], gv[6];
+#pragma unroll
+  for (int f = 0; f < 6; f++) { r[f] = mg.R[f][t]; gv[f] = mg.G[f][t]; }
+  for (int a = 0; a < 2; a++)
+    for (int bb = 0; bb < 2; bb++)
+      for (int kc = k - 1; kc <= k; kc++) {
+        if (kc < 0 || kc >= g.ncz) continue;
+        const int64_t ord = ((int64_t)ci[a] * g.ny + cj[bb]) * g.ncz + kc;
+        if (!cellflag[ord]) continue;
+        const float portion = portion_cell[((int64_t)kc * g.ny + cj[bb]) * g.nx + ci[a]];
+#pragma unroll
+        for (int f = 0; f < 6; f++) r[f] = r[f] - portion * gv[f];
+      }
+  const float mlt = (k == 0 || k == g.nzn - 1) ? 2.f : 1.f;
+#pragma unroll
+  for (int f = 0; f < 6; f++) mg.R[f][t] = r[f] * mlt;
+}
+
+__global__ void k32_mix_absorb(Geometry g, Bulk32 b, int64_t cap, int64_t norig,
+                               const uint8_t *__restrict__ keep, Mix32 mg) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= norig || keep[p] != 1) return;
+  float *F = b.f;
+  const int fld[5] = {B_B, B_H, B_P, B_Q, B_R};
+  const int32_t cell = b.cell[p];
+  const float ox = F[B_OX * cap + p], oy = F[B_OY * cap + p], oz = F[B_OZ * cap + p];
+  const float v = F[B_VOL * cap + p];
+  float fv[5];
+#pragma unroll
+  for (int f = 0; f < 5; f++) fv[f] = F[(int64_t)fld[f] * cap + p];
+  int i, j, k;
+  cell_ijk(g, cell, i, j, k);
+  int32_t o[8];
+  corner_nodes(g, i, j, k, o);
+  float w[8], g0[8];
+  w[0] = (1 - oz) * (1 - oy) * (1 - ox) * v;
+  w[1] = (1 - oz) * (1 - oy) * (ox) * v;
+  w[2] = (1 - oz) * (oy) * (1 - ox) * v;
+  w[3] = (1 - oz) * (oy) * (ox) * v;
+  w[4] = (oz) * (1 - oy) * (1 - ox) * v;
+  w[5] = (oz) * (1 - oy) * (ox) * v;
+  w[6] = (oz) * (oy) * (1 - ox) * v;
+  w[7] = (oz) * (oy) * (ox) * v;
+#pragma unroll
+  for (int c = 0; c < 8; c++) g0[c] = mg.G[0][o[c]];
+  float vn = v;
+#pragma unroll
+  for (int c = 0; c < 8; c++) vn = vn + w[c] * mg.R[0][o[c]] / g0[c];
+  float out[5];
+#pragma unroll
+  for (int f = 0; f < 5; f++) {
+    float acc = fv[f] * v;
+#pragma unroll
+    for (int c = 0; c < 8; c++) acc = acc + w[c] / g0[c] * mg.R[f + 1][o[c]];
+    out[f] = acc / vn;
+  }
+  F[B_VOL * cap + p] = vn;
+#pragma unroll
+  for (int f = 0; f < 5; f++) F[(int64_t)fld[f] * cap + p] = out[f];
+}
+
+// backfill: the state fields of the j-th kept slot from the end into the j-th removed slot
+// (the reference copies x, y, z, p, q, r, b, h, vol, tag and stretch)
+__global__ void k32_mix_backfill(Bulk32 b, int64_t cap, const int32_t *__restrict__ dst, int64_t R,
+                                 const int32_t *__restrict__ k1, int64_t K1) {
+  const int64_t jx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (jx >= R) return;
+  const int64_t s = k1[K1 - 1 - jx], d = dst[jx];
+  float *F = b.f;
+  const int fl[10] = {B_OX, B_OY, B_OZ, B_P, B_Q, B_R, B_B, B_H, B_VOL, B_STRETCH};
+#pragma unroll
+  for (int c = 0; c < 10; c++) F[(int64_t)fl[c] * cap + d] = F[(int64_t)fl[c] * cap + s];
+  b.cell[d] = b.cell[s];
+  b.tag[d] = b.tag[s];
+}
+
+Zmap zmap_of(const Geometry &g) { return Zmap{(float)g.zbot, (float)g.dz}; }
+Scale32 scale_of(const Geometry &g) {
+  return Scale32{(float)(1.0 / g.dx), (float)(1.0 / g.dy), (float)(1.0 / g.dz)};
+}
+
+template <typename T>
+int dalloc32(T **p, int64_t n) {
+  *p = nullptr;
+  if (n <= 0) n = 1;
+  if (hipMalloc((void **)p, sizeof(T) * (size_t)n) != hipSuccess) return PMPIC_ERR_NOMEM;
+  return PMPIC_OK;
+}
+
+int bin32(Ctx *c, int64_t n) {
+  const int64_t nc1 = c->ncell + 1;
+  hipLaunchKernelGGL(k32_fill_i32, dim3(cdiv(nc1, TPB)), dim3(TPB), 0, c->stream, c->cell_count,
+                     nc1, 0);
+  if (n > 0)
+    hipLaunchKernelGGL(k32_bin_count, dim3(cdiv(n, TPB)), dim3(TPB), 0, c->stream, c->b32.cell, n,
+                       c->rank, c->cell_count);
+  PMPIC_TRY(scan_exclusive_i32(c, c->cell_count, c->cell_start, nc1));
+  if (n > 0)
+    hipLaunchKernelGGL(k32_bin_scatter, dim3(cdiv(n, TPB)), dim3(TPB), 0, c->stream, c->b32.cell,
+                       c->rank, c->cell_start, n, c->sorted);
+  hipLaunchKernelGGL(k32_sort_big, dim3(cdiv(c->ncell, TPB)), dim3(TPB), 0, c->stream,
+                     c->cell_start, c->ncell, ROWS32, c->sorted);
+  PMPIC_CHECK(hipGetLastError());
+  return PMPIC_OK;
+}
+
+// FP32 corner partials live in the plain form's partial buffers (48 doubles per cell hold
+// 96 floats)
+float *partial32(Ctx *c) { return reinterpret_cast<float *>(c->partial); }
+float *partial32b(Ctx *c) { return reinterpret_cast<float *>(c->partial2); }
+
+}  // namespace
+
+int b32_alloc(Ctx *c) {
+  Bulk32 &b = c->b32;
+  PMPIC_TRY(dalloc32(&b.f, (int64_t)NF32 * c->cap));
+  PMPIC_TRY(dalloc32(&b.cell, c->cap));
+  PMPIC_TRY(dalloc32(&b.cello, c->cap));
+  PMPIC_TRY(dalloc32(&b.tag, c->cap));
+  for (int i = 0; i < 6; i++) PMPIC_TRY(dalloc32(&b.grid[i], c->gsz));
+  for (int i = 0; i < 12; i++) PMPIC_TRY(dalloc32(&b.mgrid[i], c->gsz));
+  return PMPIC_OK;
+}
+
+void b32_free(Ctx *c) {
+  Bulk32 &b = c->b32;
+  hipFree(b.f);
+  hipFree(b.cell);
+  hipFree(b.cello);
+  hipFree(b.tag);
+  for (int i = 0; i < 6; i++) hipFree(b.grid[i]);
+  for (int i = 0; i < 12; i++) hipFree(b.mgrid[i]);
+}
+
+int b32_set_parcels(Ctx *c, int64_t n, const double *const *fields) {
+  Bulk32 &b = c->b32;
+  const dim3 gb(cdiv(n > 0 ? n : 1, TPB)), tb(TPB);
+  for (int a = 0; a < 3; a++) {
+    if (n > 0) {
+      PMPIC_CHECK(hipMemcpyAsync(c->fbuf, fields[a], sizeof(double) * n, hipMemcpyHostToDevice,
+                                 c->stream));
+      hipLaunchKernelGGL(k32_pos_in, gb, tb, 0, c->stream, c->g, a, c->fbuf, n, b.cell,
+                         b.f + (int64_t)(B_OX + a) * c->cap, c->dsc);
+    }
+  }
+  for (int f = FP; f <= FSTRETCH; f++) {
+    if (n > 0) {
+      PMPIC_CHECK(hipMemcpyAsync(c->fbuf, fields[f], sizeof(double) * n, hipMemcpyHostToDevice,
+                                 c->stream));
+      hipLaunchKernelGGL(k32_from_f64, gb, tb, 0, c->stream, c->fbuf, n,
+                         b.f + (int64_t)f * c->cap);
+    }
+  }
+  if (n > 0)
+    PMPIC_CHECK(hipMemcpyAsync(b.tag, fields[FTAG], sizeof(double) * n, hipMemcpyHostToDevice,
+                               c->stream));
+  PMPIC_CHECK(hipGetLastError());
+  PMPIC_CHECK(hipStreamSynchronize(c->stream));
+  return PMPIC_OK;
+}
+
+int b32_get_parcels(Ctx *c, double *const *fields) {
+  Bulk32 &b = c->b32;
+  const int64_t n = c->n;
+  if (n == 0) return PMPIC_OK;
+  const dim3 gb(cdiv(n, TPB)), tb(TPB);
+  for (int f = 0; f < 17; f++) {
+    if (!fields[f]) continue;
+    if (f == FTAG) {
+      PMPIC_CHECK(hipMemcpyAsync(fields[f], b.tag, sizeof(double) * n, hipMemcpyDeviceToHost,
+                                 c->stream));
+      continue;
+    }
+    if (f < 3)
+      hipLaunchKernelGGL(k32_pos_out, gb, tb, 0, c->stream, c->g, f, b.cell,
+                         b.f + (int64_t)(B_OX + f) * c->cap, n, c->fbuf);
+    else
+      hipLaunchKernelGGL(k32_to_f64, gb, tb, 0, c->stream, b.f + (int64_t)f * c->cap, n, c->fbuf);
+    PMPIC_CHECK(hipMemcpyAsync(fields[f], c->fbuf, sizeof(double) * n, hipMemcpyDeviceToHost,
+                               c->stream));
+  }
+  PMPIC_CHECK(hipGetLastError());
+  PMPIC_CHECK(hipStreamSynchronize(c->stream));
+  return PMPIC_OK;
+}
+
+int b32_par2grid(Ctx *c) {
+  const int64_t n = c->n;
+  PMPIC_TRY(bin32(c, n));
+  PMPIC_TRY(timing_begin(c, 0));
+  hipLaunchKernelGGL(k32_cell_accum<A32_P2G>, dim3(cdiv(c->ncell, TILE32)), dim3(TILE32 * 8), 0,
+                     c->stream, c->b32, c->cap, c->g, zmap_of(c->g), (float)c->g.vmin,
+                     c->cell_start, c->sorted, c->ncell, partial32(c), nullptr, nullptr);
+  PMPIC_TRY(timing_end(c, 0));
+  Out32<double> out = {{c->grid[GP], c->grid[GQ], c->grid[GR], c->grid[GB], nullptr, nullptr}};
+  const int64_t nnode = (int64_t)c->g.nzn * c->g.ny * c->g.nx;
+  hipLaunchKernelGGL((k32_node_sum<4, true, true, double>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0,
+                     c->stream, c->g, partial32(c), c->ncell, out);
+  PMPIC_CHECK(hipGetLastError());
+  return PMPIC_OK;
+}
+
+int b32_grids_from(Ctx *c, int first, int count) {
+  const GridId src[6] = {GU, GV, GW, GDP, GDQ, GDR};
+  for (int f = first; f < first + count; f++)
+    hipLaunchKernelGGL(k32_grid_from_f64, dim3(cdiv(c->gsz, TPB)), dim3(TPB), 0, c->stream,
+                       c->grid[src[f]], c->b32.grid[f], c->gsz);
+  PMPIC_CHECK(hipGetLastError());
+  return PMPIC_OK;
+}
+
+int b32_rk_stage(Ctx *c, int stage, int stretch) {
+  const int64_t n = c->n;
+  if (n == 0) return PMPIC_OK;
+  PMPIC_TRY(timing_begin(c, 1));
+  Grid32Set gs = {{c->b32.grid[0], c->b32.grid[1], c->b32.grid[2], c->b32.grid[3],
+                   c->b32.grid[4], c->b32.grid[5]}};
+  const Scale32 s3 = scale_of(c->g);
+  const dim3 gb(cdiv(n, TPB)), tb(TPB);
+  switch (stage) {
+    case 1:
+      hipLaunchKernelGGL(k32_rk_stage<1>, gb, tb, 0, c->stream, c->g, c->b32, c->cap, n, gs, s3,
+                         stretch, c->dsc);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k32_rk_stage<2>, gb, tb, 0, c->stream, c->g, c->b32, c->cap, n, gs, s3,
+                         stretch, c->dsc);
+      break;
+    case 3:
+      hipLaunchKernelGGL(k32_rk_stage<3>, gb, tb, 0, c->stream, c->g, c->b32, c->cap, n, gs, s3,
+                         stretch, c->dsc);
+      break;
+    default:
+      hipLaunchKernelGGL(k32_rk_stage<4>, gb, tb, 0, c->stream, c->g, c->b32, c->cap, n, gs, s3,
+                         stretch, c->dsc);
+      break;
+  }
+  PMPIC_CHECK(hipGetLastError());
+  PMPIC_TRY(timing_end(c, 1));
+  return PMPIC_OK;
+}
+
+int b32_split(Ctx *c) {
+  const int64_t n = c->n;
+  if (n > 0) {
+    hipLaunchKernelGGL(k32_split_flags, dim3(cdiv(n, TPB)), dim3(TPB), 0, c->stream, c->b32.f,
+                       c->cap, n, c->flag);
+    PMPIC_TRY(compact_flags(c, c->flag, 1, n, c->list_b, &c->dsc->nsplit));
+  } else {
+    c->hsc.nsplit = 0;
+    PMPIC_CHECK(hipMemcpyAsync(&c->dsc->nsplit, &c->hsc.nsplit, sizeof(int64_t),
+                               hipMemcpyHostToDevice, c->stream));
+  }
+  PMPIC_TRY(sync_scalars(c));
+  const int64_t ns = c->hsc.nsplit;
+  if (ns > 0) {
+    if (n + ns > c->cap) return PMPIC_ERR_MAXPARCELS;
+    hipLaunchKernelGGL(k32_split_apply, dim3(cdiv(ns, TPB)), dim3(TPB), 0, c->stream, c->g,
+                       c->b32, c->cap, n, scale_of(c->g), c->list_b, ns, c->dsc);
+    PMPIC_CHECK(hipGetLastError());
+  }
+  c->n = n + ns;
+  return PMPIC_OK;
+}
+
+int b32_mix(Ctx *c) {
+  const Geometry &g = c->g;
+  Bulk32 &b = c->b32;
+  const int64_t norig = c->n;
+  const float vmin = (float)g.vmin;
+  // kept and residual grids (construct_grids, parcel_mixing.F90:440-624)
+  PMPIC_TRY(bin32(c, norig));
+  hipLaunchKernelGGL(k32_cell_accum<A32_MIX>, dim3(cdiv(c->ncell, TILE32)), dim3(TILE32 * 8), 0,
+                     c->stream, b, c->cap, g, zmap_of(g), vmin, c->cell_start, c->sorted,
+                     c->ncell, partial32(c), partial32b(c), c->npercell);
+  const int64_t nnode = (int64_t)g.nzn * g.ny * g.nx;
+  Out32<float> kept = {{b.mgrid[0], b.mgrid[1], b.mgrid[2], b.mgrid[3], b.mgrid[4], b.mgrid[5]}};
+  Out32<float> res = {{b.mgrid[6], b.mgrid[7], b.mgrid[8], b.mgrid[9], b.mgrid[10], b.mgrid[11]}};
+  hipLaunchKernelGGL((k32_node_sum<5, false, true, float>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0,
+                     c->stream, g, partial32(c), c->ncell, kept);
+  hipLaunchKernelGGL((k32_node_sum<5, false, false, float>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0,
+                     c->stream, g, partial32b(c), c->ncell, res);
+  if (norig > 0)
+    hipLaunchKernelGGL(k32_keep_flags, dim3(cdiv(norig, TPB)), dim3(TPB), 0, c->stream, b.f,
+                       c->cap, norig, vmin, c->flag);
+  PMPIC_TRY(compact_flags(c, c->flag, 0, norig, c->list_a, &c->dsc->nzero));
+  hipLaunchKernelGGL(k32_newcell_flags, dim3(cdiv(c->ncell, TPB)), dim3(TPB), 0, c->stream, g,
+                     c->npercell, c->cellflag, c->ncell);
+  PMPIC_TRY(compact_flags(c, c->cellflag, 1, c->ncell, c->sorted, &c->dsc->nadd));
+  PMPIC_TRY(sync_scalars(c));
+  const int64_t Z = c->hsc.nzero, nadd = c->hsc.nadd;
+  Mix32 mg = {{b.mgrid[0], b.mgrid[1], b.mgrid[2], b.mgrid[3], b.mgrid[4], b.mgrid[5]},
+              {b.mgrid[6], b.mgrid[7], b.mgrid[8], b.mgrid[9], b.mgrid[10], b.mgrid[11]}};
+  float *portion = partial32(c);  // the kept partials are consumed by now
+  if (nadd > 0)
+    hipLaunchKernelGGL(k32_mix_new, dim3(cdiv(nadd, TPB)), dim3(TPB), 0, c->stream, g, b, c->cap,
+                       norig, c->sorted, nadd, c->list_a, Z, c->flag, mg, vmin, portion, c->dsc);
+  hipLaunchKernelGGL(k32_mix_residual, dim3(cdiv(nnode, TPB)), dim3(TPB), 0, c->stream, g, mg,
+                     c->cellflag, portion);
+  if (norig > 0)
+    hipLaunchKernelGGL(k32_mix_absorb, dim3(cdiv(norig, TPB)), dim3(TPB), 0, c->stream, g, b,
+                       c->cap, norig, c->flag, mg);
+  PMPIC_CHECK(hipGetLastError());
+  // new count (parcel_mixing.F90:217), then the backfill of removed slots (:347-389)
+  int64_t np = norig;
+  if (nadd > 0) {
+    int64_t slast;
+    if (nadd - 1 < Z) {
+      int32_t v;
+      PMPIC_CHECK(hipMemcpyAsync(&v, c->list_a + (nadd - 1), sizeof(int32_t),
+                                 hipMemcpyDeviceToHost, c->stream));
+      PMPIC_CHECK(hipStreamSynchronize(c->stream));
+      slast = v;
+    } else {
+      slast = norig + (nadd - 1 - Z);
+    }
+    if (slast + 1 > np) np = slast + 1;
+  }
+  PMPIC_TRY(sync_scalars(c));
+  c->last_removed = Z;
+  c->last_added = nadd;
+  if (np == norig) {
+    int64_t R = 0;
+    const int32_t *dst = nullptr;
+    if (nadd < Z) {
+      R = Z - nadd;
+      dst = c->list_a + nadd;
+    } else if (nadd == Z && Z > 0) {
+      int32_t v;
+      PMPIC_CHECK(hipMemcpyAsync(&v, c->list_a + (Z - 1), sizeof(int32_t), hipMemcpyDeviceToHost,
+                                 c->stream));
+      PMPIC_CHECK(hipStreamSynchronize(c->stream));
+      if (v == norig - 1) {
+        R = 1;
+        dst = c->list_a + (Z - 1);
+      }
+    }
+    if (R > 0) {
+      PMPIC_TRY(compact_flags(c, c->flag, 1, norig, c->list_b, &c->dsc->nkeep1));
+      PMPIC_TRY(sync_scalars(c));
+      const int64_t K1 = c->hsc.nkeep1;
+      if (R > K1) return PMPIC_ERR_BACKFILL;
+      hipLaunchKernelGGL(k32_mix_backfill, dim3(cdiv(R, TPB)), dim3(TPB), 0, c->stream, b, c->cap,
+                         dst, R, c->list_b, K1);
+      PMPIC_CHECK(hipGetLastError());
+      np -= R;
+    }
+  }
+  c->n = np;
+  return PMPIC_OK;
+}
+
+int b32_conservation_sums(Ctx *c, double out6[6]) {
+  const int64_t n = c->n;
+  std::vector<float> v(n), f(n);
+  if (n > 0)
+    PMPIC_CHECK(hipMemcpy(v.data(), c->b32.f + (int64_t)B_VOL * c->cap, sizeof(float) * n,
+                          hipMemcpyDeviceToHost));
+  const int fl[6] = {-1, B_B, B_H, B_P, B_Q, B_R};
+  for (int k = 0; k < 6; k++) {
+    if (fl[k] >= 0 && n > 0)
+      PMPIC_CHECK(hipMemcpy(f.data(), c->b32.f + (int64_t)fl[k] * c->cap, sizeof(float) * n,
+                            hipMemcpyDeviceToHost));
+    double s = 0.0, comp = 0.0;  // Neumaier summation in parcel order, FP64
+    for (int64_t p = 0; p < n; p++) {
+      const double term = fl[k] < 0 ? (double)v[p] : (double)v[p] * (double)f[p];
+      const double t = s + term;
+      if (fabs(s) >= fabs(term)) comp += (s - t) + term;
+      else comp += (term - t) + s;
+      s = t;
+    }
+    out6[k] = s + comp;
+  }
+  return PMPIC_OK;
+}
+
+}  // namespace pmpic

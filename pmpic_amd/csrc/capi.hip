@@ -179,6 +179,7 @@ int set_dev_scalars(Ctx *c) {
 }
 
 int free_all(Ctx *c) {
+  b32_free(c);
   hipFree(c->par); hipFree(c->tmp);
   hipFree(c->cell); hipFree(c->rank); hipFree(c->sorted); hipFree(c->flag);
   hipFree(c->list_a); hipFree(c->list_b);
@@ -207,7 +208,7 @@ int free_all(Ctx *c) {
 
 int create(const pmpic_config *cf, Ctx *c) {
   if (!cf || cf->nx < 2 || cf->ny < 2 || cf->nz < 2 || cf->max_parcels < 1) return PMPIC_ERR_ARG;
-  if (cf->precision != 0) return PMPIC_ERR_ARG;
+  if (cf->precision != 0 && cf->precision != 1) return PMPIC_ERR_ARG;
   if (!check_factors(cf->nx) || !check_factors(cf->ny)) return PMPIC_ERR_FFT_SIZE;
   if (cf->max_parcels > (int64_t)INT32_MAX) return PMPIC_ERR_ARG;
   PMPIC_CHECK(hipSetDevice(cf->device));
@@ -232,8 +233,10 @@ int create(const pmpic_config *cf, Ctx *c) {
   c->csz = (int64_t)g.nzn * ((int64_t)g.ny * (g.nx / 2 + 1) > (int64_t)(g.ny / 2 + 1) * g.nx
                                   ? (int64_t)g.ny * (g.nx / 2 + 1)
                                   : (int64_t)(g.ny / 2 + 1) * g.nx);
+  if (c->gsz > (int64_t)INT32_MAX) return PMPIC_ERR_ARG;
   const int64_t big = c->cap > c->ncell ? c->cap : c->ncell;
-  PMPIC_TRY(dalloc(&c->par, (int64_t)NFIELDS * c->cap));
+  if (c->precision == 1) PMPIC_TRY(b32_alloc(c));
+  else PMPIC_TRY(dalloc(&c->par, (int64_t)NFIELDS * c->cap));
   PMPIC_TRY(dalloc(&c->cell, c->cap));
   PMPIC_TRY(dalloc(&c->rank, c->cap));
   PMPIC_TRY(dalloc(&c->sorted, big));
@@ -338,8 +341,14 @@ int pmpic_destroy(pmpic_ctx *ctx) {
 int pmpic_set_parcels(pmpic_ctx *ctx, int64_t n, const double *const *fields) {
   if (!ctx || !fields || n < 0 || n > ctx->c.cap) return PMPIC_ERR_ARG;
   Ctx *c = &ctx->c;
-  for (int f = 0; f < 17; f++) {
+  for (int f = 0; f < 17; f++)
     if (!fields[f]) return PMPIC_ERR_ARG;
+  if (c->precision == 1) {
+    PMPIC_TRY(b32_set_parcels(c, n, fields));
+    c->n = n;
+    return take_status(c);
+  }
+  for (int f = 0; f < 17; f++) {
     if (n > 0)
       PMPIC_CHECK(hipMemcpyAsync(c->par + (int64_t)f * c->cap, fields[f], sizeof(double) * n,
                                  hipMemcpyHostToDevice, c->stream));
@@ -353,6 +362,7 @@ int pmpic_get_parcels(pmpic_ctx *ctx, int64_t *n, double *const *fields) {
   if (!ctx) return PMPIC_ERR_ARG;
   Ctx *c = &ctx->c;
   if (n) *n = c->n;
+  if (fields && c->precision == 1) return b32_get_parcels(c, fields);
   if (fields)
     for (int f = 0; f < 17; f++)
       if (fields[f] && c->n > 0)
@@ -363,7 +373,7 @@ int pmpic_get_parcels(pmpic_ctx *ctx, int64_t *n, double *const *fields) {
 }
 
 int pmpic_set_rk_arrays(pmpic_ctx *ctx, const double *const *fields) {
-  if (!ctx || !fields) return PMPIC_ERR_ARG;
+  if (!ctx || !fields || ctx->c.precision != 0) return PMPIC_ERR_ARG;
   Ctx *c = &ctx->c;
   for (int f = 0; f < 12; f++) {
     if (!fields[f]) return PMPIC_ERR_ARG;
@@ -376,7 +386,7 @@ int pmpic_set_rk_arrays(pmpic_ctx *ctx, const double *const *fields) {
 }
 
 int pmpic_get_rk_arrays(pmpic_ctx *ctx, double *const *fields) {
-  if (!ctx || !fields) return PMPIC_ERR_ARG;
+  if (!ctx || !fields || ctx->c.precision != 0) return PMPIC_ERR_ARG;
   Ctx *c = &ctx->c;
   for (int f = 0; f < 12; f++)
     if (fields[f] && c->n > 0)
@@ -408,8 +418,14 @@ int pmpic_vort2vel(pmpic_ctx *ctx) {
   if (!ctx) return PMPIC_ERR_ARG;
   Ctx *c = &ctx->c;
   const int stage = stage_of(c->it_v2v);
-  PMPIC_TRY(par2grid_fused(c, c->n));
-  PMPIC_TRY(vort2vel_spectral(c, stage));
+  if (c->precision == 1) {
+    PMPIC_TRY(b32_par2grid(c));
+    PMPIC_TRY(vort2vel_spectral(c, stage));
+    PMPIC_TRY(b32_grids_from(c, 0, 3));
+  } else {
+    PMPIC_TRY(par2grid_fused(c, c->n));
+    PMPIC_TRY(vort2vel_spectral(c, stage));
+  }
   c->it_v2v++;
   return PMPIC_OK;
 }
@@ -419,6 +435,7 @@ int pmpic_vorticity_tendency(pmpic_ctx *ctx) {
   Ctx *c = &ctx->c;
   const int stage = stage_of(c->it_vt);
   PMPIC_TRY(vorticity_tendency_spectral(c, stage));
+  if (c->precision == 1) PMPIC_TRY(b32_grids_from(c, 3, 3));
   if (stage == 1) {
     PMPIC_TRY(grid_max_reduce(c));
   }
@@ -430,7 +447,8 @@ int pmpic_rk4_integrator(pmpic_ctx *ctx) {
   if (!ctx) return PMPIC_ERR_ARG;
   Ctx *c = &ctx->c;
   const int stage = c->rkstep;
-  PMPIC_TRY(rk_stage_update(c, stage, c->n, c->splitting_enabled));
+  if (c->precision == 1) PMPIC_TRY(b32_rk_stage(c, stage, c->splitting_enabled));
+  else PMPIC_TRY(rk_stage_update(c, stage, c->n, c->splitting_enabled));
   c->rkstep = (c->rkstep < 4) ? c->rkstep + 1 : 1;
   int st = take_status(c);
   c->dtm = c->hsc.dtm;
@@ -444,7 +462,7 @@ int pmpic_parcel_splitting(pmpic_ctx *ctx) {
   int st = PMPIC_OK;
   if (c->it_split % 4 == 3) {
     const int64_t n0 = c->n;
-    st = parcel_split(c);
+    st = (c->precision == 1) ? b32_split(c) : parcel_split(c);
     c->last_split = c->n - n0;
     if (st == PMPIC_OK) st = take_status(c);
   }
@@ -457,7 +475,7 @@ int pmpic_parcel_mixing(pmpic_ctx *ctx) {
   Ctx *c = &ctx->c;
   int st = PMPIC_OK;
   if (c->it_mix % 4 == 3) {
-    st = parcel_mix(c);
+    st = (c->precision == 1) ? b32_mix(c) : parcel_mix(c);
     if (st == PMPIC_OK) st = take_status(c);
   }
   c->it_mix++;
@@ -483,7 +501,7 @@ int pmpic_step(pmpic_ctx *ctx, int32_t nsteps) {
 }
 
 int pmpic_parcel_haloswap(pmpic_ctx *ctx) {
-  if (!ctx) return PMPIC_ERR_ARG;
+  if (!ctx || ctx->c.precision != 0) return PMPIC_ERR_ARG;
   Ctx *c = &ctx->c;
   PMPIC_TRY(parcel_haloswap(c, c->n, 1));
   return take_status(c);
@@ -491,7 +509,7 @@ int pmpic_parcel_haloswap(pmpic_ctx *ctx) {
 
 int pmpic_cache_parcel_interp_weights(pmpic_ctx *ctx, int32_t *is, int32_t *js, int32_t *ks,
                                       double *delxs, double *delys, double *delzs) {
-  if (!ctx) return PMPIC_ERR_ARG;
+  if (!ctx || ctx->c.precision != 0) return PMPIC_ERR_ARG;
   Ctx *c = &ctx->c;
   const Geometry &g = c->g;
   std::vector<double> x, y, z;
@@ -521,7 +539,7 @@ int pmpic_cache_parcel_interp_weights(pmpic_ctx *ctx, int32_t *is, int32_t *js, 
 }
 
 int pmpic_par2grid(pmpic_ctx *ctx, const double *var, double *grid) {
-  if (!ctx || !var || !grid) return PMPIC_ERR_ARG;
+  if (!ctx || !var || !grid || ctx->c.precision != 0) return PMPIC_ERR_ARG;
   Ctx *c = &ctx->c;
   if (c->n > 0)
     PMPIC_CHECK(hipMemcpyAsync(c->fbuf, var, sizeof(double) * c->n, hipMemcpyHostToDevice,
@@ -536,7 +554,7 @@ int pmpic_par2grid(pmpic_ctx *ctx, const double *var, double *grid) {
 }
 
 int pmpic_grid2par(pmpic_ctx *ctx, const double *grid, double *var) {
-  if (!ctx || !var || !grid) return PMPIC_ERR_ARG;
+  if (!ctx || !var || !grid || ctx->c.precision != 0) return PMPIC_ERR_ARG;
   Ctx *c = &ctx->c;
   std::vector<double> h;
   grid_to_device_layout(c->g, grid, h);
@@ -651,6 +669,7 @@ int pmpic_laplinv(pmpic_ctx *ctx, double *f, int32_t df0) {
 int pmpic_conservation_sums(pmpic_ctx *ctx, double out6[6]) {
   if (!ctx || !out6) return PMPIC_ERR_ARG;
   Ctx *c = &ctx->c;
+  if (c->precision == 1) return b32_conservation_sums(c, out6);
   std::vector<double> v, f;
   PMPIC_TRY(download(c, v, c->par + FVOL * c->cap, c->n));
   const int fl[6] = {-1, FB, FH, FP, FQ, FR};

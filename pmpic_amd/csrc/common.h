@@ -24,6 +24,25 @@ enum Field {
   FSTRETCH, FTAG, FXO, FYO, FZO, FXF, FYF, FZF, FPO, FQO, FRO, FPF, FQF, FRF, NFIELDS
 };
 
+// FP32-bulk form: per-parcel FP32 fields (the position is an int32 cell index plus the three
+// offsets B_OX..B_OZ in [0,1) inside that cell; B_OXO..B_OZO and the cell_o array hold the RK4
+// origin, B_DFX..B_DFZ the accumulated RK4 increment in cells relative to it).  Fields 3..15
+// share their numbers with Field.
+enum Field32 {
+  B_OX = 0, B_OY, B_OZ, B_P, B_Q, B_R, B_DXDT, B_DYDT, B_DZDT, B_DPDT, B_DQDT, B_DRDT, B_H, B_B,
+  B_VOL, B_STRETCH, B_OXO, B_OYO, B_OZO, B_DFX, B_DFY, B_DFZ, B_PO, B_QO, B_RO, B_PF, B_QF, B_RF,
+  NF32
+};
+
+struct Bulk32 {
+  float *f;          // NF32 * cap
+  int32_t *cell;     // cap: cell holding the parcel
+  int32_t *cello;    // cap: cell of the RK4 origin
+  double *tag;       // cap
+  float *grid[6];    // gathered grids U, V, W, DP, DQ, DR [nzn][ny][nx]
+  float *mgrid[12];  // mixing: kept vol, b, h, p, q, r (0..5), residuals (6..11)
+};
+
 // grid ids exported through the C-ABI (writebinary.F90:287-296 order, then the tendencies)
 enum GridId { GU = 0, GV, GW, GP, GQ, GR, GB, GHG, GHGLIQ, GVOL, GDP, GDQ, GDR, NGRIDS };
 
@@ -63,6 +82,7 @@ struct Ctx {
   int splitting_enabled, mixing_enabled;
   int64_t last_split, last_removed, last_added;
   hipStream_t stream;
+  Bulk32 b32;            // FP32-bulk form state (precision == 1)
   // parcels
   double *par;           // NFIELDS * cap
   double *tmp;           // NFIELDS * cap staging for permutations (allocated lazily)
@@ -204,6 +224,17 @@ int parcel_split(Ctx *c);
 int parcel_mix(Ctx *c);
 int compact_flags(Ctx *c, const uint8_t *flags, uint8_t want, int64_t n, int32_t *out,
                   int64_t *d_count);
+// bulk32.hip (FP32-bulk form)
+int b32_alloc(Ctx *c);
+void b32_free(Ctx *c);
+int b32_set_parcels(Ctx *c, int64_t n, const double *const *fields);
+int b32_get_parcels(Ctx *c, double *const *fields);
+int b32_par2grid(Ctx *c);                 // P, Q, R, B grids (FP64) from the FP32 parcels
+int b32_grids_from(Ctx *c, int first, int count);  // FP32 copies of grid U..W (0..2) / DP..DR (3..5)
+int b32_rk_stage(Ctx *c, int stage, int stretch);
+int b32_split(Ctx *c);
+int b32_mix(Ctx *c);
+int b32_conservation_sums(Ctx *c, double out6[6]);
 int scan_exclusive_i32(Ctx *c, const int32_t *in, int32_t *out, int64_t n);
 int sync_scalars(Ctx *c);
 

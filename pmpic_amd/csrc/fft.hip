@@ -31,7 +31,7 @@ struct Plan {
 };
 
 constexpr int FFT_TPB = 256;
-constexpr int LDS_CPLX = 2048;  // complex elements per LDS buffer (32 KiB in FP64)
+constexpr int LDS_CPLX = 1024;  // complex elements per LDS buffer (16 KiB in FP64)
 constexpr int MAXF = 8;
 
 struct FieldPtrs {
@@ -46,38 +46,99 @@ __device__ __forceinline__ double2 cadd(double2 a, double2 b) {
   return make_double2(a.x + b.x, a.y + b.y);
 }
 
+// DFT of length R on registers: v[o] <- sum_q v[q] W^(o q), W = exp(-2 pi i / R) for the
+// forward transform and its conjugate for the inverse (INV); products with +-1 and +-i are
+// exact swaps and sign changes.
+template <int R, bool INV>
+__device__ __forceinline__ void dft(double2 *v) {
+  if constexpr (R == 2) {
+    const double2 a = v[0], b = v[1];
+    v[0] = make_double2(a.x + b.x, a.y + b.y);
+    v[1] = make_double2(a.x - b.x, a.y - b.y);
+  } else if constexpr (R == 4) {
+    const double2 s02 = make_double2(v[0].x + v[2].x, v[0].y + v[2].y);
+    const double2 d02 = make_double2(v[0].x - v[2].x, v[0].y - v[2].y);
+    const double2 s13 = make_double2(v[1].x + v[3].x, v[1].y + v[3].y);
+    const double2 d13 = make_double2(v[1].x - v[3].x, v[1].y - v[3].y);
+    // forward: y1 = d02 - i d13, y3 = d02 + i d13 (inverse swaps them)
+    const double2 mi = make_double2(d13.y, -d13.x);  // -i * d13
+    v[0] = make_double2(s02.x + s13.x, s02.y + s13.y);
+    v[2] = make_double2(s02.x - s13.x, s02.y - s13.y);
+    if (!INV) {
+      v[1] = make_double2(d02.x + mi.x, d02.y + mi.y);
+      v[3] = make_double2(d02.x - mi.x, d02.y - mi.y);
+    } else {
+      v[1] = make_double2(d02.x - mi.x, d02.y - mi.y);
+      v[3] = make_double2(d02.x + mi.x, d02.y + mi.y);
+    }
+  } else if constexpr (R == 3) {
+    constexpr double c = -0.5, s = INV ? -0.86602540378443864676 : 0.86602540378443864676;
+    const double2 t = make_double2(v[1].x + v[2].x, v[1].y + v[2].y);
+    const double2 d = make_double2(v[1].x - v[2].x, v[1].y - v[2].y);
+    const double2 a = make_double2(v[0].x + c * t.x, v[0].y + c * t.y);
+    v[0] = make_double2(v[0].x + t.x, v[0].y + t.y);
+    // y1 = a - i s d, y2 = a + i s d;  -i (x + i y) = y - i x
+    v[1] = make_double2(a.x + s * d.y, a.y - s * d.x);
+    v[2] = make_double2(a.x - s * d.y, a.y + s * d.x);
+  } else {
+    static_assert(R == 5, "radix 2, 3, 4 or 5");
+    constexpr double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
+    constexpr double s1 = INV ? -0.95105651629515357212 : 0.95105651629515357212;
+    constexpr double s2 = INV ? -0.58778525229247312917 : 0.58778525229247312917;
+    const double2 t1 = make_double2(v[1].x + v[4].x, v[1].y + v[4].y);
+    const double2 t2 = make_double2(v[2].x + v[3].x, v[2].y + v[3].y);
+    const double2 t3 = make_double2(v[1].x - v[4].x, v[1].y - v[4].y);
+    const double2 t4 = make_double2(v[2].x - v[3].x, v[2].y - v[3].y);
+    const double2 a1 = make_double2(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
+    const double2 a2 = make_double2(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
+    const double2 b1 = make_double2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
+    const double2 b2 = make_double2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+    v[0] = make_double2(v[0].x + t1.x + t2.x, v[0].y + t1.y + t2.y);
+    // y1 = a1 - i b1, y4 = a1 + i b1, y2 = a2 - i b2, y3 = a2 + i b2
+    v[1] = make_double2(a1.x + b1.y, a1.y - b1.x);
+    v[4] = make_double2(a1.x - b1.y, a1.y + b1.x);
+    v[2] = make_double2(a2.x + b2.y, a2.y - b2.x);
+    v[3] = make_double2(a2.x - b2.y, a2.y + b2.x);
+  }
+}
+
+// One Stockham stage of radix R over nb transforms of length n (t-major in x), Ns = product of
+// the radices already applied: item (t, i) takes x[i + q n/R], q < R, twiddled by
+// W_n^(q k n/(Ns R)) with k = i mod Ns, and writes the R-point DFT to
+// y[(i / Ns) Ns R + k + o Ns].  tw[e] = W_n^e staged in LDS (conjugated for the inverse).
+template <int R, bool INV>
+__device__ __forceinline__ void stockham_stage(const double2 *x, double2 *y, const double2 *tw,
+                                               int n, int Ns, int nb) {
+  const int nq = n / R;
+  const int tstep = n / (Ns * R);
+  const int tot = nb * nq;
+  for (int bi = threadIdx.x; bi < tot; bi += blockDim.x) {
+    const int t = bi / nq, i = bi - t * nq;
+    const int j = i / Ns, k = i - j * Ns;
+    const double2 *xs = x + t * n + i;
+    double2 v[R];
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+      const double2 e = xs[q * nq];
+      v[q] = (q == 0 || k == 0) ? e : cmul(e, tw[q * k * tstep]);
+    }
+    dft<R, INV>(v);
+    double2 *ys = y + t * n + j * Ns * R + k;
+#pragma unroll
+    for (int o = 0; o < R; o++) ys[o * Ns] = v[o];
+  }
+}
+
 // nb transforms of length p.n stored t-major in `a`; result in the returned buffer.
-// tw[e] = exp(sign 2 pi i e / n) staged in LDS.
+template <bool INV>
 __device__ double2 *lds_fft(double2 *a, double2 *b, const double2 *tw, const Plan &p, int nb) {
-  const int n = p.n;
   int Ns = 1;
   for (int s = 0; s < p.nrad; s++) {
     const int r = p.rad[s];
-    const int nq = n / r;
-    const int tot = nb * nq;
-    for (int bi = threadIdx.x; bi < tot; bi += blockDim.x) {
-      const int t = bi / nq, i = bi - t * nq;
-      const int k = i % Ns;
-      const double2 *x = a + t * n;
-      double2 *y = b + t * n;
-      double2 v[5];
-      const int tstep = n / (Ns * r);
-      for (int q = 0; q < r; q++) {
-        double2 e = x[i + q * nq];
-        int te = (q * k * tstep) % n;
-        v[q] = (te == 0) ? e : cmul(e, tw[te]);
-      }
-      const int base = (i / Ns) * Ns * r + k;
-      const int rstep = n / r;
-      for (int o = 0; o < r; o++) {
-        double2 acc = v[0];
-        for (int q = 1; q < r; q++) {
-          int te = (o * q * rstep) % n;
-          acc = cadd(acc, (te == 0) ? v[q] : cmul(v[q], tw[te]));
-        }
-        y[base + o * Ns] = acc;
-      }
-    }
+    if (r == 4) stockham_stage<4, INV>(a, b, tw, p.n, Ns, nb);
+    else if (r == 2) stockham_stage<2, INV>(a, b, tw, p.n, Ns, nb);
+    else if (r == 3) stockham_stage<3, INV>(a, b, tw, p.n, Ns, nb);
+    else stockham_stage<5, INV>(a, b, tw, p.n, Ns, nb);
     __syncthreads();
     double2 *sw = a;
     a = b;
@@ -109,7 +170,7 @@ __global__ void __launch_bounds__(FFT_TPB) k_rows_r2c(Plan p, const double2 *__r
   stage_twiddles(T, tw, n, false);
   for (int e = threadIdx.x; e < nbb * n; e += blockDim.x) A[e] = make_double2(src[(int64_t)r0 * n + e], 0.0);
   __syncthreads();
-  double2 *R = lds_fft(A, B, T, p, nbb);
+  double2 *R = lds_fft<false>(A, B, T, p, nbb);
   for (int e = threadIdx.x; e < nbb * nl; e += blockDim.x) {
     int t = e / nl, l = e - t * nl;
     dst[(int64_t)(r0 + t) * nl + l] = R[t * n + l];
@@ -137,7 +198,7 @@ __global__ void __launch_bounds__(FFT_TPB) k_cols_fwd(Plan p, const double2 *__r
     A[t * ny + y] = lev[(int64_t)y * nl + l0 + t];
   }
   __syncthreads();
-  double2 *R = lds_fft(A, B, T, p, nbb);
+  double2 *R = lds_fft<false>(A, B, T, p, nbb);
   const int nm = ny / 2 + 1;
   double *out = dst + (int64_t)k * sy * sx;
   for (int e = threadIdx.x; e < nbb * nm; e += blockDim.x) {
@@ -186,7 +247,7 @@ __global__ void __launch_bounds__(FFT_TPB) k_rows_inv(Plan p, const double2 *__r
     A[t * nx + l] = h;
   }
   __syncthreads();
-  double2 *R = lds_fft(A, B, T, p, nbb);
+  double2 *R = lds_fft<true>(A, B, T, p, nbb);
   const double dn = 1.0 / (double)nx;
   for (int e = threadIdx.x; e < nbb * nx; e += blockDim.x) {
     int t = e / nx, x = e - t * nx;
@@ -219,7 +280,7 @@ __global__ void __launch_bounds__(FFT_TPB) k_cols_c2r(Plan p, const double2 *__r
     A[t * ny + y] = w;
   }
   __syncthreads();
-  double2 *R = lds_fft(A, B, T, p, nbb);
+  double2 *R = lds_fft<true>(A, B, T, p, nbb);
   const double dn = 1.0 / (double)ny;
   double *out = dst + (int64_t)k * ny * nx;
   for (int e = threadIdx.x; e < nbb * ny; e += blockDim.x) {

@@ -167,16 +167,24 @@ __global__ void k_mean_flow(Geometry g, const double *__restrict__ d, const doub
   }
 }
 
-// laplinv (fft_ops.F90:505-601), in place, one thread per column.  The right-hand side of row
-// k reads the original f(k-1), f(k), f(k+1); a sliding window keeps the originals while the
-// forward sweep overwrites f with d'.  cp is staged in scratch (same layout).
-__global__ void k_laplinv(Geometry g, double *__restrict__ f, double *__restrict__ cp,
-                          const double *__restrict__ k2a, int df0, DevScalars *sc) {
-  const int64_t plane = (int64_t)g.sy * g.sx;
-  int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= plane) return;
-  const double k2 = k2a[col];
-  if (k2 == 0) return;
+// Column solvers.  One thread owns one (kx,ky) column and runs the reference's Thomas
+// recurrence on it.  The column (and the elimination coefficients cp) sit either in HBM
+// (stride = one spectral plane) or, when they fit, in an LDS slice private to the thread
+// (stride = threads per block): the whole column is loaded with independent coalesced reads,
+// solved in LDS, and stored back, so the level-by-level recurrence never waits on HBM.  Both
+// forms run the same arithmetic.
+template <typename T>
+struct Col {
+  T *p;
+  int64_t s;
+  __device__ __forceinline__ T &operator[](int k) const { return p[(int64_t)k * s]; }
+};
+
+// laplinv (fft_ops.F90:505-601) of one column, in place.  The right-hand side of row k reads
+// the original f(k-1), f(k), f(k+1); a sliding window keeps the originals while the forward
+// sweep overwrites f with d'.  Returns PMPIC_ERR_ZERO_PIVOT on a zero pivot.
+template <typename C>
+__device__ __forceinline__ int laplinv_column(const Geometry &g, C F, C CP, double k2, int df0) {
   const int nz = g.nzn;
   const double f16 = (double)(1.f / 6.f), f13 = (double)(1.f / 3.f);
   const double f112 = (double)(1.f / 12.f), f56 = (double)(5.f / 6.f);
@@ -184,9 +192,7 @@ __global__ void k_laplinv(Geometry g, double *__restrict__ f, double *__restrict
   const double i2dz2 = 2. / dz / dz, idz2 = 1. / dz / dz;
   // interior coefficients (constant along the column)
   const double ai = idz2 - f112 * k2, bi = -i2dz2 - f56 * k2, ci = idz2 - f112 * k2;
-  double *F = f + col;
-  double *CP = cp + col;
-  double fm = 0.0, f0 = F[0], fp = F[plane];  // originals f(k-1), f(k), f(k+1)
+  double fm = 0.0, f0 = F[0], fp = F[1];  // originals f(k-1), f(k), f(k+1)
   // row 1
   double b1, c1, d1;
   if (!df0) { b1 = 1.; c1 = 0.; d1 = 0.; }
@@ -198,7 +204,7 @@ __global__ void k_laplinv(Geometry g, double *__restrict__ f, double *__restrict
   for (int k = 2; k <= nz; k++) {
     fm = f0;
     f0 = fp;
-    fp = (k < nz) ? F[(int64_t)k * plane] : 0.0;
+    fp = (k < nz) ? F[k] : 0.0;
     double a, b, c, d;
     if (k < nz) {
       a = ai; b = bi; c = ci;
@@ -210,65 +216,144 @@ __global__ void k_laplinv(Geometry g, double *__restrict__ f, double *__restrict
       d = f13 * f0 + f16 * fm;
     }
     const double cpk = cprev / denominator;
-    CP[(int64_t)(k - 1) * plane] = cpk;
+    CP[k - 1] = cpk;
     denominator = b - a * cpk;
-    if (denominator == 0) { raise_status(sc, PMPIC_ERR_ZERO_PIVOT); return; }
+    if (denominator == 0) return PMPIC_ERR_ZERO_PIVOT;
     fprev = (d - a * fprev) / (denominator);
-    F[(int64_t)(k - 1) * plane] = fprev;
+    F[k - 1] = fprev;
     cprev = c;
   }
   double fnext = fprev;
   for (int k = nz - 1; k >= 1; k--) {
-    const double v = F[(int64_t)(k - 1) * plane] - CP[(int64_t)k * plane] * fnext;
-    F[(int64_t)(k - 1) * plane] = v;
+    const double v = F[k - 1] - CP[k] * fnext;
+    F[k - 1] = v;
     fnext = v;
   }
+  return PMPIC_OK;
 }
 
-// diffz (fft_ops.F90:419-483), out of place, one thread per column
-__global__ void k_diffz(Geometry g, const double *__restrict__ f, double *__restrict__ out,
-                        double *__restrict__ cp, const double *__restrict__ bot,
-                        const double *__restrict__ top, DevScalars *sc) {
-  const int64_t plane = (int64_t)g.sy * g.sx;
-  int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= plane) return;
+// diffz (fft_ops.F90:419-483) of one column; O may alias F (the original f(k-2), f(k-1) are
+// kept in registers, and f(k) is read before d'(k-1) is written).
+template <typename CI, typename CO>
+__device__ __forceinline__ int diffz_column(const Geometry &g, CI F, CO O, CO CP, bool bt, double bot,
+                                            double top) {
   const int nz = g.nzn;
-  const bool bt = (bot != nullptr && top != nullptr);
   const double a1 = (double)(1.f / 6.f), b1 = (double)(2.f / 3.f), c1 = (double)(1.f / 6.f);
   const double al = (double)(1.f / 3.f), cl = (double)(1.f / 3.f);
   const double dz = g.dz;
-  const double *F = f + col;
-  double *O = out + col;
-  double *CP = cp + col;
+  double fkm2 = F[0], fkm1 = F[1];  // f(k-2), f(k-1) of the next row
   // row 1
-  double d = bt ? F[plane] / dz - a1 * bot[col] * dz : 0.0;
+  double d = bt ? fkm1 / dz - a1 * bot * dz : 0.0;
   double denominator = 1.;  // b(1) = b0
   double fprev = d / denominator;
   O[0] = fprev;
   double cprev = bt ? cl : 0.;  // c(1)
   for (int k = 2; k <= nz; k++) {
-    double a, b, c;
+    double a, b, c, fk = 0.0;
     if (k < nz) {
       a = a1; b = b1; c = c1;
-      d = (F[(int64_t)k * plane] - F[(int64_t)(k - 2) * plane]) / 2. / dz;
+      fk = F[k];
+      d = (fk - fkm2) / 2. / dz;
     } else {
       a = bt ? al : 0.; b = 1.; c = 0.;
-      d = bt ? -F[(int64_t)(nz - 2) * plane] / dz + a1 * top[col] * dz : 0.0;
+      d = bt ? -fkm2 / dz + a1 * top * dz : 0.0;
     }
     const double cpk = cprev / denominator;
-    CP[(int64_t)(k - 1) * plane] = cpk;
+    CP[k - 1] = cpk;
     denominator = b - a * cpk;
-    if (denominator == 0) { raise_status(sc, PMPIC_ERR_ZERO_PIVOT); return; }
+    if (denominator == 0) return PMPIC_ERR_ZERO_PIVOT;
     fprev = (d - a * fprev) / (denominator);
-    O[(int64_t)(k - 1) * plane] = fprev;
+    O[k - 1] = fprev;
     cprev = c;
+    fkm2 = fkm1;
+    fkm1 = fk;
   }
   double fnext = fprev;
   for (int k = nz - 1; k >= 1; k--) {
-    const double v = O[(int64_t)(k - 1) * plane] - CP[(int64_t)k * plane] * fnext;
-    O[(int64_t)(k - 1) * plane] = v;
+    const double v = O[k - 1] - CP[k] * fnext;
+    O[k - 1] = v;
     fnext = v;
   }
+  return PMPIC_OK;
+}
+
+// up to 4 fields per launch (blockIdx.y)
+struct ColFields {
+  const double *in[4];
+  double *out[4];
+  const double *bot[4];
+  const double *top[4];
+  int df0[4];
+};
+
+__global__ void k_laplinv(Geometry g, ColFields cf, double *__restrict__ cp,
+                          const double *__restrict__ k2a, DevScalars *sc) {
+  const int64_t plane = (int64_t)g.sy * g.sx;
+  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= plane) return;
+  const double k2 = k2a[col];
+  if (k2 == 0) return;
+  Col<double> F{cf.out[blockIdx.y] + col, plane}, CP{cp + (int64_t)blockIdx.y * g.nzn * plane + col, plane};
+  const int st = laplinv_column(g, F, CP, k2, cf.df0[blockIdx.y]);
+  if (st != PMPIC_OK) raise_status(sc, st);
+}
+
+__global__ void k_laplinv_lds(Geometry g, ColFields cf, const double *__restrict__ k2a,
+                              DevScalars *sc) {
+  extern __shared__ double sh[];
+  const int nc = blockDim.x, t = threadIdx.x, nz = g.nzn;
+  const int64_t plane = (int64_t)g.sy * g.sx;
+  const int64_t col = (int64_t)blockIdx.x * nc + t;
+  if (col >= plane) return;  // no barriers below: each thread touches only its own slice
+  const double k2 = k2a[col];
+  if (k2 == 0) return;
+  double *f = cf.out[blockIdx.y] + col;
+  Col<double> F{sh + t, nc}, CP{sh + (int64_t)nz * nc + t, nc};
+#pragma unroll 8
+  for (int k = 0; k < nz; k++) F[k] = f[(int64_t)k * plane];
+  const int st = laplinv_column(g, F, CP, k2, cf.df0[blockIdx.y]);
+  if (st != PMPIC_OK) {
+    raise_status(sc, st);
+    return;
+  }
+#pragma unroll 8
+  for (int k = 0; k < nz; k++) f[(int64_t)k * plane] = F[k];
+}
+
+__global__ void k_diffz(Geometry g, ColFields cf, double *__restrict__ cp, DevScalars *sc) {
+  const int64_t plane = (int64_t)g.sy * g.sx;
+  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= plane) return;
+  const int fi = blockIdx.y;
+  const bool bt = cf.bot[fi] != nullptr && cf.top[fi] != nullptr;
+  Col<const double> F{cf.in[fi] + col, plane};
+  Col<double> O{cf.out[fi] + col, plane}, CP{cp + (int64_t)fi * g.nzn * plane + col, plane};
+  const int st = diffz_column(g, F, O, CP, bt, bt ? cf.bot[fi][col] : 0.0,
+                              bt ? cf.top[fi][col] : 0.0);
+  if (st != PMPIC_OK) raise_status(sc, st);
+}
+
+__global__ void k_diffz_lds(Geometry g, ColFields cf, DevScalars *sc) {
+  extern __shared__ double sh[];
+  const int nc = blockDim.x, t = threadIdx.x, nz = g.nzn;
+  const int64_t plane = (int64_t)g.sy * g.sx;
+  const int64_t col = (int64_t)blockIdx.x * nc + t;
+  if (col >= plane) return;  // no barriers below: each thread touches only its own slice
+  const int fi = blockIdx.y;
+  const bool bt = cf.bot[fi] != nullptr && cf.top[fi] != nullptr;
+  const double *f = cf.in[fi] + col;
+  double *o = cf.out[fi] + col;
+  Col<double> F{sh + t, nc}, CP{sh + (int64_t)nz * nc + t, nc};
+#pragma unroll 8
+  for (int k = 0; k < nz; k++) F[k] = f[(int64_t)k * plane];
+  const int st = diffz_column(g, F, F, CP, bt, bt ? cf.bot[fi][col] : 0.0,
+                              bt ? cf.top[fi][col] : 0.0);
+  if (st != PMPIC_OK) {
+    raise_status(sc, st);
+    return;
+  }
+#pragma unroll 8
+  for (int k = 0; k < nz; k++) o[(int64_t)k * plane] = F[k];
 }
 
 // u, v, w spectra (vort2vel.F90:251-320): fu = dzb - diffy(c), column (1,1) = ubar;
@@ -460,18 +545,72 @@ int op_filter(Ctx *c, const double *in, double *out) {
   PMPIC_CHECK(hipGetLastError());
   return PMPIC_OK;
 }
+// threads per block of the LDS column kernels: the most columns (64 down to 8) whose two
+// slices (f and cp) fit in 64 KiB of LDS; 0 = columns too long, solve straight from HBM
+static int lds_cols(const Geometry &g) {
+  for (int nc = 64; nc >= 8; nc /= 2)
+    if (2 * (int64_t)nc * g.nzn * (int64_t)sizeof(double) <= 65536) return nc;
+  return 0;
+}
+
+int op_diffz_n(Ctx *c, int nf, const double *const *in, double *const *out,
+               const double *const *bot, const double *const *top) {
+  if (nf < 1 || nf > 4) return PMPIC_ERR_ARG;
+  const int64_t plane = plane_of(c->g);
+  const int nc = lds_cols(c->g);
+  if (nc > 0) {
+    ColFields cf{};
+    for (int f = 0; f < nf; f++) {
+      cf.in[f] = in[f];
+      cf.out[f] = out[f];
+      cf.bot[f] = bot ? bot[f] : nullptr;
+      cf.top[f] = top ? top[f] : nullptr;
+    }
+    hipLaunchKernelGGL(k_diffz_lds, dim3(cdiv(plane, nc), nf), dim3(nc),
+                       (size_t)2 * nc * c->g.nzn * sizeof(double), c->stream, c->g, cf, c->dsc);
+  } else {
+    for (int f = 0; f < nf; f++) {  // one field at a time: sder[7] holds one field's cp
+      ColFields one{};
+      one.in[0] = in[f];
+      one.out[0] = out[f];
+      one.bot[0] = bot ? bot[f] : nullptr;
+      one.top[0] = top ? top[f] : nullptr;
+      hipLaunchKernelGGL(k_diffz, dim3(cdiv(plane, 64), 1), dim3(64), 0, c->stream, c->g, one,
+                         c->sder[7], c->dsc);
+    }
+  }
+  PMPIC_CHECK(hipGetLastError());
+  return PMPIC_OK;
+}
 int op_diffz(Ctx *c, const double *in, double *out, const double *bot, const double *top) {
-  hipLaunchKernelGGL(k_diffz, dim3(cdiv(plane_of(c->g), 64)), dim3(64), 0, c->stream, c->g, in,
-                     out, c->sder[7], bot, top, c->dsc);
+  return op_diffz_n(c, 1, &in, &out, bot ? &bot : nullptr, top ? &top : nullptr);
+}
+int op_laplinv_n(Ctx *c, int nf, double *const *f, const int *df0) {
+  if (nf < 1 || nf > 4) return PMPIC_ERR_ARG;
+  const int64_t plane = plane_of(c->g);
+  const int nc = lds_cols(c->g);
+  if (nc > 0) {
+    ColFields cf{};
+    for (int q = 0; q < nf; q++) {
+      cf.out[q] = f[q];
+      cf.df0[q] = df0[q];
+    }
+    hipLaunchKernelGGL(k_laplinv_lds, dim3(cdiv(plane, nc), nf), dim3(nc),
+                       (size_t)2 * nc * c->g.nzn * sizeof(double), c->stream, c->g, cf, c->k2,
+                       c->dsc);
+  } else {
+    for (int q = 0; q < nf; q++) {
+      ColFields one{};
+      one.out[0] = f[q];
+      one.df0[0] = df0[q];
+      hipLaunchKernelGGL(k_laplinv, dim3(cdiv(plane, 64), 1), dim3(64), 0, c->stream, c->g, one,
+                         c->sder[7], c->k2, c->dsc);
+    }
+  }
   PMPIC_CHECK(hipGetLastError());
   return PMPIC_OK;
 }
-int op_laplinv(Ctx *c, double *f, int df0) {
-  hipLaunchKernelGGL(k_laplinv, dim3(cdiv(plane_of(c->g), 64)), dim3(64), 0, c->stream, c->g, f,
-                     c->sder[7], c->k2, df0, c->dsc);
-  PMPIC_CHECK(hipGetLastError());
-  return PMPIC_OK;
-}
+int op_laplinv(Ctx *c, double *f, int df0) { return op_laplinv_n(c, 1, &f, &df0); }
 
 // vort2vel timestep_callback (vort2vel.F90:134-320) after par2grid of p, q, r into the grids:
 // corrected and filtered vorticity back into grid[GP..GR], velocities into grid[GU..GW], the
@@ -497,12 +636,18 @@ int vort2vel_spectral(Ctx *c, int stage) {
                      c->sd, c->se, c->sf, c->filt, c->atop, c->abot, c->btop, c->bbot);
   hipLaunchKernelGGL(k_mean_flow, dim3(1), dim3(64), 0, c->stream, g, c->sd, c->se, c->ubar,
                      c->vbar);
-  PMPIC_TRY(op_laplinv(c, c->sa, 0));
-  PMPIC_TRY(op_laplinv(c, c->sb, 0));
-  PMPIC_TRY(op_laplinv(c, c->sc, 1));
-  // dz(b) with b's cached top/bottom into sder[0], dz(a) into sder[1]
-  PMPIC_TRY(op_diffz(c, c->sb, c->sder[0], c->bbot, c->btop));
-  PMPIC_TRY(op_diffz(c, c->sa, c->sder[1], c->abot, c->atop));
+  {
+    double *fl[3] = {c->sa, c->sb, c->sc};
+    const int df0[3] = {0, 0, 1};
+    PMPIC_TRY(op_laplinv_n(c, 3, fl, df0));
+  }
+  {
+    // dz(b) with b's cached top/bottom into sder[0], dz(a) into sder[1]
+    const double *in[2] = {c->sb, c->sa};
+    double *out[2] = {c->sder[0], c->sder[1]};
+    const double *bot[2] = {c->bbot, c->abot}, *top[2] = {c->btop, c->atop};
+    PMPIC_TRY(op_diffz_n(c, 2, in, out, bot, top));
+  }
   hipLaunchKernelGGL(k_velocity_spectra, eb, et, 0, c->stream, g, c->sa, c->sb, c->sc, c->sder[0],
                      c->sder[1], c->ubar, c->vbar, c->sder[2], c->sder[3], c->sder[4], c->su,
                      c->sv, c->sw, c->kx, c->ky, c->filt);

@@ -130,13 +130,18 @@ class Context:
 
     def __init__(self, nx, ny, nz, xmin, xmax, ymin, ymax, zmin, zmax, max_parcels,
                  dtm=0.01, dtmax=0.01, timestep_prefactor=0.5, splitting=True, mixing=True,
-                 device=0):
+                 device=0, precision=0):
+        """precision 0 = plain FP64 form; 1 = FP32-bulk form (cell index + FP32 offsets, FP32
+        parcel fields and interpolation; components, step, parcel transfer and conservation
+        sums only)."""
         self.L = load()
         cfg = Config(nx, ny, nz, xmin, xmax, ymin, ymax, zmin, zmax, int(max_parcels), dtm, dtmax,
-                     timestep_prefactor, 0, int(bool(splitting)), int(bool(mixing)), device)
+                     timestep_prefactor, int(precision), int(bool(splitting)), int(bool(mixing)),
+                     device)
         h = ctypes.c_void_p()
         self._chk(self.L.pmpic_create(ctypes.byref(cfg), ctypes.byref(h)), "pmpic_create")
         self.h = h
+        self.precision = int(precision)
         self.nx, self.ny, self.nz = nx, ny, nz
         self.max_parcels = int(max_parcels)
         self.gshape = (nx + 4, ny + 4, nz + 1)

@@ -1,0 +1,85 @@
+"""FP32-bulk form (precision 1): the plain form's step with each position held as a cell index
+plus three FP32 offsets, FP32 parcel fields, FP32 par2grid sums and interpolation.
+
+It is checked against the plain FP64 GPU form on the same inputs; the FP64 form is itself pinned
+to the oracle by test_gpu_parity.py.  Bars (written per test): the parcel transfer round trip to
+FP32 rounding of the offsets; grids and velocities within 1e-5 relative (the north-star
+velocity bar for the FP32-bulk form) after the steps run; identical split/mix counts on cases
+where no parcel sits within FP32 rounding of a threshold; volume conserved to FP32 rounding.
+"""
+import numpy as np
+import pytest
+
+from tests.cases import LX, LZ, box, rel_err, thermal_case
+
+pytestmark = pytest.mark.gpu
+
+
+def make(shape, precision, ic, cap_factor=3):
+    from pmpic_amd.lib import Context
+    nx, ny, nz = shape
+    b = box(nx, ny, nz)
+    cap = cap_factor * len(ic["x"]) + 64
+    g = Context(nx, ny, nz, b["xmin"], b["xmax"], b["ymin"], b["ymax"], b["zmin"], b["zmax"], cap,
+                dtm=0.01, dtmax=0.01, timestep_prefactor=0.5, precision=precision)
+    g.set_parcels(ic)
+    return g
+
+
+def test_transfer_roundtrip():
+    shape = (12, 10, 8)
+    ic = thermal_case(*shape, 2, vort=1.0)
+    g = make(shape, 1, ic)
+    got = g.get_parcels()
+    assert g.n == len(ic["x"])
+    d = (LX / shape[0], LX / shape[1], LZ / shape[2])
+    for k, dk in zip(("x", "y", "z"), d):
+        # offsets carry 24 bits inside the cell
+        assert np.max(np.abs(got[k] - ic[k])) <= dk * 2.0 ** -23, k
+    for k in ("p", "q", "r", "b", "h", "vol", "stretch"):
+        np.testing.assert_array_equal(got[k], ic[k].astype(np.float32).astype(np.float64))
+    np.testing.assert_array_equal(got["tag"], ic["tag"])
+
+
+def test_fp64_only_entry_points_refuse():
+    from pmpic_amd.lib import PmpicError
+    shape = (12, 10, 8)
+    ic = thermal_case(*shape, 2)
+    g = make(shape, 1, ic)
+    with pytest.raises(PmpicError):
+        g.par2grid(ic["b"])
+    with pytest.raises(PmpicError):
+        g.parcel_haloswap()
+
+
+@pytest.mark.parametrize("shape,kw,steps", [
+    ((12, 10, 8), dict(vort=1.0), 1),
+    ((16, 16, 8), dict(vort=2.0, shrink=0.05, thin=0.1), 6),
+    ((10, 12, 6), dict(vort=0.5, thin=0.2), 10),
+])
+def test_steps_close_to_fp64(shape, kw, steps):
+    ic = thermal_case(*shape, 2, **kw)
+    g64 = make(shape, 0, ic)
+    g32 = make(shape, 1, ic)
+    for step in range(steps):
+        g64.step()
+        g32.step()
+        assert g32.n == g64.n, f"step {step}: parcel count {g32.n} vs {g64.n}"
+        assert g32.last_counts() == g64.last_counts(), step
+        assert abs(g32.time - g64.time) <= 1e-12
+    for name in ("u", "v", "w"):
+        assert rel_err(g32.grid(name), g64.grid(name)) < 1e-5, name
+    for name in ("p", "q", "r", "b"):
+        assert rel_err(g32.grid(name), g64.grid(name)) < 1e-5, name
+
+
+def test_volume_conserved():
+    shape = (16, 16, 8)
+    ic = thermal_case(*shape, 2, vort=3.0, shrink=0.05, thin=0.1, near_split=0.05)
+    g = make(shape, 1, ic)
+    s0 = g.conservation_sums()
+    for _ in range(3):
+        g.step()
+    s1 = g.conservation_sums()
+    # split halves volumes exactly; mixing moves volume between parcels in FP32
+    assert abs(s1[0] - s0[0]) <= 1e-6 * s0[0]

@@ -156,3 +156,20 @@ def test_steps_match_oracle(oracle_mod, shape, kw):
         for k in ("x", "y", "z", "p", "q", "r", "vol", "b", "h", "stretch"):
             assert rel_err(gp[k], op[k]) < 1e-10, (step, k)
         assert abs(g.time - o.time) <= 1e-12
+
+
+def test_twenty_steps_counts_match_oracle(oracle_mod):
+    """The north-star count bar: identical parcel counts after split/merge for the first 20
+    steps (and the split/removed/added counts of every step), with the fields still within
+    1e-9 relative at the end."""
+    o, g, _ = make_pair(oracle_mod, (12, 10, 8), vort=3.0, shrink=0.03, thin=0.08, near_split=0.04)
+    for step in range(20):
+        o.step()
+        g.step()
+        assert g.n == o.n, f"step {step}: parcel count {g.n} vs {o.n}"
+        assert g.last_counts() == o.last_counts(), step
+    gp, op = g.get_parcels(), o.get_parcels()
+    for k in ("x", "y", "z", "p", "q", "r", "vol", "b", "h"):
+        assert rel_err(gp[k], op[k]) < 1e-9, k
+    for name in ("u", "v", "w"):
+        assert rel_err(interior(g.grid(name)), interior(o.grid(name))) < 1e-9, name
