@@ -26,6 +26,10 @@
 #include <math.h>
 #include <vector>
 
+// The FP32-bulk form is not held to the reference's operation order (its bar is the velocity
+// error against the FP64 form), so multiply-adds may fuse here.
+#pragma clang fp contract(fast)
+
 namespace pmpic {
 
 namespace {
@@ -33,11 +37,29 @@ namespace {
 constexpr int TPB = 256;
 constexpr int NWAVE = TPB / 64;
 
-__device__ __forceinline__ void cell_ijk(const Geometry &g, int32_t c, int &i, int &j, int &k) {
-  i = c % g.nx;
-  const int r = c / g.nx;
-  j = r % g.ny;
-  k = r / g.ny;
+// A parcel's cell is stored packed as k << 20 | j << 10 | i (nx, ny <= 1024, nz < 2048), so
+// unpacking is shifts and masks; the linear cell index (k*ny + j)*nx + i is formed only where
+// buckets are indexed.
+__device__ __forceinline__ int32_t pack_cell(int i, int j, int k) { return (k << 20) | (j << 10) | i; }
+__device__ __forceinline__ void cell_ijk(const Geometry &, int32_t c, int &i, int &j, int &k) {
+  i = c & 1023;
+  j = (c >> 10) & 1023;
+  k = c >> 20;
+}
+__device__ __forceinline__ int32_t linear_cell(const Geometry &g, int32_t c) {
+  return ((c >> 20) * g.ny + ((c >> 10) & 1023)) * g.nx + (c & 1023);
+}
+
+// Loads and stores at a uniform base plus a 32-bit byte offset (the global_load/store form
+// with a scalar base), valid while index * sizeof(T) < 2^32: capacities are checked at set-up.
+template <typename T>
+__device__ __forceinline__ T ld(const T *base, uint32_t i) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) +
+                                      i * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void st(T *base, uint32_t i, T v) {
+  *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + i * (uint32_t)sizeof(T)) = v;
 }
 
 // fold an offset outside [0,1) into the cell index
@@ -60,10 +82,15 @@ __device__ __forceinline__ bool place(const Geometry &g, int i, int j, int k, fl
   fold1(i, tx);
   fold1(j, ty);
   fold1(k, tz);
-  i %= g.nx;
-  if (i < 0) i += g.nx;
-  j %= g.ny;
-  if (j < 0) j += g.ny;
+  // a stage moves a parcel by far less than a period, so one correction is the common case
+  if (i < 0 || i >= g.nx) {
+    i %= g.nx;
+    if (i < 0) i += g.nx;
+  }
+  if (j < 0 || j >= g.ny) {
+    j %= g.ny;
+    if (j < 0) j += g.ny;
+  }
   const int nz = g.ncz;
   if (k < 0) {  // z' = -z
     if (tz == 0.0f) {
@@ -87,7 +114,7 @@ __device__ __forceinline__ bool place(const Geometry &g, int i, int j, int k, fl
     tz = 1.0f;
   }
   if (k < 0 || k >= nz) return false;
-  cell = (k * g.ny + j) * g.nx + i;
+  cell = pack_cell(i, j, k);
   ox = tx;
   oy = ty;
   oz = tz;
@@ -107,10 +134,10 @@ __device__ __forceinline__ void corner_nodes(const Geometry &g, int i, int j, in
 // trilinear interpolation in the reference's order (parcel_interpolation.F90:309-321)
 __device__ __forceinline__ float interp(const float *__restrict__ G, const int32_t o[8], float ox,
                                         float oy, float oz) {
-  const float c00 = G[o[0]] * (1 - oz) + G[o[4]] * oz;
-  const float c01 = G[o[1]] * (1 - oz) + G[o[5]] * oz;
-  const float c10 = G[o[2]] * (1 - oz) + G[o[6]] * oz;
-  const float c11 = G[o[3]] * (1 - oz) + G[o[7]] * oz;
+  const float c00 = ld(G, o[0]) * (1 - oz) + ld(G, o[4]) * oz;
+  const float c01 = ld(G, o[1]) * (1 - oz) + ld(G, o[5]) * oz;
+  const float c10 = ld(G, o[2]) * (1 - oz) + ld(G, o[6]) * oz;
+  const float c11 = ld(G, o[3]) * (1 - oz) + ld(G, o[7]) * oz;
   const float c0 = c00 * (1 - oy) + c10 * oy;
   const float c1 = c01 * (1 - oy) + c11 * oy;
   return c0 * (1 - ox) + c1 * ox;
@@ -141,8 +168,7 @@ __global__ void k32_pos_in(Geometry g, int axis, const double *__restrict__ x, i
     }
   }
   off[p] = o;
-  const int32_t stride = axis == 0 ? 1 : (axis == 1 ? g.nx : g.nx * g.ny);
-  cell[p] = (axis == 0 ? 0 : cell[p]) + i * stride;
+  cell[p] = (axis == 0 ? 0 : cell[p]) | (i << (10 * axis));
 }
 
 __global__ void k32_pos_out(Geometry g, int axis, const int32_t *__restrict__ cell,
@@ -168,30 +194,41 @@ __global__ void k32_to_f64(const float *__restrict__ a, int64_t n, double *__res
 }
 
 // ---- cell binning (the plain form's, with the cell read instead of located) ----
-__global__ void k32_bin_count(const int32_t *__restrict__ cellv, int64_t n,
+// Each wavefront walks BIN_SEQ consecutive groups of 64 parcels in order, so a cell whose
+// parcels are consecutive in slot order gets its bucket slots in ascending parcel order unless
+// it straddles two wavefronts' ranges; the accumulation only has to reorder those cells.
+constexpr int BIN_SEQ = 16;
+
+__global__ void k32_bin_count(Geometry g, const int32_t *__restrict__ cellv, int64_t n,
                               int32_t *__restrict__ rank, int32_t *__restrict__ count) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
-  const int32_t c = (p < n) ? cellv[p] : -1;
-  const int32_t cprev = __shfl_up(c, 1);
-  const bool head = (lane == 0) || (c != cprev);
-  const unsigned long long hm = __ballot(head);
-  const unsigned long long upto = hm & ((2ull << lane) - 1ull);
-  const int hl = 63 - __clzll(upto);
-  const unsigned long long above = hm & ~((2ull << hl) - 1ull);
-  const int next = above ? (__ffsll(above) - 1) : 64;
-  int base = 0;
-  if (head && c >= 0) base = atomicAdd(&count[c], next - hl);
-  base = __shfl(base, hl);
-  if (p < n) rank[p] = base + (lane - hl);
+  const int64_t w0 = (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64 * BIN_SEQ;
+  for (int it = 0; it < BIN_SEQ; it++) {
+    const int64_t g0 = w0 + (int64_t)it * 64;
+    if (g0 >= n) break;  // uniform across the wavefront
+    const int64_t p = g0 + lane;
+    const int32_t c = (p < n) ? linear_cell(g, cellv[p]) : -1;
+    const int32_t cprev = __shfl_up(c, 1);
+    const bool head = (lane == 0) || (c != cprev);
+    const unsigned long long hm = __ballot(head);
+    const unsigned long long upto = hm & ((2ull << lane) - 1ull);
+    const int hl = 63 - __clzll(upto);
+    const unsigned long long above = hm & ~((2ull << hl) - 1ull);
+    const int next = above ? (__ffsll(above) - 1) : 64;
+    int base = 0;
+    if (head && c >= 0) base = atomicAdd(&count[c], next - hl);
+    base = __shfl(base, hl);
+    if (p < n) rank[p] = base + (lane - hl);
+  }
 }
 
-__global__ void k32_bin_scatter(const int32_t *__restrict__ cellv, const int32_t *__restrict__ rank,
+__global__ void k32_bin_scatter(Geometry g, const int32_t *__restrict__ cellv,
+                                const int32_t *__restrict__ rank,
                                 const int32_t *__restrict__ start, int64_t n,
                                 int32_t *__restrict__ sorted) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  sorted[start[cellv[p]] + rank[p]] = (int32_t)p;
+  sorted[start[linear_cell(g, cellv[p])] + rank[p]] = (int32_t)p;
 }
 
 __global__ void k32_sort_big(const int32_t *__restrict__ start, int64_t ncell, int rows,
@@ -247,6 +284,10 @@ __global__ void __launch_bounds__(256) k32_cell_accum(Bulk32 b, int64_t cap, Geo
   const int cl = tid >> 3;
   const int q = tid & 7;
   const int bx = q & 1, by = (q >> 1) & 1, bz = (q >> 2) & 1;
+  // corner factor f = s * offset + t: offset for the upper corner, 1 - offset for the lower
+  const float sx = bx ? 1.f : -1.f, tx = bx ? 0.f : 1.f;
+  const float sy = by ? 1.f : -1.f, ty = by ? 0.f : 1.f;
+  const float sz = bz ? 1.f : -1.f, tz = bz ? 0.f : 1.f;
   const int64_t c0 = (int64_t)blockIdx.x * TILE32;
   const int ntile = (int)min((int64_t)TILE32, ncell - c0);
   for (int t = tid; t <= ntile; t += blockDim.x) sstart[t] = start[c0 + t];
@@ -260,7 +301,9 @@ __global__ void __launch_bounds__(256) k32_cell_accum(Bulk32 b, int64_t cap, Geo
   int first = 0;
   while (first < ntile) {
     int last = first + 1;
-    while (last < ntile && sstart[last + 1] - sstart[first] <= ROWS32) last++;
+    if (sstart[ntile] - sstart[first] <= ROWS32) last = ntile;  // the usual case: the rest fits
+    else
+      while (last < ntile && sstart[last + 1] - sstart[first] <= ROWS32) last++;
     const int32_t r0 = sstart[first];
     const int nrows = sstart[last] - r0;
     const bool fits = nrows <= ROWS32;
@@ -270,7 +313,7 @@ __global__ void __launch_bounds__(256) k32_cell_accum(Bulk32 b, int64_t cap, Geo
 #pragma unroll
       for (int k = 0; k < RPT; k++) {
         const int r = tid + k * TILE32 * 8;
-        pr[k] = (r < nrows) ? sorted[r0 + r] : -1;
+        pr[k] = (r < nrows) ? ld(sorted, (uint32_t)(r0 + r)) : -1;
       }
       float4 geo[RPT], va[RPT];
       float vb[RPT];
@@ -279,18 +322,19 @@ __global__ void __launch_bounds__(256) k32_cell_accum(Bulk32 b, int64_t cap, Geo
       for (int k = 0; k < RPT; k++) {
         vb[k] = 0.f;
         if (pr[k] >= 0) {
-          const int64_t pp = pr[k];
-          geo[k] = make_float4(F[B_OX * cap + pp], F[B_OY * cap + pp], F[B_OZ * cap + pp],
-                               F[B_VOL * cap + pp]);
+          const uint32_t pp = (uint32_t)pr[k];
+          auto fld = [&](int f) { return F + (size_t)f * cap; };
+          geo[k] = make_float4(ld(fld(B_OX), pp), ld(fld(B_OY), pp), ld(fld(B_OZ), pp),
+                               ld(fld(B_VOL), pp));
           if constexpr (!MIX) {
-            pc[k] = b.cell[pp];
-            va[k] = make_float4(F[B_P * cap + pp], F[B_Q * cap + pp], F[B_R * cap + pp],
-                                F[B_B * cap + pp]);
-            vb[k] = F[B_H * cap + pp];  // humidity, for the total buoyancy below
+            pc[k] = ld(b.cell, pp);
+            va[k] = make_float4(ld(fld(B_P), pp), ld(fld(B_Q), pp), ld(fld(B_R), pp),
+                                ld(fld(B_B), pp));
+            vb[k] = ld(fld(B_H), pp);  // humidity, for the total buoyancy below
           } else {
-            va[k] = make_float4(F[B_B * cap + pp], F[B_H * cap + pp], F[B_P * cap + pp],
-                                F[B_Q * cap + pp]);
-            vb[k] = F[B_R * cap + pp];
+            va[k] = make_float4(ld(fld(B_B), pp), ld(fld(B_H), pp), ld(fld(B_P), pp),
+                                ld(fld(B_Q), pp));
+            vb[k] = ld(fld(B_R), pp);
           }
         }
       }
@@ -301,7 +345,7 @@ __global__ void __launch_bounds__(256) k32_cell_accum(Bulk32 b, int64_t cap, Geo
         float4 v = va[k];
         if constexpr (!MIX) {
           // total buoyancy (vorticity_tendency.F90:121-126)
-          const int kc = pc[k] / (g.nx * g.ny);
+          const int kc = pc[k] >> 20;
           const float z = zm.zbot + ((float)kc + geo[k].z) * zm.dz;
           const float hl = vb[k] - expf(-z);
           v.w = v.w + 12.5f * (hl > 0.f ? hl : 0.f);
@@ -337,9 +381,9 @@ __global__ void __launch_bounds__(256) k32_cell_accum(Bulk32 b, int64_t cap, Geo
       const int r = sperm[a];
       const float4 gr = sgeo[r];
       const float4 v = sva[r];
-      const float fx = bx ? gr.x : 1 - gr.x;
-      const float fy = by ? gr.y : 1 - gr.y;
-      const float fz = bz ? gr.z : 1 - gr.z;
+      const float fx = fmaf(sx, gr.x, tx);
+      const float fy = fmaf(sy, gr.y, ty);
+      const float fz = fmaf(sz, gr.z, tz);
       const float w = fz * fy * fx * gr.w;
       float vals[5] = {v.x, v.y, v.z, v.w, 0.f};
       if constexpr (MIX) vals[4] = svb[r];
@@ -367,7 +411,7 @@ __global__ void __launch_bounds__(256) k32_cell_accum(Bulk32 b, int64_t cap, Geo
       const float vol = F[B_VOL * cap + pp];
       float vals[5];
       if constexpr (!MIX) {
-        const int kc = b.cell[pp] / (g.nx * g.ny);
+        const int kc = b.cell[pp] >> 20;
         const float z = zm.zbot + ((float)kc + oz) * zm.dz;
         const float hl = F[B_H * cap + pp] - expf(-z);
         vals[0] = F[B_P * cap + pp];
@@ -463,6 +507,11 @@ __global__ void k32_grid_from_f64(const double *__restrict__ a, float *__restric
   if (t < n) b[t] = (float)a[t];
 }
 
+__global__ void k32_grid_to_f64(const float *__restrict__ a, double *__restrict__ b, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) b[t] = (double)a[t];
+}
+
 // ---- RK4 stage: gather the six tendencies, update, place (wrap x, y; reflect z) ----
 struct Grid32Set {
   const float *g[6];
@@ -476,11 +525,13 @@ template <int STAGE>
 __global__ void __launch_bounds__(TPB) k32_rk_stage(Geometry g, Bulk32 b, int64_t cap, int64_t n,
                                                     Grid32Set gs, Scale32 sc3, int stretch,
                                                     DevScalars *sc) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
+  const int64_t p64 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p64 >= n) return;
+  const uint32_t p = (uint32_t)p64;
   float *F = b.f;
-  const int32_t cell = b.cell[p];
-  const float ox = F[B_OX * cap + p], oy = F[B_OY * cap + p], oz = F[B_OZ * cap + p];
+  auto fld = [&](int f) { return F + (size_t)f * cap; };
+  const int32_t cell = ld(b.cell, p);
+  const float ox = ld(fld(B_OX), p), oy = ld(fld(B_OY), p), oz = ld(fld(B_OZ), p);
   int32_t cello;
   float oo[3], df[3] = {0.f, 0.f, 0.f}, vo[3] = {0.f, 0.f, 0.f}, vf[3] = {0.f, 0.f, 0.f};
   if (STAGE == 1) {
@@ -489,18 +540,18 @@ __global__ void __launch_bounds__(TPB) k32_rk_stage(Geometry g, Bulk32 b, int64_
     oo[1] = oy;
     oo[2] = oz;
 #pragma unroll
-    for (int a = 0; a < 3; a++) vo[a] = F[(B_P + a) * cap + p];
+    for (int a = 0; a < 3; a++) vo[a] = ld(fld(B_P + a), p);
   } else {
-    cello = b.cello[p];
+    cello = ld(b.cello, p);
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-      oo[a] = F[(B_OXO + a) * cap + p];
-      df[a] = F[(B_DFX + a) * cap + p];
-      if (STAGE != 4) vo[a] = F[(B_PO + a) * cap + p];
-      vf[a] = F[(B_PF + a) * cap + p];
+      oo[a] = ld(fld(B_OXO + a), p);
+      df[a] = ld(fld(B_DFX + a), p);
+      if (STAGE != 4) vo[a] = ld(fld(B_PO + a), p);
+      vf[a] = ld(fld(B_PF + a), p);
     }
   }
-  const float s_old = (STAGE == 4 && stretch) ? F[B_STRETCH * cap + p] : 0.f;
+  const float s_old = (STAGE == 4 && stretch) ? ld(fld(B_STRETCH), p) : 0.f;
   int i, j, k;
   cell_ijk(g, cell, i, j, k);
   int32_t o[8];
@@ -545,34 +596,34 @@ __global__ void __launch_bounds__(TPB) k32_rk_stage(Geometry g, Bulk32 b, int64_
     raise_status(sc, PMPIC_ERR_OUT_OF_BOX_Z);
     return;
   }
-  b.cell[p] = ncell_;
-  F[B_OX * cap + p] = nox;
-  F[B_OY * cap + p] = noy;
-  F[B_OZ * cap + p] = noz;
+  st(b.cell, p, ncell_);
+  st(fld(B_OX), p, nox);
+  st(fld(B_OY), p, noy);
+  st(fld(B_OZ), p, noz);
 #pragma unroll
-  for (int a = 0; a < 3; a++) F[(B_P + a) * cap + p] = vn[a];
+  for (int a = 0; a < 3; a++) st(fld(B_P + a), p, vn[a]);
   if (STAGE == 1) {
-    b.cello[p] = cello;
+    st(b.cello, p, cello);
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-      F[(B_OXO + a) * cap + p] = oo[a];
-      F[(B_PO + a) * cap + p] = vo[a];
+      st(fld(B_OXO + a), p, oo[a]);
+      st(fld(B_PO + a), p, vo[a]);
     }
   }
   if (STAGE != 4) {
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-      F[(B_DFX + a) * cap + p] = dfn[a];
-      F[(B_PF + a) * cap + p] = vfn[a];
+      st(fld(B_DFX + a), p, dfn[a]);
+      st(fld(B_PF + a), p, vfn[a]);
     }
   } else {
 #pragma unroll
-    for (int f = 0; f < 6; f++) F[(B_DXDT + f) * cap + p] = kv[f];
+    for (int f = 0; f < 6; f++) st(fld(B_DXDT + f), p, kv[f]);
     if (stretch) {
       const float third = 1.f / 3.f;
       float ds = vn[0] * kv[3] + vn[1] * kv[4] + vn[2] * kv[5];
       ds = powf(fabsf(ds), third);
-      F[B_STRETCH * cap + p] = s_old + ds * dt;
+      st(fld(B_STRETCH), p, s_old + ds * dt);
     }
   }
 }
@@ -692,7 +743,7 @@ __global__ void k32_mix_new(Geometry g, Bulk32 b, int64_t cap, int64_t norig,
   }
   const float v = s6[0];
   float *F = b.f;
-  b.cell[s] = (k * g.ny + j) * g.nx + i;
+  b.cell[s] = pack_cell(i, j, k);
   F[B_OX * cap + s] = 0.5f;  // the cell centre
   F[B_OY * cap + s] = 0.5f;
   F[B_OZ * cap + s] = 0.5f;
@@ -738,16 +789,18 @@ __global__ void k32_mix_residual(Geometry g, Mix32 mg, const uint8_t *__restrict
 
 __global__ void k32_mix_absorb(Geometry g, Bulk32 b, int64_t cap, int64_t norig,
                                const uint8_t *__restrict__ keep, Mix32 mg) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= norig || keep[p] != 1) return;
+  const int64_t p64 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p64 >= norig || keep[p64] != 1) return;
+  const uint32_t p = (uint32_t)p64;
   float *F = b.f;
+  auto fp = [&](int f) { return F + (size_t)f * cap; };
   const int fld[5] = {B_B, B_H, B_P, B_Q, B_R};
-  const int32_t cell = b.cell[p];
-  const float ox = F[B_OX * cap + p], oy = F[B_OY * cap + p], oz = F[B_OZ * cap + p];
-  const float v = F[B_VOL * cap + p];
+  const int32_t cell = ld(b.cell, p);
+  const float ox = ld(fp(B_OX), p), oy = ld(fp(B_OY), p), oz = ld(fp(B_OZ), p);
+  const float v = ld(fp(B_VOL), p);
   float fv[5];
 #pragma unroll
-  for (int f = 0; f < 5; f++) fv[f] = F[(int64_t)fld[f] * cap + p];
+  for (int f = 0; f < 5; f++) fv[f] = ld(fp(fld[f]), p);
   int i, j, k;
   cell_ijk(g, cell, i, j, k);
   int32_t o[8];
@@ -762,21 +815,26 @@ __global__ void k32_mix_absorb(Geometry g, Bulk32 b, int64_t cap, int64_t norig,
   w[6] = (oz) * (oy) * (1 - ox) * v;
   w[7] = (oz) * (oy) * (ox) * v;
 #pragma unroll
-  for (int c = 0; c < 8; c++) g0[c] = mg.G[0][o[c]];
+  for (int c = 0; c < 8; c++) g0[c] = ld(mg.G[0], (uint32_t)o[c]);
+  // w / vol_g once per corner, 1 / vol_new once per parcel
+  float wg[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) wg[c] = w[c] / g0[c];
   float vn = v;
 #pragma unroll
-  for (int c = 0; c < 8; c++) vn = vn + w[c] * mg.R[0][o[c]] / g0[c];
+  for (int c = 0; c < 8; c++) vn = vn + wg[c] * ld((const float *)mg.R[0], (uint32_t)o[c]);
+  const float ivn = 1.f / vn;
   float out[5];
 #pragma unroll
   for (int f = 0; f < 5; f++) {
     float acc = fv[f] * v;
 #pragma unroll
-    for (int c = 0; c < 8; c++) acc = acc + w[c] / g0[c] * mg.R[f + 1][o[c]];
-    out[f] = acc / vn;
+    for (int c = 0; c < 8; c++) acc = acc + wg[c] * ld((const float *)mg.R[f + 1], (uint32_t)o[c]);
+    out[f] = acc * ivn;
   }
-  F[B_VOL * cap + p] = vn;
+  st(fp(B_VOL), p, vn);
 #pragma unroll
-  for (int f = 0; f < 5; f++) F[(int64_t)fld[f] * cap + p] = out[f];
+  for (int f = 0; f < 5; f++) st(fp(fld[f]), p, out[f]);
 }
 
 // backfill: the state fields of the j-th kept slot from the end into the j-th removed slot
@@ -812,11 +870,11 @@ int bin32(Ctx *c, int64_t n) {
   hipLaunchKernelGGL(k32_fill_i32, dim3(cdiv(nc1, TPB)), dim3(TPB), 0, c->stream, c->cell_count,
                      nc1, 0);
   if (n > 0)
-    hipLaunchKernelGGL(k32_bin_count, dim3(cdiv(n, TPB)), dim3(TPB), 0, c->stream, c->b32.cell, n,
-                       c->rank, c->cell_count);
+    hipLaunchKernelGGL(k32_bin_count, dim3(cdiv(n, (int64_t)TPB * BIN_SEQ)), dim3(TPB), 0,
+                       c->stream, c->g, c->b32.cell, n, c->rank, c->cell_count);
   PMPIC_TRY(scan_exclusive_i32(c, c->cell_count, c->cell_start, nc1));
   if (n > 0)
-    hipLaunchKernelGGL(k32_bin_scatter, dim3(cdiv(n, TPB)), dim3(TPB), 0, c->stream, c->b32.cell,
+    hipLaunchKernelGGL(k32_bin_scatter, dim3(cdiv(n, TPB)), dim3(TPB), 0, c->stream, c->g, c->b32.cell,
                        c->rank, c->cell_start, n, c->sorted);
   hipLaunchKernelGGL(k32_sort_big, dim3(cdiv(c->ncell, TPB)), dim3(TPB), 0, c->stream,
                      c->cell_start, c->ncell, ROWS32, c->sorted);
@@ -1000,6 +1058,12 @@ int b32_mix(Ctx *c) {
                      c->stream, g, partial32(c), c->ncell, kept);
   hipLaunchKernelGGL((k32_node_sum<5, false, false, float>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0,
                      c->stream, g, partial32b(c), c->ncell, res);
+  // the reference builds the kept grids in the model's vol, b, hg, p, q, r grids
+  // (parcel_mixing.F90:131), which is what a grid export shows after mixing
+  const GridId kept_ids[6] = {GVOL, GB, GHG, GP, GQ, GR};
+  for (int f = 0; f < 6; f++)
+    hipLaunchKernelGGL(k32_grid_to_f64, dim3(cdiv(nnode, TPB)), dim3(TPB), 0, c->stream,
+                       b.mgrid[f], c->grid[kept_ids[f]], nnode);
   if (norig > 0)
     hipLaunchKernelGGL(k32_keep_flags, dim3(cdiv(norig, TPB)), dim3(TPB), 0, c->stream, b.f,
                        c->cap, norig, vmin, c->flag);

@@ -234,6 +234,8 @@ int create(const pmpic_config *cf, Ctx *c) {
                                   ? (int64_t)g.ny * (g.nx / 2 + 1)
                                   : (int64_t)(g.ny / 2 + 1) * g.nx);
   if (c->gsz > (int64_t)INT32_MAX) return PMPIC_ERR_ARG;
+  // the FP32-bulk form packs a cell as k << 20 | j << 10 | i
+  if (c->precision == 1 && (g.nx > 1024 || g.ny > 1024 || g.ncz > 2047)) return PMPIC_ERR_ARG;
   const int64_t big = c->cap > c->ncell ? c->cap : c->ncell;
   if (c->precision == 1) PMPIC_TRY(b32_alloc(c));
   else PMPIC_TRY(dalloc(&c->par, (int64_t)NFIELDS * c->cap));

@@ -38,34 +38,43 @@ __global__ void k_fill_f64(double *a, int64_t n, double v) {
 
 // cell of every parcel and its slot in the cell's bucket.  Parcels that are neighbours in slot
 // order usually share a cell, so lanes are grouped into runs of equal cell and one lane per run
-// takes the whole run's slots with a single integer atomic.
+// takes the whole run's slots with a single integer atomic.  Each wavefront walks BIN_SEQ
+// consecutive groups of 64 parcels in order, so a cell whose parcels are consecutive in slot
+// order gets ascending bucket slots unless it straddles two wavefronts' ranges.
+constexpr int BIN_SEQ = 16;
+
 __global__ void k_bin_count(Geometry g, const double *__restrict__ X, const double *__restrict__ Y,
                             const double *__restrict__ Z, int64_t n, int32_t *__restrict__ cell,
                             int32_t *__restrict__ rank, int32_t *__restrict__ count,
                             DevScalars *sc) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
-  int32_t c = -1;
-  if (p < n) {
-    int i0, j0, k0;
-    double dx_, dy_, dz_;
-    const int st = locate(g, X[p], Y[p], Z[p], i0, j0, k0, dx_, dy_, dz_);
-    if (st != PMPIC_OK) raise_status(sc, st);
-    else c = (int32_t)(((int64_t)k0 * g.ny + j0) * g.nx + i0);
-  }
-  const int32_t cprev = __shfl_up(c, 1);
-  const bool head = (lane == 0) || (c != cprev);
-  const unsigned long long hm = __ballot(head);
-  const unsigned long long upto = hm & ((2ull << lane) - 1ull);
-  const int hl = 63 - __clzll(upto);
-  const unsigned long long above = hm & ~((2ull << hl) - 1ull);
-  const int next = above ? (__ffsll(above) - 1) : 64;
-  int base = 0;
-  if (head && c >= 0) base = atomicAdd(&count[c], next - hl);
-  base = __shfl(base, hl);
-  if (p < n) {
-    cell[p] = c;
-    rank[p] = base + (lane - hl);
+  const int64_t w0 = (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64 * BIN_SEQ;
+  for (int it = 0; it < BIN_SEQ; it++) {
+    const int64_t g0 = w0 + (int64_t)it * 64;
+    if (g0 >= n) break;  // uniform across the wavefront
+    const int64_t p = g0 + lane;
+    int32_t c = -1;
+    if (p < n) {
+      int i0, j0, k0;
+      double dx_, dy_, dz_;
+      const int st = locate(g, X[p], Y[p], Z[p], i0, j0, k0, dx_, dy_, dz_);
+      if (st != PMPIC_OK) raise_status(sc, st);
+      else c = (int32_t)(((int64_t)k0 * g.ny + j0) * g.nx + i0);
+    }
+    const int32_t cprev = __shfl_up(c, 1);
+    const bool head = (lane == 0) || (c != cprev);
+    const unsigned long long hm = __ballot(head);
+    const unsigned long long upto = hm & ((2ull << lane) - 1ull);
+    const int hl = 63 - __clzll(upto);
+    const unsigned long long above = hm & ~((2ull << hl) - 1ull);
+    const int next = above ? (__ffsll(above) - 1) : 64;
+    int base = 0;
+    if (head && c >= 0) base = atomicAdd(&count[c], next - hl);
+    base = __shfl(base, hl);
+    if (p < n) {
+      cell[p] = c;
+      rank[p] = base + (lane - hl);
+    }
   }
 }
 
@@ -174,7 +183,9 @@ __global__ void __launch_bounds__(256) k_cell_accum_lds(Geometry g, const double
   while (first < ntile) {
     // the group of cells [first, last) whose rows fit in LDS (at least one cell)
     int last = first + 1;
-    while (last < ntile && sstart[last + 1] - sstart[first] <= ROWS) last++;
+    if (sstart[ntile] - sstart[first] <= ROWS) last = ntile;  // the usual case: the rest fits
+    else
+      while (last < ntile && sstart[last + 1] - sstart[first] <= ROWS) last++;
     const int32_t r0 = sstart[first];
     const int nrows = sstart[last] - r0;
     const bool fits = nrows <= ROWS;
@@ -406,7 +417,7 @@ int bin_parcels(Ctx *c, int64_t n) {
   hipLaunchKernelGGL(k_fill_i32, dim3(cdiv(nc1, TPB)), dim3(TPB), 0, c->stream, c->cell_count,
                      nc1, 0);
   if (n > 0)
-    hipLaunchKernelGGL(k_bin_count, dim3(cdiv(n, TPB)), dim3(TPB), 0, c->stream, g,
+    hipLaunchKernelGGL(k_bin_count, dim3(cdiv(n, (int64_t)TPB * BIN_SEQ)), dim3(TPB), 0, c->stream, g,
                        c->par + FX * c->cap, c->par + FY * c->cap, c->par + FZ * c->cap, n,
                        c->cell, c->rank, c->cell_count, c->dsc);
   PMPIC_TRY(scan_exclusive_i32(c, c->cell_count, c->cell_start, nc1));
