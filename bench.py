@@ -101,6 +101,22 @@ def cpu_baseline(steps):
                       f"({len(ic['x'])} parcels), {steps} steps, single thread, {dt:.1f} s"}
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_thermal128_27ppc.txt")
+
+
+def measured_traffic(form, family, n, nz, ppcd):
+    """HBM bytes per launch of a kernel family from the committed rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel calibration); None when the summary
+    does not cover this workload."""
+    if (n, nz, ppcd) != (128, 128, 3) or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        parts = line.split()
+        if len(parts) == 7 and parts[0] == "traffic" and parts[1] == form and parts[2] == family:
+            return float(parts[6])
+    return None
+
+
 def time_form(precision, ic, args, n, nz):
     """Warm up, then time args.steps whole steps of one numeric form; returns the figures."""
     from pmpic_amd.lib import Context
@@ -128,24 +144,28 @@ def time_form(precision, ic, args, n, nz):
     ncell = n * n * nz
     nnode = n * n * (nz + 1)
     avg_np = total / max(args.steps, 1)
+    form = "fp32" if precision else "fp64"
     if p2g_ms >= rk_ms:
         name = "k_cell_accum (par2grid per-cell corner sums)" if not precision else \
             "k32_cell_accum (FP32 par2grid per-cell corner sums)"
+        family = "k32_cell_accum<0>" if precision else "k_cell_accum<0>"
         bp, bc = ACC_BYTES[precision]
         per_launch = avg_np * bp + ncell * bc
         avg_ms = p2g_ms / max(p2g_calls, 1)
     else:
         name = "k_rk_stage (grid2par of 6 fields + RK4 stage update)" if not precision else \
             "k32_rk_stage (FP32 grid2par of 6 fields + RK4 stage update)"
+        family = "k32_rk_stage" if precision else "k_rk_stage"
         table = RK32_STAGE_BYTES if precision else RK_STAGE_BYTES
         per_launch = avg_np * sum(table.values()) / 4 + nnode * GATHER_NODE_BYTES[precision]
         avg_ms = rk_ms / max(rk_calls, 1)
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
+    traffic = measured_traffic(form, family, n, nz, args.ppc)
     return {
         "value": total / elapsed,
         "ms_per_step": 1e3 * elapsed / max(args.steps, 1),
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "avg_launch_ms": avg_ms, "bytes_per_launch": per_launch,
                      "p2g_ms_total": p2g_ms, "p2g_launches": p2g_calls,
                      "rk_ms_total": rk_ms, "rk_launches": rk_calls},

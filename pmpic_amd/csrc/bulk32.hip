@@ -203,10 +203,11 @@ __global__ void k32_bin_count(Geometry g, const int32_t *__restrict__ cellv, int
                               int32_t *__restrict__ rank, int32_t *__restrict__ count) {
   const int lane = threadIdx.x & 63;
   const int64_t w0 = (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64 * BIN_SEQ;
+  // all atomics of the wavefront are issued before any result is used, in group order
+  int base[BIN_SEQ], hls[BIN_SEQ];
+#pragma unroll
   for (int it = 0; it < BIN_SEQ; it++) {
-    const int64_t g0 = w0 + (int64_t)it * 64;
-    if (g0 >= n) break;  // uniform across the wavefront
-    const int64_t p = g0 + lane;
+    const int64_t p = w0 + (int64_t)it * 64 + lane;
     const int32_t c = (p < n) ? linear_cell(g, cellv[p]) : -1;
     const int32_t cprev = __shfl_up(c, 1);
     const bool head = (lane == 0) || (c != cprev);
@@ -215,10 +216,14 @@ __global__ void k32_bin_count(Geometry g, const int32_t *__restrict__ cellv, int
     const int hl = 63 - __clzll(upto);
     const unsigned long long above = hm & ~((2ull << hl) - 1ull);
     const int next = above ? (__ffsll(above) - 1) : 64;
-    int base = 0;
-    if (head && c >= 0) base = atomicAdd(&count[c], next - hl);
-    base = __shfl(base, hl);
-    if (p < n) rank[p] = base + (lane - hl);
+    base[it] = (head && c >= 0) ? atomicAdd(&count[c], next - hl) : 0;
+    hls[it] = hl;
+  }
+#pragma unroll
+  for (int it = 0; it < BIN_SEQ; it++) {
+    const int64_t p = w0 + (int64_t)it * 64 + lane;
+    const int b = __shfl(base[it], hls[it]);
+    if (p < n) rank[p] = b + (lane - hls[it]);
   }
 }
 

@@ -49,10 +49,11 @@ __global__ void k_bin_count(Geometry g, const double *__restrict__ X, const doub
                             DevScalars *sc) {
   const int lane = threadIdx.x & 63;
   const int64_t w0 = (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64 * BIN_SEQ;
+  // all atomics of the wavefront are issued before any result is used, in group order
+  int base[BIN_SEQ], hls[BIN_SEQ];
+#pragma unroll
   for (int it = 0; it < BIN_SEQ; it++) {
-    const int64_t g0 = w0 + (int64_t)it * 64;
-    if (g0 >= n) break;  // uniform across the wavefront
-    const int64_t p = g0 + lane;
+    const int64_t p = w0 + (int64_t)it * 64 + lane;
     int32_t c = -1;
     if (p < n) {
       int i0, j0, k0;
@@ -60,6 +61,7 @@ __global__ void k_bin_count(Geometry g, const double *__restrict__ X, const doub
       const int st = locate(g, X[p], Y[p], Z[p], i0, j0, k0, dx_, dy_, dz_);
       if (st != PMPIC_OK) raise_status(sc, st);
       else c = (int32_t)(((int64_t)k0 * g.ny + j0) * g.nx + i0);
+      cell[p] = c;
     }
     const int32_t cprev = __shfl_up(c, 1);
     const bool head = (lane == 0) || (c != cprev);
@@ -68,13 +70,14 @@ __global__ void k_bin_count(Geometry g, const double *__restrict__ X, const doub
     const int hl = 63 - __clzll(upto);
     const unsigned long long above = hm & ~((2ull << hl) - 1ull);
     const int next = above ? (__ffsll(above) - 1) : 64;
-    int base = 0;
-    if (head && c >= 0) base = atomicAdd(&count[c], next - hl);
-    base = __shfl(base, hl);
-    if (p < n) {
-      cell[p] = c;
-      rank[p] = base + (lane - hl);
-    }
+    base[it] = (head && c >= 0) ? atomicAdd(&count[c], next - hl) : 0;
+    hls[it] = hl;
+  }
+#pragma unroll
+  for (int it = 0; it < BIN_SEQ; it++) {
+    const int64_t p = w0 + (int64_t)it * 64 + lane;
+    const int b = __shfl(base[it], hls[it]);
+    if (p < n) rank[p] = b + (lane - hls[it]);
   }
 }
 

@@ -144,26 +144,54 @@ __global__ void k_correct_c_filter(Geometry g, const double *__restrict__ a,
 // mean flow, vort2vel.F90:230-241 (sequential over z; 1./nz is a default-real quotient)
 __global__ void k_mean_flow(Geometry g, const double *__restrict__ d, const double *__restrict__ e,
                             double *__restrict__ ubar, double *__restrict__ vbar) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  // the (1,1) columns are staged in LDS by the whole block; one thread then runs the
+  // reference's sequential recurrences and sums on them, and the block writes the result
+  extern __shared__ double sm[];
   const int64_t plane = (int64_t)g.sy * g.sx;
   const int nz = g.nzn;
-  const double dz = g.dz;
-  ubar[0] = 0;
-  vbar[0] = 0;
-  for (int k = 1; k <= nz - 1; k++) {
-    ubar[k] = ubar[k - 1] + dz / 2. * (e[(int64_t)(k - 1) * plane] + e[(int64_t)k * plane]);
-    vbar[k] = vbar[k - 1] - dz / 2. * (d[(int64_t)(k - 1) * plane] + d[(int64_t)k * plane]);
+  double *se = sm, *sd = sm + nz, *ub = sm + 2 * nz, *vb = sm + 3 * nz, *avg = sm + 4 * nz;
+  for (int k = threadIdx.x; k < nz; k += blockDim.x) {
+    se[k] = e[(int64_t)k * plane];
+    sd[k] = d[(int64_t)k * plane];
   }
-  const double rnz = (double)(1.f / (float)nz);
-  double su = 0, sv = 0;
-  for (int k = 1; k <= nz - 1; k++) {
-    su += ubar[k - 1] + 0.5 * ubar[nz - 1];
-    sv += vbar[k - 1] + 0.5 * vbar[nz - 1];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double dz = g.dz;
+    ub[0] = 0;
+    vb[0] = 0;
+    for (int k = 1; k <= nz - 1; k++) {
+      ub[k] = ub[k - 1] + dz / 2. * (se[k - 1] + se[k]);
+      vb[k] = vb[k - 1] - dz / 2. * (sd[k - 1] + sd[k]);
+    }
+    const double rnz = (double)(1.f / (float)nz);
+    double su = 0, sv = 0;
+    for (int k = 1; k <= nz - 1; k++) {
+      su += ub[k - 1] + 0.5 * ub[nz - 1];
+      sv += vb[k - 1] + 0.5 * vb[nz - 1];
+    }
+    avg[0] = rnz * su;
+    avg[1] = rnz * sv;
   }
-  const double uavg = rnz * su, vavg = rnz * sv;
-  for (int k = 0; k < nz; k++) {
-    ubar[k] = ubar[k] - uavg;
-    vbar[k] = vbar[k] - vavg;
+  __syncthreads();
+  for (int k = threadIdx.x; k < nz; k += blockDim.x) {
+    ubar[k] = ub[k] - avg[0];
+    vbar[k] = vb[k] - avg[1];
+  }
+}
+
+// Column stage-in with many loads in flight: a chunk of LVL levels is read into registers
+// (independent loads, issued back to back) before any is written to the LDS slice.
+constexpr int LVL = 16;
+__device__ __forceinline__ void column_to_lds(const double *__restrict__ f, int64_t plane, int nz,
+                                              double *F, int nc) {
+  for (int k0 = 0; k0 < nz; k0 += LVL) {
+    double t[LVL];
+#pragma unroll
+    for (int u = 0; u < LVL; u++)
+      if (k0 + u < nz) t[u] = f[(int64_t)(k0 + u) * plane];
+#pragma unroll
+    for (int u = 0; u < LVL; u++)
+      if (k0 + u < nz) F[(k0 + u) * nc] = t[u];
   }
 }
 
@@ -201,6 +229,7 @@ __device__ __forceinline__ int laplinv_column(const Geometry &g, C F, C CP, doub
   double fprev = d1 / denominator;
   F[0] = fprev;
   double cprev = c1;  // c(i-1)
+#pragma unroll 4
   for (int k = 2; k <= nz; k++) {
     fm = f0;
     f0 = fp;
@@ -224,6 +253,7 @@ __device__ __forceinline__ int laplinv_column(const Geometry &g, C F, C CP, doub
     cprev = c;
   }
   double fnext = fprev;
+#pragma unroll 4
   for (int k = nz - 1; k >= 1; k--) {
     const double v = F[k - 1] - CP[k] * fnext;
     F[k - 1] = v;
@@ -248,6 +278,7 @@ __device__ __forceinline__ int diffz_column(const Geometry &g, CI F, CO O, CO CP
   double fprev = d / denominator;
   O[0] = fprev;
   double cprev = bt ? cl : 0.;  // c(1)
+#pragma unroll 4
   for (int k = 2; k <= nz; k++) {
     double a, b, c, fk = 0.0;
     if (k < nz) {
@@ -269,6 +300,7 @@ __device__ __forceinline__ int diffz_column(const Geometry &g, CI F, CO O, CO CP
     fkm1 = fk;
   }
   double fnext = fprev;
+#pragma unroll 4
   for (int k = nz - 1; k >= 1; k--) {
     const double v = O[k - 1] - CP[k] * fnext;
     O[k - 1] = v;
@@ -298,7 +330,7 @@ __global__ void k_laplinv(Geometry g, ColFields cf, double *__restrict__ cp,
   if (st != PMPIC_OK) raise_status(sc, st);
 }
 
-__global__ void k_laplinv_lds(Geometry g, ColFields cf, const double *__restrict__ k2a,
+__global__ void __launch_bounds__(64) k_laplinv_lds(Geometry g, ColFields cf, const double *__restrict__ k2a,
                               DevScalars *sc) {
   extern __shared__ double sh[];
   const int nc = blockDim.x, t = threadIdx.x, nz = g.nzn;
@@ -309,8 +341,7 @@ __global__ void k_laplinv_lds(Geometry g, ColFields cf, const double *__restrict
   if (k2 == 0) return;
   double *f = cf.out[blockIdx.y] + col;
   Col<double> F{sh + t, nc}, CP{sh + (int64_t)nz * nc + t, nc};
-#pragma unroll 8
-  for (int k = 0; k < nz; k++) F[k] = f[(int64_t)k * plane];
+  column_to_lds(f, plane, nz, sh + t, nc);
   const int st = laplinv_column(g, F, CP, k2, cf.df0[blockIdx.y]);
   if (st != PMPIC_OK) {
     raise_status(sc, st);
@@ -333,7 +364,7 @@ __global__ void k_diffz(Geometry g, ColFields cf, double *__restrict__ cp, DevSc
   if (st != PMPIC_OK) raise_status(sc, st);
 }
 
-__global__ void k_diffz_lds(Geometry g, ColFields cf, DevScalars *sc) {
+__global__ void __launch_bounds__(64) k_diffz_lds(Geometry g, ColFields cf, DevScalars *sc) {
   extern __shared__ double sh[];
   const int nc = blockDim.x, t = threadIdx.x, nz = g.nzn;
   const int64_t plane = (int64_t)g.sy * g.sx;
@@ -344,8 +375,7 @@ __global__ void k_diffz_lds(Geometry g, ColFields cf, DevScalars *sc) {
   const double *f = cf.in[fi] + col;
   double *o = cf.out[fi] + col;
   Col<double> F{sh + t, nc}, CP{sh + (int64_t)nz * nc + t, nc};
-#pragma unroll 8
-  for (int k = 0; k < nz; k++) F[k] = f[(int64_t)k * plane];
+  column_to_lds(f, plane, nz, sh + t, nc);
   const int st = diffz_column(g, F, F, CP, bt, bt ? cf.bot[fi][col] : 0.0,
                               bt ? cf.top[fi][col] : 0.0);
   if (st != PMPIC_OK) {
@@ -634,8 +664,8 @@ int vort2vel_spectral(Ctx *c, int stage) {
   PMPIC_TRY(op_diffz(c, c->sf, c->sd, nullptr, nullptr));
   hipLaunchKernelGGL(k_correct_c_filter, eb, et, 0, c->stream, g, c->sa, c->sb, c->sc, c->sd,
                      c->sd, c->se, c->sf, c->filt, c->atop, c->abot, c->btop, c->bbot);
-  hipLaunchKernelGGL(k_mean_flow, dim3(1), dim3(64), 0, c->stream, g, c->sd, c->se, c->ubar,
-                     c->vbar);
+  hipLaunchKernelGGL(k_mean_flow, dim3(1), dim3(256), (size_t)(4 * g.nzn + 2) * sizeof(double),
+                     c->stream, g, c->sd, c->se, c->ubar, c->vbar);
   {
     double *fl[3] = {c->sa, c->sb, c->sc};
     const int df0[3] = {0, 0, 1};
