@@ -82,6 +82,12 @@ int pmpic_parcel_mixing(pmpic_ctx *ctx);        /* parcel_mixing.F90:68-431 */
 /* nsteps full timesteps: the integrator group rksteps times, then time += dtm
  * (monc.F90:216-228, timestepper.F90:89-106) */
 int pmpic_step(pmpic_ctx *ctx, int32_t nsteps);
+/* Ensemble batching: nsteps timesteps of `count` independent contexts (members of any grid size
+ * and either precision) at once, one host thread per member, each on its own HIP stream, so
+ * the members' kernels overlap on the GPU.  Returns the first non-zero status in member order.
+ * The contexts must be distinct.  No reference counterpart: the reference runs one model per
+ * MPI job (monc.F90:200-235). */
+int pmpic_step_concurrent(pmpic_ctx *const *ctxs, int32_t count, int32_t nsteps);
 
 /* parcel_interpolation_mod entry points used by other components
  * (parcel_interpolation.F90:186-262, 267-332, 336-481); host arrays */

@@ -61,6 +61,18 @@ template <typename T>
 __device__ __forceinline__ void st(T *base, uint32_t i, T v) {
   *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + i * (uint32_t)sizeof(T)) = v;
 }
+// streaming forms for parcel fields that one kernel reads or writes once: they bypass the
+// caches' retention so the grids being gathered stay resident
+template <typename T>
+__device__ __forceinline__ T ldnt(const T *base, uint32_t i) {
+  return __builtin_nontemporal_load(reinterpret_cast<const T *>(
+      reinterpret_cast<const char *>(base) + i * (uint32_t)sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ void stnt(T *base, uint32_t i, T v) {
+  __builtin_nontemporal_store(v, reinterpret_cast<T *>(reinterpret_cast<char *>(base) +
+                                                       i * (uint32_t)sizeof(T)));
+}
 
 // fold an offset outside [0,1) into the cell index
 __device__ __forceinline__ void fold1(int &i, float &o) {
@@ -535,8 +547,8 @@ __global__ void __launch_bounds__(TPB) k32_rk_stage(Geometry g, Bulk32 b, int64_
   const uint32_t p = (uint32_t)p64;
   float *F = b.f;
   auto fld = [&](int f) { return F + (size_t)f * cap; };
-  const int32_t cell = ld(b.cell, p);
-  const float ox = ld(fld(B_OX), p), oy = ld(fld(B_OY), p), oz = ld(fld(B_OZ), p);
+  const int32_t cell = ldnt(b.cell, p);
+  const float ox = ldnt(fld(B_OX), p), oy = ldnt(fld(B_OY), p), oz = ldnt(fld(B_OZ), p);
   int32_t cello;
   float oo[3], df[3] = {0.f, 0.f, 0.f}, vo[3] = {0.f, 0.f, 0.f}, vf[3] = {0.f, 0.f, 0.f};
   if (STAGE == 1) {
@@ -545,18 +557,18 @@ __global__ void __launch_bounds__(TPB) k32_rk_stage(Geometry g, Bulk32 b, int64_
     oo[1] = oy;
     oo[2] = oz;
 #pragma unroll
-    for (int a = 0; a < 3; a++) vo[a] = ld(fld(B_P + a), p);
+    for (int a = 0; a < 3; a++) vo[a] = ldnt(fld(B_P + a), p);
   } else {
-    cello = ld(b.cello, p);
+    cello = ldnt(b.cello, p);
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-      oo[a] = ld(fld(B_OXO + a), p);
-      df[a] = ld(fld(B_DFX + a), p);
-      if (STAGE != 4) vo[a] = ld(fld(B_PO + a), p);
-      vf[a] = ld(fld(B_PF + a), p);
+      oo[a] = ldnt(fld(B_OXO + a), p);
+      df[a] = ldnt(fld(B_DFX + a), p);
+      if (STAGE != 4) vo[a] = ldnt(fld(B_PO + a), p);
+      vf[a] = ldnt(fld(B_PF + a), p);
     }
   }
-  const float s_old = (STAGE == 4 && stretch) ? ld(fld(B_STRETCH), p) : 0.f;
+  const float s_old = (STAGE == 4 && stretch) ? ldnt(fld(B_STRETCH), p) : 0.f;
   int i, j, k;
   cell_ijk(g, cell, i, j, k);
   int32_t o[8];
@@ -601,34 +613,34 @@ __global__ void __launch_bounds__(TPB) k32_rk_stage(Geometry g, Bulk32 b, int64_
     raise_status(sc, PMPIC_ERR_OUT_OF_BOX_Z);
     return;
   }
-  st(b.cell, p, ncell_);
-  st(fld(B_OX), p, nox);
-  st(fld(B_OY), p, noy);
-  st(fld(B_OZ), p, noz);
+  stnt(b.cell, p, ncell_);
+  stnt(fld(B_OX), p, nox);
+  stnt(fld(B_OY), p, noy);
+  stnt(fld(B_OZ), p, noz);
 #pragma unroll
-  for (int a = 0; a < 3; a++) st(fld(B_P + a), p, vn[a]);
+  for (int a = 0; a < 3; a++) stnt(fld(B_P + a), p, vn[a]);
   if (STAGE == 1) {
-    st(b.cello, p, cello);
+    stnt(b.cello, p, cello);
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-      st(fld(B_OXO + a), p, oo[a]);
-      st(fld(B_PO + a), p, vo[a]);
+      stnt(fld(B_OXO + a), p, oo[a]);
+      stnt(fld(B_PO + a), p, vo[a]);
     }
   }
   if (STAGE != 4) {
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-      st(fld(B_DFX + a), p, dfn[a]);
-      st(fld(B_PF + a), p, vfn[a]);
+      stnt(fld(B_DFX + a), p, dfn[a]);
+      stnt(fld(B_PF + a), p, vfn[a]);
     }
   } else {
 #pragma unroll
-    for (int f = 0; f < 6; f++) st(fld(B_DXDT + f), p, kv[f]);
+    for (int f = 0; f < 6; f++) stnt(fld(B_DXDT + f), p, kv[f]);
     if (stretch) {
       const float third = 1.f / 3.f;
       float ds = vn[0] * kv[3] + vn[1] * kv[4] + vn[2] * kv[5];
       ds = powf(fabsf(ds), third);
-      st(fld(B_STRETCH), p, s_old + ds * dt);
+      stnt(fld(B_STRETCH), p, s_old + ds * dt);
     }
   }
 }

@@ -5,6 +5,7 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <thread>
 #include <vector>
 
 #include "../../include/pmpic_hip.h"
@@ -194,6 +195,7 @@ int free_all(Ctx *c) {
   hipFree(c->atop); hipFree(c->abot); hipFree(c->btop); hipFree(c->bbot);
   hipFree(c->ubar); hipFree(c->vbar);
   hipFree(c->kx); hipFree(c->ky); hipFree(c->k2); hipFree(c->filt); hipFree(c->coords);
+  hipFree(c->lap_den); hipFree(c->lap_cp); hipFree(c->dz_den); hipFree(c->dz_cp);
   hipFree(c->fbuf); hipFree(c->ibuf);
   hipFree(c->cbuf); hipFree(c->tw_x); hipFree(c->tw_y);
   hipFree(c->scan_tmp); hipFree(c->blk); hipFree(c->dsc);
@@ -223,6 +225,7 @@ int create(const pmpic_config *cf, Ctx *c) {
   c->prefactor = cf->timestep_prefactor;
   c->rkstep = 1;
   c->precision = cf->precision;
+  c->device = cf->device;
   c->splitting_enabled = cf->splitting_enabled;
   c->mixing_enabled = cf->mixing_enabled;
   PMPIC_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -268,6 +271,10 @@ int create(const pmpic_config *cf, Ctx *c) {
   PMPIC_TRY(dalloc(&c->ky, plane));
   PMPIC_TRY(dalloc(&c->k2, plane));
   PMPIC_TRY(dalloc(&c->filt, plane));
+  PMPIC_TRY(dalloc(&c->lap_den, 2 * (int64_t)g.nzn * plane));
+  PMPIC_TRY(dalloc(&c->lap_cp, 2 * (int64_t)g.nzn * plane));
+  PMPIC_TRY(dalloc(&c->dz_den, 2 * (int64_t)g.nzn));
+  PMPIC_TRY(dalloc(&c->dz_cp, 2 * (int64_t)g.nzn));
   PMPIC_TRY(dalloc(&c->coords, (int64_t)coords.size()));
   PMPIC_CHECK(hipMemcpy(c->coords, coords.data(), sizeof(double) * coords.size(),
                         hipMemcpyHostToDevice));
@@ -499,6 +506,30 @@ int pmpic_step(pmpic_ctx *ctx, int32_t nsteps) {
     }
     c->time = c->time + c->dtm;
   }
+  return PMPIC_OK;
+}
+
+int pmpic_step_concurrent(pmpic_ctx *const *ctxs, int32_t count, int32_t nsteps) {
+  if (!ctxs || count < 0 || nsteps < 0) return PMPIC_ERR_ARG;
+  for (int32_t i = 0; i < count; i++) {
+    if (!ctxs[i]) return PMPIC_ERR_ARG;
+    for (int32_t j = 0; j < i; j++)
+      if (ctxs[j] == ctxs[i]) return PMPIC_ERR_ARG;
+  }
+  std::vector<int> st(count, PMPIC_OK);
+  std::vector<std::thread> th;
+  th.reserve(count);
+  for (int32_t i = 0; i < count; i++)
+    th.emplace_back([&, i]() {
+      if (hipSetDevice(ctxs[i]->c.device) != hipSuccess) {
+        st[i] = PMPIC_ERR_HIP;
+        return;
+      }
+      st[i] = pmpic_step(ctxs[i], nsteps);
+    });
+  for (auto &t : th) t.join();
+  for (int32_t i = 0; i < count; i++)
+    if (st[i] != PMPIC_OK) return st[i];
   return PMPIC_OK;
 }
 

@@ -79,6 +79,7 @@ struct Ctx {
   double time, dtm, dtmax, prefactor;
   int it_v2v, it_vt, rkstep, it_split, it_mix;  // per-component iteration counters
   int precision;
+  int device;
   int splitting_enabled, mixing_enabled;
   int64_t last_split, last_removed, last_added;
   hipStream_t stream;
@@ -108,6 +109,10 @@ struct Ctx {
   double *atop, *abot, *btop, *bbot;  // [sy][sx]
   double *ubar, *vbar;                // [nzn]
   double *kx, *ky, *k2, *filt;        // [sy][sx]
+  // Thomas elimination coefficients, which depend only on the column's k^2 and the boundary
+  // condition: laplinv [df0][k][sy*sx] denominators and c' (per column); diffz [bt][k] (the
+  // same for every column)
+  double *lap_den, *lap_cp, *dz_den, *dz_cp;
   double *coords;                     // x_coords (nx+4), y_coords (ny+4), z_coords (nzn)
   double *fbuf;                       // host-array staging for standalone ops (max(gsz, ssz, cap))
   int32_t *ibuf;                      // cap int32 staging

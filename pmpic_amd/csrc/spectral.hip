@@ -386,6 +386,201 @@ __global__ void __launch_bounds__(64) k_diffz_lds(Geometry g, ColFields cf, DevS
   for (int k = 0; k < nz; k++) o[(int64_t)k * plane] = F[k];
 }
 
+// Register-resident column solvers.  The Thomas denominators and c' of a column depend only
+// on its k^2 and the boundary condition, so they are built once (k_lap_tables, k_dz_tables,
+// with the same operations as laplinv_column / diffz_column) and each call runs only the data
+// recurrences: one division per level forward, none backward.  The column itself stays in
+// registers (NZ levels, unrolled), so a full wavefront of columns runs per SIMD with no LDS.
+// Results are bit-identical to the LDS and HBM forms.
+__global__ void k_lap_tables(Geometry g, const double *__restrict__ k2a, double *__restrict__ den,
+                             double *__restrict__ cp) {
+  const int64_t plane = (int64_t)g.sy * g.sx;
+  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= plane) return;
+  const double k2 = k2a[col];
+  const int nz = g.nzn;
+  const double f16 = (double)(1.f / 6.f), f13 = (double)(1.f / 3.f);
+  const double f112 = (double)(1.f / 12.f), f56 = (double)(5.f / 6.f);
+  const double dz = g.dz;
+  const double i2dz2 = 2. / dz / dz, idz2 = 1. / dz / dz;
+  const double ai = idz2 - f112 * k2, bi = -i2dz2 - f56 * k2, ci = idz2 - f112 * k2;
+  for (int df0 = 0; df0 < 2; df0++) {
+    double *D = den + (int64_t)df0 * nz * plane + col, *C = cp + (int64_t)df0 * nz * plane + col;
+    double b1, c1;
+    if (!df0) { b1 = 1.; c1 = 0.; }
+    else { b1 = -idz2 - f13 * k2; c1 = idz2 - f16 * k2; }
+    double denominator = b1;
+    D[0] = denominator;
+    C[0] = 0.;
+    double cprev = c1;
+    for (int k = 2; k <= nz; k++) {
+      double a, b, c;
+      if (k < nz) { a = ai; b = bi; c = ci; }
+      else if (!df0) { a = 0.; b = 1.; c = 0.; }
+      else { a = idz2 - f16 * k2; b = -idz2 - f13 * k2; c = 0.; }
+      const double cpk = cprev / denominator;
+      C[(int64_t)(k - 1) * plane] = cpk;
+      denominator = b - a * cpk;
+      D[(int64_t)(k - 1) * plane] = denominator;
+      cprev = c;
+    }
+  }
+}
+
+__global__ void k_dz_tables(Geometry g, double *__restrict__ den, double *__restrict__ cp) {
+  if (threadIdx.x || blockIdx.x) return;
+  const int nz = g.nzn;
+  const double a1 = (double)(1.f / 6.f), b1 = (double)(2.f / 3.f), c1 = (double)(1.f / 6.f);
+  const double al = (double)(1.f / 3.f), cl = (double)(1.f / 3.f);
+  for (int bt = 0; bt < 2; bt++) {
+    double *D = den + bt * nz, *C = cp + bt * nz;
+    double denominator = 1.;
+    D[0] = denominator;
+    C[0] = 0.;
+    double cprev = bt ? cl : 0.;
+    for (int k = 2; k <= nz; k++) {
+      double a, b, c;
+      if (k < nz) { a = a1; b = b1; c = c1; }
+      else { a = bt ? al : 0.; b = 1.; c = 0.; }
+      const double cpk = cprev / denominator;
+      C[k - 1] = cpk;
+      denominator = b - a * cpk;
+      D[k - 1] = denominator;
+      cprev = c;
+    }
+  }
+}
+
+// levels [0, R) of the column live in registers, levels [R, NZ) (long columns only) in an LDS
+// slice of the thread; every index below is a compile-time constant after unrolling
+template <int NZ, int R>
+struct RegColumn {
+  double F[R];
+  double *S;  // LDS slice, stride 64
+  __device__ __forceinline__ double get(int k) const { return k < R ? F[k < R ? k : 0] : S[(k >= R ? k - R : 0) * 64]; }
+  __device__ __forceinline__ void set(int k, double v) {
+    if (k < R) F[k < R ? k : 0] = v;
+    else S[(k >= R ? k - R : 0) * 64] = v;
+  }
+};
+
+template <int NZ, int R>
+__global__ void __launch_bounds__(64) k_laplinv_reg(Geometry g, ColFields cf,
+                                                    const double *__restrict__ k2a,
+                                                    const double *__restrict__ tden,
+                                                    const double *__restrict__ tcp,
+                                                    DevScalars *sc) {
+  __shared__ double lds[(NZ > R ? NZ - R : 1) * 64];
+  const int64_t plane = (int64_t)g.sy * g.sx;
+  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= plane) return;
+  const double k2 = k2a[col];
+  if (k2 == 0) return;
+  const int df0 = cf.df0[blockIdx.y];
+  double *f = cf.out[blockIdx.y] + col;
+  const double *den = tden + (int64_t)df0 * NZ * plane + col;
+  const double *cpt = tcp + (int64_t)df0 * NZ * plane + col;
+  RegColumn<NZ, R> F;
+  F.S = lds + threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < NZ; k++) F.set(k, f[(int64_t)k * plane]);
+  const double f16 = (double)(1.f / 6.f), f13 = (double)(1.f / 3.f);
+  const double f112 = (double)(1.f / 12.f), f56 = (double)(5.f / 6.f);
+  const double dz = g.dz;
+  const double idz2 = 1. / dz / dz;
+  const double ai = idz2 - f112 * k2;
+  double fm = 0.0, f0 = F.get(0), fp = F.get(1);  // originals f(k-1), f(k), f(k+1)
+  const double d1 = df0 ? f13 * f0 + f16 * fp : 0.;
+  double fprev = d1 / den[0];
+  F.set(0, fprev);
+#pragma unroll
+  for (int k = 2; k <= NZ; k++) {
+    fm = f0;
+    f0 = fp;
+    fp = (k < NZ) ? F.get(k < NZ ? k : 0) : 0.0;
+    double a, d;
+    if (k < NZ) {
+      a = ai;
+      d = f112 * fm + f56 * f0 + f112 * fp;
+    } else if (!df0) {
+      a = 0.;
+      d = 0.;
+    } else {
+      a = idz2 - f16 * k2;
+      d = f13 * f0 + f16 * fm;
+    }
+    // coefficient loads are hoisted at most 16 levels ahead, which keeps the column itself
+    // in registers
+    if ((k & 31) == 0) __asm__ volatile("" ::: "memory");
+    const double dn = den[(int64_t)(k - 1) * plane];
+    if (dn == 0) { raise_status(sc, PMPIC_ERR_ZERO_PIVOT); return; }
+    fprev = (d - a * fprev) / (dn);
+    F.set(k - 1, fprev);
+  }
+  double fnext = fprev;
+#pragma unroll
+  for (int k = NZ - 1; k >= 1; k--) {
+    if ((k & 31) == 0) __asm__ volatile("" ::: "memory");
+    const double v = F.get(k - 1) - cpt[(int64_t)k * plane] * fnext;
+    F.set(k - 1, v);
+    fnext = v;
+  }
+#pragma unroll
+  for (int k = 0; k < NZ; k++) f[(int64_t)k * plane] = F.get(k);
+}
+
+template <int NZ>
+__global__ void __launch_bounds__(64) k_diffz_reg(Geometry g, ColFields cf,
+                                                  const double *__restrict__ tden,
+                                                  const double *__restrict__ tcp,
+                                                  DevScalars *sc) {
+  const int64_t plane = (int64_t)g.sy * g.sx;
+  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= plane) return;
+  const int fi = blockIdx.y;
+  const bool bt = cf.bot[fi] != nullptr && cf.top[fi] != nullptr;
+  const double *den = tden + (bt ? NZ : 0), *cpt = tcp + (bt ? NZ : 0);
+  const double *fin = cf.in[fi] + col;
+  double *o = cf.out[fi] + col;
+  double F[NZ];
+#pragma unroll
+  for (int k = 0; k < NZ; k++) F[k] = fin[(int64_t)k * plane];
+  const double bot = bt ? cf.bot[fi][col] : 0.0, top = bt ? cf.top[fi][col] : 0.0;
+  const double a1 = (double)(1.f / 6.f), al = (double)(1.f / 3.f);
+  const double dz = g.dz;
+  double fkm2 = F[0], fkm1 = F[1];  // originals f(k-2), f(k-1) of the next row
+  const double d0 = bt ? fkm1 / dz - a1 * bot * dz : 0.0;
+  double fprev = d0 / den[0];
+  F[0] = fprev;
+#pragma unroll
+  for (int k = 2; k <= NZ; k++) {
+    double a, d, fk = 0.0;
+    if (k < NZ) {
+      a = a1;
+      fk = F[k < NZ ? k : 0];
+      d = (fk - fkm2) / 2. / dz;
+    } else {
+      a = bt ? al : 0.;
+      d = bt ? -fkm2 / dz + a1 * top * dz : 0.0;
+    }
+    const double dn = den[k - 1];
+    if (dn == 0) { raise_status(sc, PMPIC_ERR_ZERO_PIVOT); return; }
+    fprev = (d - a * fprev) / (dn);
+    F[k - 1] = fprev;
+    fkm2 = fkm1;
+    fkm1 = fk;
+  }
+  double fnext = fprev;
+#pragma unroll
+  for (int k = NZ - 1; k >= 1; k--) {
+    const double v = F[k - 1] - cpt[k] * fnext;
+    F[k - 1] = v;
+    fnext = v;
+  }
+#pragma unroll
+  for (int k = 0; k < NZ; k++) o[(int64_t)k * plane] = F[k];
+}
+
 // u, v, w spectra (vort2vel.F90:251-320): fu = dzb - diffy(c), column (1,1) = ubar;
 // fv = diffx(c) - dza, column (1,1) = vbar; fw = diffy(a) - diffx(b); u_s.. = filtered
 __global__ void k_velocity_spectra(Geometry g, const double *__restrict__ a,
@@ -554,6 +749,12 @@ int spectral_setup(Ctx *c) {
   PMPIC_CHECK(hipMemcpy(c->ky, ky.data(), b, hipMemcpyHostToDevice));
   PMPIC_CHECK(hipMemcpy(c->k2, k2.data(), b, hipMemcpyHostToDevice));
   PMPIC_CHECK(hipMemcpy(c->filt, fl.data(), b, hipMemcpyHostToDevice));
+  const int64_t plane = (int64_t)sx * sy;
+  hipLaunchKernelGGL(k_lap_tables, dim3(cdiv(plane, 64)), dim3(64), 0, c->stream, g, c->k2,
+                     c->lap_den, c->lap_cp);
+  hipLaunchKernelGGL(k_dz_tables, dim3(1), dim3(1), 0, c->stream, g, c->dz_den, c->dz_cp);
+  PMPIC_CHECK(hipGetLastError());
+  PMPIC_CHECK(hipStreamSynchronize(c->stream));
   return PMPIC_OK;
 }
 
@@ -583,11 +784,33 @@ static int lds_cols(const Geometry &g) {
   return 0;
 }
 
+// column lengths (z nodes) with a register-resident solver instance
+#define PMPIC_REG_COLUMNS(X) X(9) X(17) X(33) X(65) X(129)
+
 int op_diffz_n(Ctx *c, int nf, const double *const *in, double *const *out,
                const double *const *bot, const double *const *top) {
   if (nf < 1 || nf > 4) return PMPIC_ERR_ARG;
   const int64_t plane = plane_of(c->g);
   const int nc = lds_cols(c->g);
+  ColFields rf{};
+  for (int f = 0; f < nf; f++) {
+    rf.in[f] = in[f];
+    rf.out[f] = out[f];
+    rf.bot[f] = bot ? bot[f] : nullptr;
+    rf.top[f] = top ? top[f] : nullptr;
+  }
+  switch (c->g.nzn) {
+#define PMPIC_DZ_CASE(NZ)                                                                     \
+  case NZ:                                                                                     \
+    hipLaunchKernelGGL(k_diffz_reg<NZ>, dim3(cdiv(plane, 64), nf), dim3(64), 0, c->stream,   \
+                       c->g, rf, c->dz_den, c->dz_cp, c->dsc);                                 \
+    PMPIC_CHECK(hipGetLastError());                                                            \
+    return PMPIC_OK;
+    PMPIC_REG_COLUMNS(PMPIC_DZ_CASE)
+#undef PMPIC_DZ_CASE
+    default:
+      break;
+  }
   if (nc > 0) {
     ColFields cf{};
     for (int f = 0; f < nf; f++) {
@@ -619,6 +842,23 @@ int op_laplinv_n(Ctx *c, int nf, double *const *f, const int *df0) {
   if (nf < 1 || nf > 4) return PMPIC_ERR_ARG;
   const int64_t plane = plane_of(c->g);
   const int nc = lds_cols(c->g);
+  ColFields rf{};
+  for (int q = 0; q < nf; q++) {
+    rf.out[q] = f[q];
+    rf.df0[q] = df0[q] ? 1 : 0;
+  }
+  switch (c->g.nzn) {
+#define PMPIC_LAP_CASE(NZ)                                                                    \
+  case NZ:                                                                                     \
+    hipLaunchKernelGGL((k_laplinv_reg<NZ, (NZ > 100 ? 64 : NZ)>), dim3(cdiv(plane, 64), nf),    \
+                       dim3(64), 0, c->stream, c->g, rf, c->k2, c->lap_den, c->lap_cp, c->dsc); \
+    PMPIC_CHECK(hipGetLastError());                                                            \
+    return PMPIC_OK;
+    PMPIC_REG_COLUMNS(PMPIC_LAP_CASE)
+#undef PMPIC_LAP_CASE
+    default:
+      break;
+  }
   if (nc > 0) {
     ColFields cf{};
     for (int q = 0; q < nf; q++) {

@@ -36,7 +36,7 @@ EXPORTS = [
     "pmpic_parcel_haloswap", "pmpic_get_grid", "pmpic_get_spectral", "pmpic_fft_forward",
     "pmpic_fft_backward", "pmpic_diffx", "pmpic_diffy", "pmpic_spectral_filter", "pmpic_diffz",
     "pmpic_laplinv", "pmpic_conservation_sums", "pmpic_last_counts", "pmpic_timing_enable",
-    "pmpic_timing_read",
+    "pmpic_timing_read", "pmpic_step_concurrent",
 ]
 
 
@@ -87,6 +87,7 @@ def load():
                      "pmpic_parcel_splitting", "pmpic_parcel_mixing", "pmpic_parcel_haloswap"):
             getattr(L, name).argtypes = [vp]
         L.pmpic_step.argtypes = [vp, i32]
+        L.pmpic_step_concurrent.argtypes = [pp, i32, i32]
         L.pmpic_cache_parcel_interp_weights.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.pmpic_par2grid.argtypes = [vp, vp, vp]
         L.pmpic_grid2par.argtypes = [vp, vp, vp]
@@ -123,6 +124,18 @@ def _ptrs(arrays):
     for i, a in enumerate(arrays):
         arr[i] = None if a is None else a.ctypes.data
     return arr
+
+
+def step_concurrent(contexts, nsteps=1):
+    """Advance an ensemble of distinct contexts by nsteps timesteps at once (one host thread
+    and HIP stream per member, pmpic_step_concurrent)."""
+    L = load()
+    arr = (ctypes.c_void_p * len(contexts))()
+    for i, cx in enumerate(contexts):
+        arr[i] = cx.h.value
+    st = L.pmpic_step_concurrent(arr, len(contexts), int(nsteps))
+    if st != 0:
+        raise PmpicError(st, "pmpic_step_concurrent")
 
 
 class Context:

@@ -173,3 +173,23 @@ def test_twenty_steps_counts_match_oracle(oracle_mod):
         assert rel_err(gp[k], op[k]) < 1e-9, k
     for name in ("u", "v", "w"):
         assert rel_err(interior(g.grid(name)), interior(o.grid(name))) < 1e-9, name
+
+
+@pytest.mark.parametrize("shape", [(8, 6, 16), (8, 6, 32), (6, 8, 64), (6, 4, 128), (4, 6, 40)])
+def test_column_solvers_bitwise_all_forms(oracle_mod, shape):
+    """diffz and laplinv bit-identical to the oracle for column lengths served by the
+    register-resident solvers (17, 33, 65, 129 z nodes) and by the LDS form (41)."""
+    from pmpic_amd.lib import Context
+    nx, ny, nz = shape
+    b = box(nx, ny, nz)
+    o = oracle_mod.Oracle(nx, ny, nz, b["xmin"], b["xmax"], b["ymin"], b["ymax"], b["zmin"],
+                          b["zmax"], 64, 0.01, 0.01, 0.5)
+    g = Context(nx, ny, nz, b["xmin"], b["xmax"], b["ymin"], b["ymax"], b["zmin"], b["zmax"], 64)
+    rng = np.random.default_rng(nz)
+    s = rng.standard_normal(o.sshape)
+    bot = rng.standard_normal(o.sshape[:2])
+    top = rng.standard_normal(o.sshape[:2])
+    np.testing.assert_array_equal(g.diffz(s), o.diffz(s))
+    np.testing.assert_array_equal(g.diffz(s, bot, top), o.diffz(s, bot, top))
+    for df0 in (False, True):
+        np.testing.assert_array_equal(g.laplinv(s, df0), o.laplinv(s, df0))
