@@ -341,17 +341,17 @@ __global__ void __launch_bounds__(256) k32_cell_accum(Bulk32 b, int64_t cap, Geo
         if (pr[k] >= 0) {
           const uint32_t pp = (uint32_t)pr[k];
           auto fld = [&](int f) { return F + (size_t)f * cap; };
-          geo[k] = make_float4(ld(fld(B_OX), pp), ld(fld(B_OY), pp), ld(fld(B_OZ), pp),
-                               ld(fld(B_VOL), pp));
+          geo[k] = make_float4(ldnt(fld(B_OX), pp), ldnt(fld(B_OY), pp), ldnt(fld(B_OZ), pp),
+                               ldnt(fld(B_VOL), pp));
           if constexpr (!MIX) {
-            pc[k] = ld(b.cell, pp);
-            va[k] = make_float4(ld(fld(B_P), pp), ld(fld(B_Q), pp), ld(fld(B_R), pp),
-                                ld(fld(B_B), pp));
-            vb[k] = ld(fld(B_H), pp);  // humidity, for the total buoyancy below
+            pc[k] = ldnt(b.cell, pp);
+            va[k] = make_float4(ldnt(fld(B_P), pp), ldnt(fld(B_Q), pp), ldnt(fld(B_R), pp),
+                                ldnt(fld(B_B), pp));
+            vb[k] = ldnt(fld(B_H), pp);  // humidity, for the total buoyancy below
           } else {
-            va[k] = make_float4(ld(fld(B_B), pp), ld(fld(B_H), pp), ld(fld(B_P), pp),
-                                ld(fld(B_Q), pp));
-            vb[k] = ld(fld(B_R), pp);
+            va[k] = make_float4(ldnt(fld(B_B), pp), ldnt(fld(B_H), pp), ldnt(fld(B_P), pp),
+                                ldnt(fld(B_Q), pp));
+            vb[k] = ldnt(fld(B_R), pp);
           }
         }
       }
@@ -812,12 +812,12 @@ __global__ void k32_mix_absorb(Geometry g, Bulk32 b, int64_t cap, int64_t norig,
   float *F = b.f;
   auto fp = [&](int f) { return F + (size_t)f * cap; };
   const int fld[5] = {B_B, B_H, B_P, B_Q, B_R};
-  const int32_t cell = ld(b.cell, p);
-  const float ox = ld(fp(B_OX), p), oy = ld(fp(B_OY), p), oz = ld(fp(B_OZ), p);
-  const float v = ld(fp(B_VOL), p);
+  const int32_t cell = ldnt(b.cell, p);
+  const float ox = ldnt(fp(B_OX), p), oy = ldnt(fp(B_OY), p), oz = ldnt(fp(B_OZ), p);
+  const float v = ldnt(fp(B_VOL), p);
   float fv[5];
 #pragma unroll
-  for (int f = 0; f < 5; f++) fv[f] = ld(fp(fld[f]), p);
+  for (int f = 0; f < 5; f++) fv[f] = ldnt(fp(fld[f]), p);
   int i, j, k;
   cell_ijk(g, cell, i, j, k);
   int32_t o[8];
@@ -849,9 +849,9 @@ __global__ void k32_mix_absorb(Geometry g, Bulk32 b, int64_t cap, int64_t norig,
     for (int c = 0; c < 8; c++) acc = acc + wg[c] * ld((const float *)mg.R[f + 1], (uint32_t)o[c]);
     out[f] = acc * ivn;
   }
-  st(fp(B_VOL), p, vn);
+  stnt(fp(B_VOL), p, vn);
 #pragma unroll
-  for (int f = 0; f < 5; f++) st(fp(fld[f]), p, out[f]);
+  for (int f = 0; f < 5; f++) stnt(fp(fld[f]), p, out[f]);
 }
 
 // backfill: the state fields of the j-th kept slot from the end into the j-th removed slot

@@ -194,6 +194,13 @@ __device__ __forceinline__ void corner_weights(double delx, double dely, double 
   w[7] = (delz) * (dely) * (delx) * v;
 }
 
+// streaming access for parcel fields a kernel touches once: the caches do not retain them, so
+// the grids being gathered stay resident
+template <typename T>
+__device__ __forceinline__ T ld_stream(const T *p) { return __builtin_nontemporal_load(p); }
+template <typename T>
+__device__ __forceinline__ void st_stream(T *p, T v) { __builtin_nontemporal_store(v, p); }
+
 __device__ __forceinline__ int64_t gidx(const Geometry &g, int k, int j, int i) {
   return ((int64_t)k * g.ny + j) * g.nx + i;
 }

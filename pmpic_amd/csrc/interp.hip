@@ -132,20 +132,20 @@ __device__ __forceinline__ void load_values(const double *__restrict__ par, int6
                                             const double *__restrict__ var, int64_t p, double z,
                                             double *val) {
   if constexpr (MODE == ACC_P2G) {
-    val[0] = par[FP * cap + p];
-    val[1] = par[FQ * cap + p];
-    val[2] = par[FR * cap + p];
+    val[0] = ld_stream(&par[FP * cap + p]);
+    val[1] = ld_stream(&par[FQ * cap + p]);
+    val[2] = ld_stream(&par[FR * cap + p]);
     // total buoyancy, vorticity_tendency.F90:121-126
-    const double hl = par[FH * cap + p] - exp(-z);
-    val[3] = par[FB * cap + p] + 12.5 * (hl > 0.0 ? hl : 0.0);
+    const double hl = ld_stream(&par[FH * cap + p]) - exp(-z);
+    val[3] = ld_stream(&par[FB * cap + p]) + 12.5 * (hl > 0.0 ? hl : 0.0);
   } else if constexpr (MODE == ACC_SINGLE) {
-    val[0] = var[p];
+    val[0] = ld_stream(&var[p]);
   } else {
-    val[0] = par[FB * cap + p];
-    val[1] = par[FH * cap + p];
-    val[2] = par[FP * cap + p];
-    val[3] = par[FQ * cap + p];
-    val[4] = par[FR * cap + p];
+    val[0] = ld_stream(&par[FB * cap + p]);
+    val[1] = ld_stream(&par[FH * cap + p]);
+    val[2] = ld_stream(&par[FP * cap + p]);
+    val[3] = ld_stream(&par[FQ * cap + p]);
+    val[4] = ld_stream(&par[FR * cap + p]);
   }
 }
 
@@ -208,10 +208,10 @@ __global__ void __launch_bounds__(256) k_cell_accum_lds(Geometry g, const double
         pval[k][2 * NVS - 1] = 0.0;
         if (pr[k] >= 0) {
           const int64_t pp = pr[k];
-          px[k] = X[pp];
-          py[k] = Y[pp];
-          pz[k] = Z[pp];
-          pv[k] = VOL[pp];
+          px[k] = ld_stream(&X[pp]);
+          py[k] = ld_stream(&Y[pp]);
+          pz[k] = ld_stream(&Z[pp]);
+          pv[k] = ld_stream(&VOL[pp]);
           load_values<MODE, NV>(par, cap, var, pp, pz[k], pval[k]);
         }
       }

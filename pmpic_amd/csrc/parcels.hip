@@ -98,22 +98,22 @@ __global__ void __launch_bounds__(TPB) k_rk_stage(Geometry g, double *__restrict
   if (active) {
     // every parcel value the stage reads is loaded up front, so the loads overlap each other
     // and the grid gathers instead of waiting behind the stores
-    const double x = par[FX * cap + p], y = par[FY * cap + p], z0 = par[FZ * cap + p];
+    const double x = ld_stream(&par[FX * cap + p]), y = ld_stream(&par[FY * cap + p]), z0 = ld_stream(&par[FZ * cap + p]);
     double yo[6], yf[6];
     if (STAGE == 1) {
       yo[0] = x;
       yo[1] = y;
       yo[2] = z0;
 #pragma unroll
-      for (int f = 3; f < 6; f++) yo[f] = par[(FX + f) * cap + p];
+      for (int f = 3; f < 6; f++) yo[f] = ld_stream(&par[(FX + f) * cap + p]);
     } else {
 #pragma unroll
       for (int f = 0; f < 6; f++) {
-        if (STAGE != 4) yo[f] = par[(f < 3 ? FXO + f : FPO + f - 3) * cap + p];
-        yf[f] = par[(f < 3 ? FXF + f : FPF + f - 3) * cap + p];
+        if (STAGE != 4) yo[f] = ld_stream(&par[(f < 3 ? FXO + f : FPO + f - 3) * cap + p]);
+        yf[f] = ld_stream(&par[(f < 3 ? FXF + f : FPF + f - 3) * cap + p]);
       }
     }
-    const double s_old = (STAGE == 4 && stretch) ? par[FSTRETCH * cap + p] : 0.0;
+    const double s_old = (STAGE == 4 && stretch) ? ld_stream(&par[FSTRETCH * cap + p]) : 0.0;
     int i0, j0, k0;
     double delx, dely, delz;
     const int st = locate(g, x, y, z0, i0, j0, k0, delx, dely, delz);
@@ -164,16 +164,16 @@ __global__ void __launch_bounds__(TPB) k_rk_stage(Geometry g, double *__restrict
       yv[2] = z;
 #pragma unroll
       for (int f = 0; f < 6; f++) {
-        par[(FDXDT + f) * cap + p] = kv[f];
-        if (STAGE == 1) par[(f < 3 ? FXO + f : FPO + f - 3) * cap + p] = yo[f];
-        par[(f < 3 ? FXF + f : FPF + f - 3) * cap + p] = yfn[f];
-        par[(FX + f) * cap + p] = yv[f];
+        st_stream(&par[(FDXDT + f) * cap + p], kv[f]);
+        if (STAGE == 1) st_stream(&par[(f < 3 ? FXO + f : FPO + f - 3) * cap + p], yo[f]);
+        st_stream(&par[(f < 3 ? FXF + f : FPF + f - 3) * cap + p], yfn[f]);
+        st_stream(&par[(FX + f) * cap + p], yv[f]);
       }
       if (STAGE == 4 && stretch) {
         const double third = (double)(1.f / 3.f);
         double ds = yv[3] * kv[3] + yv[4] * kv[4] + yv[5] * kv[5];
         ds = pow(fabs(ds), third);
-        par[FSTRETCH * cap + p] = s_old + ds * dt;
+        st_stream(&par[FSTRETCH * cap + p], s_old + ds * dt);
       }
       dir = direction(g, yv[0], yv[1]);
     }
@@ -451,11 +451,12 @@ __global__ void k_mix_absorb(Geometry g, double *__restrict__ par, int64_t cap, 
   if (p >= norig || keep[p] != 1) return;
   // all loads first (parcel values, then the gathers), all stores last
   const int fld[5] = {FB, FH, FP, FQ, FR};
-  const double x = par[FX * cap + p], y = par[FY * cap + p], z = par[FZ * cap + p];
-  const double v = par[FVOL * cap + p];
+  const double x = ld_stream(&par[FX * cap + p]), y = ld_stream(&par[FY * cap + p]),
+               z = ld_stream(&par[FZ * cap + p]);
+  const double v = ld_stream(&par[FVOL * cap + p]);
   double fv[5];
 #pragma unroll
-  for (int f = 0; f < 5; f++) fv[f] = par[(int64_t)fld[f] * cap + p];
+  for (int f = 0; f < 5; f++) fv[f] = ld_stream(&par[(int64_t)fld[f] * cap + p]);
   int i0, j0, k0;
   double delx, dely, delz;
   locate(g, x, y, z, i0, j0, k0, delx, dely, delz);
@@ -480,9 +481,9 @@ __global__ void k_mix_absorb(Geometry g, double *__restrict__ par, int64_t cap, 
     for (int c = 0; c < 8; c++) acc = acc + w[c] / g0[c] * mg.R[f + 1][q[c]];
     out[f] = acc / vn;
   }
-  par[FVOL * cap + p] = vn;
+  st_stream(&par[FVOL * cap + p], vn);
 #pragma unroll
-  for (int f = 0; f < 5; f++) par[(int64_t)fld[f] * cap + p] = out[f];
+  for (int f = 0; f < 5; f++) st_stream(&par[(int64_t)fld[f] * cap + p], out[f]);
 }
 
 // backfill (parcel_mixing.F90:347-389): j-th remaining removed slot <- j-th kept slot from the
