@@ -140,6 +140,8 @@ def time_form(precision, ic, args, n, nz):
     print(f"bench[{tag}]: {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
     p2g_ms, p2g_calls = ctx.timing_read(0)
     rk_ms, rk_calls = ctx.timing_read(1)
+    timers = {k: {"ms_per_step": v[0] / max(args.steps, 1), "count": v[1]}
+              for k, v in ctx.timer_table().items()}
     ctx.close()
     ncell = n * n * nz
     nnode = n * n * (nz + 1)
@@ -169,6 +171,7 @@ def time_form(precision, ic, args, n, nz):
                      "avg_launch_ms": avg_ms, "bytes_per_launch": per_launch,
                      "p2g_ms_total": p2g_ms, "p2g_launches": p2g_calls,
                      "rk_ms_total": rk_ms, "rk_launches": rk_calls},
+        "timers": timers,
     }
 
 
@@ -247,10 +250,11 @@ def main():
                                f"({npar} parcels), {form}, dt=0.01, split/mix on",
                    "parcels": npar, "grid": [n, n, nz], "form": form},
         "roofline": head["roofline"],
+        "timers": head["timers"],
     }
     if "fp64" in res:
         out["fp64"] = {"value": res["fp64"]["value"], "ms_per_step": res["fp64"]["ms_per_step"],
-                       "roofline": res["fp64"]["roofline"]}
+                       "roofline": res["fp64"]["roofline"], "timers": res["fp64"]["timers"]}
     if "fp32" in res and "fp64" in res:
         out["fp32_speedup_vs_fp64"] = res["fp32"]["value"] / res["fp64"]["value"]
     if args.err_steps > 0 and "fp32" in res:

@@ -122,11 +122,15 @@ int pmpic_conservation_sums(pmpic_ctx *ctx, double out6[6]);
  * (parcel_splitting.F90:149-155, parcel_mixing.F90:408-410) */
 int pmpic_last_counts(pmpic_ctx *ctx, int64_t out3[3]);
 
-/* HIP-event timing of the hot kernels on the context's stream: enable, then read the
- * accumulated milliseconds and launch counts of kernel `which` (0 = per-cell par2grid
- * accumulation, 1 = fused grid2par + RK stage update) */
+/* HIP-event timing on the context's stream: enable, then read the accumulated milliseconds and
+ * counts of timer `which`: 0 = per-cell par2grid accumulation kernel, 1 = fused grid2par + RK
+ * stage kernel, 2..6 = the component callbacks vort2vel, vorticity_tendency, rk4_integrator,
+ * parcel_splitting (active stages), parcel_mixing (active stages).  pmpic_timing_name gives a
+ * timer's region name (the reference's timer names for the components, timer.F90:247-406),
+ * NULL past the last. */
 int pmpic_timing_enable(pmpic_ctx *ctx, int32_t enable);
 int pmpic_timing_read(pmpic_ctx *ctx, int32_t which, double *total_ms, int64_t *launches);
+const char *pmpic_timing_name(int32_t which);
 
 #ifdef __cplusplus
 }

@@ -33,7 +33,7 @@ int timing_end(Ctx *c, int which) {
 }
 
 int timing_harvest(Ctx *c) {
-  for (int w = 0; w < 2; w++) {
+  for (int w = 0; w < NTIMERS; w++) {
     for (int e = 0; e < c->nev[w]; e++) {
       PMPIC_CHECK(hipEventSynchronize(c->ev[w][e][1]));
       float ms = 0.f;
@@ -199,7 +199,7 @@ int free_all(Ctx *c) {
   hipFree(c->fbuf); hipFree(c->ibuf);
   hipFree(c->cbuf); hipFree(c->tw_x); hipFree(c->tw_y);
   hipFree(c->scan_tmp); hipFree(c->blk); hipFree(c->dsc);
-  for (int w = 0; w < 2; w++)
+  for (int w = 0; w < NTIMERS; w++)
     for (int e = 0; e < 64; e++) {
       if (c->ev[w][e][0]) hipEventDestroy(c->ev[w][e][0]);
       if (c->ev[w][e][1]) hipEventDestroy(c->ev[w][e][1]);
@@ -286,7 +286,7 @@ int create(const pmpic_config *cf, Ctx *c) {
   c->nblk_max = cdiv(big, 256) + 1;
   PMPIC_TRY(dalloc(&c->blk, 20 * c->nblk_max + 64));
   PMPIC_TRY(dalloc(&c->dsc, 1));
-  for (int w = 0; w < 2; w++)
+  for (int w = 0; w < NTIMERS; w++)
     for (int e = 0; e < 64; e++) {
       PMPIC_CHECK(hipEventCreate(&c->ev[w][e][0]));
       PMPIC_CHECK(hipEventCreate(&c->ev[w][e][1]));
@@ -427,6 +427,7 @@ int pmpic_vort2vel(pmpic_ctx *ctx) {
   if (!ctx) return PMPIC_ERR_ARG;
   Ctx *c = &ctx->c;
   const int stage = stage_of(c->it_v2v);
+  PMPIC_TRY(timing_begin(c, T_VORT2VEL));
   if (c->precision == 1) {
     PMPIC_TRY(b32_par2grid(c));
     PMPIC_TRY(vort2vel_spectral(c, stage));
@@ -435,6 +436,7 @@ int pmpic_vort2vel(pmpic_ctx *ctx) {
     PMPIC_TRY(par2grid_fused(c, c->n));
     PMPIC_TRY(vort2vel_spectral(c, stage));
   }
+  PMPIC_TRY(timing_end(c, T_VORT2VEL));
   c->it_v2v++;
   return PMPIC_OK;
 }
@@ -443,11 +445,13 @@ int pmpic_vorticity_tendency(pmpic_ctx *ctx) {
   if (!ctx) return PMPIC_ERR_ARG;
   Ctx *c = &ctx->c;
   const int stage = stage_of(c->it_vt);
+  PMPIC_TRY(timing_begin(c, T_VORT_TEND));
   PMPIC_TRY(vorticity_tendency_spectral(c, stage));
   if (c->precision == 1) PMPIC_TRY(b32_grids_from(c, 3, 3));
   if (stage == 1) {
     PMPIC_TRY(grid_max_reduce(c));
   }
+  PMPIC_TRY(timing_end(c, T_VORT_TEND));
   c->it_vt++;
   return PMPIC_OK;
 }
@@ -456,8 +460,10 @@ int pmpic_rk4_integrator(pmpic_ctx *ctx) {
   if (!ctx) return PMPIC_ERR_ARG;
   Ctx *c = &ctx->c;
   const int stage = c->rkstep;
+  PMPIC_TRY(timing_begin(c, T_RK4));
   if (c->precision == 1) PMPIC_TRY(b32_rk_stage(c, stage, c->splitting_enabled));
   else PMPIC_TRY(rk_stage_update(c, stage, c->n, c->splitting_enabled));
+  PMPIC_TRY(timing_end(c, T_RK4));
   c->rkstep = (c->rkstep < 4) ? c->rkstep + 1 : 1;
   int st = take_status(c);
   c->dtm = c->hsc.dtm;
@@ -471,7 +477,9 @@ int pmpic_parcel_splitting(pmpic_ctx *ctx) {
   int st = PMPIC_OK;
   if (c->it_split % 4 == 3) {
     const int64_t n0 = c->n;
+    PMPIC_TRY(timing_begin(c, T_SPLIT));
     st = (c->precision == 1) ? b32_split(c) : parcel_split(c);
+    if (st == PMPIC_OK) st = timing_end(c, T_SPLIT);
     c->last_split = c->n - n0;
     if (st == PMPIC_OK) st = take_status(c);
   }
@@ -484,7 +492,9 @@ int pmpic_parcel_mixing(pmpic_ctx *ctx) {
   Ctx *c = &ctx->c;
   int st = PMPIC_OK;
   if (c->it_mix % 4 == 3) {
+    PMPIC_TRY(timing_begin(c, T_MIX));
     st = (c->precision == 1) ? b32_mix(c) : parcel_mix(c);
+    if (st == PMPIC_OK) st = timing_end(c, T_MIX);
     if (st == PMPIC_OK) st = take_status(c);
   }
   c->it_mix++;
@@ -734,13 +744,24 @@ int pmpic_timing_enable(pmpic_ctx *ctx, int32_t enable) {
   Ctx *c = &ctx->c;
   PMPIC_TRY(timing_harvest(c));
   c->timing = enable ? 1 : 0;
-  c->t_ms[0] = c->t_ms[1] = 0;
-  c->t_calls[0] = c->t_calls[1] = 0;
+  for (int w = 0; w < NTIMERS; w++) {
+    c->t_ms[w] = 0;
+    c->t_calls[w] = 0;
+  }
   return PMPIC_OK;
 }
 
+const char *pmpic_timing_name(int32_t which) {
+  // the kernels, then the components under the reference's timer region names
+  // (vort2vel.F90:88, vorticity_tendency.F90:97, rk4_integrator.F90:71,
+  // parcel_splitting.F90:40, parcel_mixing.F90:49)
+  static const char *names[NTIMERS] = {"par2grid_cells", "grid2par_rk_stage", "vort2vel",
+                                       "vort_tend", "RK4_all_steps", "splitting", "mixing"};
+  return (which >= 0 && which < NTIMERS) ? names[which] : nullptr;
+}
+
 int pmpic_timing_read(pmpic_ctx *ctx, int32_t which, double *total_ms, int64_t *launches) {
-  if (!ctx || which < 0 || which > 1) return PMPIC_ERR_ARG;
+  if (!ctx || which < 0 || which >= NTIMERS) return PMPIC_ERR_ARG;
   Ctx *c = &ctx->c;
   PMPIC_TRY(timing_harvest(c));
   if (total_ms) *total_ms = c->t_ms[which];

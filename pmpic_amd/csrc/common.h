@@ -72,6 +72,8 @@ struct DevScalars {
   int pad;
 };
 
+enum TimerId { T_ACCUM = 0, T_RK = 1, T_VORT2VEL, T_VORT_TEND, T_RK4, T_SPLIT, T_MIX, NTIMERS };
+
 struct Ctx {
   Geometry g;
   int64_t cap;
@@ -125,12 +127,13 @@ struct Ctx {
   int64_t nblk_max;
   DevScalars *dsc;                    // device scalars
   DevScalars hsc;                     // host mirror
-  // HIP-event timing of the hot kernels (0 = par2grid cell accumulation, 1 = gather + RK)
+  // HIP-event timing: 0 = par2grid cell accumulation kernel, 1 = gather + RK stage kernel,
+  // 2..6 = the component callbacks (timer regions named as the reference's timers)
   int timing;
-  hipEvent_t ev[2][64][2];
-  int nev[2];
-  double t_ms[2];
-  int64_t t_calls[2];
+  hipEvent_t ev[NTIMERS][64][2];
+  int nev[NTIMERS];
+  double t_ms[NTIMERS];
+  int64_t t_calls[NTIMERS];
 };
 
 #define PMPIC_CHECK(call)                                   \

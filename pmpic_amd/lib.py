@@ -36,7 +36,7 @@ EXPORTS = [
     "pmpic_parcel_haloswap", "pmpic_get_grid", "pmpic_get_spectral", "pmpic_fft_forward",
     "pmpic_fft_backward", "pmpic_diffx", "pmpic_diffy", "pmpic_spectral_filter", "pmpic_diffz",
     "pmpic_laplinv", "pmpic_conservation_sums", "pmpic_last_counts", "pmpic_timing_enable",
-    "pmpic_timing_read", "pmpic_step_concurrent",
+    "pmpic_timing_read", "pmpic_step_concurrent", "pmpic_timing_name",
 ]
 
 
@@ -104,6 +104,8 @@ def load():
         L.pmpic_last_counts.argtypes = [vp, vp]
         L.pmpic_timing_enable.argtypes = [vp, i32]
         L.pmpic_timing_read.argtypes = [vp, i32, ctypes.POINTER(d), ctypes.POINTER(i64)]
+        L.pmpic_timing_name.restype = ctypes.c_char_p
+        L.pmpic_timing_name.argtypes = [i32]
         _lib = L
     return _lib
 
@@ -341,3 +343,14 @@ class Context:
         self._chk(self.L.pmpic_timing_read(self.h, int(which), ctypes.byref(ms), ctypes.byref(calls)),
                   "pmpic_timing_read")
         return ms.value, calls.value
+
+    def timer_table(self):
+        """{region name: (total ms, count)} for every timer (timing must be enabled)."""
+        out = {}
+        w = 0
+        while True:
+            name = self.L.pmpic_timing_name(w)
+            if name is None:
+                return out
+            out[name.decode()] = self.timing_read(w)
+            w += 1
