@@ -272,8 +272,8 @@ __global__ void k32_fill_i32(int32_t *a, int64_t n, int32_t v) {
 
 // ---- per-cell corner partial sums (the plain form's LDS kernel in FP32) ----
 enum Acc32 { A32_P2G = 0, A32_MIX = 1 };
-constexpr int TILE32 = 32;
-constexpr int ROWS32 = 1024;
+constexpr int TILE32 = 16;
+constexpr int ROWS32 = 512;
 
 struct Zmap {
   float zbot, dz;  // z of a parcel = zbot + (k + oz) dz

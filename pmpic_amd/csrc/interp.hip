@@ -124,8 +124,8 @@ enum AccMode { ACC_P2G = 0, ACC_SINGLE = 1, ACC_MIX_KEEP = 2, ACC_MIX_RES = 3 };
 // MIX mode sums kept parcels into `partial` (and counts them) and residual parcels into
 // `partial2`.  Cells with more than ROWS parcels (never at the target workloads) are summed
 // straight from global memory after k_bin_sort_big ordered their buckets.
-constexpr int TILE = 32;  // cells per workgroup: 8 threads (corners) each
-constexpr int ROWS_P2G = 1024, ROWS_MIX = 896;
+constexpr int TILE = 16;  // cells per workgroup: 8 threads (corners) each
+constexpr int ROWS_P2G = 512, ROWS_MIX = 448;
 
 template <int MODE, int NV>
 __device__ __forceinline__ void load_values(const double *__restrict__ par, int64_t cap,
