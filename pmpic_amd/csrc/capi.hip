@@ -239,6 +239,8 @@ int create(const pmpic_config *cf, Ctx *c) {
   if (c->gsz > (int64_t)INT32_MAX) return PMPIC_ERR_ARG;
   // the FP32-bulk form packs a cell as k << 20 | j << 10 | i
   if (c->precision == 1 && (g.nx > 1024 || g.ny > 1024 || g.ncz > 2047)) return PMPIC_ERR_ARG;
+  // ... and addresses its FP32 parcel fields with 32-bit byte offsets
+  if (c->precision == 1 && c->cap >= ((int64_t)1 << 30)) return PMPIC_ERR_ARG;
   const int64_t big = c->cap > c->ncell ? c->cap : c->ncell;
   if (c->precision == 1) PMPIC_TRY(b32_alloc(c));
   else PMPIC_TRY(dalloc(&c->par, (int64_t)NFIELDS * c->cap));
