@@ -386,47 +386,13 @@ __global__ void __launch_bounds__(64) k_diffz_lds(Geometry g, ColFields cf, DevS
   for (int k = 0; k < nz; k++) o[(int64_t)k * plane] = F[k];
 }
 
-// Register-resident column solvers.  The Thomas denominators and c' of a column depend only
-// on its k^2 and the boundary condition, so they are built once (k_lap_tables, k_dz_tables,
-// with the same operations as laplinv_column / diffz_column) and each call runs only the data
-// recurrences: one division per level forward, none backward.  The column itself stays in
-// registers (NZ levels, unrolled), so a full wavefront of columns runs per SIMD with no LDS.
+// Register-resident column solvers.  The column itself stays in registers (NZ levels,
+// unrolled; beyond R levels in an LDS slice), so a full wavefront of columns runs per SIMD.
+// diffz's Thomas denominators and c' are the same for every column, so they are built once
+// (k_dz_tables, with the same operations as diffz_column) and each call runs only the data
+// recurrence.  laplinv's depend on the column's k^2: they are recomputed alongside the data
+// recurrence (an independent chain), c' going to a scratch column for the backward sweep.
 // Results are bit-identical to the LDS and HBM forms.
-__global__ void k_lap_tables(Geometry g, const double *__restrict__ k2a, double *__restrict__ den,
-                             double *__restrict__ cp) {
-  const int64_t plane = (int64_t)g.sy * g.sx;
-  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= plane) return;
-  const double k2 = k2a[col];
-  const int nz = g.nzn;
-  const double f16 = (double)(1.f / 6.f), f13 = (double)(1.f / 3.f);
-  const double f112 = (double)(1.f / 12.f), f56 = (double)(5.f / 6.f);
-  const double dz = g.dz;
-  const double i2dz2 = 2. / dz / dz, idz2 = 1. / dz / dz;
-  const double ai = idz2 - f112 * k2, bi = -i2dz2 - f56 * k2, ci = idz2 - f112 * k2;
-  for (int df0 = 0; df0 < 2; df0++) {
-    double *D = den + (int64_t)df0 * nz * plane + col, *C = cp + (int64_t)df0 * nz * plane + col;
-    double b1, c1;
-    if (!df0) { b1 = 1.; c1 = 0.; }
-    else { b1 = -idz2 - f13 * k2; c1 = idz2 - f16 * k2; }
-    double denominator = b1;
-    D[0] = denominator;
-    C[0] = 0.;
-    double cprev = c1;
-    for (int k = 2; k <= nz; k++) {
-      double a, b, c;
-      if (k < nz) { a = ai; b = bi; c = ci; }
-      else if (!df0) { a = 0.; b = 1.; c = 0.; }
-      else { a = idz2 - f16 * k2; b = -idz2 - f13 * k2; c = 0.; }
-      const double cpk = cprev / denominator;
-      C[(int64_t)(k - 1) * plane] = cpk;
-      denominator = b - a * cpk;
-      D[(int64_t)(k - 1) * plane] = denominator;
-      cprev = c;
-    }
-  }
-}
-
 __global__ void k_dz_tables(Geometry g, double *__restrict__ den, double *__restrict__ cp) {
   if (threadIdx.x || blockIdx.x) return;
   const int nz = g.nzn;
@@ -467,9 +433,7 @@ struct RegColumn {
 template <int NZ, int R>
 __global__ void __launch_bounds__(64) k_laplinv_reg(Geometry g, ColFields cf,
                                                     const double *__restrict__ k2a,
-                                                    const double *__restrict__ tden,
-                                                    const double *__restrict__ tcp,
-                                                    DevScalars *sc) {
+                                                    double *__restrict__ cpscr, DevScalars *sc) {
   __shared__ double lds[(NZ > R ? NZ - R : 1) * 64];
   const int64_t plane = (int64_t)g.sy * g.sx;
   const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -478,8 +442,7 @@ __global__ void __launch_bounds__(64) k_laplinv_reg(Geometry g, ColFields cf,
   if (k2 == 0) return;
   const int df0 = cf.df0[blockIdx.y];
   double *f = cf.out[blockIdx.y] + col;
-  const double *den = tden + (int64_t)df0 * NZ * plane + col;
-  const double *cpt = tcp + (int64_t)df0 * NZ * plane + col;
+  double *CP = cpscr + (int64_t)blockIdx.y * NZ * plane + col;
   RegColumn<NZ, R> F;
   F.S = lds + threadIdx.x;
 #pragma unroll
@@ -487,41 +450,45 @@ __global__ void __launch_bounds__(64) k_laplinv_reg(Geometry g, ColFields cf,
   const double f16 = (double)(1.f / 6.f), f13 = (double)(1.f / 3.f);
   const double f112 = (double)(1.f / 12.f), f56 = (double)(5.f / 6.f);
   const double dz = g.dz;
-  const double idz2 = 1. / dz / dz;
-  const double ai = idz2 - f112 * k2;
+  const double i2dz2 = 2. / dz / dz, idz2 = 1. / dz / dz;
+  const double ai = idz2 - f112 * k2, bi = -i2dz2 - f56 * k2, ci = idz2 - f112 * k2;
   double fm = 0.0, f0 = F.get(0), fp = F.get(1);  // originals f(k-1), f(k), f(k+1)
-  const double d1 = df0 ? f13 * f0 + f16 * fp : 0.;
-  double fprev = d1 / den[0];
+  double b1, c1, d1;
+  if (!df0) { b1 = 1.; c1 = 0.; d1 = 0.; }
+  else { b1 = -idz2 - f13 * k2; c1 = idz2 - f16 * k2; d1 = f13 * f0 + f16 * fp; }
+  double denominator = b1;
+  double fprev = d1 / denominator;
   F.set(0, fprev);
+  double cprev = c1;
 #pragma unroll
   for (int k = 2; k <= NZ; k++) {
     fm = f0;
     f0 = fp;
     fp = (k < NZ) ? F.get(k < NZ ? k : 0) : 0.0;
-    double a, d;
+    double a, b, c, d;
     if (k < NZ) {
-      a = ai;
+      a = ai; b = bi; c = ci;
       d = f112 * fm + f56 * f0 + f112 * fp;
     } else if (!df0) {
-      a = 0.;
-      d = 0.;
+      a = 0.; b = 1.; c = 0.; d = 0.;
     } else {
-      a = idz2 - f16 * k2;
+      a = idz2 - f16 * k2; b = -idz2 - f13 * k2; c = 0.;
       d = f13 * f0 + f16 * fm;
     }
-    // coefficient loads are hoisted at most 16 levels ahead, which keeps the column itself
-    // in registers
-    if ((k & 31) == 0) __asm__ volatile("" ::: "memory");
-    const double dn = den[(int64_t)(k - 1) * plane];
-    if (dn == 0) { raise_status(sc, PMPIC_ERR_ZERO_PIVOT); return; }
-    fprev = (d - a * fprev) / (dn);
+    const double cpk = cprev / denominator;
+    CP[(int64_t)(k - 1) * plane] = cpk;
+    denominator = b - a * cpk;
+    if (denominator == 0) { raise_status(sc, PMPIC_ERR_ZERO_PIVOT); return; }
+    fprev = (d - a * fprev) / (denominator);
     F.set(k - 1, fprev);
+    cprev = c;
   }
   double fnext = fprev;
 #pragma unroll
   for (int k = NZ - 1; k >= 1; k--) {
+    // c' loads are hoisted at most 32 levels ahead, which keeps the column in registers
     if ((k & 31) == 0) __asm__ volatile("" ::: "memory");
-    const double v = F.get(k - 1) - cpt[(int64_t)k * plane] * fnext;
+    const double v = F.get(k - 1) - CP[(int64_t)k * plane] * fnext;
     F.set(k - 1, v);
     fnext = v;
   }
@@ -750,8 +717,6 @@ int spectral_setup(Ctx *c) {
   PMPIC_CHECK(hipMemcpy(c->k2, k2.data(), b, hipMemcpyHostToDevice));
   PMPIC_CHECK(hipMemcpy(c->filt, fl.data(), b, hipMemcpyHostToDevice));
   const int64_t plane = (int64_t)sx * sy;
-  hipLaunchKernelGGL(k_lap_tables, dim3(cdiv(plane, 64)), dim3(64), 0, c->stream, g, c->k2,
-                     c->lap_den, c->lap_cp);
   hipLaunchKernelGGL(k_dz_tables, dim3(1), dim3(1), 0, c->stream, g, c->dz_den, c->dz_cp);
   PMPIC_CHECK(hipGetLastError());
   PMPIC_CHECK(hipStreamSynchronize(c->stream));
@@ -851,7 +816,7 @@ int op_laplinv_n(Ctx *c, int nf, double *const *f, const int *df0) {
 #define PMPIC_LAP_CASE(NZ)                                                                    \
   case NZ:                                                                                     \
     hipLaunchKernelGGL((k_laplinv_reg<NZ, (NZ > 100 ? 64 : NZ)>), dim3(cdiv(plane, 64), nf),    \
-                       dim3(64), 0, c->stream, c->g, rf, c->k2, c->lap_den, c->lap_cp, c->dsc); \
+                       dim3(64), 0, c->stream, c->g, rf, c->k2, c->lap_cp, c->dsc);             \
     PMPIC_CHECK(hipGetLastError());                                                            \
     return PMPIC_OK;
     PMPIC_REG_COLUMNS(PMPIC_LAP_CASE)
