@@ -156,7 +156,9 @@ __device__ void stage_twiddles(double2 *tws, const double2 *tw, int n, bool conj
   }
 }
 
-// forward pass 1: r2c along x of every row (k, y) -> C1[k][y][l], l <= nx/2
+// forward pass 1: r2c along x of every row (k, y) -> C1[k][y][l], l <= nx/2.  Two real rows
+// a, b share one complex transform Z = FFT(a + i b): A[l] = (Z[l] + conj Z[n-l]) / 2 and
+// B[l] = (Z[l] - conj Z[n-l]) / (2i).
 __global__ void __launch_bounds__(FFT_TPB) k_rows_r2c(Plan p, const double2 *__restrict__ tw,
                                                       FieldPtrs fp, double2 *__restrict__ c1,
                                                       int64_t c1_field, int nrows, int nb) {
@@ -165,15 +167,25 @@ __global__ void __launch_bounds__(FFT_TPB) k_rows_r2c(Plan p, const double2 *__r
   const double *src = fp.in[f];
   double2 *dst = c1 + (int64_t)f * c1_field;
   const int n = p.n, nl = n / 2 + 1;
-  const int r0 = blockIdx.x * nb;
-  const int nbb = min(nb, nrows - r0);
+  const int r0 = blockIdx.x * 2 * nb;                  // first row of this workgroup
+  const int nbb = min(nb, (nrows - r0 + 1) / 2);       // transforms (row pairs)
   stage_twiddles(T, tw, n, false);
-  for (int e = threadIdx.x; e < nbb * n; e += blockDim.x) A[e] = make_double2(src[(int64_t)r0 * n + e], 0.0);
+  for (int e = threadIdx.x; e < nbb * n; e += blockDim.x) {
+    const int t = e / n, x = e - t * n;
+    const int ra = r0 + 2 * t, rb = ra + 1;
+    const double va = src[(int64_t)ra * n + x];
+    const double vb = (rb < nrows) ? src[(int64_t)rb * n + x] : 0.0;
+    A[e] = make_double2(va, vb);
+  }
   __syncthreads();
   double2 *R = lds_fft<false>(A, B, T, p, nbb);
   for (int e = threadIdx.x; e < nbb * nl; e += blockDim.x) {
-    int t = e / nl, l = e - t * nl;
-    dst[(int64_t)(r0 + t) * nl + l] = R[t * n + l];
+    const int t = e / nl, l = e - t * nl;
+    const int ra = r0 + 2 * t, rb = ra + 1;
+    const double2 z = R[t * n + l];
+    const double2 w = R[t * n + (l == 0 ? 0 : n - l)];  // Z[n-l]
+    dst[(int64_t)ra * nl + l] = make_double2((z.x + w.x) / 2, (z.y - w.y) / 2);
+    if (rb < nrows) dst[(int64_t)rb * nl + l] = make_double2((z.y + w.y) / 2, (w.x - z.x) / 2);
   }
 }
 
@@ -256,7 +268,9 @@ __global__ void __launch_bounds__(FFT_TPB) k_rows_inv(Plan p, const double2 *__r
   }
 }
 
-// inverse pass 2: c2r along y for nb adjacent x columns of one z level
+// inverse pass 2: c2r along y for 2 nb adjacent x columns of one z level.  The Hermitian
+// spectra of two columns a, b share one complex inverse transform of H_a + i H_b, whose real
+// and imaginary parts are the two real columns.
 __global__ void __launch_bounds__(FFT_TPB) k_cols_c2r(Plan p, const double2 *__restrict__ tw,
                                                       FieldPtrs fp, const double2 *__restrict__ c2,
                                                       int64_t c2_field, int nx, int nb) {
@@ -266,26 +280,31 @@ __global__ void __launch_bounds__(FFT_TPB) k_cols_c2r(Plan p, const double2 *__r
   double *dst = fp.out[f];
   const int ny = p.n, nm = ny / 2 + 1;
   const int k = blockIdx.y;
-  const int x0 = blockIdx.x * nb;
-  const int nbb = min(nb, nx - x0);
+  const int x0 = blockIdx.x * 2 * nb;
+  const int nbb = min(nb, (nx - x0 + 1) / 2);
   stage_twiddles(T, tw, ny, true);
   const double2 *lev = src + (int64_t)k * nm * nx;
   const bool even = (ny % 2) == 0;
   for (int e = threadIdx.x; e < nbb * ny; e += blockDim.x) {
-    int y = e / nbb, t = e - y * nbb;
-    int m = (y < nm) ? y : ny - y;
-    double2 w = lev[(int64_t)m * nx + x0 + t];
-    if (m == 0 || (even && m == ny / 2)) w.y = 0.0;
-    if (y >= nm) w.y = -w.y;
-    A[t * ny + y] = w;
+    const int y = e / nbb, t = e - y * nbb;
+    const int xa = x0 + 2 * t, xb = xa + 1;
+    const int m = (y < nm) ? y : ny - y;
+    double2 wa = lev[(int64_t)m * nx + xa];
+    double2 wb = (xb < nx) ? lev[(int64_t)m * nx + xb] : make_double2(0.0, 0.0);
+    if (m == 0 || (even && m == ny / 2)) { wa.y = 0.0; wb.y = 0.0; }
+    if (y >= nm) { wa.y = -wa.y; wb.y = -wb.y; }
+    A[t * ny + y] = make_double2(wa.x - wb.y, wa.y + wb.x);
   }
   __syncthreads();
   double2 *R = lds_fft<true>(A, B, T, p, nbb);
   const double dn = 1.0 / (double)ny;
   double *out = dst + (int64_t)k * ny * nx;
   for (int e = threadIdx.x; e < nbb * ny; e += blockDim.x) {
-    int y = e / nbb, t = e - y * nbb;
-    out[(int64_t)y * nx + x0 + t] = R[t * ny + y].x * dn;
+    const int y = e / nbb, t = e - y * nbb;
+    const int xa = x0 + 2 * t, xb = xa + 1;
+    const double2 v = R[t * ny + y];
+    out[(int64_t)y * nx + xa] = v.x * dn;
+    if (xb < nx) out[(int64_t)y * nx + xb] = v.y * dn;
   }
 }
 
@@ -340,8 +359,8 @@ int fft_forward(Ctx *c, int nf, const double *const *src, double *const *dst) {
   Plan px = make_plan(g.nx), py = make_plan(g.ny);
   const int64_t c1f = c1_field_size(g);
   const int nrows = g.nzn * g.ny;
-  const int nbx = LDS_CPLX / g.nx;
-  hipLaunchKernelGGL(k_rows_r2c, dim3(cdiv(nrows, nbx), nf), dim3(FFT_TPB), 0, c->stream, px,
+  const int nbx = LDS_CPLX / g.nx;  // transforms per workgroup, two rows each
+  hipLaunchKernelGGL(k_rows_r2c, dim3(cdiv(nrows, 2 * nbx), nf), dim3(FFT_TPB), 0, c->stream, px,
                      c->tw_x, fp, c->cbuf, c1f, nrows, nbx);
   const int nl = g.nx / 2 + 1;
   int nby = LDS_CPLX / g.ny;
@@ -364,10 +383,10 @@ int fft_backward(Ctx *c, int nf, const double *const *src, double *const *dst) {
   const int nbx = LDS_CPLX / g.nx;
   hipLaunchKernelGGL(k_rows_inv, dim3(cdiv(nrows, nbx), nf), dim3(FFT_TPB), 0, c->stream, px,
                      c->tw_x, fp, c->cbuf, c2f, nrows, nm, nbx, g.sx);
-  int nby = LDS_CPLX / g.ny;
+  int nby = LDS_CPLX / g.ny;  // transforms per workgroup, two x columns each
   if (nby > 32) nby = 32;
-  hipLaunchKernelGGL(k_cols_c2r, dim3(cdiv(g.nx, nby), g.nzn, nf), dim3(FFT_TPB), 0, c->stream,
-                     py, c->tw_y, fp, c->cbuf, c2f, g.nx, nby);
+  hipLaunchKernelGGL(k_cols_c2r, dim3(cdiv(g.nx, 2 * nby), g.nzn, nf), dim3(FFT_TPB), 0,
+                     c->stream, py, c->tw_y, fp, c->cbuf, c2f, g.nx, nby);
   PMPIC_CHECK(hipGetLastError());
   return PMPIC_OK;
 }
