@@ -466,9 +466,9 @@ int pmpic_rk4_integrator(pmpic_ctx *ctx) {
   else PMPIC_TRY(rk_stage_update(c, stage, c->n, c->splitting_enabled));
   PMPIC_TRY(timing_end(c, T_RK4));
   c->rkstep = (c->rkstep < 4) ? c->rkstep + 1 : 1;
+  if (c->defer_status) return PMPIC_OK;  // pmpic_step checks once per step
   int st = take_status(c);
   c->dtm = c->hsc.dtm;
-  PMPIC_TRY(timing_harvest(c));
   return st;
 }
 
@@ -482,7 +482,7 @@ int pmpic_parcel_splitting(pmpic_ctx *ctx) {
     st = (c->precision == 1) ? b32_split(c) : parcel_split(c);
     if (st == PMPIC_OK) st = timing_end(c, T_SPLIT);
     c->last_split = c->n - n0;
-    if (st == PMPIC_OK) st = take_status(c);
+    if (st == PMPIC_OK && !c->defer_status) st = take_status(c);
   }
   c->it_split++;
   return st;
@@ -496,25 +496,40 @@ int pmpic_parcel_mixing(pmpic_ctx *ctx) {
     PMPIC_TRY(timing_begin(c, T_MIX));
     st = (c->precision == 1) ? b32_mix(c) : parcel_mix(c);
     if (st == PMPIC_OK) st = timing_end(c, T_MIX);
-    if (st == PMPIC_OK) st = take_status(c);
+    if (st == PMPIC_OK && !c->defer_status) st = take_status(c);
   }
   c->it_mix++;
   return st;
+}
+
+// one timestep with the kernels' status checked once at its end (the components called one
+// by one check after each call); no kernel indexes memory from data a failed kernel left, so
+// the later launches of a failing step are harmless
+static int step_once(pmpic_ctx *ctx) {
+  Ctx *c = &ctx->c;
+  for (int r = 0; r < 4; r++) {
+    PMPIC_TRY(pmpic_vort2vel(ctx));
+    PMPIC_TRY(pmpic_vorticity_tendency(ctx));
+    PMPIC_TRY(pmpic_rk4_integrator(ctx));
+    if (c->splitting_enabled) PMPIC_TRY(pmpic_parcel_splitting(ctx));
+    else c->it_split++;
+    if (c->mixing_enabled) PMPIC_TRY(pmpic_parcel_mixing(ctx));
+    else c->it_mix++;
+  }
+  return PMPIC_OK;
 }
 
 int pmpic_step(pmpic_ctx *ctx, int32_t nsteps) {
   if (!ctx || nsteps < 0) return PMPIC_ERR_ARG;
   Ctx *c = &ctx->c;
   for (int s = 0; s < nsteps; s++) {
-    for (int r = 0; r < 4; r++) {
-      PMPIC_TRY(pmpic_vort2vel(ctx));
-      PMPIC_TRY(pmpic_vorticity_tendency(ctx));
-      PMPIC_TRY(pmpic_rk4_integrator(ctx));
-      if (c->splitting_enabled) PMPIC_TRY(pmpic_parcel_splitting(ctx));
-      else c->it_split++;
-      if (c->mixing_enabled) PMPIC_TRY(pmpic_parcel_mixing(ctx));
-      else c->it_mix++;
-    }
+    c->defer_status = 1;
+    int st = step_once(ctx);
+    c->defer_status = 0;
+    const int ks = take_status(c);
+    if (st == PMPIC_OK) st = ks;
+    if (st != PMPIC_OK) return st;
+    c->dtm = c->hsc.dtm;
     c->time = c->time + c->dtm;
   }
   return PMPIC_OK;

@@ -82,6 +82,7 @@ struct Ctx {
   int it_v2v, it_vt, rkstep, it_split, it_mix;  // per-component iteration counters
   int precision;
   int device;
+  int defer_status;      // set inside pmpic_step: kernel status checked once per step
   int splitting_enabled, mixing_enabled;
   int64_t last_split, last_removed, last_added;
   hipStream_t stream;
