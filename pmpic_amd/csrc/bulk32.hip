@@ -541,6 +541,7 @@ struct Scale32 {
 template <int STAGE>
 __global__ void __launch_bounds__(TPB) k32_rk_stage(Geometry g, Bulk32 b, int64_t cap, int64_t n,
                                                     Grid32Set gs, Scale32 sc3, int stretch,
+                                                    uint8_t *__restrict__ split_flag,
                                                     DevScalars *sc) {
   const int64_t p64 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p64 >= n) return;
@@ -640,7 +641,9 @@ __global__ void __launch_bounds__(TPB) k32_rk_stage(Geometry g, Bulk32 b, int64_
       const float third = 1.f / 3.f;
       float ds = vn[0] * kv[3] + vn[1] * kv[4] + vn[2] * kv[5];
       ds = powf(fabsf(ds), third);
-      stnt(fld(B_STRETCH), p, s_old + ds * dt);
+      const float snew = s_old + ds * dt;
+      stnt(fld(B_STRETCH), p, snew);
+      split_flag[p] = (snew > 4.f) ? 1 : 0;  // the test of parcel_splitting.F90:76
     }
   }
 }
@@ -1015,31 +1018,35 @@ int b32_rk_stage(Ctx *c, int stage, int stretch) {
   switch (stage) {
     case 1:
       hipLaunchKernelGGL(k32_rk_stage<1>, gb, tb, 0, c->stream, c->g, c->b32, c->cap, n, gs, s3,
-                         stretch, c->dsc);
+                         stretch, c->flag, c->dsc);
       break;
     case 2:
       hipLaunchKernelGGL(k32_rk_stage<2>, gb, tb, 0, c->stream, c->g, c->b32, c->cap, n, gs, s3,
-                         stretch, c->dsc);
+                         stretch, c->flag, c->dsc);
       break;
     case 3:
       hipLaunchKernelGGL(k32_rk_stage<3>, gb, tb, 0, c->stream, c->g, c->b32, c->cap, n, gs, s3,
-                         stretch, c->dsc);
+                         stretch, c->flag, c->dsc);
       break;
     default:
       hipLaunchKernelGGL(k32_rk_stage<4>, gb, tb, 0, c->stream, c->g, c->b32, c->cap, n, gs, s3,
-                         stretch, c->dsc);
+                         stretch, c->flag, c->dsc);
       break;
   }
   PMPIC_CHECK(hipGetLastError());
   PMPIC_TRY(timing_end(c, 1));
+  c->split_flags_fresh = (stage == 4 && stretch) ? 1 : 0;
   return PMPIC_OK;
 }
 
 int b32_split(Ctx *c) {
   const int64_t n = c->n;
   if (n > 0) {
-    hipLaunchKernelGGL(k32_split_flags, dim3(cdiv(n, TPB)), dim3(TPB), 0, c->stream, c->b32.f,
-                       c->cap, n, c->flag);
+    // the stage-4 RK kernel already flagged the parcels to split
+    if (!c->split_flags_fresh)
+      hipLaunchKernelGGL(k32_split_flags, dim3(cdiv(n, TPB)), dim3(TPB), 0, c->stream,
+                         c->b32.f, c->cap, n, c->flag);
+    c->split_flags_fresh = 0;
     PMPIC_TRY(compact_flags(c, c->flag, 1, n, c->list_b, &c->dsc->nsplit));
   } else {
     c->hsc.nsplit = 0;

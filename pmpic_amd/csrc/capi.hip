@@ -353,6 +353,7 @@ int pmpic_set_parcels(pmpic_ctx *ctx, int64_t n, const double *const *fields) {
   Ctx *c = &ctx->c;
   for (int f = 0; f < 17; f++)
     if (!fields[f]) return PMPIC_ERR_ARG;
+  c->split_flags_fresh = 0;
   if (c->precision == 1) {
     PMPIC_TRY(b32_set_parcels(c, n, fields));
     c->n = n;

@@ -83,6 +83,7 @@ struct Ctx {
   int precision;
   int device;
   int defer_status;      // set inside pmpic_step: kernel status checked once per step
+  int split_flags_fresh; // flag[] holds the split test of the last stage-4 RK update
   int splitting_enabled, mixing_enabled;
   int64_t last_split, last_removed, last_added;
   hipStream_t stream;
