@@ -193,3 +193,30 @@ def test_column_solvers_bitwise_all_forms(oracle_mod, shape):
     np.testing.assert_array_equal(g.diffz(s, bot, top), o.diffz(s, bot, top))
     for df0 in (False, True):
         np.testing.assert_array_equal(g.laplinv(s, df0), o.laplinv(s, df0))
+
+
+@pytest.mark.parametrize("vort", [4.0, 2.0])
+def test_adaptive_timestep_steps_match_oracle(oracle_mod, vort):
+    """With dtmax large the CFL or vorticity bound sets every step's dt: the GPU's device-side
+    selection gives the oracle's dt sequence (and time), counts and fields over 5 steps."""
+    from pmpic_amd.lib import Context
+    shape = (12, 10, 8)
+    nx, ny, nz = shape
+    ic = thermal_case(nx, ny, nz, 2, vort=vort, shrink=0.02)
+    b = box(nx, ny, nz)
+    cap = 3 * len(ic["x"]) + 64
+    o = oracle_mod.Oracle(nx, ny, nz, b["xmin"], b["xmax"], b["ymin"], b["ymax"], b["zmin"],
+                          b["zmax"], cap, 0.01, 10.0, 0.5)
+    o.set_parcels(ic)
+    g = Context(nx, ny, nz, b["xmin"], b["xmax"], b["ymin"], b["ymax"], b["zmin"], b["zmax"], cap,
+                dtm=0.01, dtmax=10.0, timestep_prefactor=0.5)
+    g.set_parcels(ic)
+    for step in range(5):
+        o.step()
+        g.step()
+        assert g.dtm == pytest.approx(o.dtm, rel=1e-11), step
+        assert g.n == o.n, step
+    assert g.time == pytest.approx(o.time, rel=1e-11)
+    gp, op = g.get_parcels(), o.get_parcels()
+    for k in ("x", "y", "z", "p", "q", "r", "vol", "b"):
+        assert rel_err(gp[k], op[k]) < 1e-9, k

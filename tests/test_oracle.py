@@ -533,3 +533,32 @@ def test_step_is_deterministic(oracle_mod):
         res.append(o.get_parcels())
     for k in res[0]:
         np.testing.assert_array_equal(res[0][k], res[1][k])
+
+
+@pytest.mark.parametrize("vort,dtmax", [(4.0, 10.0), (0.2, 10.0), (4.0, 0.01)])
+def test_adaptive_timestep_rule(oracle_mod, vort, dtmax):
+    """The stage-1 timestep is min(dtmax, Lx/umax, Ly/vmax, Lz'/wmax, prefactor/omax)
+    (vort2vel.F90:329-381, vorticity_tendency.F90:277-321): recomputed here in numpy from the
+    oracle's own exported grids (interior maxima, Lz' = (nz+1) dz), with dtmax large enough for
+    the CFL or vorticity bound to bind and small enough for it not to."""
+    nx, ny, nz = 12, 10, 8
+    b = box(nx, ny, nz)
+    ic = thermal_case(nx, ny, nz, 2, vort=vort)
+    o = oracle_mod.Oracle(nx, ny, nz, b["xmin"], b["xmax"], b["ymin"], b["ymax"], b["zmin"],
+                          b["zmax"], 3 * len(ic["x"]), 0.01, dtmax, 0.5)
+    o.set_parcels(ic)
+    o.vort2vel()
+    o.vorticity_tendency()
+    inner = lambda a: a[2:-2, 2:-2, :]
+    u, v, w = (inner(o.grid(k)) for k in ("u", "v", "w"))
+    p, q, r = (inner(o.grid(k)) for k in ("p", "q", "r"))
+    dx, dy, dz = LX / nx, LX / ny, LZ / nz
+    vmax = [max(float(np.sqrt(np.max(a ** 2))), 1e-15) for a in (u, v, w)]
+    cfl = min(nx * dx / vmax[0], ny * dy / vmax[1], (nz + 1) * dz / vmax[2])
+    dt = dtmax if dtmax <= cfl else cfl
+    omax = float(np.sqrt(np.max(p ** 2 + q ** 2 + r ** 2)))
+    if omax > 0:
+        dt = min(dt, 0.5 / omax)
+    assert o.dtm == pytest.approx(dt, rel=1e-13, abs=0)
+    if dtmax == 10.0:
+        assert o.dtm < dtmax  # an adaptive bound did bind
