@@ -562,3 +562,44 @@ def test_adaptive_timestep_rule(oracle_mod, vort, dtmax):
     assert o.dtm == pytest.approx(dt, rel=1e-13, abs=0)
     if dtmax == 10.0:
         assert o.dtm < dtmax  # an adaptive bound did bind
+
+
+def test_mean_flow_mode(oracle_mod):
+    """The horizontal mean of the velocity is the zero-wavenumber mode of vort2vel.F90:221-247:
+    ubar(k) = ubar(k-1) + dz/2 (e(k-1) + e(k)), vbar with -d, from the (1,1) columns of the
+    y- and x-vorticity spectra, minus the reference's average
+    fl32(1/nz') * sum_{k<nz'} (ubar(k) + ubar(nz')/2).  Recomputed in numpy from the
+    horizontal means of the oracle's own vorticity grids (the mean mode is not touched by the
+    divergence correction or the filter), and checked against its velocity grids."""
+    nx, ny, nz = 12, 10, 8
+    b = box(nx, ny, nz)
+    ic = thermal_case(nx, ny, nz, 2)
+    rng = np.random.default_rng(3)
+    n = len(ic["x"])
+    # vorticity with a horizontal mean that varies in z
+    ic["p"] = 0.7 * np.cos(ic["z"]) + 0.1 * rng.uniform(-1, 1, n)
+    ic["q"] = 0.5 + 0.3 * ic["z"] + 0.1 * rng.uniform(-1, 1, n)
+    ic["r"] = 0.1 * rng.uniform(-1, 1, n)
+    o = oracle_mod.Oracle(nx, ny, nz, b["xmin"], b["xmax"], b["ymin"], b["ymax"], b["zmin"],
+                          b["zmax"], 3 * n, 0.01, 0.01, 0.5)
+    o.set_parcels(ic)
+    o.vort2vel()
+    inner = lambda a: a[2:-2, 2:-2, :]
+    mean_xy = lambda a: inner(a).mean(axis=(0, 1))
+    dz = LZ / nz
+    nzn = nz + 1
+    e = mean_xy(o.grid("q"))  # (1,1) column of the y-vorticity spectrum / (nx ny)
+    d = mean_xy(o.grid("p"))
+    ub = np.zeros(nzn)
+    vb = np.zeros(nzn)
+    for k in range(1, nzn):
+        ub[k] = ub[k - 1] + dz / 2.0 * (e[k - 1] + e[k])
+        vb[k] = vb[k - 1] - dz / 2.0 * (d[k - 1] + d[k])
+    rnz = float(np.float32(1.0) / np.float32(nzn))
+    ref_avg = lambda a: rnz * sum(a[k - 1] + 0.5 * a[nzn - 1] for k in range(1, nzn))
+    uref, vref = ub - ref_avg(ub), vb - ref_avg(vb)
+    np.testing.assert_allclose(mean_xy(o.grid("u")), uref, rtol=0, atol=1e-12 * np.abs(uref).max())
+    np.testing.assert_allclose(mean_xy(o.grid("v")), vref, rtol=0, atol=1e-12 * np.abs(vref).max())
+    # removing the true vertical mean (trapezoid) instead would differ visibly
+    ualt = ub - np.trapezoid(ub, dx=1.0) / (nzn - 1)
+    assert np.abs(ualt - uref).max() > 1e-6 * np.abs(uref).max()
