@@ -23,7 +23,10 @@
 #include "common.h"
 
 #include <hipcub/hipcub.hpp>
+#include <rocprim/iterator/transform_iterator.hpp>
+#include <algorithm>
 #include <math.h>
+#include <stdlib.h>
 #include <vector>
 
 // The FP32-bulk form is not held to the reference's operation order (its bar is the velocity
@@ -205,22 +208,43 @@ __global__ void k32_to_f64(const float *__restrict__ a, int64_t n, double *__res
   if (p < n) b[p] = (double)a[p];
 }
 
-// ---- cell binning (the plain form's, with the cell read instead of located) ----
-// Each wavefront walks BIN_SEQ consecutive groups of 64 parcels in order, so a cell whose
-// parcels are consecutive in slot order gets its bucket slots in ascending parcel order unless
-// it straddles two wavefronts' ranges; the accumulation only has to reorder those cells.
+// ---- cell binning ----
+// A wavefront walks BIN_SEQ consecutive groups of 64 parcels in order and takes one atomic per
+// run of equal cells, so a cell whose parcels are consecutive in slot order gets its ranks in
+// ascending parcel order unless it straddles two wavefronts' ranges; the accumulation only has
+// to reorder those cells.  A parcel of rank < capc is listed at slot[cell * capc + rank], so a
+// typical cell needs no scan and no scatter pass.  A parcel of higher rank is appended, with
+// its rank, to an overflow list; every cell holding more than capc parcels is then laid out
+// densely, all its parcels in rank order, in `sorted` from ostart[cell] (an exclusive scan of
+// those cells' counts).  capc = 0 lays out every cell that way.
 constexpr int BIN_SEQ = 16;
+constexpr int CAPC = 32;  // slot capacity per cell when the slot table is present
 
-__global__ void k32_bin_count(Geometry g, const int32_t *__restrict__ cellv, int64_t n,
-                              int32_t *__restrict__ rank, int32_t *__restrict__ count) {
+__global__ void k32_bin_reset(int32_t *__restrict__ count, int64_t nc1, DevScalars *dsc) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nc1) count[i] = 0;
+  if (i == 0) dsc->novf = 0;
+}
+
+__global__ void k32_bin_slot(Geometry g, const int32_t *__restrict__ cellv, int64_t n, int capc,
+                             int32_t *__restrict__ slot, int2 *__restrict__ ovf,
+                             int32_t *__restrict__ count, DevScalars *dsc) {
+  // parcel indices fit 32 bits (capacity < 2^30)
   const int lane = threadIdx.x & 63;
-  const int64_t w0 = (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64 * BIN_SEQ;
-  // all atomics of the wavefront are issued before any result is used, in group order
-  int base[BIN_SEQ], hls[BIN_SEQ];
+  const int32_t w0 = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64 * BIN_SEQ;
+  const int32_t n32 = (int32_t)n;
+  // all count atomics of the wavefront are issued before any result is used, in group order;
+  // ch[it] = cell << 6 | head lane of the parcel's run (the cell part is only used while the
+  // slot table exists, ncell * capc < 2^30)
+  int base[BIN_SEQ], ch[BIN_SEQ];
+  int32_t cv[BIN_SEQ];  // every load issued before the first is used (clamped index, n >= 1)
+#pragma unroll
+  for (int it = 0; it < BIN_SEQ; it++)
+    cv[it] = ld(cellv, (uint32_t)min(w0 + it * 64 + lane, n32 - 1));
 #pragma unroll
   for (int it = 0; it < BIN_SEQ; it++) {
-    const int64_t p = w0 + (int64_t)it * 64 + lane;
-    const int32_t c = (p < n) ? linear_cell(g, cellv[p]) : -1;
+    const int32_t p = w0 + it * 64 + lane;
+    const int32_t c = (p < n32) ? linear_cell(g, cv[it]) : -1;
     const int32_t cprev = __shfl_up(c, 1);
     const bool head = (lane == 0) || (c != cprev);
     const unsigned long long hm = __ballot(head);
@@ -229,23 +253,59 @@ __global__ void k32_bin_count(Geometry g, const int32_t *__restrict__ cellv, int
     const unsigned long long above = hm & ~((2ull << hl) - 1ull);
     const int next = above ? (__ffsll(above) - 1) : 64;
     base[it] = (head && c >= 0) ? atomicAdd(&count[c], next - hl) : 0;
-    hls[it] = hl;
+    ch[it] = (int)((uint32_t)c << 6) | hl;
   }
+  bool anyover = false;
 #pragma unroll
   for (int it = 0; it < BIN_SEQ; it++) {
-    const int64_t p = w0 + (int64_t)it * 64 + lane;
-    const int b = __shfl(base[it], hls[it]);
-    if (p < n) rank[p] = b + (lane - hls[it]);
+    const int32_t p = w0 + it * 64 + lane;
+    const int hl = ch[it] & 63;
+    const int32_t rk = __shfl(base[it], hl) + (lane - hl);
+    if (p < n32) {
+      if (rk < capc) st(slot, (uint32_t)((ch[it] >> 6) * capc + rk), p);
+      else anyover = true;
+    }
+  }
+  if (__ballot(anyover) == 0ull) return;
+#pragma unroll
+  for (int it = 0; it < BIN_SEQ; it++) {
+    const int32_t p = w0 + it * 64 + lane;
+    const int hl = ch[it] & 63;
+    const int32_t rk = __shfl(base[it], hl) + (lane - hl);
+    const bool over = (p < n32) && rk >= capc;
+    const unsigned long long om = __ballot(over);
+    if (om) {
+      const int lead = __ffsll(om) - 1;
+      int32_t ob = 0;
+      if (lane == lead) ob = atomicAdd(&dsc->novf, (int32_t)__popcll(om));
+      ob = __shfl(ob, lead);
+      if (over) ovf[ob + __popcll(om & ((1ull << lane) - 1ull))] = make_int2(p, rk);
+    }
   }
 }
 
-__global__ void k32_bin_scatter(Geometry g, const int32_t *__restrict__ cellv,
-                                const int32_t *__restrict__ rank,
-                                const int32_t *__restrict__ start, int64_t n,
+struct OvfCount {
+  int capc;
+  __host__ __device__ int32_t operator()(int32_t k) const { return k > capc ? k : 0; }
+};
+
+// the overflow list into the dense region; the parcel of rank capc also copies its cell's
+// capc slot entries there, so the region holds ranks 0 .. count-1
+__global__ void k32_ovf_scatter(Geometry g, const int32_t *__restrict__ cellv,
+                                const int2 *__restrict__ ovf, const DevScalars *dsc,
+                                const int32_t *__restrict__ ostart,
+                                const int32_t *__restrict__ slot, int capc,
                                 int32_t *__restrict__ sorted) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  sorted[start[linear_cell(g, cellv[p])] + rank[p]] = (int32_t)p;
+  const int32_t no = dsc->novf;
+  const int32_t stride = gridDim.x * blockDim.x;
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < no; i += stride) {
+    const int2 e = ovf[i];
+    const int32_t c = linear_cell(g, cellv[e.x]);
+    const int32_t b0 = ostart[c];
+    sorted[b0 + e.y] = e.x;
+    if (e.y == capc)
+      for (int r = 0; r < capc; r++) sorted[b0 + r] = slot[c * capc + r];
+  }
 }
 
 __global__ void k32_sort_big(const int32_t *__restrict__ start, int64_t ncell, int rows,
@@ -265,37 +325,42 @@ __global__ void k32_sort_big(const int32_t *__restrict__ start, int64_t ncell, i
   }
 }
 
-__global__ void k32_fill_i32(int32_t *a, int64_t n, int32_t v) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) a[i] = v;
-}
-
 // ---- per-cell corner partial sums (the plain form's LDS kernel in FP32) ----
+// A workgroup of 8 threads per cell takes a tile of TILE32 cells.  The tile's slot entries are
+// loaded together with the cell counts (their addresses do not depend on the counts), the
+// parcels' values are staged in LDS in dense rows, cell after cell, and thread q of a cell sums
+// corner q over the cell's parcels in ascending parcel order, so the sums are deterministic.
 enum Acc32 { A32_P2G = 0, A32_MIX = 1 };
 constexpr int TILE32 = 16;
 constexpr int ROWS32 = 512;
+constexpr int ATPB = TILE32 * 8;  // threads of the accumulation workgroup
 
 struct Zmap {
   float zbot, dz;  // z of a parcel = zbot + (k + oz) dz
 };
 
 template <int MODE>
-__global__ void __launch_bounds__(256) k32_cell_accum(Bulk32 b, int64_t cap, Geometry g, Zmap zm,
-                                                      float vmin, const int32_t *__restrict__ start,
-                                                      const int32_t *__restrict__ sorted,
-                                                      int64_t ncell, float *__restrict__ partial,
-                                                      float *__restrict__ partial2,
-                                                      int32_t *__restrict__ npercell) {
+__global__ void __launch_bounds__(ATPB) k32_cell_accum(Bulk32 b, int64_t cap, Geometry g, Zmap zm,
+                                                       float vmin, const int32_t *__restrict__ count,
+                                                       const int32_t *__restrict__ ostart,
+                                                       const int32_t *__restrict__ sorted,
+                                                       int64_t ncell, float *__restrict__ partial,
+                                                       float *__restrict__ partial2,
+                                                       int32_t *__restrict__ npercell) {
   constexpr bool MIX = (MODE == A32_MIX);
   constexpr int NV = MIX ? 5 : 4;
   constexpr int NF = NV + 1;
+  constexpr int SPT = TILE32 * CAPC / ATPB;  // slot entries per thread
+  constexpr int RPT = ROWS32 / ATPB;         // dense rows per thread
   __shared__ float4 sgeo[ROWS32];  // ox, oy, oz, vol
   __shared__ float4 sva[ROWS32];   // values 0..3
   __shared__ float svb[MIX ? ROWS32 : 1];
   __shared__ int32_t sidx[ROWS32];
   __shared__ int16_t sperm[ROWS32];
   __shared__ uint8_t skeep[MIX ? ROWS32 : 1];
-  __shared__ int32_t sstart[TILE32 + 1];
+  __shared__ int32_t soff[TILE32 + 1];  // first dense row of each cell of the tile
+  __shared__ int32_t scnt[TILE32], sost[TILE32];
+  __shared__ uint32_t sover;            // bit cl: cell cl is listed in the dense overflow region
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int cl = tid >> 3;
@@ -307,9 +372,67 @@ __global__ void __launch_bounds__(256) k32_cell_accum(Bulk32 b, int64_t cap, Geo
   const float sz = bz ? 1.f : -1.f, tz = bz ? 0.f : 1.f;
   const int64_t c0 = (int64_t)blockIdx.x * TILE32;
   const int ntile = (int)min((int64_t)TILE32, ncell - c0);
-  for (int t = tid; t <= ntile; t += blockDim.x) sstart[t] = start[c0 + t];
+  const int capc = b.capc;
+  int32_t sl[SPT];
+#pragma unroll
+  for (int k = 0; k < SPT; k++) {
+    const int e = tid + k * ATPB;
+    sl[k] = (capc > 0 && e < ntile * CAPC) ? ld(b.slot, (uint32_t)(c0 * CAPC + e)) : -1;
+  }
+  if (tid < 64) {
+    const int32_t v = (tid < ntile) ? count[c0 + tid] : 0;
+    const int32_t o = (tid < ntile) ? ostart[c0 + tid] : 0;
+    int32_t incl = v;
+#pragma unroll
+    for (int d = 1; d < TILE32; d <<= 1) {
+      const int32_t u = __shfl_up(incl, d);
+      if (lane >= d) incl += u;
+    }
+    const unsigned long long om = __ballot(tid < TILE32 && v > capc);
+    if (tid < TILE32) {
+      scnt[tid] = v;
+      sost[tid] = o;
+      soff[tid + 1] = incl;
+    }
+    if (tid == 0) {
+      soff[0] = 0;
+      sover = (uint32_t)om;
+    }
+  }
   __syncthreads();
   const float *F = b.f;
+  auto fld = [&](int f) { return F + (size_t)f * cap; };
+  // one parcel's staged values
+  auto fetch = [&](uint32_t pp, float4 &geo, float4 &va, float &vb, int32_t &pc) {
+    geo = make_float4(ldnt(fld(B_OX), pp), ldnt(fld(B_OY), pp), ldnt(fld(B_OZ), pp),
+                      ldnt(fld(B_VOL), pp));
+    if constexpr (!MIX) {
+      pc = ldnt(b.cell, pp);
+      va = make_float4(ldnt(fld(B_P), pp), ldnt(fld(B_Q), pp), ldnt(fld(B_R), pp),
+                       ldnt(fld(B_B), pp));
+      vb = ldnt(fld(B_H), pp);  // humidity, for the total buoyancy below
+    } else {
+      pc = 0;
+      va = make_float4(ldnt(fld(B_B), pp), ldnt(fld(B_H), pp), ldnt(fld(B_P), pp),
+                       ldnt(fld(B_Q), pp));
+      vb = ldnt(fld(B_R), pp);
+    }
+  };
+  auto put = [&](int r, int32_t pp, float4 geo, float4 v, float vb, int32_t pc) {
+    if constexpr (!MIX) {
+      // total buoyancy (vorticity_tendency.F90:121-126)
+      const int kc = pc >> 20;
+      const float z = zm.zbot + ((float)kc + geo.z) * zm.dz;
+      const float hl = vb - expf(-z);
+      v.w = v.w + 12.5f * (hl > 0.f ? hl : 0.f);
+    } else {
+      svb[r] = vb;
+      skeep[r] = (geo.w < vmin) ? 0 : 1;
+    }
+    sgeo[r] = geo;
+    sva[r] = v;
+    sidx[r] = pp;
+  };
   float acc[NF], res[NF];
 #pragma unroll
   for (int f = 0; f < NF; f++) { acc[f] = 0.f; res[f] = 0.f; }
@@ -318,66 +441,61 @@ __global__ void __launch_bounds__(256) k32_cell_accum(Bulk32 b, int64_t cap, Geo
   int first = 0;
   while (first < ntile) {
     int last = first + 1;
-    if (sstart[ntile] - sstart[first] <= ROWS32) last = ntile;  // the usual case: the rest fits
+    if (soff[ntile] - soff[first] <= ROWS32) last = ntile;  // the usual case: the rest fits
     else
-      while (last < ntile && sstart[last + 1] - sstart[first] <= ROWS32) last++;
-    const int32_t r0 = sstart[first];
-    const int nrows = sstart[last] - r0;
+      while (last < ntile && soff[last + 1] - soff[first] <= ROWS32) last++;
+    const int32_t r0 = soff[first];
+    const int nrows = soff[last] - r0;
     const bool fits = nrows <= ROWS32;
     if (fits) {
-      constexpr int RPT = ROWS32 / (TILE32 * 8);
-      int32_t pr[RPT];
+      // rows of the cells listed in the slot table, from the entries already loaded
+      int32_t pr[SPT], row[SPT];
 #pragma unroll
-      for (int k = 0; k < RPT; k++) {
-        const int r = tid + k * TILE32 * 8;
-        pr[k] = (r < nrows) ? ld(sorted, (uint32_t)(r0 + r)) : -1;
+      for (int k = 0; k < SPT; k++) {
+        const int e = tid + k * ATPB;
+        const int ce = e / CAPC, j = e % CAPC;
+        pr[k] = -1;
+        row[k] = 0;
+        if (sl[k] >= 0 && ce >= first && ce < last && j < scnt[ce] && scnt[ce] <= capc) {
+          pr[k] = sl[k];
+          row[k] = soff[ce] - r0 + j;
+        }
       }
-      float4 geo[RPT], va[RPT];
-      float vb[RPT];
-      int32_t pc[RPT];
+      float4 geo[SPT], va[SPT];
+      float vb[SPT];
+      int32_t pc[SPT];
 #pragma unroll
-      for (int k = 0; k < RPT; k++) {
-        vb[k] = 0.f;
-        if (pr[k] >= 0) {
-          const uint32_t pp = (uint32_t)pr[k];
-          auto fld = [&](int f) { return F + (size_t)f * cap; };
-          geo[k] = make_float4(ldnt(fld(B_OX), pp), ldnt(fld(B_OY), pp), ldnt(fld(B_OZ), pp),
-                               ldnt(fld(B_VOL), pp));
-          if constexpr (!MIX) {
-            pc[k] = ldnt(b.cell, pp);
-            va[k] = make_float4(ldnt(fld(B_P), pp), ldnt(fld(B_Q), pp), ldnt(fld(B_R), pp),
-                                ldnt(fld(B_B), pp));
-            vb[k] = ldnt(fld(B_H), pp);  // humidity, for the total buoyancy below
-          } else {
-            va[k] = make_float4(ldnt(fld(B_B), pp), ldnt(fld(B_H), pp), ldnt(fld(B_P), pp),
-                                ldnt(fld(B_Q), pp));
-            vb[k] = ldnt(fld(B_R), pp);
+      for (int k = 0; k < SPT; k++)
+        if (pr[k] >= 0) fetch((uint32_t)pr[k], geo[k], va[k], vb[k], pc[k]);
+#pragma unroll
+      for (int k = 0; k < SPT; k++)
+        if (pr[k] >= 0) put(row[k], pr[k], geo[k], va[k], vb[k], pc[k]);
+      // rows of the cells listed in the dense overflow region
+      const uint32_t pass = (last - first >= 32 ? ~0u : ((1u << (last - first)) - 1u)) << first;
+      if (sover & pass) {
+        for (int k = 0; k < RPT; k++) {
+          const int r = tid + k * ATPB;
+          if (r >= nrows) break;
+          // the cell of dense row r: the last cell of the pass starting at or before it
+          int lo_ = first, hi_ = last - 1;
+          while (lo_ < hi_) {
+            const int mid = (lo_ + hi_ + 1) >> 1;
+            if (soff[mid] - r0 <= r) lo_ = mid;
+            else hi_ = mid - 1;
           }
+          if (!((sover >> lo_) & 1u)) continue;
+          const int32_t pp = ld(sorted, (uint32_t)(sost[lo_] + (r + r0 - soff[lo_])));
+          float4 g1, v1;
+          float b1;
+          int32_t c1;
+          fetch((uint32_t)pp, g1, v1, b1, c1);
+          put(r, pp, g1, v1, b1, c1);
         }
-      }
-#pragma unroll
-      for (int k = 0; k < RPT; k++) {
-        const int r = tid + k * TILE32 * 8;
-        if (pr[k] < 0) continue;
-        float4 v = va[k];
-        if constexpr (!MIX) {
-          // total buoyancy (vorticity_tendency.F90:121-126)
-          const int kc = pc[k] >> 20;
-          const float z = zm.zbot + ((float)kc + geo[k].z) * zm.dz;
-          const float hl = vb[k] - expf(-z);
-          v.w = v.w + 12.5f * (hl > 0.f ? hl : 0.f);
-        } else {
-          svb[r] = vb[k];
-          skeep[r] = (geo[k].w < vmin) ? 0 : 1;
-        }
-        sgeo[r] = geo[k];
-        sva[r] = v;
-        sidx[r] = pr[k];
       }
     }
     __syncthreads();
     const bool mine = fits && cl >= first && cl < last;
-    const int lo = mine ? sstart[cl] - r0 : 0, hi = mine ? sstart[cl + 1] - r0 : 0;
+    const int lo = mine ? soff[cl] - r0 : 0, hi = mine ? soff[cl + 1] - r0 : 0;
     bool ord = true;
     for (int a = lo + 1 + q; a < hi; a += 8) ord &= (sidx[a - 1] < sidx[a]);
     const unsigned long long bad = __ballot(!ord);
@@ -422,7 +540,9 @@ __global__ void __launch_bounds__(256) k32_cell_accum(Bulk32 b, int64_t cap, Geo
   if (cl >= ntile) return;
   const int64_t c = c0 + cl;
   if (big) {
-    for (int32_t s = sstart[cl]; s < sstart[cl + 1]; s++) {
+    // more parcels than LDS rows: always in the overflow region, sorted by parcel there
+    const int32_t s0 = sost[cl], s1 = s0 + scnt[cl];
+    for (int32_t s = s0; s < s1; s++) {
       const int64_t pp = sorted[s];
       const float ox = F[B_OX * cap + pp], oy = F[B_OY * cap + pp], oz = F[B_OZ * cap + pp];
       const float vol = F[B_VOL * cap + pp];
@@ -887,15 +1007,32 @@ int dalloc32(T **p, int64_t n) {
 
 int bin32(Ctx *c, int64_t n) {
   const int64_t nc1 = c->ncell + 1;
-  hipLaunchKernelGGL(k32_fill_i32, dim3(cdiv(nc1, TPB)), dim3(TPB), 0, c->stream, c->cell_count,
-                     nc1, 0);
+  const Bulk32 &b = c->b32;
+  hipLaunchKernelGGL(k32_bin_reset, dim3(cdiv(nc1, TPB)), dim3(TPB), 0, c->stream, c->cell_count,
+                     nc1, c->dsc);
   if (n > 0)
-    hipLaunchKernelGGL(k32_bin_count, dim3(cdiv(n, (int64_t)TPB * BIN_SEQ)), dim3(TPB), 0,
-                       c->stream, c->g, c->b32.cell, n, c->rank, c->cell_count);
-  PMPIC_TRY(scan_exclusive_i32(c, c->cell_count, c->cell_start, nc1));
-  if (n > 0)
-    hipLaunchKernelGGL(k32_bin_scatter, dim3(cdiv(n, TPB)), dim3(TPB), 0, c->stream, c->g, c->b32.cell,
-                       c->rank, c->cell_start, n, c->sorted);
+    hipLaunchKernelGGL(k32_bin_slot, dim3(cdiv(n, (int64_t)TPB * BIN_SEQ)), dim3(TPB), 0,
+                       c->stream, c->g, b.cell, n, b.capc, b.slot, b.ovf, c->cell_count, c->dsc);
+  // dense-region offsets of the cells past the slot capacity
+  auto it = rocprim::make_transform_iterator(static_cast<const int32_t *>(c->cell_count),
+                                             OvfCount{b.capc});
+  size_t need = 0;
+  PMPIC_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, need, it, c->cell_start, (int)nc1,
+                                               c->stream));
+  if (need > c->scan_tmp_bytes) {
+    if (c->scan_tmp) PMPIC_CHECK(hipFree(c->scan_tmp));
+    c->scan_tmp = nullptr;
+    PMPIC_CHECK(hipMalloc(&c->scan_tmp, need));
+    c->scan_tmp_bytes = need;
+  }
+  size_t have = c->scan_tmp_bytes;
+  PMPIC_CHECK(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp, have, it, c->cell_start, (int)nc1,
+                                               c->stream));
+  if (n > 0) {
+    const int64_t nb = std::min<int64_t>(cdiv(n, TPB), 1024);
+    hipLaunchKernelGGL(k32_ovf_scatter, dim3(nb), dim3(TPB), 0, c->stream, c->g, b.cell, b.ovf,
+                       c->dsc, c->cell_start, b.slot, b.capc, c->sorted);
+  }
   hipLaunchKernelGGL(k32_sort_big, dim3(cdiv(c->ncell, TPB)), dim3(TPB), 0, c->stream,
                      c->cell_start, c->ncell, ROWS32, c->sorted);
   PMPIC_CHECK(hipGetLastError());
@@ -917,6 +1054,14 @@ int b32_alloc(Ctx *c) {
   PMPIC_TRY(dalloc32(&b.tag, c->cap));
   for (int i = 0; i < 6; i++) PMPIC_TRY(dalloc32(&b.grid[i], c->gsz));
   for (int i = 0; i < 12; i++) PMPIC_TRY(dalloc32(&b.mgrid[i], c->gsz));
+  // 32-bit byte offsets into the slot table; PMPIC_SLOT_CAPACITY=0 in the environment drops the
+  // table (every cell then binned through the dense region, as grids beyond that range are)
+  b.capc = (c->ncell * CAPC < ((int64_t)1 << 30)) ? CAPC : 0;
+  const char *env = getenv("PMPIC_SLOT_CAPACITY");
+  if (env && atoi(env) == 0) b.capc = 0;
+  b.slot = nullptr;
+  if (b.capc > 0) PMPIC_TRY(dalloc32(&b.slot, c->ncell * CAPC));
+  PMPIC_TRY(dalloc32(&b.ovf, c->cap));
   return PMPIC_OK;
 }
 
@@ -928,6 +1073,8 @@ void b32_free(Ctx *c) {
   hipFree(b.tag);
   for (int i = 0; i < 6; i++) hipFree(b.grid[i]);
   for (int i = 0; i < 12; i++) hipFree(b.mgrid[i]);
+  hipFree(b.slot);
+  hipFree(b.ovf);
 }
 
 int b32_set_parcels(Ctx *c, int64_t n, const double *const *fields) {
@@ -986,9 +1133,10 @@ int b32_par2grid(Ctx *c) {
   const int64_t n = c->n;
   PMPIC_TRY(bin32(c, n));
   PMPIC_TRY(timing_begin(c, 0));
-  hipLaunchKernelGGL(k32_cell_accum<A32_P2G>, dim3(cdiv(c->ncell, TILE32)), dim3(TILE32 * 8), 0,
+  hipLaunchKernelGGL(k32_cell_accum<A32_P2G>, dim3(cdiv(c->ncell, TILE32)), dim3(ATPB), 0,
                      c->stream, c->b32, c->cap, c->g, zmap_of(c->g), (float)c->g.vmin,
-                     c->cell_start, c->sorted, c->ncell, partial32(c), nullptr, nullptr);
+                     c->cell_count, c->cell_start, c->sorted, c->ncell, partial32(c), nullptr,
+                     nullptr);
   PMPIC_TRY(timing_end(c, 0));
   Out32<double> out = {{c->grid[GP], c->grid[GQ], c->grid[GR], c->grid[GB], nullptr, nullptr}};
   const int64_t nnode = (int64_t)c->g.nzn * c->g.ny * c->g.nx;
@@ -1072,9 +1220,9 @@ int b32_mix(Ctx *c) {
   const float vmin = (float)g.vmin;
   // kept and residual grids (construct_grids, parcel_mixing.F90:440-624)
   PMPIC_TRY(bin32(c, norig));
-  hipLaunchKernelGGL(k32_cell_accum<A32_MIX>, dim3(cdiv(c->ncell, TILE32)), dim3(TILE32 * 8), 0,
-                     c->stream, b, c->cap, g, zmap_of(g), vmin, c->cell_start, c->sorted,
-                     c->ncell, partial32(c), partial32b(c), c->npercell);
+  hipLaunchKernelGGL(k32_cell_accum<A32_MIX>, dim3(cdiv(c->ncell, TILE32)), dim3(ATPB), 0,
+                     c->stream, b, c->cap, g, zmap_of(g), vmin, c->cell_count, c->cell_start,
+                     c->sorted, c->ncell, partial32(c), partial32b(c), c->npercell);
   const int64_t nnode = (int64_t)g.nzn * g.ny * g.nx;
   Out32<float> kept = {{b.mgrid[0], b.mgrid[1], b.mgrid[2], b.mgrid[3], b.mgrid[4], b.mgrid[5]}};
   Out32<float> res = {{b.mgrid[6], b.mgrid[7], b.mgrid[8], b.mgrid[9], b.mgrid[10], b.mgrid[11]}};

@@ -41,6 +41,12 @@ struct Bulk32 {
   double *tag;       // cap
   float *grid[6];    // gathered grids U, V, W, DP, DQ, DR [nzn][ny][nx]
   float *mgrid[12];  // mixing: kept vol, b, h, p, q, r (0..5), residuals (6..11)
+  // cell binning: slot[c * capc + rank] lists the parcels of cell c while rank < capc; a cell
+  // with more parcels lists all of them in the dense overflow region (ncell * capc, capc = 32,
+  // or capc = 0 and no slot table when its byte offsets would exceed 32 bits)
+  int32_t *slot;
+  int2 *ovf;         // cap: (parcel, rank) of the parcels past the slot capacity
+  int capc;
 };
 
 // grid ids exported through the C-ABI (writebinary.F90:287-296 order, then the tendencies)
@@ -69,7 +75,7 @@ struct DevScalars {
   int64_t nsplit;    // parcels split this step
   int64_t nremove, nadd, nzero, nkeep1, nbackfill;
   int status;        // first error code raised by a kernel
-  int pad;
+  int32_t novf;      // parcels past their cell's slot capacity in the last FP32 binning
 };
 
 enum TimerId { T_ACCUM = 0, T_RK = 1, T_VORT2VEL, T_VORT_TEND, T_RK4, T_SPLIT, T_MIX, NTIMERS };

@@ -54,3 +54,32 @@ def rel_err(a, b):
     b = np.asarray(b)
     scale = max(np.max(np.abs(b)), 1e-300)
     return float(np.max(np.abs(a - b)) / scale) if a.size else 0.0
+
+
+def crowded_case(nx, ny, nz, seed=3, extra=(24, 25, 36, 700)):
+    """The vortical thermal (8 parcels per cell) with `extra` parcels appended in as many random
+    cells, so those cells hold 8 + extra parcels: exactly 32, one past 32, a few past 32, and
+    more than one workgroup's LDS rows (512), the cell-binning capacity edges."""
+    ic = thermal_case(nx, ny, nz, 2, seed=seed, vort=1.0)
+    rng = np.random.default_rng(seed)
+    dx, dy, dz = LX / nx, LX / ny, LZ / nz
+    cells = rng.choice(nx * ny * nz, size=len(extra), replace=False)
+    add = {k: [] for k in ic}
+    for m, c in zip(extra, cells):
+        i, j, k = c % nx, (c // nx) % ny, c // (nx * ny)
+        add["x"].append((i + rng.uniform(0.02, 0.98, m)) * dx)
+        add["y"].append((j + rng.uniform(0.02, 0.98, m)) * dy)
+        add["z"].append((k + rng.uniform(0.02, 0.98, m)) * dz)
+        for f in ("p", "q", "r"):
+            add[f].append(rng.uniform(-1, 1, m))
+        add["b"].append(rng.uniform(0.0, 0.5, m))
+        add["h"].append(rng.uniform(0.0, 0.4, m))
+        add["vol"].append(np.full(m, dx * dy * dz / 8.0))
+    n0 = len(ic["x"])
+    for k in ic:
+        if k == "tag":
+            continue
+        ic[k] = np.concatenate([ic[k]] + add[k])
+    ic["tag"] = np.arange(len(ic["x"]), dtype=np.float64)
+    assert len(ic["x"]) == n0 + sum(extra)
+    return ic

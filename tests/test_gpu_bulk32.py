@@ -10,7 +10,7 @@ where no parcel sits within FP32 rounding of a threshold; volume conserved to FP
 import numpy as np
 import pytest
 
-from tests.cases import LX, LZ, box, rel_err, thermal_case
+from tests.cases import LX, LZ, box, crowded_case, rel_err, thermal_case
 
 pytestmark = pytest.mark.gpu
 
@@ -71,6 +71,26 @@ def test_steps_close_to_fp64(shape, kw, steps):
     for name in ("u", "v", "w"):
         assert rel_err(g32.grid(name), g64.grid(name)) < 1e-5, name
     for name in ("p", "q", "r", "b"):
+        assert rel_err(g32.grid(name), g64.grid(name)) < 1e-5, name
+
+
+@pytest.mark.parametrize("slots", ["table", "dense"])
+def test_crowded_cells_close_to_fp64(monkeypatch, slots):
+    """Cells of 32, 33, 44 and 708 parcels through the FP32 binning: with the per-cell slot
+    table (cells past 32 parcels in the dense overflow region, the 708-parcel cell past the
+    LDS rows of a workgroup) and without it (PMPIC_SLOT_CAPACITY=0, every cell dense, the
+    layout of grids too large for the table).  Same bars as the steps above."""
+    if slots == "dense":
+        monkeypatch.setenv("PMPIC_SLOT_CAPACITY", "0")
+    shape = (12, 10, 8)
+    ic = crowded_case(*shape)
+    g64 = make(shape, 0, ic)
+    g32 = make(shape, 1, ic)
+    for step in range(2):
+        g64.step()
+        g32.step()
+        assert g32.n == g64.n and g32.last_counts() == g64.last_counts(), step
+    for name in ("u", "v", "w", "p", "q", "r", "b"):
         assert rel_err(g32.grid(name), g64.grid(name)) < 1e-5, name
 
 

@@ -11,17 +11,18 @@ Bars (written per test):
 import numpy as np
 import pytest
 
-from tests.cases import box, rel_err, thermal_case
+from tests.cases import box, crowded_case, rel_err, thermal_case
 
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(12, 10, 8), (16, 16, 6), (10, 12, 5)]
 
 
-def make_pair(oracle_mod, shape, ppc_dir=2, cap_factor=3, **kw):
+def make_pair(oracle_mod, shape, ppc_dir=2, cap_factor=3, ic=None, **kw):
     from pmpic_amd.lib import Context
     nx, ny, nz = shape
-    ic = thermal_case(nx, ny, nz, ppc_dir, **kw)
+    if ic is None:
+        ic = thermal_case(nx, ny, nz, ppc_dir, **kw)
     b = box(nx, ny, nz)
     cap = cap_factor * len(ic["x"]) + 64
     o = oracle_mod.Oracle(nx, ny, nz, b["xmin"], b["xmax"], b["ymin"], b["ymax"], b["zmin"],
@@ -68,6 +69,25 @@ def test_par2grid(oracle_mod, shape):
         want = o.par2grid(var)
         got = g.par2grid(var)
         assert rel_err(interior(got), interior(want)) < 1e-12
+
+
+def test_par2grid_and_steps_crowded_cells(oracle_mod):
+    """Cells holding 32, 33, 44 and 708 parcels: the binning's per-cell capacity edges and a
+    cell with more parcels than one workgroup stages in LDS."""
+    shape = (12, 10, 8)
+    o, g, ic = make_pair(oracle_mod, shape, ic=crowded_case(*shape))
+    o.cache_weights()
+    for var in (ic["b"], ic["p"], np.sin(ic["x"]) * np.cos(ic["z"])):
+        want = o.par2grid(var)
+        got = g.par2grid(var)
+        assert rel_err(interior(got), interior(want)) < 1e-12
+    for step in range(2):
+        o.step()
+        g.step()
+        assert g.n == o.n and g.last_counts() == o.last_counts(), step
+    gp, op = g.get_parcels(), o.get_parcels()
+    for k in ("x", "y", "z", "p", "q", "r", "vol", "b", "h"):
+        assert rel_err(gp[k], op[k]) < 1e-10, k
 
 
 @pytest.mark.parametrize("shape", SHAPES)
