@@ -195,7 +195,7 @@ int free_all(Ctx *c) {
   hipFree(c->atop); hipFree(c->abot); hipFree(c->btop); hipFree(c->bbot);
   hipFree(c->ubar); hipFree(c->vbar);
   hipFree(c->kx); hipFree(c->ky); hipFree(c->k2); hipFree(c->filt); hipFree(c->coords);
-  hipFree(c->lap_cp); hipFree(c->dz_den); hipFree(c->dz_cp);
+  hipFree(c->dz_den); hipFree(c->dz_cp);
   hipFree(c->fbuf); hipFree(c->ibuf);
   hipFree(c->cbuf); hipFree(c->tw_x); hipFree(c->tw_y);
   hipFree(c->scan_tmp); hipFree(c->blk); hipFree(c->dsc);
@@ -273,7 +273,6 @@ int create(const pmpic_config *cf, Ctx *c) {
   PMPIC_TRY(dalloc(&c->ky, plane));
   PMPIC_TRY(dalloc(&c->k2, plane));
   PMPIC_TRY(dalloc(&c->filt, plane));
-  PMPIC_TRY(dalloc(&c->lap_cp, 4 * (int64_t)g.nzn * plane));
   PMPIC_TRY(dalloc(&c->dz_den, 2 * (int64_t)g.nzn));
   PMPIC_TRY(dalloc(&c->dz_cp, 2 * (int64_t)g.nzn));
   PMPIC_TRY(dalloc(&c->coords, (int64_t)coords.size()));

@@ -119,9 +119,8 @@ struct Ctx {
   double *atop, *abot, *btop, *bbot;  // [sy][sx]
   double *ubar, *vbar;                // [nzn]
   double *kx, *ky, *k2, *filt;        // [sy][sx]
-  // Thomas elimination: laplinv's c' scratch for up to 4 fields [field][k][sy*sx]; diffz's
-  // denominators and c' [bt][k] (the same for every column)
-  double *lap_cp, *dz_den, *dz_cp;
+  // Thomas elimination: diffz's denominators and c' [bt][k] (the same for every column)
+  double *dz_den, *dz_cp;
   double *coords;                     // x_coords (nx+4), y_coords (ny+4), z_coords (nzn)
   double *fbuf;                       // host-array staging for standalone ops (max(gsz, ssz, cap))
   int32_t *ibuf;                      // cap int32 staging

@@ -386,13 +386,12 @@ __global__ void __launch_bounds__(64) k_diffz_lds(Geometry g, ColFields cf, DevS
   for (int k = 0; k < nz; k++) o[(int64_t)k * plane] = F[k];
 }
 
-// Register-resident column solvers.  The column itself stays in registers (NZ levels,
-// unrolled; beyond R levels in an LDS slice), so a full wavefront of columns runs per SIMD.
-// diffz's Thomas denominators and c' are the same for every column, so they are built once
-// (k_dz_tables, with the same operations as diffz_column) and each call runs only the data
-// recurrence.  laplinv's depend on the column's k^2: they are recomputed alongside the data
-// recurrence (an independent chain), c' going to a scratch column for the backward sweep.
-// Results are bit-identical to the LDS and HBM forms.
+// Register-resident column solvers: the column stays in registers (NZ levels, unrolled) and
+// the level-by-level recurrence never waits on memory.  diffz's Thomas denominators and c' are
+// the same for every column, so they are built once (k_dz_tables, with the same operations as
+// diffz_column) and each call runs only the data recurrence.  laplinv's depend on the column's
+// k^2: they are computed alongside the data recurrence (an independent chain).  Results are
+// bit-identical to the LDS and HBM forms.
 __global__ void k_dz_tables(Geometry g, double *__restrict__ den, double *__restrict__ cp) {
   if (threadIdx.x || blockIdx.x) return;
   const int nz = g.nzn;
@@ -417,83 +416,117 @@ __global__ void k_dz_tables(Geometry g, double *__restrict__ den, double *__rest
   }
 }
 
-// levels [0, R) of the column live in registers, levels [R, NZ) (long columns only) in an LDS
-// slice of the thread; every index below is a compile-time constant after unrolling
-template <int NZ, int R>
-struct RegColumn {
-  double F[R];
-  double *S;  // LDS slice, stride 64
-  __device__ __forceinline__ double get(int k) const { return k < R ? F[k < R ? k : 0] : S[(k >= R ? k - R : 0) * 64]; }
-  __device__ __forceinline__ void set(int k, double v) {
-    if (k < R) F[k < R ? k : 0] = v;
-    else S[(k >= R ? k - R : 0) * 64] = v;
-  }
-};
-
-template <int NZ, int R>
-__global__ void __launch_bounds__(64) k_laplinv_reg(Geometry g, ColFields cf,
-                                                    const double *__restrict__ k2a,
-                                                    double *__restrict__ cpscr, DevScalars *sc) {
-  __shared__ double lds[(NZ > R ? NZ - R : 1) * 64];
+// laplinv with four lanes per column: lane q * 16 + c holds levels [q L, q L + L) of column c
+// of the wavefront's 16, both the column values and the forward sweep's c', in registers.  The
+// sweeps run quarter after quarter (the others masked), the recurrence state crossing to the
+// next quarter by a lane shuffle, so no level waits on memory and every operation is the one of
+// laplinv_column in the same order: results are bit-identical.  A zero pivot is flagged and
+// raised after the sweeps (the reference stops there; the column's values are then moot).
+template <int NZ>
+__global__ void __launch_bounds__(64) k_laplinv_q4(Geometry g, ColFields cf,
+                                                   const double *__restrict__ k2a,
+                                                   DevScalars *sc) {
+  constexpr int L = (NZ + 3) / 4;
+  const int lane = threadIdx.x;
+  const int q = lane >> 4;
   const int64_t plane = (int64_t)g.sy * g.sx;
-  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t col = (int64_t)blockIdx.x * 16 + (lane & 15);
   if (col >= plane) return;
   const double k2 = k2a[col];
   if (k2 == 0) return;
   const int df0 = cf.df0[blockIdx.y];
   double *f = cf.out[blockIdx.y] + col;
-  double *CP = cpscr + (int64_t)blockIdx.y * NZ * plane + col;
-  RegColumn<NZ, R> F;
-  F.S = lds + threadIdx.x;
+  double F[L], CPl[L];
 #pragma unroll
-  for (int k = 0; k < NZ; k++) F.set(k, f[(int64_t)k * plane]);
+  for (int i = 0; i < L; i++) {
+    F[i] = 0.0;
+    CPl[i] = 0.0;
+  }
+#pragma unroll
+  for (int i = 0; i < L; i++)
+    if (q * L + i < NZ) F[i] = f[(int64_t)(q * L + i) * plane];
   const double f16 = (double)(1.f / 6.f), f13 = (double)(1.f / 3.f);
   const double f112 = (double)(1.f / 12.f), f56 = (double)(5.f / 6.f);
   const double dz = g.dz;
   const double i2dz2 = 2. / dz / dz, idz2 = 1. / dz / dz;
   const double ai = idz2 - f112 * k2, bi = -i2dz2 - f56 * k2, ci = idz2 - f112 * k2;
-  double fm = 0.0, f0 = F.get(0), fp = F.get(1);  // originals f(k-1), f(k), f(k+1)
-  double b1, c1, d1;
-  if (!df0) { b1 = 1.; c1 = 0.; d1 = 0.; }
-  else { b1 = -idz2 - f13 * k2; c1 = idz2 - f16 * k2; d1 = f13 * f0 + f16 * fp; }
-  double denominator = b1;
-  double fprev = d1 / denominator;
-  F.set(0, fprev);
-  double cprev = c1;
+  // the original of level q L + L, the next quarter's first
+  const double nxt0 = __shfl(F[0], lane + 16);
+  constexpr int PTOP = (NZ - 1) / L;  // quarter holding level NZ - 1
+  double denominator = 0.0, fprev = 0.0, cprev = 0.0, f0 = 0.0, fp = 0.0;
+  bool bad = false;
 #pragma unroll
-  for (int k = 2; k <= NZ; k++) {
-    fm = f0;
-    f0 = fp;
-    fp = (k < NZ) ? F.get(k < NZ ? k : 0) : 0.0;
-    double a, b, c, d;
-    if (k < NZ) {
-      a = ai; b = bi; c = ci;
-      d = f112 * fm + f56 * f0 + f112 * fp;
-    } else if (!df0) {
-      a = 0.; b = 1.; c = 0.; d = 0.;
-    } else {
-      a = idz2 - f16 * k2; b = -idz2 - f13 * k2; c = 0.;
-      d = f13 * f0 + f16 * fm;
+  for (int p = 0; p <= PTOP; p++) {
+    if (p > 0) {
+      denominator = __shfl(denominator, lane - 16);
+      fprev = __shfl(fprev, lane - 16);
+      cprev = __shfl(cprev, lane - 16);
+      f0 = __shfl(f0, lane - 16);
+      fp = __shfl(fp, lane - 16);
     }
-    const double cpk = cprev / denominator;
-    CP[(int64_t)(k - 1) * plane] = cpk;
-    denominator = b - a * cpk;
-    if (denominator == 0) { raise_status(sc, PMPIC_ERR_ZERO_PIVOT); return; }
-    fprev = (d - a * fprev) / (denominator);
-    F.set(k - 1, fprev);
-    cprev = c;
+    if (q == p) {
+#pragma unroll
+      for (int i = 0; i < L; i++) {
+        const int j = p * L + i;  // level (0-based); row j + 1 of the reference
+        if (j >= NZ) break;
+        const double onext = (i + 1 < L) ? F[i + 1 < L ? i + 1 : 0] : nxt0;  // original j + 1
+        if (j == 0) {
+          f0 = F[0];
+          fp = onext;
+          double b1, c1, d1;
+          if (!df0) { b1 = 1.; c1 = 0.; d1 = 0.; }
+          else { b1 = -idz2 - f13 * k2; c1 = idz2 - f16 * k2; d1 = f13 * f0 + f16 * fp; }
+          denominator = b1;
+          fprev = d1 / denominator;
+          F[0] = fprev;
+          cprev = c1;
+          continue;
+        }
+        const double fm = f0;
+        f0 = fp;
+        fp = (j + 1 < NZ) ? onext : 0.0;
+        double a, b, c, d;
+        if (j + 1 < NZ) {
+          a = ai; b = bi; c = ci;
+          d = f112 * fm + f56 * f0 + f112 * fp;
+        } else if (!df0) {
+          a = 0.; b = 1.; c = 0.; d = 0.;
+        } else {
+          a = idz2 - f16 * k2; b = -idz2 - f13 * k2; c = 0.;
+          d = f13 * f0 + f16 * fm;
+        }
+        const double cpk = cprev / denominator;
+        CPl[i] = cpk;
+        denominator = b - a * cpk;
+        bad |= (denominator == 0);
+        fprev = (d - a * fprev) / (denominator);
+        F[i] = fprev;
+        cprev = c;
+      }
+    }
   }
+  // backward sweep from the top quarter down: level j takes c' of level j + 1
+  const double cpn = __shfl(CPl[0], lane + 16);
   double fnext = fprev;
 #pragma unroll
-  for (int k = NZ - 1; k >= 1; k--) {
-    // c' loads are hoisted at most 32 levels ahead, which keeps the column in registers
-    if ((k & 31) == 0) __asm__ volatile("" ::: "memory");
-    const double v = F.get(k - 1) - CP[(int64_t)k * plane] * fnext;
-    F.set(k - 1, v);
-    fnext = v;
-  }
+  for (int p = PTOP; p >= 0; p--) {
+    if (p < PTOP) fnext = __shfl(fnext, lane + 16);
+    if (q == p) {
 #pragma unroll
-  for (int k = 0; k < NZ; k++) f[(int64_t)k * plane] = F.get(k);
+      for (int i = L - 1; i >= 0; i--) {
+        const int j = p * L + i;
+        if (j >= NZ - 1) continue;
+        const double cpj1 = (i + 1 < L) ? CPl[i + 1 < L ? i + 1 : 0] : cpn;
+        const double v = F[i] - cpj1 * fnext;
+        F[i] = v;
+        fnext = v;
+      }
+    }
+  }
+  if (bad) raise_status(sc, PMPIC_ERR_ZERO_PIVOT);
+#pragma unroll
+  for (int i = 0; i < L; i++)
+    if (q * L + i < NZ) f[(int64_t)(q * L + i) * plane] = F[i];
 }
 
 template <int NZ>
@@ -501,12 +534,19 @@ __global__ void __launch_bounds__(64) k_diffz_reg(Geometry g, ColFields cf,
                                                   const double *__restrict__ tden,
                                                   const double *__restrict__ tcp,
                                                   DevScalars *sc) {
+  // the Thomas tables are the same for every column: staged once per block in LDS, so the
+  // recurrence reads them without a memory round trip per level
+  __shared__ double den[NZ], cpt[NZ];
   const int64_t plane = (int64_t)g.sy * g.sx;
   const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= plane) return;
   const int fi = blockIdx.y;
   const bool bt = cf.bot[fi] != nullptr && cf.top[fi] != nullptr;
-  const double *den = tden + (bt ? NZ : 0), *cpt = tcp + (bt ? NZ : 0);
+  for (int k = threadIdx.x; k < NZ; k += blockDim.x) {
+    den[k] = tden[(bt ? NZ : 0) + k];
+    cpt[k] = tcp[(bt ? NZ : 0) + k];
+  }
+  __syncthreads();
+  if (col >= plane) return;
   const double *fin = cf.in[fi] + col;
   double *o = cf.out[fi] + col;
   double F[NZ];
@@ -815,8 +855,8 @@ int op_laplinv_n(Ctx *c, int nf, double *const *f, const int *df0) {
   switch (c->g.nzn) {
 #define PMPIC_LAP_CASE(NZ)                                                                    \
   case NZ:                                                                                     \
-    hipLaunchKernelGGL((k_laplinv_reg<NZ, (NZ > 100 ? 64 : NZ)>), dim3(cdiv(plane, 64), nf),    \
-                       dim3(64), 0, c->stream, c->g, rf, c->k2, c->lap_cp, c->dsc);             \
+    hipLaunchKernelGGL((k_laplinv_q4<NZ>), dim3(cdiv(plane, 16), nf), dim3(64), 0, c->stream,   \
+                       c->g, rf, c->k2, c->dsc);                                                \
     PMPIC_CHECK(hipGetLastError());                                                            \
     return PMPIC_OK;
     PMPIC_REG_COLUMNS(PMPIC_LAP_CASE)
