@@ -11,8 +11,8 @@ Both numeric forms run on the same initial condition: the FP32-bulk form (positi
 index + FP32 offsets, FP32 parcel arithmetic; `value`) and the plain FP64 form.  The velocity
 error of the FP32-bulk form against the FP64 form is measured after --err-steps steps on the
 --err-n grid.  Prints one JSON line: parcel-steps/s of each form, the roofline of each form's
-dominant kernel (HIP events on the library's stream around every launch in the timed region)
-and a CPU baseline (the C restatement of the same path, single-threaded, on a bounded sample).
+dominant kernel and of the other parcel<->grid kernel (HIP events on the library's stream around
+every launch in the timed region) and a CPU baseline (the C restatement of the same path, single-threaded, on a bounded sample).
 """
 import argparse
 import json
@@ -26,15 +26,16 @@ sys.path.insert(0, ROOT)
 METRIC = "parcel-steps/s FP32-bulk vs FP64 on 1 MI355X at ≤1e-5 vel err; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
 
-# algorithmic bytes per launch of the two hot kernels (see DESIGN.md "Roofline accounting")
+# algorithmic bytes per launch of the two hot kernels
 #  cell accumulation: per parcel the bucket entry (4 B) and x,y,z,vol,p,q,r,b,h (72 B);
 #    per cell two bucket offsets (8 B) and 40 corner partial sums written (320 B)
 #  gather + RK stage: per parcel position (24 B), RK state read/written for the stage, six
 #    tendencies written (48 B), direction flag (1 B); per node the six gathered grids (48 B)
 RK_STAGE_BYTES = {1: 24 + 24 + 18 * 8 + 48 + 1, 2: 24 + 12 * 8 + 12 * 8 + 48 + 1,
                   3: 24 + 12 * 8 + 12 * 8 + 48 + 1, 4: 24 + 7 * 8 + 13 * 8 + 48 + 1}
-# FP32-bulk form: cell accumulation reads per parcel the bucket entry, cell, offsets, vol,
-# p,q,r,b,h (44 B) and per cell two offsets and 40 FP32 partials (168 B); the RK stages move
+# FP32-bulk form: cell accumulation reads per parcel its slot entry, cell, offsets, vol,
+# p,q,r,b,h (44 B) and per cell its count and overflow offset and writes 40 FP32 partials
+# (168 B); the RK stages move
 # cell (4 B) + offsets (12 B) for the position and the FP32 RK state (see bulk32.hip)
 ACC_BYTES = {0: (76, 328), 1: (44, 168)}
 RK32_STAGE_BYTES = {1: 28 + 80, 2: 68 + 52, 3: 68 + 52, 4: 60 + 56}
@@ -151,30 +152,38 @@ def time_form(precision, ic, args, n, nz):
     nnode = n * n * (nz + 1)
     avg_np = total / max(args.steps, 1)
     form = "fp32" if precision else "fp64"
-    if p2g_ms >= rk_ms:
-        name = "k_cell_accum (par2grid per-cell corner sums)" if not precision else \
-            "k32_cell_accum (FP32 par2grid per-cell corner sums)"
-        family = "k32_cell_accum<0>" if precision else "k_cell_accum<0>"
-        bp, bc = ACC_BYTES[precision]
-        per_launch = avg_np * bp + ncell * bc
-        avg_ms = p2g_ms / max(p2g_calls, 1)
-    else:
-        name = "k_rk_stage (grid2par of 6 fields + RK4 stage update)" if not precision else \
-            "k32_rk_stage (FP32 grid2par of 6 fields + RK4 stage update)"
-        family = "k32_rk_stage" if precision else "k_rk_stage"
-        table = RK32_STAGE_BYTES if precision else RK_STAGE_BYTES
-        per_launch = avg_np * sum(table.values()) / 4 + nnode * GATHER_NODE_BYTES[precision]
-        avg_ms = rk_ms / max(rk_calls, 1)
-    achieved = per_launch / (avg_ms * 1e-3) / 1e9
-    traffic = measured_traffic(form, family, n, nz, args.ppc)
+    acc = {
+        "kernel": ("k32_cell_accum (FP32 par2grid per-cell corner sums)" if precision else
+                   "k_cell_accum (par2grid per-cell corner sums)"),
+        "family": "k32_cell_accum<0>" if precision else "k_cell_accum<0>",
+        "bytes": avg_np * ACC_BYTES[precision][0] + ncell * ACC_BYTES[precision][1],
+        "ms": p2g_ms, "calls": p2g_calls,
+    }
+    table = RK32_STAGE_BYTES if precision else RK_STAGE_BYTES
+    rks = {
+        "kernel": ("k32_rk_stage (FP32 grid2par of 6 fields + RK4 stage update)" if precision else
+                   "k_rk_stage (grid2par of 6 fields + RK4 stage update)"),
+        "family": "k32_rk_stage" if precision else "k_rk_stage",
+        "bytes": avg_np * sum(table.values()) / 4 + nnode * GATHER_NODE_BYTES[precision],
+        "ms": rk_ms, "calls": rk_calls,
+    }
+
+    def roof(k):
+        avg_ms = k["ms"] / max(k["calls"], 1)
+        achieved = k["bytes"] / (avg_ms * 1e-3) / 1e9
+        return {"bound": "hbm", "kernel": k["kernel"], "achieved": achieved,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                "traffic": measured_traffic(form, k["family"], n, nz, args.ppc),
+                "avg_launch_ms": avg_ms, "bytes_per_launch": k["bytes"],
+                "launches": k["calls"], "ms_total": k["ms"]}
+
+    dominant, other = (acc, rks) if p2g_ms >= rk_ms else (rks, acc)
     return {
         "value": total / elapsed,
         "ms_per_step": 1e3 * elapsed / max(args.steps, 1),
-        "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "avg_launch_ms": avg_ms, "bytes_per_launch": per_launch,
-                     "p2g_ms_total": p2g_ms, "p2g_launches": p2g_calls,
-                     "rk_ms_total": rk_ms, "rk_launches": rk_calls},
+        # the dominant kernel, and the other half of the parcel<->grid pair beside it
+        "roofline": roof(dominant),
+        "roofline_other": roof(other),
         "timers": timers,
     }
 
@@ -309,11 +318,14 @@ def main():
                                f"({npar} parcels), {form}, dt=0.01, split/mix on",
                    "parcels": npar, "grid": [n, n, nz], "form": form},
         "roofline": head["roofline"],
+        "roofline_other": head["roofline_other"],
         "timers": head["timers"],
     }
     if "fp64" in res:
         out["fp64"] = {"value": res["fp64"]["value"], "ms_per_step": res["fp64"]["ms_per_step"],
-                       "roofline": res["fp64"]["roofline"], "timers": res["fp64"]["timers"]}
+                       "roofline": res["fp64"]["roofline"],
+                       "roofline_other": res["fp64"]["roofline_other"],
+                       "timers": res["fp64"]["timers"]}
     if "fp32" in res and "fp64" in res:
         out["fp32_speedup_vs_fp64"] = res["fp32"]["value"] / res["fp64"]["value"]
     if args.err_steps > 0 and "fp32" in res:

@@ -24,10 +24,12 @@ namespace pmpic {
 
 namespace {
 
+// radix schedule: radix of stage s in bits [4 s, 4 s + 4) of `code` (held in registers; no
+// per-stage memory read)
 struct Plan {
   int n;
   int nrad;
-  int rad[16];
+  uint64_t code;
 };
 
 constexpr int FFT_TPB = 256;
@@ -112,7 +114,7 @@ __device__ __forceinline__ void stockham_stage(const double2 *x, double2 *y, con
   const int nq = n / R;
   const int tstep = n / (Ns * R);
   const int tot = nb * nq;
-  for (int bi = threadIdx.x; bi < tot; bi += blockDim.x) {
+  for (int bi = threadIdx.x; bi < tot; bi += FFT_TPB) {
     const int t = bi / nq, i = bi - t * nq;
     const int j = i / Ns, k = i - j * Ns;
     const double2 *xs = x + t * n + i;
@@ -134,7 +136,7 @@ template <bool INV>
 __device__ double2 *lds_fft(double2 *a, double2 *b, const double2 *tw, const Plan &p, int nb) {
   int Ns = 1;
   for (int s = 0; s < p.nrad; s++) {
-    const int r = p.rad[s];
+    const int r = (int)((p.code >> (4 * s)) & 15u);
     if (r == 4) stockham_stage<4, INV>(a, b, tw, p.n, Ns, nb);
     else if (r == 2) stockham_stage<2, INV>(a, b, tw, p.n, Ns, nb);
     else if (r == 3) stockham_stage<3, INV>(a, b, tw, p.n, Ns, nb);
@@ -148,11 +150,24 @@ __device__ double2 *lds_fft(double2 *a, double2 *b, const double2 *tw, const Pla
   return a;
 }
 
-__device__ void stage_twiddles(double2 *tws, const double2 *tw, int n, bool conj_) {
-  for (int e = threadIdx.x; e < n; e += blockDim.x) {
-    double2 w = tw[e];
-    if (conj_) w.y = -w.y;
-    tws[e] = w;
+// Loads into LDS are issued as a batch (each thread's elements loaded into registers, then
+// stored), so a workgroup waits on memory once per pass rather than once per element.
+constexpr int EPT = LDS_CPLX / FFT_TPB;  // elements per thread of a full LDS buffer
+constexpr int TPT = 512 / FFT_TPB;       // twiddles per thread (n <= 512)
+
+__device__ __forceinline__ void load_twiddles(double2 (&w)[TPT], const double2 *tw, int n) {
+#pragma unroll
+  for (int i = 0; i < TPT; i++) {
+    const int e = threadIdx.x + i * FFT_TPB;
+    if (e < n) w[i] = tw[e];
+  }
+}
+__device__ __forceinline__ void put_twiddles(double2 *tws, const double2 (&w)[TPT], int n,
+                                             bool conj_) {
+#pragma unroll
+  for (int i = 0; i < TPT; i++) {
+    const int e = threadIdx.x + i * FFT_TPB;
+    if (e < n) tws[e] = conj_ ? make_double2(w[i].x, -w[i].y) : w[i];
   }
 }
 
@@ -169,17 +184,27 @@ __global__ void __launch_bounds__(FFT_TPB) k_rows_r2c(Plan p, const double2 *__r
   const int n = p.n, nl = n / 2 + 1;
   const int r0 = blockIdx.x * 2 * nb;                  // first row of this workgroup
   const int nbb = min(nb, (nrows - r0 + 1) / 2);       // transforms (row pairs)
-  stage_twiddles(T, tw, n, false);
-  for (int e = threadIdx.x; e < nbb * n; e += blockDim.x) {
-    const int t = e / n, x = e - t * n;
-    const int ra = r0 + 2 * t, rb = ra + 1;
-    const double va = src[(int64_t)ra * n + x];
-    const double vb = (rb < nrows) ? src[(int64_t)rb * n + x] : 0.0;
-    A[e] = make_double2(va, vb);
+  double2 w[TPT], v[EPT];
+  load_twiddles(w, tw, n);
+#pragma unroll
+  for (int i = 0; i < EPT; i++) {
+    const int e = threadIdx.x + i * FFT_TPB;
+    if (e < nbb * n) {
+      const int t = e / n, x = e - t * n;
+      const int ra = r0 + 2 * t, rb = ra + 1;
+      v[i].x = src[(int64_t)ra * n + x];
+      v[i].y = (rb < nrows) ? src[(int64_t)rb * n + x] : 0.0;
+    }
+  }
+  put_twiddles(T, w, n, false);
+#pragma unroll
+  for (int i = 0; i < EPT; i++) {
+    const int e = threadIdx.x + i * FFT_TPB;
+    if (e < nbb * n) A[e] = v[i];
   }
   __syncthreads();
   double2 *R = lds_fft<false>(A, B, T, p, nbb);
-  for (int e = threadIdx.x; e < nbb * nl; e += blockDim.x) {
+  for (int e = threadIdx.x; e < nbb * nl; e += FFT_TPB) {
     const int t = e / nl, l = e - t * nl;
     const int ra = r0 + 2 * t, rb = ra + 1;
     const double2 z = R[t * n + l];
@@ -203,17 +228,33 @@ __global__ void __launch_bounds__(FFT_TPB) k_cols_fwd(Plan p, const double2 *__r
   const int k = blockIdx.y;
   const int l0 = blockIdx.x * nb;
   const int nbb = min(nb, nl - l0);
-  stage_twiddles(T, tw, ny, false);
   const double2 *lev = src + (int64_t)k * ny * nl;
-  for (int e = threadIdx.x; e < nbb * ny; e += blockDim.x) {
-    int y = e / nbb, t = e - y * nbb;
-    A[t * ny + y] = lev[(int64_t)y * nl + l0 + t];
+  double2 w[TPT];
+  double vx[EPT], vy[EPT];
+  int at[EPT];
+  load_twiddles(w, tw, ny);
+#pragma unroll
+  for (int i = 0; i < EPT; i++) {
+    const int e = threadIdx.x + i * FFT_TPB;
+    const int y = e / nbb, t = e - y * nbb;
+    at[i] = (e < nbb * ny) ? t * ny + y : -1;
+    vx[i] = 0.0;
+    vy[i] = 0.0;
+    if (at[i] >= 0) {
+      const double2 u = lev[(int64_t)y * nl + l0 + t];
+      vx[i] = u.x;
+      vy[i] = u.y;
+    }
   }
+  put_twiddles(T, w, ny, false);
+#pragma unroll
+  for (int i = 0; i < EPT; i++)
+    if (at[i] >= 0) A[at[i]] = make_double2(vx[i], vy[i]);
   __syncthreads();
   double2 *R = lds_fft<false>(A, B, T, p, nbb);
   const int nm = ny / 2 + 1;
   double *out = dst + (int64_t)k * sy * sx;
-  for (int e = threadIdx.x; e < nbb * nm; e += blockDim.x) {
+  for (int e = threadIdx.x; e < nbb * nm; e += FFT_TPB) {
     int m = e / nbb, t = e - m * nbb;
     int l = l0 + t;
     double2 g = R[t * ny + m];
@@ -241,27 +282,43 @@ __global__ void __launch_bounds__(FFT_TPB) k_rows_inv(Plan p, const double2 *__r
   const int nx = p.n;
   const int r0 = blockIdx.x * nb;
   const int nbb = min(nb, nrows - r0);
-  stage_twiddles(T, tw, nx, true);
   const int half = nx / 2;
   const bool even = (nx % 2) == 0;
-  for (int e = threadIdx.x; e < nbb * nx; e += blockDim.x) {
-    int t = e / nx, l = e - t * nx;
-    int row = r0 + t;          // row = k * nm + m
-    int k = row / nm, m = row - k * nm;
-    const double *sa = src + ((int64_t)k * (2 * nm) + 2 * m) * sx;
-    const double *sb = sa + sx;
-    int ll = (l <= half) ? l : nx - l;
-    double ar = sa[2 * ll], ai = sa[2 * ll + 1], br = sb[2 * ll], bi = sb[2 * ll + 1];
-    if (ll == 0 || (even && ll == half)) { ai = 0.0; bi = 0.0; }
-    double2 h;
-    if (l <= half) h = make_double2(ar - bi, ai + br);
-    else h = make_double2(ar + bi, -ai + br);
-    A[t * nx + l] = h;
+  double2 w[TPT], va[EPT], vb[EPT];
+  load_twiddles(w, tw, nx);
+#pragma unroll
+  for (int i = 0; i < EPT; i++) {
+    const int e = threadIdx.x + i * FFT_TPB;
+    if (e < nbb * nx) {
+      const int t = e / nx, l = e - t * nx;
+      const int row = r0 + t;  // row = k * nm + m
+      const int k = row / nm, m = row - k * nm;
+      const double *sa = src + ((int64_t)k * (2 * nm) + 2 * m) * sx;
+      const int ll = (l <= half) ? l : nx - l;
+      va[i] = *reinterpret_cast<const double2 *>(sa + 2 * ll);       // (ar, ai)
+      vb[i] = *reinterpret_cast<const double2 *>(sa + sx + 2 * ll);  // (br, bi)
+    }
+  }
+  put_twiddles(T, w, nx, true);
+#pragma unroll
+  for (int i = 0; i < EPT; i++) {
+    const int e = threadIdx.x + i * FFT_TPB;
+    if (e < nbb * nx) {
+      const int t = e / nx, l = e - t * nx;
+      const int ll = (l <= half) ? l : nx - l;
+      const double ar = va[i].x, br = vb[i].x;
+      double ai = va[i].y, bi = vb[i].y;
+      if (ll == 0 || (even && ll == half)) { ai = 0.0; bi = 0.0; }
+      double2 h;
+      if (l <= half) h = make_double2(ar - bi, ai + br);
+      else h = make_double2(ar + bi, -ai + br);
+      A[t * nx + l] = h;
+    }
   }
   __syncthreads();
   double2 *R = lds_fft<true>(A, B, T, p, nbb);
   const double dn = 1.0 / (double)nx;
-  for (int e = threadIdx.x; e < nbb * nx; e += blockDim.x) {
+  for (int e = threadIdx.x; e < nbb * nx; e += FFT_TPB) {
     int t = e / nx, x = e - t * nx;
     double2 v = R[t * nx + x];
     dst[(int64_t)(r0 + t) * nx + x] = make_double2(v.x * dn, v.y * dn);
@@ -282,24 +339,39 @@ __global__ void __launch_bounds__(FFT_TPB) k_cols_c2r(Plan p, const double2 *__r
   const int k = blockIdx.y;
   const int x0 = blockIdx.x * 2 * nb;
   const int nbb = min(nb, (nx - x0 + 1) / 2);
-  stage_twiddles(T, tw, ny, true);
   const double2 *lev = src + (int64_t)k * nm * nx;
   const bool even = (ny % 2) == 0;
-  for (int e = threadIdx.x; e < nbb * ny; e += blockDim.x) {
-    const int y = e / nbb, t = e - y * nbb;
-    const int xa = x0 + 2 * t, xb = xa + 1;
-    const int m = (y < nm) ? y : ny - y;
-    double2 wa = lev[(int64_t)m * nx + xa];
-    double2 wb = (xb < nx) ? lev[(int64_t)m * nx + xb] : make_double2(0.0, 0.0);
-    if (m == 0 || (even && m == ny / 2)) { wa.y = 0.0; wb.y = 0.0; }
-    if (y >= nm) { wa.y = -wa.y; wb.y = -wb.y; }
-    A[t * ny + y] = make_double2(wa.x - wb.y, wa.y + wb.x);
+  double2 w[TPT], va[EPT], vb[EPT];
+  load_twiddles(w, tw, ny);
+#pragma unroll
+  for (int i = 0; i < EPT; i++) {
+    const int e = threadIdx.x + i * FFT_TPB;
+    if (e < nbb * ny) {
+      const int y = e / nbb, t = e - y * nbb;
+      const int xa = x0 + 2 * t, xb = xa + 1;
+      const int m = (y < nm) ? y : ny - y;
+      va[i] = lev[(int64_t)m * nx + xa];
+      vb[i] = (xb < nx) ? lev[(int64_t)m * nx + xb] : make_double2(0.0, 0.0);
+    }
+  }
+  put_twiddles(T, w, ny, true);
+#pragma unroll
+  for (int i = 0; i < EPT; i++) {
+    const int e = threadIdx.x + i * FFT_TPB;
+    if (e < nbb * ny) {
+      const int y = e / nbb, t = e - y * nbb;
+      const int m = (y < nm) ? y : ny - y;
+      double2 wa = va[i], wb = vb[i];
+      if (m == 0 || (even && m == ny / 2)) { wa.y = 0.0; wb.y = 0.0; }
+      if (y >= nm) { wa.y = -wa.y; wb.y = -wb.y; }
+      A[t * ny + y] = make_double2(wa.x - wb.y, wa.y + wb.x);
+    }
   }
   __syncthreads();
   double2 *R = lds_fft<true>(A, B, T, p, nbb);
   const double dn = 1.0 / (double)ny;
   double *out = dst + (int64_t)k * ny * nx;
-  for (int e = threadIdx.x; e < nbb * ny; e += blockDim.x) {
+  for (int e = threadIdx.x; e < nbb * ny; e += FFT_TPB) {
     const int y = e / nbb, t = e - y * nbb;
     const int xa = x0 + 2 * t, xb = xa + 1;
     const double2 v = R[t * ny + y];
@@ -312,11 +384,15 @@ Plan make_plan(int n) {
   Plan p{};
   p.n = n;
   int m = n;
-  while (m % 4 == 0) { p.rad[p.nrad++] = 4; m /= 4; }
-  while (m % 2 == 0) { p.rad[p.nrad++] = 2; m /= 2; }
-  while (m % 3 == 0) { p.rad[p.nrad++] = 3; m /= 3; }
-  while (m % 5 == 0) { p.rad[p.nrad++] = 5; m /= 5; }
-  if (m != 1) p.nrad = -1;
+  auto add = [&](int r) {
+    p.code |= (uint64_t)r << (4 * p.nrad);
+    p.nrad++;
+  };
+  while (m % 4 == 0) { add(4); m /= 4; }
+  while (m % 2 == 0) { add(2); m /= 2; }
+  while (m % 3 == 0) { add(3); m /= 3; }
+  while (m % 5 == 0) { add(5); m /= 5; }
+  if (m != 1 || p.nrad > 16) p.nrad = -1;
   return p;
 }
 
