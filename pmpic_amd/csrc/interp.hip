@@ -335,28 +335,36 @@ __global__ void k_node_sum(Geometry g, const double *__restrict__ partial, int64
   const int i = (int)(t % g.nx);
   const int j = (int)((t / g.nx) % g.ny);
   const int k = (int)(t / ((int64_t)g.nx * g.ny));
+  // every corner's partials are loaded first (a corner below the bottom or above the top cell
+  // reads cell 0 and is left out of the sum), then summed in the fixed corner order
+  double v[8][NV + 1];
+  bool ok[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int bx = q & 1, by = (q >> 1) & 1, bz = q >> 2;
+    int ci = i - bx;
+    if (ci < 0) ci += g.nx;
+    int cj = j - by;
+    if (cj < 0) cj += g.ny;
+    const int ck = k - bz;
+    ok[q] = ck >= 0 && ck < g.ncz;
+    const int64_t c = ok[q] ? ((int64_t)ck * g.ny + cj) * g.nx + ci : 0;
+#pragma unroll
+    for (int f = 0; f <= NV; f++) v[q][f] = partial[(int64_t)(q * (NV + 1) + f) * ncell + c];
+  }
   double s[NV + 1];
 #pragma unroll
   for (int f = 0; f <= NV; f++) s[f] = 0.0;
 #pragma unroll
-  for (int bx = 0; bx < 2; bx++) {
-    int ci = i - bx;
-    if (ci < 0) ci += g.nx;
+  for (int bx = 0; bx < 2; bx++)
 #pragma unroll
-    for (int by = 0; by < 2; by++) {
-      int cj = j - by;
-      if (cj < 0) cj += g.ny;
+    for (int by = 0; by < 2; by++)
 #pragma unroll
       for (int bz = 0; bz < 2; bz++) {
-        const int ck = k - bz;
-        if (ck < 0 || ck >= g.ncz) continue;
-        const int64_t c = ((int64_t)ck * g.ny + cj) * g.nx + ci;
         const int q = (bz << 2) | (by << 1) | bx;
 #pragma unroll
-        for (int f = 0; f <= NV; f++) s[f] += partial[(int64_t)(q * (NV + 1) + f) * ncell + c];
+        for (int f = 0; f <= NV; f++) s[f] = ok[q] ? s[f] + v[q][f] : s[f];
       }
-    }
-  }
   if (NORM) {
     double w = s[0];
     if (w == 0.) w = 1.;
