@@ -51,14 +51,27 @@ __global__ void k_bin_count(Geometry g, const double *__restrict__ X, const doub
   const int64_t w0 = (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64 * BIN_SEQ;
   // all atomics of the wavefront are issued before any result is used, in group order
   int base[BIN_SEQ], hls[BIN_SEQ];
+  // the positions of the next BIN_AHEAD groups are loaded before the current group is located
+  // (clamped index, n >= 1), so the wavefront does not wait on memory once per group
+  constexpr int BIN_AHEAD = 4;
+  double px[BIN_SEQ], py[BIN_SEQ], pz[BIN_SEQ];
+  auto load_group = [&](int it) {
+    const int64_t p = min(w0 + (int64_t)it * 64 + lane, n - 1);
+    px[it] = X[p];
+    py[it] = Y[p];
+    pz[it] = Z[p];
+  };
+#pragma unroll
+  for (int it = 0; it < BIN_AHEAD; it++) load_group(it);
 #pragma unroll
   for (int it = 0; it < BIN_SEQ; it++) {
+    if (it + BIN_AHEAD < BIN_SEQ) load_group(it + BIN_AHEAD);
     const int64_t p = w0 + (int64_t)it * 64 + lane;
     int32_t c = -1;
     if (p < n) {
       int i0, j0, k0;
       double dx_, dy_, dz_;
-      const int st = locate(g, X[p], Y[p], Z[p], i0, j0, k0, dx_, dy_, dz_);
+      const int st = locate(g, px[it], py[it], pz[it], i0, j0, k0, dx_, dy_, dz_);
       if (st != PMPIC_OK) raise_status(sc, st);
       else c = (int32_t)(((int64_t)k0 * g.ny + j0) * g.nx + i0);
       cell[p] = c;
