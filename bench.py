@@ -33,11 +33,11 @@ HBM_PEAK_GBS = 8000.0
 #    tendencies written (48 B), direction flag (1 B); per node the six gathered grids (48 B)
 RK_STAGE_BYTES = {1: 24 + 24 + 18 * 8 + 48 + 1, 2: 24 + 12 * 8 + 12 * 8 + 48 + 1,
                   3: 24 + 12 * 8 + 12 * 8 + 48 + 1, 4: 24 + 7 * 8 + 13 * 8 + 48 + 1}
-# FP32-bulk form: cell accumulation reads per parcel its slot entry, cell, offsets, vol,
-# p,q,r,b,h (44 B) and per cell its count and overflow offset and writes 40 FP32 partials
-# (168 B); the RK stages move
+# FP32-bulk form: cell accumulation reads per parcel its slot entry, offsets, vol, p,q,r,b,h
+# (40 B; the z level comes from the cell) and per cell its count and overflow offset and
+# writes 40 FP32 partials (168 B); the RK stages move
 # cell (4 B) + offsets (12 B) for the position and the FP32 RK state (see bulk32.hip)
-ACC_BYTES = {0: (76, 328), 1: (44, 168)}
+ACC_BYTES = {0: (76, 328), 1: (40, 168)}
 RK32_STAGE_BYTES = {1: 28 + 80, 2: 68 + 52, 3: 68 + 52, 4: 60 + 56}
 GATHER_NODE_BYTES = {0: 48, 1: 24}
 

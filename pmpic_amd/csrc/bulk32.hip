@@ -361,6 +361,7 @@ __global__ void __launch_bounds__(ATPB) k32_cell_accum(Bulk32 b, int64_t cap, Ge
   __shared__ int32_t soff[TILE32 + 1];  // first dense row of each cell of the tile
   __shared__ int32_t scnt[TILE32], sost[TILE32];
   __shared__ uint32_t sover;            // bit cl: cell cl is listed in the dense overflow region
+  __shared__ float skz[TILE32];         // z level of each cell of the tile
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int cl = tid >> 3;
@@ -393,6 +394,7 @@ __global__ void __launch_bounds__(ATPB) k32_cell_accum(Bulk32 b, int64_t cap, Ge
       scnt[tid] = v;
       sost[tid] = o;
       soff[tid + 1] = incl;
+      skz[tid] = (tid < ntile) ? (float)((c0 + tid) / ((int64_t)g.nx * g.ny)) : 0.f;
     }
     if (tid == 0) {
       soff[0] = 0;
@@ -403,26 +405,24 @@ __global__ void __launch_bounds__(ATPB) k32_cell_accum(Bulk32 b, int64_t cap, Ge
   const float *F = b.f;
   auto fld = [&](int f) { return F + (size_t)f * cap; };
   // one parcel's staged values
-  auto fetch = [&](uint32_t pp, float4 &geo, float4 &va, float &vb, int32_t &pc) {
+  auto fetch = [&](uint32_t pp, float4 &geo, float4 &va, float &vb) {
     geo = make_float4(ldnt(fld(B_OX), pp), ldnt(fld(B_OY), pp), ldnt(fld(B_OZ), pp),
                       ldnt(fld(B_VOL), pp));
     if constexpr (!MIX) {
-      pc = ldnt(b.cell, pp);
       va = make_float4(ldnt(fld(B_P), pp), ldnt(fld(B_Q), pp), ldnt(fld(B_R), pp),
                        ldnt(fld(B_B), pp));
       vb = ldnt(fld(B_H), pp);  // humidity, for the total buoyancy below
     } else {
-      pc = 0;
       va = make_float4(ldnt(fld(B_B), pp), ldnt(fld(B_H), pp), ldnt(fld(B_P), pp),
                        ldnt(fld(B_Q), pp));
       vb = ldnt(fld(B_R), pp);
     }
   };
-  auto put = [&](int r, int32_t pp, float4 geo, float4 v, float vb, int32_t pc) {
+  // kz: the z level of the parcel's cell (known from the tile, so the cell is not read)
+  auto put = [&](int r, int32_t pp, float4 geo, float4 v, float vb, float kz) {
     if constexpr (!MIX) {
       // total buoyancy (vorticity_tendency.F90:121-126)
-      const int kc = pc >> 20;
-      const float z = zm.zbot + ((float)kc + geo.z) * zm.dz;
+      const float z = zm.zbot + (kz + geo.z) * zm.dz;
       const float hl = vb - expf(-z);
       v.w = v.w + 12.5f * (hl > 0.f ? hl : 0.f);
     } else {
@@ -463,13 +463,12 @@ __global__ void __launch_bounds__(ATPB) k32_cell_accum(Bulk32 b, int64_t cap, Ge
       }
       float4 geo[SPT], va[SPT];
       float vb[SPT];
-      int32_t pc[SPT];
 #pragma unroll
       for (int k = 0; k < SPT; k++)
-        if (pr[k] >= 0) fetch((uint32_t)pr[k], geo[k], va[k], vb[k], pc[k]);
+        if (pr[k] >= 0) fetch((uint32_t)pr[k], geo[k], va[k], vb[k]);
 #pragma unroll
       for (int k = 0; k < SPT; k++)
-        if (pr[k] >= 0) put(row[k], pr[k], geo[k], va[k], vb[k], pc[k]);
+        if (pr[k] >= 0) put(row[k], pr[k], geo[k], va[k], vb[k], skz[(tid + k * ATPB) / CAPC]);
       // rows of the cells listed in the dense overflow region
       const uint32_t pass = (last - first >= 32 ? ~0u : ((1u << (last - first)) - 1u)) << first;
       if (sover & pass) {
@@ -487,9 +486,8 @@ __global__ void __launch_bounds__(ATPB) k32_cell_accum(Bulk32 b, int64_t cap, Ge
           const int32_t pp = ld(sorted, (uint32_t)(sost[lo_] + (r + r0 - soff[lo_])));
           float4 g1, v1;
           float b1;
-          int32_t c1;
-          fetch((uint32_t)pp, g1, v1, b1, c1);
-          put(r, pp, g1, v1, b1, c1);
+          fetch((uint32_t)pp, g1, v1, b1);
+          put(r, pp, g1, v1, b1, skz[lo_]);
         }
       }
     }
@@ -548,8 +546,7 @@ __global__ void __launch_bounds__(ATPB) k32_cell_accum(Bulk32 b, int64_t cap, Ge
       const float vol = F[B_VOL * cap + pp];
       float vals[5];
       if constexpr (!MIX) {
-        const int kc = b.cell[pp] >> 20;
-        const float z = zm.zbot + ((float)kc + oz) * zm.dz;
+        const float z = zm.zbot + (skz[cl] + oz) * zm.dz;
         const float hl = F[B_H * cap + pp] - expf(-z);
         vals[0] = F[B_P * cap + pp];
         vals[1] = F[B_Q * cap + pp];
