@@ -125,7 +125,9 @@ int pmpic_last_counts(pmpic_ctx *ctx, int64_t out3[3]);
 /* HIP-event timing on the context's stream: enable, then read the accumulated milliseconds and
  * counts of timer `which`: 0 = per-cell par2grid accumulation kernel, 1 = fused grid2par + RK
  * stage kernel, 2..6 = the component callbacks vort2vel, vorticity_tendency, rk4_integrator,
- * parcel_splitting (active stages), parcel_mixing (active stages).  pmpic_timing_name gives a
+ * parcel_splitting (active stages), parcel_mixing (active stages).  enable: 0 = off, 1 = every
+ * timer, 2 = timers 0 and 1 only.  Timers 0 and 1 are taken by the kernel dispatches themselves;
+ * timers 2..6 record events around the component on the stream.  pmpic_timing_name gives a
  * timer's region name (the reference's timer names for the components, timer.F90:247-406),
  * NULL past the last. */
 int pmpic_timing_enable(pmpic_ctx *ctx, int32_t enable);

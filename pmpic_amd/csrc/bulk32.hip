@@ -1137,12 +1137,10 @@ int b32_get_parcels(Ctx *c, double *const *fields) {
 int b32_par2grid(Ctx *c) {
   const int64_t n = c->n;
   PMPIC_TRY(bin32(c, n));
-  PMPIC_TRY(timing_begin(c, 0));
-  hipLaunchKernelGGL(k32_cell_accum<A32_P2G>, dim3(cdiv(c->ncell, TILE32)), dim3(ATPB), 0,
-                     c->stream, c->b32, c->cap, c->g, zmap_of(c->g), (float)c->g.vmin,
-                     c->cell_count, c->cell_start, c->sorted, c->ncell, partial32(c), nullptr,
-                     nullptr);
-  PMPIC_TRY(timing_end(c, 0));
+  PMPIC_TRY(launch_timed(c, T_ACCUM, k32_cell_accum<A32_P2G>, dim3(cdiv(c->ncell, TILE32)),
+                         dim3(ATPB), 0, c->b32, c->cap, c->g, zmap_of(c->g), (float)c->g.vmin,
+                         c->cell_count, c->cell_start, c->sorted, c->ncell, partial32(c), nullptr,
+                         nullptr));
   Out32<double> out = {{c->grid[GP], c->grid[GQ], c->grid[GR], c->grid[GB], nullptr, nullptr}};
   const int64_t nnode = (int64_t)c->g.nzn * c->g.ny * c->g.nx;
   hipLaunchKernelGGL((k32_node_sum<4, true, true, double>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0,
@@ -1163,31 +1161,29 @@ int b32_grids_from(Ctx *c, int first, int count) {
 int b32_rk_stage(Ctx *c, int stage, int stretch) {
   const int64_t n = c->n;
   if (n == 0) return PMPIC_OK;
-  PMPIC_TRY(timing_begin(c, 1));
   Grid32Set gs = {{c->b32.grid[0], c->b32.grid[1], c->b32.grid[2], c->b32.grid[3],
                    c->b32.grid[4], c->b32.grid[5]}};
   const Scale32 s3 = scale_of(c->g);
   const dim3 gb(cdiv(n, TPB)), tb(TPB);
   switch (stage) {
     case 1:
-      hipLaunchKernelGGL(k32_rk_stage<1>, gb, tb, 0, c->stream, c->g, c->b32, c->cap, n, gs, s3,
-                         stretch, c->flag, c->dsc);
+      PMPIC_TRY(launch_timed(c, T_RK, k32_rk_stage<1>, gb, tb, 0, c->g, c->b32, c->cap, n, gs,
+                             s3, stretch, c->flag, c->dsc));
       break;
     case 2:
-      hipLaunchKernelGGL(k32_rk_stage<2>, gb, tb, 0, c->stream, c->g, c->b32, c->cap, n, gs, s3,
-                         stretch, c->flag, c->dsc);
+      PMPIC_TRY(launch_timed(c, T_RK, k32_rk_stage<2>, gb, tb, 0, c->g, c->b32, c->cap, n, gs,
+                             s3, stretch, c->flag, c->dsc));
       break;
     case 3:
-      hipLaunchKernelGGL(k32_rk_stage<3>, gb, tb, 0, c->stream, c->g, c->b32, c->cap, n, gs, s3,
-                         stretch, c->flag, c->dsc);
+      PMPIC_TRY(launch_timed(c, T_RK, k32_rk_stage<3>, gb, tb, 0, c->g, c->b32, c->cap, n, gs,
+                             s3, stretch, c->flag, c->dsc));
       break;
     default:
-      hipLaunchKernelGGL(k32_rk_stage<4>, gb, tb, 0, c->stream, c->g, c->b32, c->cap, n, gs, s3,
-                         stretch, c->flag, c->dsc);
+      PMPIC_TRY(launch_timed(c, T_RK, k32_rk_stage<4>, gb, tb, 0, c->g, c->b32, c->cap, n, gs,
+                             s3, stretch, c->flag, c->dsc));
       break;
   }
   PMPIC_CHECK(hipGetLastError());
-  PMPIC_TRY(timing_end(c, 1));
   c->split_flags_fresh = (stage == 4 && stretch) ? 1 : 0;
   return PMPIC_OK;
 }

@@ -19,14 +19,14 @@ struct pmpic_ctx {
 namespace pmpic {
 
 int timing_begin(Ctx *c, int which) {
-  if (!c->timing) return PMPIC_OK;
+  if (c->timing != 1) return PMPIC_OK;
   if (c->nev[which] >= 64) PMPIC_TRY(timing_harvest(c));
   PMPIC_CHECK(hipEventRecord(c->ev[which][c->nev[which]][0], c->stream));
   return PMPIC_OK;
 }
 
 int timing_end(Ctx *c, int which) {
-  if (!c->timing) return PMPIC_OK;
+  if (c->timing != 1) return PMPIC_OK;
   PMPIC_CHECK(hipEventRecord(c->ev[which][c->nev[which]][1], c->stream));
   c->nev[which]++;
   return PMPIC_OK;
@@ -759,7 +759,7 @@ int pmpic_timing_enable(pmpic_ctx *ctx, int32_t enable) {
   if (!ctx) return PMPIC_ERR_ARG;
   Ctx *c = &ctx->c;
   PMPIC_TRY(timing_harvest(c));
-  c->timing = enable ? 1 : 0;
+  c->timing = (enable == 2) ? 2 : (enable ? 1 : 0);
   for (int w = 0; w < NTIMERS; w++) {
     c->t_ms[w] = 0;
     c->t_calls[w] = 0;

@@ -11,6 +11,7 @@
 //    that one thread per (kx,ky) column walks z with unit-stride, coalesced across threads;
 //  * cells: c = (k*ny + j)*nx + i over ncz*ny*nx cells.
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stddef.h>
@@ -237,9 +238,31 @@ int op_laplinv(Ctx *c, double *f, int df0);
 int grid_max_reduce(Ctx *c);
 // parcels.hip
 int rk_stage_update(Ctx *c, int stage, int64_t n, int stretch);
+// Component timers (timing mode 1) bracket a component with events on the stream.
 int timing_begin(Ctx *c, int which);
 int timing_end(Ctx *c, int which);
 int timing_harvest(Ctx *c);
+
+// A hot kernel (timer T_ACCUM or T_RK, timing modes 1 and 2): when timing is on, the start
+// and stop timestamps are taken by the dispatch itself (hipExtLaunchKernelGGL), so the timer
+// puts no packet between kernels on the stream.
+template <typename... KArgs, typename... Args>
+int launch_timed(Ctx *c, int which, void (*kernel)(KArgs...), dim3 grid, dim3 block,
+                 uint32_t shmem, Args... args) {
+  static_assert(sizeof...(KArgs) == sizeof...(Args), "kernel argument count");
+  if (!c->timing) {
+    hipLaunchKernelGGL(kernel, grid, block, shmem, c->stream, static_cast<KArgs>(args)...);
+    return PMPIC_OK;
+  }
+  if (c->nev[which] >= 64) {
+    const int st = timing_harvest(c);
+    if (st != PMPIC_OK) return st;
+  }
+  hipExtLaunchKernelGGL(kernel, grid, block, shmem, c->stream, c->ev[which][c->nev[which]][0],
+                        c->ev[which][c->nev[which]][1], 0u, static_cast<KArgs>(args)...);
+  c->nev[which]++;
+  return PMPIC_OK;
+}
 int parcel_haloswap(Ctx *c, int64_t n, int reflect);
 int parcel_split(Ctx *c);
 int parcel_mix(Ctx *c);

@@ -465,9 +465,10 @@ static void launch_accum_lds(Ctx *c, const double *var, double *partial, double 
 int par2grid_fused(Ctx *c, int64_t n) {
   PMPIC_TRY(bin_parcels(c, n));
   const int64_t ncell = c->ncell;
-  PMPIC_TRY(timing_begin(c, 0));
-  launch_accum_lds<ACC_P2G, 4, ROWS_P2G>(c, nullptr, c->partial, nullptr, nullptr);
-  PMPIC_TRY(timing_end(c, 0));
+  PMPIC_TRY(launch_timed(c, T_ACCUM, k_cell_accum_lds<ACC_P2G, 4, ROWS_P2G>,
+                         dim3(cdiv(c->ncell, TILE)), dim3(TILE * 8), 0, c->g, c->par, c->cap,
+                         nullptr, c->cell_start, c->sorted, c->ncell, c->partial, nullptr,
+                         nullptr));
   OutTable table = {{c->grid[GP], c->grid[GQ], c->grid[GR], c->grid[GB], nullptr, nullptr}};
   const int64_t nnode = (int64_t)c->g.nzn * c->g.ny * c->g.nx;
   hipLaunchKernelGGL((k_node_sum<4, true>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0, c->stream, c->g,

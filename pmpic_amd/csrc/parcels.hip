@@ -556,28 +556,26 @@ static int haloswap_permute(Ctx *c, int64_t n, int64_t nblk) {
 int rk_stage_update(Ctx *c, int stage, int64_t n, int stretch) {
   const int64_t nblk = cdiv(n > 0 ? n : 1, TPB);
   if (nblk > c->nblk_max) return PMPIC_ERR_ARG;
-  PMPIC_TRY(timing_begin(c, 1));
   GridSet gs = {{c->grid[GU], c->grid[GV], c->grid[GW], c->grid[GDP], c->grid[GDQ], c->grid[GDR]}};
   switch (stage) {
     case 1:
-      hipLaunchKernelGGL(k_rk_stage<1>, dim3(nblk), dim3(TPB), 0, c->stream, c->g, c->par, c->cap,
-                         n, gs, stretch, c->dsc, c->flag, c->blk, nblk);
+      PMPIC_TRY(launch_timed(c, T_RK, k_rk_stage<1>, dim3(nblk), dim3(TPB), 0, c->g, c->par,
+                             c->cap, n, gs, stretch, c->dsc, c->flag, c->blk, nblk));
       break;
     case 2:
-      hipLaunchKernelGGL(k_rk_stage<2>, dim3(nblk), dim3(TPB), 0, c->stream, c->g, c->par, c->cap,
-                         n, gs, stretch, c->dsc, c->flag, c->blk, nblk);
+      PMPIC_TRY(launch_timed(c, T_RK, k_rk_stage<2>, dim3(nblk), dim3(TPB), 0, c->g, c->par,
+                             c->cap, n, gs, stretch, c->dsc, c->flag, c->blk, nblk));
       break;
     case 3:
-      hipLaunchKernelGGL(k_rk_stage<3>, dim3(nblk), dim3(TPB), 0, c->stream, c->g, c->par, c->cap,
-                         n, gs, stretch, c->dsc, c->flag, c->blk, nblk);
+      PMPIC_TRY(launch_timed(c, T_RK, k_rk_stage<3>, dim3(nblk), dim3(TPB), 0, c->g, c->par,
+                             c->cap, n, gs, stretch, c->dsc, c->flag, c->blk, nblk));
       break;
     default:
-      hipLaunchKernelGGL(k_rk_stage<4>, dim3(nblk), dim3(TPB), 0, c->stream, c->g, c->par, c->cap,
-                         n, gs, stretch, c->dsc, c->flag, c->blk, nblk);
+      PMPIC_TRY(launch_timed(c, T_RK, k_rk_stage<4>, dim3(nblk), dim3(TPB), 0, c->g, c->par,
+                             c->cap, n, gs, stretch, c->dsc, c->flag, c->blk, nblk));
       break;
   }
   PMPIC_CHECK(hipGetLastError());
-  PMPIC_TRY(timing_end(c, 1));
   return haloswap_permute(c, n, nblk);
 }
 

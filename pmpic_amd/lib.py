@@ -335,8 +335,11 @@ class Context:
         return f
 
     # ---- timing ----
-    def timing_enable(self, on=True):
-        self._chk(self.L.pmpic_timing_enable(self.h, 1 if on else 0), "pmpic_timing_enable")
+    def timing_enable(self, on=True, kernels_only=False):
+        """Timers on (every timer) or off; kernels_only keeps just the two hot kernels' timers,
+        which are taken by the kernel dispatches themselves and add nothing between kernels."""
+        mode = (2 if kernels_only else 1) if on else 0
+        self._chk(self.L.pmpic_timing_enable(self.h, mode), "pmpic_timing_enable")
 
     def timing_read(self, which):
         ms, calls = ctypes.c_double(), ctypes.c_int64()

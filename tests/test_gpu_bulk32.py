@@ -124,3 +124,23 @@ def test_ensemble_concurrent_matches_sequential():
         pa, pb = a.get_parcels(), b.get_parcels()
         for k in ("x", "y", "z", "p", "q", "r", "vol", "b", "h", "stretch"):
             np.testing.assert_array_equal(pa[k], pb[k])
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_timers_kernels_only_and_all(precision):
+    """Timing mode 2 keeps only the two hot kernels' timers (taken by their dispatches); mode 1
+    adds the component timers.  Four accumulations and four RK stage launches per step."""
+    shape = (12, 10, 8)
+    ic = thermal_case(*shape, 2, vort=1.0)
+    g = make(shape, precision, ic)
+    g.timing_enable(True, kernels_only=True)
+    g.step(2)
+    t = g.timer_table()
+    assert t["par2grid_cells"][1] == 8 and t["grid2par_rk_stage"][1] == 8
+    assert t["par2grid_cells"][0] > 0 and t["grid2par_rk_stage"][0] > 0
+    assert t["vort2vel"][1] == 0 and t["RK4_all_steps"][1] == 0
+    g.timing_enable(True)
+    g.step(1)
+    t = g.timer_table()
+    assert t["par2grid_cells"][1] == 4 and t["vort2vel"][1] == 4 and t["RK4_all_steps"][1] == 4
+    assert t["vort2vel"][0] > 0
