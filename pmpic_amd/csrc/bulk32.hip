@@ -644,11 +644,6 @@ __global__ void k32_node_sum(Geometry g, const float *__restrict__ partial, int6
   }
 }
 
-__global__ void k32_grid_from_f64(const double *__restrict__ a, float *__restrict__ b, int64_t n) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) b[t] = (float)a[t];
-}
-
 __global__ void k32_grid_to_f64(const float *__restrict__ a, double *__restrict__ b, int64_t n) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) b[t] = (double)a[t];
@@ -1149,15 +1144,6 @@ int b32_par2grid(Ctx *c) {
   return PMPIC_OK;
 }
 
-int b32_grids_from(Ctx *c, int first, int count) {
-  const GridId src[6] = {GU, GV, GW, GDP, GDQ, GDR};
-  for (int f = first; f < first + count; f++)
-    hipLaunchKernelGGL(k32_grid_from_f64, dim3(cdiv(c->gsz, TPB)), dim3(TPB), 0, c->stream,
-                       c->grid[src[f]], c->b32.grid[f], c->gsz);
-  PMPIC_CHECK(hipGetLastError());
-  return PMPIC_OK;
-}
-
 int b32_rk_stage(Ctx *c, int stage, int stretch) {
   const int64_t n = c->n;
   if (n == 0) return PMPIC_OK;
@@ -1264,9 +1250,7 @@ int b32_mix(Ctx *c) {
     int64_t slast;
     if (nadd - 1 < Z) {
       int32_t v;
-      PMPIC_CHECK(hipMemcpyAsync(&v, c->list_a + (nadd - 1), sizeof(int32_t),
-                                 hipMemcpyDeviceToHost, c->stream));
-      PMPIC_CHECK(hipStreamSynchronize(c->stream));
+      PMPIC_TRY(read_back(c, &v, c->list_a + (nadd - 1), sizeof(int32_t)));
       slast = v;
     } else {
       slast = norig + (nadd - 1 - Z);
@@ -1284,9 +1268,7 @@ int b32_mix(Ctx *c) {
       dst = c->list_a + nadd;
     } else if (nadd == Z && Z > 0) {
       int32_t v;
-      PMPIC_CHECK(hipMemcpyAsync(&v, c->list_a + (Z - 1), sizeof(int32_t), hipMemcpyDeviceToHost,
-                                 c->stream));
-      PMPIC_CHECK(hipStreamSynchronize(c->stream));
+      PMPIC_TRY(read_back(c, &v, c->list_a + (Z - 1), sizeof(int32_t)));
       if (v == norig - 1) {
         R = 1;
         dst = c->list_a + (Z - 1);

@@ -204,6 +204,8 @@ int free_all(Ctx *c) {
       if (c->ev[w][e][0]) hipEventDestroy(c->ev[w][e][0]);
       if (c->ev[w][e][1]) hipEventDestroy(c->ev[w][e][1]);
     }
+  if (c->sync_ev) hipEventDestroy(c->sync_ev);
+  if (c->pin) hipHostFree(c->pin);
   if (c->stream) hipStreamDestroy(c->stream);
   return PMPIC_OK;
 }
@@ -286,6 +288,8 @@ int create(const pmpic_config *cf, Ctx *c) {
   c->nblk_max = cdiv(big, 256) + 1;
   PMPIC_TRY(dalloc(&c->blk, 20 * c->nblk_max + 64));
   PMPIC_TRY(dalloc(&c->dsc, 1));
+  PMPIC_CHECK(hipHostMalloc(&c->pin, 256));
+  PMPIC_CHECK(hipEventCreateWithFlags(&c->sync_ev, hipEventDisableTiming));
   for (int w = 0; w < NTIMERS; w++)
     for (int e = 0; e < 64; e++) {
       PMPIC_CHECK(hipEventCreate(&c->ev[w][e][0]));
@@ -431,8 +435,7 @@ int pmpic_vort2vel(pmpic_ctx *ctx) {
   PMPIC_TRY(timing_begin(c, T_VORT2VEL));
   if (c->precision == 1) {
     PMPIC_TRY(b32_par2grid(c));
-    PMPIC_TRY(vort2vel_spectral(c, stage));
-    PMPIC_TRY(b32_grids_from(c, 0, 3));
+    PMPIC_TRY(vort2vel_spectral(c, stage));  // also writes the FP32 u, v, w
   } else {
     PMPIC_TRY(par2grid_fused(c, c->n));
     PMPIC_TRY(vort2vel_spectral(c, stage));
@@ -447,8 +450,7 @@ int pmpic_vorticity_tendency(pmpic_ctx *ctx) {
   Ctx *c = &ctx->c;
   const int stage = stage_of(c->it_vt);
   PMPIC_TRY(timing_begin(c, T_VORT_TEND));
-  PMPIC_TRY(vorticity_tendency_spectral(c, stage));
-  if (c->precision == 1) PMPIC_TRY(b32_grids_from(c, 3, 3));
+  PMPIC_TRY(vorticity_tendency_spectral(c, stage));  // and the FP32 tendencies
   if (stage == 1) {
     PMPIC_TRY(grid_max_reduce(c));
   }

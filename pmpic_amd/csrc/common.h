@@ -134,6 +134,8 @@ struct Ctx {
   int64_t nblk_max;
   DevScalars *dsc;                    // device scalars
   DevScalars hsc;                     // host mirror
+  void *pin;                          // pinned host staging for small read-backs (256 B)
+  hipEvent_t sync_ev;                 // read-back completion (spin wait)
   // HIP-event timing: 0 = par2grid cell accumulation kernel, 1 = gather + RK stage kernel,
   // 2..6 = the component callbacks (timer regions named as the reference's timers)
   int timing;
@@ -225,7 +227,9 @@ int mixing_grids(Ctx *c, int64_t n);
 // fft.hip
 int fft_setup(Ctx *c);
 int fft_forward(Ctx *c, int nf, const double *const *src, double *const *dst);
-int fft_backward(Ctx *c, int nf, const double *const *src, double *const *dst);
+// dst32 (optional, entries may be NULL): an FP32 copy of each real output
+int fft_backward(Ctx *c, int nf, const double *const *src, double *const *dst,
+                 float *const *dst32 = nullptr);
 // spectral.hip
 int spectral_setup(Ctx *c);
 int vort2vel_spectral(Ctx *c, int stage);
@@ -242,6 +246,8 @@ int rk_stage_update(Ctx *c, int stage, int64_t n, int stretch);
 int timing_begin(Ctx *c, int which);
 int timing_end(Ctx *c, int which);
 int timing_harvest(Ctx *c);
+// device-to-host read of a few bytes (<= 256) in stream order, through pinned staging
+int read_back(Ctx *c, void *host, const void *dev, size_t bytes);
 
 // A hot kernel (timer T_ACCUM or T_RK, timing modes 1 and 2): when timing is on, the start
 // and stop timestamps are taken by the dispatch itself (hipExtLaunchKernelGGL), so the timer
@@ -274,7 +280,6 @@ void b32_free(Ctx *c);
 int b32_set_parcels(Ctx *c, int64_t n, const double *const *fields);
 int b32_get_parcels(Ctx *c, double *const *fields);
 int b32_par2grid(Ctx *c);                 // P, Q, R, B grids (FP64) from the FP32 parcels
-int b32_grids_from(Ctx *c, int first, int count);  // FP32 copies of grid U..W (0..2) / DP..DR (3..5)
 int b32_rk_stage(Ctx *c, int stage, int stretch);
 int b32_split(Ctx *c);
 int b32_mix(Ctx *c);

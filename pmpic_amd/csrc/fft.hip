@@ -39,6 +39,7 @@ constexpr int MAXF = 8;
 struct FieldPtrs {
   const double *in[MAXF];
   double *out[MAXF];
+  float *out32[MAXF];  // optional FP32 copy of a real output (the FP32-bulk form's gathers)
 };
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
@@ -371,12 +372,17 @@ __global__ void __launch_bounds__(FFT_TPB) k_cols_c2r(Plan p, const double2 *__r
   double2 *R = lds_fft<true>(A, B, T, p, nbb);
   const double dn = 1.0 / (double)ny;
   double *out = dst + (int64_t)k * ny * nx;
+  float *o32 = fp.out32[f] ? fp.out32[f] + (int64_t)k * ny * nx : nullptr;
   for (int e = threadIdx.x; e < nbb * ny; e += FFT_TPB) {
     const int y = e / nbb, t = e - y * nbb;
     const int xa = x0 + 2 * t, xb = xa + 1;
     const double2 v = R[t * ny + y];
     out[(int64_t)y * nx + xa] = v.x * dn;
     if (xb < nx) out[(int64_t)y * nx + xb] = v.y * dn;
+    if (o32) {
+      o32[(int64_t)y * nx + xa] = (float)(v.x * dn);
+      if (xb < nx) o32[(int64_t)y * nx + xb] = (float)(v.y * dn);
+    }
   }
 }
 
@@ -447,11 +453,16 @@ int fft_forward(Ctx *c, int nf, const double *const *src, double *const *dst) {
   return PMPIC_OK;
 }
 
-int fft_backward(Ctx *c, int nf, const double *const *src, double *const *dst) {
+int fft_backward(Ctx *c, int nf, const double *const *src, double *const *dst,
+                 float *const *dst32) {
   const Geometry &g = c->g;
   if (nf < 1 || nf > MAXF) return PMPIC_ERR_ARG;
   FieldPtrs fp{};
-  for (int f = 0; f < nf; f++) { fp.in[f] = src[f]; fp.out[f] = dst[f]; }
+  for (int f = 0; f < nf; f++) {
+    fp.in[f] = src[f];
+    fp.out[f] = dst[f];
+    fp.out32[f] = dst32 ? dst32[f] : nullptr;
+  }
   Plan px = make_plan(g.nx), py = make_plan(g.ny);
   const int64_t c2f = c2_field_size(g);
   const int nm = g.ny / 2 + 1;

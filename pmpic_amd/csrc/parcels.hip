@@ -20,6 +20,8 @@
 //    compaction and permutation is deterministic.
 #include "common.h"
 
+#include <string.h>
+
 #include <hipcub/hipcub.hpp>
 #include <math.h>
 
@@ -501,9 +503,20 @@ __global__ void k_mix_backfill(double *__restrict__ par, int64_t cap,
 
 }  // namespace
 
+// The step's small read-backs go through pinned staging and wait on an event without the
+// blocking-sync flag, which the host spins on: a pageable copy plus a stream synchronisation
+// costs several times the latency.
+int read_back(Ctx *c, void *host, const void *dev, size_t bytes) {
+  if (bytes > 256) return PMPIC_ERR_ARG;
+  PMPIC_CHECK(hipMemcpyAsync(c->pin, dev, bytes, hipMemcpyDeviceToHost, c->stream));
+  PMPIC_CHECK(hipEventRecord(c->sync_ev, c->stream));
+  PMPIC_CHECK(hipEventSynchronize(c->sync_ev));
+  memcpy(host, c->pin, bytes);
+  return PMPIC_OK;
+}
+
 int sync_scalars(Ctx *c) {
-  PMPIC_CHECK(hipMemcpyAsync(&c->hsc, c->dsc, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
-  PMPIC_CHECK(hipStreamSynchronize(c->stream));
+  PMPIC_TRY(read_back(c, &c->hsc, c->dsc, sizeof(DevScalars)));
   return c->hsc.status;
 }
 
@@ -537,9 +550,7 @@ static int haloswap_permute(Ctx *c, int64_t n, int64_t nblk) {
   PMPIC_TRY(scan_exclusive_i32(c, blk, pre, 9 * nblk));
   hipLaunchKernelGGL(k_cross_total, dim3(1), dim3(1), 0, c->stream, pre, blk, nblk, c->dsc);
   // the staging buffer must hold every crossing parcel; read the count back
-  PMPIC_CHECK(hipMemcpyAsync(&c->hsc.cross, &c->dsc->cross, sizeof(int64_t), hipMemcpyDeviceToHost,
-                             c->stream));
-  PMPIC_CHECK(hipStreamSynchronize(c->stream));
+  PMPIC_TRY(read_back(c, &c->hsc.cross, &c->dsc->cross, sizeof(int64_t)));
   const int64_t K = c->hsc.cross;
   if (K == 0) return PMPIC_OK;
   PMPIC_TRY(ensure_tmp(c, K));
@@ -650,9 +661,7 @@ int parcel_mix(Ctx *c) {
     int64_t slast;
     if (nadd - 1 < Z) {
       int32_t v;
-      PMPIC_CHECK(hipMemcpyAsync(&v, c->list_a + (nadd - 1), sizeof(int32_t), hipMemcpyDeviceToHost,
-                                 c->stream));
-      PMPIC_CHECK(hipStreamSynchronize(c->stream));
+      PMPIC_TRY(read_back(c, &v, c->list_a + (nadd - 1), sizeof(int32_t)));
       slast = v;
     } else {
       slast = norig + (nadd - 1 - Z);
@@ -671,7 +680,7 @@ int parcel_mix(Ctx *c) {
       dst = c->list_a + nadd;
     } else if (nadd == Z && Z > 0) {
       int32_t v;
-      PMPIC_CHECK(hipMemcpy(&v, c->list_a + (Z - 1), sizeof(int32_t), hipMemcpyDeviceToHost));
+      PMPIC_TRY(read_back(c, &v, c->list_a + (Z - 1), sizeof(int32_t)));
       if (v == norig - 1) {
         R = 1;
         dst = c->list_a + (Z - 1);

@@ -649,7 +649,8 @@ __global__ void k_tendency(int64_t nnode, const double *__restrict__ P,
                            const double *__restrict__ dbdx, const double *__restrict__ dvdx,
                            const double *__restrict__ dvdy, const double *__restrict__ dwdy,
                            double *__restrict__ DP, double *__restrict__ DQ,
-                           double *__restrict__ DR) {
+                           double *__restrict__ DR, float *__restrict__ DP32,
+                           float *__restrict__ DQ32, float *__restrict__ DR32) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nnode) return;
   const double p = P[q], qq = Q[q], r = R[q];
@@ -666,6 +667,11 @@ __global__ void k_tendency(int64_t nnode, const double *__restrict__ P,
   dr = dr + r * (-dudx[q]);
   dr = dr + r * (-dvdy[q]);
   DP[q] = dp; DQ[q] = dq; DR[q] = dr;
+  if (DP32) {
+    DP32[q] = (float)dp;
+    DQ32[q] = (float)dq;
+    DR32[q] = (float)dr;
+  }
 }
 
 // maxima of u^2, v^2, w^2 and p^2+q^2+r^2 over the interior (vort2vel.F90:331-337,
@@ -930,7 +936,11 @@ int vort2vel_spectral(Ctx *c, int stage) {
     const double *src[6] = {c->sd, c->se, c->sf, c->sder[2], c->sder[3], c->sder[4]};
     double *dst[6] = {c->grid[GP], c->grid[GQ], c->grid[GR], c->grid[GU], c->grid[GV],
                       c->grid[GW]};
-    PMPIC_TRY(fft_backward(c, 6, src, dst));
+    // the FP32-bulk form gathers u, v, w from FP32 copies written by the same pass
+    float *dst32[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (c->precision == 1)
+      for (int a = 0; a < 3; a++) dst32[3 + a] = c->b32.grid[a];
+    PMPIC_TRY(fft_backward(c, 6, src, dst, dst32));
   }
   PMPIC_CHECK(hipGetLastError());
   return PMPIC_OK;
@@ -958,10 +968,12 @@ int vorticity_tendency_spectral(Ctx *c, int stage) {
     PMPIC_TRY(fft_backward(c, 8, src, dst));
   }
   const int64_t nnode = (int64_t)g.nzn * g.ny * g.nx;
+  const bool b32 = c->precision == 1;  // FP32 copies of the tendencies for the FP32-bulk gathers
   hipLaunchKernelGGL(k_tendency, dim3(cdiv(nnode, TPB)), dim3(TPB), 0, c->stream, nnode,
                      c->grid[GP], c->grid[GQ], c->grid[GR], c->gtmp[0], c->gtmp[1], c->gtmp[2],
                      c->gtmp[3], c->gtmp[4], c->gtmp[5], c->gtmp[6], c->gtmp[7], c->grid[GDP],
-                     c->grid[GDQ], c->grid[GDR]);
+                     c->grid[GDQ], c->grid[GDR], b32 ? c->b32.grid[3] : nullptr,
+                     b32 ? c->b32.grid[4] : nullptr, b32 ? c->b32.grid[5] : nullptr);
   PMPIC_CHECK(hipGetLastError());
   return PMPIC_OK;
 }
