@@ -828,11 +828,18 @@ __global__ void k32_split_apply(Geometry g, Bulk32 b, int64_t cap, int64_t n, Sc
 }
 
 // ---- mixing (parcel_mixing.F90:157-389) ----
+// KPT parcels per thread, at stride TPB (each a coalesced pass), the loads issued together
+constexpr int KPT = 4;
 __global__ void k32_keep_flags(const float *__restrict__ F, int64_t cap, int64_t n, float vmin,
                                uint8_t *__restrict__ keep) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  keep[p] = (F[B_VOL * cap + p] < vmin) ? 0 : 1;
+  const int64_t p0 = (int64_t)blockIdx.x * (TPB * KPT) + threadIdx.x;
+  const float *vol = F + (int64_t)B_VOL * cap;
+  float v[KPT];
+#pragma unroll
+  for (int k = 0; k < KPT; k++) v[k] = vol[min(p0 + (int64_t)k * TPB, n - 1)];
+#pragma unroll
+  for (int k = 0; k < KPT; k++)
+    if (p0 + (int64_t)k * TPB < n) keep[p0 + (int64_t)k * TPB] = (v[k] < vmin) ? 0 : 1;
 }
 
 __global__ void k32_newcell_flags(Geometry g, const int32_t *__restrict__ npercell,
@@ -1224,7 +1231,7 @@ int b32_mix(Ctx *c) {
     hipLaunchKernelGGL(k32_grid_to_f64, dim3(cdiv(nnode, TPB)), dim3(TPB), 0, c->stream,
                        b.mgrid[f], c->grid[kept_ids[f]], nnode);
   if (norig > 0)
-    hipLaunchKernelGGL(k32_keep_flags, dim3(cdiv(norig, TPB)), dim3(TPB), 0, c->stream, b.f,
+    hipLaunchKernelGGL(k32_keep_flags, dim3(cdiv(norig, TPB * KPT)), dim3(TPB), 0, c->stream, b.f,
                        c->cap, norig, vmin, c->flag);
   PMPIC_TRY(compact_flags(c, c->flag, 0, norig, c->list_a, &c->dsc->nzero));
   hipLaunchKernelGGL(k32_newcell_flags, dim3(cdiv(c->ncell, TPB)), dim3(TPB), 0, c->stream, g,

@@ -263,14 +263,61 @@ __global__ void k_hs_scatter(double *__restrict__ par, int64_t cap, const double
 }
 
 // ---- generic compaction: ascending indices i < n with flags[i] == want ----
+// A thread takes FPT consecutive flags (one 16-byte load when all are in range), so a block
+// covers FPB flags; the flags arrays start at allocation boundaries, so the loads are aligned.
+constexpr int FPT = 16;
+constexpr int FPB = FPT * TPB;
+
+// bit q set when flags[i0 + q] == want (i0 < n)
+__device__ __forceinline__ uint32_t flag_bits(const uint8_t *__restrict__ flags, uint8_t want,
+                                              int64_t i0, int64_t n) {
+  uint32_t m = 0;
+  if (i0 + FPT <= n) {
+    const uint4 v = *reinterpret_cast<const uint4 *>(flags + i0);
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+#pragma unroll
+      for (int b = 0; b < 4; b++)
+        if (((w[k] >> (8 * b)) & 0xffu) == want) m |= 1u << (4 * k + b);
+  } else {
+    for (int q = 0; q < FPT && i0 + q < n; q++)
+      if (flags[i0 + q] == want) m |= 1u << q;
+  }
+  return m;
+}
+
+// exclusive prefix sum of v over the block in thread order; total = the block's sum
+__device__ __forceinline__ int block_excl_scan(int v, int &total, int *lds) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int incl = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int u = __shfl_up(incl, d);
+    if (lane >= d) incl += u;
+  }
+  if (lane == 63) lds[wave] = incl;
+  __syncthreads();
+  int off = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < NWAVE; w++) {
+    const int sw = lds[w];
+    if (w < wave) off += sw;
+    tot += sw;
+  }
+  __syncthreads();
+  total = tot;
+  return off + incl - v;
+}
+
 __global__ void __launch_bounds__(TPB) k_flag_count(const uint8_t *__restrict__ flags,
                                                     uint8_t want, int64_t n,
                                                     int32_t *__restrict__ blk) {
   __shared__ int lds[NWAVE];
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool pr = (i < n) && flags[i] == want;
-  int r, t;
-  block_rank(pr, r, t, lds);
+  const int64_t i0 = ((int64_t)blockIdx.x * TPB + threadIdx.x) * FPT;
+  const uint32_t m = (i0 < n) ? flag_bits(flags, want, i0, n) : 0u;
+  int t;
+  block_excl_scan(__popc(m), t, lds);
   if (threadIdx.x == 0) blk[blockIdx.x] = t;
 }
 
@@ -281,11 +328,14 @@ __global__ void __launch_bounds__(TPB) k_flag_scatter(const uint8_t *__restrict_
                                                       int64_t nblk, int32_t *__restrict__ out,
                                                       int64_t *__restrict__ count) {
   __shared__ int lds[NWAVE];
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool pr = (i < n) && flags[i] == want;
-  int r, t;
-  block_rank(pr, r, t, lds);
-  if (pr) out[pre[blockIdx.x] + r] = (int32_t)i;
+  const int64_t i0 = ((int64_t)blockIdx.x * TPB + threadIdx.x) * FPT;
+  uint32_t m = (i0 < n) ? flag_bits(flags, want, i0, n) : 0u;
+  int t;
+  int64_t o = (int64_t)pre[blockIdx.x] + block_excl_scan(__popc(m), t, lds);
+  while (m) {
+    out[o++] = (int32_t)(i0 + (__ffs(m) - 1));
+    m &= m - 1u;
+  }
   if (blockIdx.x == nblk - 1 && threadIdx.x == 0 && count)
     *count = (int64_t)pre[nblk - 1] + blk[nblk - 1];
 }
@@ -338,11 +388,18 @@ __global__ void k_split_apply(double *__restrict__ par, int64_t cap, int64_t n,
 }
 
 // ---- mixing (parcel_mixing.F90:157-389) ----
+// KPT parcels per thread, at stride TPB (each a coalesced pass), the loads issued together
+constexpr int KPT = 4;
 __global__ void k_keep_flags(const double *__restrict__ par, int64_t cap, int64_t n, double vmin,
                              uint8_t *__restrict__ keep) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  keep[p] = (par[FVOL * cap + p] < vmin) ? 0 : 1;
+  const int64_t p0 = (int64_t)blockIdx.x * (TPB * KPT) + threadIdx.x;
+  const double *vol = par + (int64_t)FVOL * cap;
+  double v[KPT];
+#pragma unroll
+  for (int k = 0; k < KPT; k++) v[k] = vol[min(p0 + (int64_t)k * TPB, n - 1)];
+#pragma unroll
+  for (int k = 0; k < KPT; k++)
+    if (p0 + (int64_t)k * TPB < n) keep[p0 + (int64_t)k * TPB] = (v[k] < vmin) ? 0 : 1;
 }
 
 // flags over cells in the reference's loop order (i outer, j, k inner; parcel_mixing.F90:160-162)
@@ -522,7 +579,7 @@ int sync_scalars(Ctx *c) {
 
 int compact_flags(Ctx *c, const uint8_t *flags, uint8_t want, int64_t n, int32_t *out,
                   int64_t *d_count) {
-  const int64_t nblk = cdiv(n > 0 ? n : 1, TPB);
+  const int64_t nblk = cdiv(n > 0 ? n : 1, FPB);
   if (nblk > c->nblk_max) return PMPIC_ERR_ARG;
   int32_t *blk = c->blk, *pre = c->blk + nblk;
   hipLaunchKernelGGL(k_flag_count, dim3(nblk), dim3(TPB), 0, c->stream, flags, want, n, blk);
@@ -630,7 +687,7 @@ int parcel_mix(Ctx *c) {
   const int64_t norig = c->n;
   PMPIC_TRY(mixing_grids(c, norig));
   if (norig > 0)
-    hipLaunchKernelGGL(k_keep_flags, dim3(cdiv(norig, TPB)), dim3(TPB), 0, c->stream, c->par,
+    hipLaunchKernelGGL(k_keep_flags, dim3(cdiv(norig, TPB * KPT)), dim3(TPB), 0, c->stream, c->par,
                        c->cap, norig, g.vmin, c->flag);
   // removed slots, ascending
   PMPIC_TRY(compact_flags(c, c->flag, 0, norig, c->list_a, &c->dsc->nzero));
