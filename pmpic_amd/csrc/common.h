@@ -227,9 +227,12 @@ int mixing_grids(Ctx *c, int64_t n);
 // fft.hip
 int fft_setup(Ctx *c);
 int fft_forward(Ctx *c, int nf, const double *const *src, double *const *dst);
-// dst32 (optional, entries may be NULL): an FP32 copy of each real output
+// dst32 (optional, entries may be NULL): an FP32 copy of each real output.
+// deriv (optional): per field, SPEC_PLAIN, or SPEC_DX / SPEC_DY to transform diffx / diffy of
+// the input (applied while the input is read, so the derivative is never stored)
+enum SpecOp { SPEC_PLAIN = 0, SPEC_DX = 1, SPEC_DY = 2 };
 int fft_backward(Ctx *c, int nf, const double *const *src, double *const *dst,
-                 float *const *dst32 = nullptr);
+                 float *const *dst32 = nullptr, const int *deriv = nullptr);
 // spectral.hip
 int spectral_setup(Ctx *c);
 int vort2vel_spectral(Ctx *c, int stage);

@@ -40,6 +40,8 @@ struct FieldPtrs {
   const double *in[MAXF];
   double *out[MAXF];
   float *out32[MAXF];  // optional FP32 copy of a real output (the FP32-bulk form's gathers)
+  int op[MAXF];        // SpecOp applied to the input of the inverse (k_rows_inv)
+  const double *kx, *ky;  // wavenumber planes for SPEC_DX / SPEC_DY
 };
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
@@ -279,6 +281,7 @@ __global__ void __launch_bounds__(FFT_TPB) k_rows_inv(Plan p, const double2 *__r
   __shared__ double2 A[LDS_CPLX], B[LDS_CPLX], T[512];
   const int f = blockIdx.y;
   const double *src = fp.in[f];
+  const int op = fp.op[f];
   double2 *dst = c2 + (int64_t)f * c2_field;
   const int nx = p.n;
   const int r0 = blockIdx.x * nb;
@@ -298,6 +301,22 @@ __global__ void __launch_bounds__(FFT_TPB) k_rows_inv(Plan p, const double2 *__r
       const int ll = (l <= half) ? l : nx - l;
       va[i] = *reinterpret_cast<const double2 *>(sa + 2 * ll);       // (ar, ai)
       vb[i] = *reinterpret_cast<const double2 *>(sa + sx + 2 * ll);  // (br, bi)
+      if (op != SPEC_PLAIN) {
+        // the derivative of spectral rows 2m (a) and 2m+1 (b), element for element as the
+        // standalone diffx / diffy compute it (fft_ops.F90:266-269, :366-369); the
+        // wavenumber planes are indexed by (row, x) like the spectral plane
+        const double *kp = (op == SPEC_DX ? fp.kx : fp.ky) + (int64_t)(2 * m) * sx + 2 * ll;
+        const double2 ka = *reinterpret_cast<const double2 *>(kp);
+        const double2 kb = *reinterpret_cast<const double2 *>(kp + sx);
+        const double2 a0 = va[i], b0 = vb[i];
+        if (op == SPEC_DX) {  // pair swap along x within each row
+          va[i] = make_double2((-1. * a0.y) * ka.x, a0.x * ka.y);
+          vb[i] = make_double2((-1. * b0.y) * kb.x, b0.x * kb.y);
+        } else {  // pair swap along y between the two rows
+          va[i] = make_double2((-1. * b0.x) * ka.x, (-1. * b0.y) * ka.y);
+          vb[i] = make_double2(a0.x * kb.x, a0.y * kb.y);
+        }
+      }
     }
   }
   put_twiddles(T, w, nx, true);
@@ -454,14 +473,18 @@ int fft_forward(Ctx *c, int nf, const double *const *src, double *const *dst) {
 }
 
 int fft_backward(Ctx *c, int nf, const double *const *src, double *const *dst,
-                 float *const *dst32) {
+                 float *const *dst32, const int *deriv) {
   const Geometry &g = c->g;
   if (nf < 1 || nf > MAXF) return PMPIC_ERR_ARG;
   FieldPtrs fp{};
+  fp.kx = c->kx;
+  fp.ky = c->ky;
   for (int f = 0; f < nf; f++) {
     fp.in[f] = src[f];
     fp.out[f] = dst[f];
     fp.out32[f] = dst32 ? dst32[f] : nullptr;
+    fp.op[f] = deriv ? deriv[f] : SPEC_PLAIN;
+    if (fp.op[f] < SPEC_PLAIN || fp.op[f] > SPEC_DY) return PMPIC_ERR_ARG;
   }
   Plan px = make_plan(g.nx), py = make_plan(g.ny);
   const int64_t c2f = c2_field_size(g);

@@ -616,31 +616,6 @@ __global__ void k_velocity_spectra(Geometry g, const double *__restrict__ a,
   us[q] = u * fl; vs[q] = v * fl; ws[q] = w * fl;
 }
 
-// the eight spectral derivatives of vorticity_tendency.F90:155-232
-__global__ void k_tendency_spectra(Geometry g, const double *__restrict__ bs,
-                                   const double *__restrict__ us, const double *__restrict__ vs,
-                                   const double *__restrict__ ws, double *__restrict__ o0,
-                                   double *__restrict__ o1, double *__restrict__ o2,
-                                   double *__restrict__ o3, double *__restrict__ o4,
-                                   double *__restrict__ o5, double *__restrict__ o6,
-                                   double *__restrict__ o7, const double *__restrict__ kx,
-                                   const double *__restrict__ ky) {
-  const int64_t plane = (int64_t)g.sy * g.sx;
-  const int64_t ssz = (int64_t)g.nzn * plane;
-  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= ssz) return;
-  const int64_t col = q % plane;
-  const int ii = (int)(q % g.sx), jj = (int)(col / g.sx);
-  o0[q] = dy_at(bs, ky, q, jj, g.sx, col);   // db/dy
-  o1[q] = dx_at(us, kx, q, ii, col);         // du/dx
-  o2[q] = dy_at(us, ky, q, jj, g.sx, col);   // du/dy
-  o3[q] = dx_at(ws, kx, q, ii, col);         // dw/dx
-  o4[q] = dx_at(bs, kx, q, ii, col);         // db/dx
-  o5[q] = dx_at(vs, kx, q, ii, col);         // dv/dx
-  o6[q] = dy_at(vs, ky, q, jj, g.sx, col);   // dv/dy
-  o7[q] = dy_at(ws, ky, q, jj, g.sx, col);   // dw/dy
-}
-
 // vorticity_tendency.F90:169-265, physical interior, accumulation order as written
 __global__ void k_tendency(int64_t nnode, const double *__restrict__ P,
                            const double *__restrict__ Q, const double *__restrict__ R,
@@ -683,14 +658,27 @@ __global__ void k_max4(int64_t nnode, const double *__restrict__ U, const double
                        DevScalars *sc) {
   __shared__ double sm[4][TPB];
   double m[4] = {0, 0, 0, 0};
-  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nnode;
-       q += (int64_t)gridDim.x * blockDim.x) {
-    const double u = U[q], v = V[q], w = W[q];
-    const double o = P[q] * P[q] + Q[q] * Q[q] + R[q] * R[q];
-    m[0] = fmax(m[0], u * u);
-    m[1] = fmax(m[1], v * v);
-    m[2] = fmax(m[2], w * w);
-    m[3] = fmax(m[3], o);
+  // MPT nodes per pass, their loads issued together; an index past the end is clamped to the
+  // last node, which the maximum tolerates
+  constexpr int MPT = 4;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t q0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q0 < nnode;
+       q0 += MPT * stride) {
+    double u[MPT], v[MPT], w[MPT], pp[MPT], qq[MPT], rr[MPT];
+#pragma unroll
+    for (int k = 0; k < MPT; k++) {
+      const int64_t q = min(q0 + k * stride, nnode - 1);
+      u[k] = U[q]; v[k] = V[q]; w[k] = W[q];
+      pp[k] = P[q]; qq[k] = Q[q]; rr[k] = R[q];
+    }
+#pragma unroll
+    for (int k = 0; k < MPT; k++) {
+      const double o = pp[k] * pp[k] + qq[k] * qq[k] + rr[k] * rr[k];
+      m[0] = fmax(m[0], u[k] * u[k]);
+      m[1] = fmax(m[1], v[k] * v[k]);
+      m[2] = fmax(m[2], w[k] * w[k]);
+      m[3] = fmax(m[3], o);
+    }
   }
   for (int f = 0; f < 4; f++) sm[f][threadIdx.x] = m[f];
   __syncthreads();
@@ -951,21 +939,19 @@ int vort2vel_spectral(Ctx *c, int stage) {
 int vorticity_tendency_spectral(Ctx *c, int stage) {
   (void)stage;
   const Geometry &g = c->g;
-  const int64_t ssz = ssz_of(g);
-  const dim3 eb(cdiv(ssz, TPB)), et(TPB);
   {
     const double *src[1] = {c->grid[GB]};
     double *dst[1] = {c->sbs};
     PMPIC_TRY(fft_forward(c, 1, src, dst));
   }
-  hipLaunchKernelGGL(k_tendency_spectra, eb, et, 0, c->stream, g, c->sbs, c->su, c->sv, c->sw,
-                     c->sder[0], c->sder[1], c->sder[2], c->sder[3], c->sder[4], c->sder[5],
-                     c->sder[6], c->sder[7], c->kx, c->ky);
   {
-    const double *src[8];
+    // the eight spectral derivatives of vorticity_tendency.F90:155-232 (db/dy, du/dx, du/dy,
+    // dw/dx, db/dx, dv/dx, dv/dy, dw/dy), each taken while its inverse transform reads it
+    const double *src[8] = {c->sbs, c->su, c->su, c->sw, c->sbs, c->sv, c->sv, c->sw};
+    const int op[8] = {SPEC_DY, SPEC_DX, SPEC_DY, SPEC_DX, SPEC_DX, SPEC_DX, SPEC_DY, SPEC_DY};
     double *dst[8];
-    for (int f = 0; f < 8; f++) { src[f] = c->sder[f]; dst[f] = c->gtmp[f]; }
-    PMPIC_TRY(fft_backward(c, 8, src, dst));
+    for (int f = 0; f < 8; f++) dst[f] = c->gtmp[f];
+    PMPIC_TRY(fft_backward(c, 8, src, dst, nullptr, op));
   }
   const int64_t nnode = (int64_t)g.nzn * g.ny * g.nx;
   const bool b32 = c->precision == 1;  // FP32 copies of the tendencies for the FP32-bulk gathers
