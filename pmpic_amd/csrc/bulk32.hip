@@ -855,7 +855,7 @@ __global__ void k32_newcell_flags(Geometry g, const int32_t *__restrict__ nperce
 
 struct Mix32 {
   const float *G[6];  // kept vol, b, h, p, q, r (folded, doubled)
-  float *R[6];        // residual vol .. r
+  float *R[6];        // residual vol .. r; after k32_mix_residual, divided by the kept volume
 };
 
 __global__ void k32_mix_new(Geometry g, Bulk32 b, int64_t cap, int64_t norig,
@@ -930,8 +930,10 @@ __global__ void k32_mix_residual(Geometry g, Mix32 mg, const uint8_t *__restrict
         for (int f = 0; f < 6; f++) r[f] = r[f] - portion * gv[f];
       }
   const float mlt = (k == 0 || k == g.nzn - 1) ? 2.f : 1.f;
+  // stored divided by the node's kept volume: the absorption weights each node's residual by
+  // w / vol_g (parcel_mixing.F90:247-340), so the division is done once per node here
 #pragma unroll
-  for (int f = 0; f < 6; f++) mg.R[f][t] = r[f] * mlt;
+  for (int f = 0; f < 6; f++) mg.R[f][t] = (r[f] * mlt) / gv[0];
 }
 
 __global__ void k32_mix_absorb(Geometry g, Bulk32 b, int64_t cap, int64_t norig,
@@ -952,7 +954,7 @@ __global__ void k32_mix_absorb(Geometry g, Bulk32 b, int64_t cap, int64_t norig,
   cell_ijk(g, cell, i, j, k);
   int32_t o[8];
   corner_nodes(g, i, j, k, o);
-  float w[8], g0[8];
+  float w[8];
   w[0] = (1 - oz) * (1 - oy) * (1 - ox) * v;
   w[1] = (1 - oz) * (1 - oy) * (ox) * v;
   w[2] = (1 - oz) * (oy) * (1 - ox) * v;
@@ -961,22 +963,17 @@ __global__ void k32_mix_absorb(Geometry g, Bulk32 b, int64_t cap, int64_t norig,
   w[5] = (oz) * (1 - oy) * (ox) * v;
   w[6] = (oz) * (oy) * (1 - ox) * v;
   w[7] = (oz) * (oy) * (ox) * v;
-#pragma unroll
-  for (int c = 0; c < 8; c++) g0[c] = ld(mg.G[0], (uint32_t)o[c]);
-  // w / vol_g once per corner, 1 / vol_new once per parcel
-  float wg[8];
-#pragma unroll
-  for (int c = 0; c < 8; c++) wg[c] = w[c] / g0[c];
+  // the residual grids already hold residual / vol_g; 1 / vol_new once per parcel
   float vn = v;
 #pragma unroll
-  for (int c = 0; c < 8; c++) vn = vn + wg[c] * ld((const float *)mg.R[0], (uint32_t)o[c]);
+  for (int c = 0; c < 8; c++) vn = vn + w[c] * ld((const float *)mg.R[0], (uint32_t)o[c]);
   const float ivn = 1.f / vn;
   float out[5];
 #pragma unroll
   for (int f = 0; f < 5; f++) {
     float acc = fv[f] * v;
 #pragma unroll
-    for (int c = 0; c < 8; c++) acc = acc + wg[c] * ld((const float *)mg.R[f + 1], (uint32_t)o[c]);
+    for (int c = 0; c < 8; c++) acc = acc + w[c] * ld((const float *)mg.R[f + 1], (uint32_t)o[c]);
     out[f] = acc * ivn;
   }
   stnt(fp(B_VOL), p, vn);
