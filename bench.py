@@ -50,7 +50,7 @@ def parse():
     ap.add_argument("--n", type=int, default=128, help="cells per horizontal direction")
     ap.add_argument("--nz", type=int, default=128)
     ap.add_argument("--ppc", type=int, default=3, help="parcels per cell per direction")
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--forms", default="fp32,fp64",
                     help="numeric forms to time: fp32 (FP32-bulk), fp64 (plain), or both")
