@@ -216,9 +216,13 @@ __global__ void k32_to_f64(const float *__restrict__ a, int64_t n, double *__res
 // typical cell needs no scan and no scatter pass.  A parcel of higher rank is appended, with
 // its rank, to an overflow list; every cell holding more than capc parcels is then laid out
 // densely, all its parcels in rank order, in `sorted` from ostart[cell] (an exclusive scan of
-// those cells' counts).  capc = 0 lays out every cell that way.
+// those cells' counts).  capc = 0 lays out every cell that way.  capc is chosen per binning:
+// CAPC_LOW while the mean is at most LOW_PPC parcels per cell, so a sparse grid does not read
+// mostly empty entries, else CAPC.
 constexpr int BIN_SEQ = 16;
-constexpr int CAPC = 32;  // slot capacity per cell when the slot table is present
+constexpr int CAPC = 32;      // slot entries per cell of the table, and the full capacity
+constexpr int CAPC_LOW = 16;
+constexpr int LOW_PPC = 10;
 
 __global__ void k32_bin_reset(int32_t *__restrict__ count, int64_t nc1, DevScalars *dsc) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -332,14 +336,17 @@ __global__ void k32_sort_big(const int32_t *__restrict__ start, int64_t ncell, i
 // corner q over the cell's parcels in ascending parcel order, so the sums are deterministic.
 enum Acc32 { A32_P2G = 0, A32_MIX = 1 };
 constexpr int TILE32 = 16;
-constexpr int ROWS32 = 512;
+constexpr int ROWS32 = 512;     // LDS rows with the full capacity
+constexpr int ROWS_LOW = 256;   // LDS rows with CAPC_LOW
+__host__ __device__ constexpr int rows_of(int capc) { return capc == CAPC_LOW ? ROWS_LOW : ROWS32; }
 constexpr int ATPB = TILE32 * 8;  // threads of the accumulation workgroup
 
 struct Zmap {
   float zbot, dz;  // z of a parcel = zbot + (k + oz) dz
 };
 
-template <int MODE>
+// CAP: the binning's slot capacity (any value when capc = 0); ROWS: LDS rows of a pass
+template <int MODE, int CAP, int ROWS>
 __global__ void __launch_bounds__(ATPB) k32_cell_accum(Bulk32 b, int64_t cap, Geometry g, Zmap zm,
                                                        float vmin, const int32_t *__restrict__ count,
                                                        const int32_t *__restrict__ ostart,
@@ -350,14 +357,14 @@ __global__ void __launch_bounds__(ATPB) k32_cell_accum(Bulk32 b, int64_t cap, Ge
   constexpr bool MIX = (MODE == A32_MIX);
   constexpr int NV = MIX ? 5 : 4;
   constexpr int NF = NV + 1;
-  constexpr int SPT = TILE32 * CAPC / ATPB;  // slot entries per thread
-  constexpr int RPT = ROWS32 / ATPB;         // dense rows per thread
-  __shared__ float4 sgeo[ROWS32];  // ox, oy, oz, vol
-  __shared__ float4 sva[ROWS32];   // values 0..3
-  __shared__ float svb[MIX ? ROWS32 : 1];
-  __shared__ int32_t sidx[ROWS32];
-  __shared__ int16_t sperm[ROWS32];
-  __shared__ uint8_t skeep[MIX ? ROWS32 : 1];
+  constexpr int SPT = TILE32 * CAP / ATPB;  // slot entries per thread
+  constexpr int RPT = ROWS / ATPB;         // dense rows per thread
+  __shared__ float4 sgeo[ROWS];  // ox, oy, oz, vol
+  __shared__ float4 sva[ROWS];   // values 0..3
+  __shared__ float svb[MIX ? ROWS : 1];
+  __shared__ int32_t sidx[ROWS];
+  __shared__ int16_t sperm[ROWS];
+  __shared__ uint8_t skeep[MIX ? ROWS : 1];
   __shared__ int32_t soff[TILE32 + 1];  // first dense row of each cell of the tile
   __shared__ int32_t scnt[TILE32], sost[TILE32];
   __shared__ uint32_t sover;            // bit cl: cell cl is listed in the dense overflow region
@@ -378,7 +385,7 @@ __global__ void __launch_bounds__(ATPB) k32_cell_accum(Bulk32 b, int64_t cap, Ge
 #pragma unroll
   for (int k = 0; k < SPT; k++) {
     const int e = tid + k * ATPB;
-    sl[k] = (capc > 0 && e < ntile * CAPC) ? ld(b.slot, (uint32_t)(c0 * CAPC + e)) : -1;
+    sl[k] = (capc > 0 && e < ntile * CAP) ? ld(b.slot, (uint32_t)(c0 * CAP + e)) : -1;
   }
   if (tid < 64) {
     const int32_t v = (tid < ntile) ? count[c0 + tid] : 0;
@@ -441,19 +448,19 @@ __global__ void __launch_bounds__(ATPB) k32_cell_accum(Bulk32 b, int64_t cap, Ge
   int first = 0;
   while (first < ntile) {
     int last = first + 1;
-    if (soff[ntile] - soff[first] <= ROWS32) last = ntile;  // the usual case: the rest fits
+    if (soff[ntile] - soff[first] <= ROWS) last = ntile;  // the usual case: the rest fits
     else
-      while (last < ntile && soff[last + 1] - soff[first] <= ROWS32) last++;
+      while (last < ntile && soff[last + 1] - soff[first] <= ROWS) last++;
     const int32_t r0 = soff[first];
     const int nrows = soff[last] - r0;
-    const bool fits = nrows <= ROWS32;
+    const bool fits = nrows <= ROWS;
     if (fits) {
       // rows of the cells listed in the slot table, from the entries already loaded
       int32_t pr[SPT], row[SPT];
 #pragma unroll
       for (int k = 0; k < SPT; k++) {
         const int e = tid + k * ATPB;
-        const int ce = e / CAPC, j = e % CAPC;
+        const int ce = e / CAP, j = e % CAP;
         pr[k] = -1;
         row[k] = 0;
         if (sl[k] >= 0 && ce >= first && ce < last && j < scnt[ce] && scnt[ce] <= capc) {
@@ -468,7 +475,7 @@ __global__ void __launch_bounds__(ATPB) k32_cell_accum(Bulk32 b, int64_t cap, Ge
         if (pr[k] >= 0) fetch((uint32_t)pr[k], geo[k], va[k], vb[k]);
 #pragma unroll
       for (int k = 0; k < SPT; k++)
-        if (pr[k] >= 0) put(row[k], pr[k], geo[k], va[k], vb[k], skz[(tid + k * ATPB) / CAPC]);
+        if (pr[k] >= 0) put(row[k], pr[k], geo[k], va[k], vb[k], skz[(tid + k * ATPB) / CAP]);
       // rows of the cells listed in the dense overflow region
       const uint32_t pass = (last - first >= 32 ? ~0u : ((1u << (last - first)) - 1u)) << first;
       if (sover & pass) {
@@ -1011,7 +1018,12 @@ int dalloc32(T **p, int64_t n) {
 
 int bin32(Ctx *c, int64_t n) {
   const int64_t nc1 = c->ncell + 1;
-  const Bulk32 &b = c->b32;
+  Bulk32 &b = c->b32;
+  // the capacity of this binning: 16 entries while the mean is at most LOW_PPC parcels per
+  // cell (fewer empty entries to read), else 32
+  if (b.capmax == 0) b.capc = 0;
+  else if (b.capfix) b.capc = b.capfix;
+  else b.capc = (n <= (int64_t)LOW_PPC * c->ncell) ? CAPC_LOW : CAPC;
   hipLaunchKernelGGL(k32_bin_reset, dim3(cdiv(nc1, TPB)), dim3(TPB), 0, c->stream, c->cell_count,
                      nc1, c->dsc);
   if (n > 0)
@@ -1038,7 +1050,7 @@ int bin32(Ctx *c, int64_t n) {
                        c->dsc, c->cell_start, b.slot, b.capc, c->sorted);
   }
   hipLaunchKernelGGL(k32_sort_big, dim3(cdiv(c->ncell, TPB)), dim3(TPB), 0, c->stream,
-                     c->cell_start, c->ncell, ROWS32, c->sorted);
+                     c->cell_start, c->ncell, rows_of(b.capc), c->sorted);
   PMPIC_CHECK(hipGetLastError());
   return PMPIC_OK;
 }
@@ -1058,13 +1070,20 @@ int b32_alloc(Ctx *c) {
   PMPIC_TRY(dalloc32(&b.tag, c->cap));
   for (int i = 0; i < 6; i++) PMPIC_TRY(dalloc32(&b.grid[i], c->gsz));
   for (int i = 0; i < 12; i++) PMPIC_TRY(dalloc32(&b.mgrid[i], c->gsz));
-  // 32-bit byte offsets into the slot table; PMPIC_SLOT_CAPACITY=0 in the environment drops the
-  // table (every cell then binned through the dense region, as grids beyond that range are)
-  b.capc = (c->ncell * CAPC < ((int64_t)1 << 30)) ? CAPC : 0;
+  // 32-bit byte offsets into the slot table.  PMPIC_SLOT_CAPACITY in the environment: 0 drops
+  // the table (every cell then binned through the dense region, as grids beyond that range
+  // are); 16 or 32 fixes the capacity instead of choosing it by the mean parcels per cell
+  b.capmax = (c->ncell * CAPC < ((int64_t)1 << 30)) ? CAPC : 0;
+  b.capfix = 0;
   const char *env = getenv("PMPIC_SLOT_CAPACITY");
-  if (env && atoi(env) == 0) b.capc = 0;
+  if (env) {
+    const int v = atoi(env);
+    if (v == 0) b.capmax = 0;
+    else if (v == CAPC_LOW || v == CAPC) b.capfix = v;
+  }
+  b.capc = b.capmax;
   b.slot = nullptr;
-  if (b.capc > 0) PMPIC_TRY(dalloc32(&b.slot, c->ncell * CAPC));
+  if (b.capmax > 0) PMPIC_TRY(dalloc32(&b.slot, c->ncell * CAPC));
   PMPIC_TRY(dalloc32(&b.ovf, c->cap));
   return PMPIC_OK;
 }
@@ -1136,10 +1155,16 @@ int b32_get_parcels(Ctx *c, double *const *fields) {
 int b32_par2grid(Ctx *c) {
   const int64_t n = c->n;
   PMPIC_TRY(bin32(c, n));
-  PMPIC_TRY(launch_timed(c, T_ACCUM, k32_cell_accum<A32_P2G>, dim3(cdiv(c->ncell, TILE32)),
-                         dim3(ATPB), 0, c->b32, c->cap, c->g, zmap_of(c->g), (float)c->g.vmin,
-                         c->cell_count, c->cell_start, c->sorted, c->ncell, partial32(c), nullptr,
-                         nullptr));
+  if (c->b32.capc == CAPC_LOW)
+    PMPIC_TRY(launch_timed(c, T_ACCUM, k32_cell_accum<A32_P2G, CAPC_LOW, ROWS_LOW>,
+                           dim3(cdiv(c->ncell, TILE32)), dim3(ATPB), 0, c->b32, c->cap, c->g,
+                           zmap_of(c->g), (float)c->g.vmin, c->cell_count, c->cell_start,
+                           c->sorted, c->ncell, partial32(c), nullptr, nullptr));
+  else
+    PMPIC_TRY(launch_timed(c, T_ACCUM, k32_cell_accum<A32_P2G, CAPC, ROWS32>,
+                           dim3(cdiv(c->ncell, TILE32)), dim3(ATPB), 0, c->b32, c->cap, c->g,
+                           zmap_of(c->g), (float)c->g.vmin, c->cell_count, c->cell_start,
+                           c->sorted, c->ncell, partial32(c), nullptr, nullptr));
   Out32<double> out = {{c->grid[GP], c->grid[GQ], c->grid[GR], c->grid[GB], nullptr, nullptr}};
   const int64_t nnode = (int64_t)c->g.nzn * c->g.ny * c->g.nx;
   hipLaunchKernelGGL((k32_node_sum<4, true, true, double>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0,
@@ -1211,9 +1236,16 @@ int b32_mix(Ctx *c) {
   const float vmin = (float)g.vmin;
   // kept and residual grids (construct_grids, parcel_mixing.F90:440-624)
   PMPIC_TRY(bin32(c, norig));
-  hipLaunchKernelGGL(k32_cell_accum<A32_MIX>, dim3(cdiv(c->ncell, TILE32)), dim3(ATPB), 0,
-                     c->stream, b, c->cap, g, zmap_of(g), vmin, c->cell_count, c->cell_start,
-                     c->sorted, c->ncell, partial32(c), partial32b(c), c->npercell);
+  if (b.capc == CAPC_LOW)
+    hipLaunchKernelGGL((k32_cell_accum<A32_MIX, CAPC_LOW, ROWS_LOW>), dim3(cdiv(c->ncell, TILE32)),
+                       dim3(ATPB), 0, c->stream, b, c->cap, g, zmap_of(g), vmin, c->cell_count,
+                       c->cell_start, c->sorted, c->ncell, partial32(c), partial32b(c),
+                       c->npercell);
+  else
+    hipLaunchKernelGGL((k32_cell_accum<A32_MIX, CAPC, ROWS32>), dim3(cdiv(c->ncell, TILE32)),
+                       dim3(ATPB), 0, c->stream, b, c->cap, g, zmap_of(g), vmin, c->cell_count,
+                       c->cell_start, c->sorted, c->ncell, partial32(c), partial32b(c),
+                       c->npercell);
   const int64_t nnode = (int64_t)g.nzn * g.ny * g.nx;
   Out32<float> kept = {{b.mgrid[0], b.mgrid[1], b.mgrid[2], b.mgrid[3], b.mgrid[4], b.mgrid[5]}};
   Out32<float> res = {{b.mgrid[6], b.mgrid[7], b.mgrid[8], b.mgrid[9], b.mgrid[10], b.mgrid[11]}};

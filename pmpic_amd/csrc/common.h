@@ -43,11 +43,14 @@ struct Bulk32 {
   float *grid[6];    // gathered grids U, V, W, DP, DQ, DR [nzn][ny][nx]
   float *mgrid[12];  // mixing: kept vol, b, h, p, q, r (0..5), residuals (6..11)
   // cell binning: slot[c * capc + rank] lists the parcels of cell c while rank < capc; a cell
-  // with more parcels lists all of them in the dense overflow region (ncell * capc, capc = 32,
-  // or capc = 0 and no slot table when its byte offsets would exceed 32 bits)
+  // with more parcels lists all of them in the dense overflow region.  The table holds 32
+  // entries per cell (none, and capc = 0, when its byte offsets would exceed 32 bits); each
+  // binning uses capc = 16 or 32 of them by the mean parcels per cell.
   int32_t *slot;
   int2 *ovf;         // cap: (parcel, rank) of the parcels past the slot capacity
-  int capc;
+  int capc;          // capacity of the current binning (0, 16 or 32)
+  int capmax;        // entries per cell allocated (0 or 32)
+  int capfix;        // capacity forced by PMPIC_SLOT_CAPACITY (16 or 32), or 0 for the mean rule
 };
 
 // grid ids exported through the C-ABI (writebinary.F90:287-296 order, then the tendencies)

@@ -74,14 +74,16 @@ def test_steps_close_to_fp64(shape, kw, steps):
         assert rel_err(g32.grid(name), g64.grid(name)) < 1e-5, name
 
 
-@pytest.mark.parametrize("slots", ["table", "dense"])
+@pytest.mark.parametrize("slots", ["auto", "16", "32", "dense"])
 def test_crowded_cells_close_to_fp64(monkeypatch, slots):
     """Cells of 32, 33, 44 and 708 parcels through the FP32 binning: with the per-cell slot
-    table (cells past 32 parcels in the dense overflow region, the 708-parcel cell past the
-    LDS rows of a workgroup) and without it (PMPIC_SLOT_CAPACITY=0, every cell dense, the
-    layout of grids too large for the table).  Same bars as the steps above."""
-    if slots == "dense":
-        monkeypatch.setenv("PMPIC_SLOT_CAPACITY", "0")
+    table at the capacity its mean picks (16 here, about 9 parcels per cell), fixed at 16 or
+    32 (PMPIC_SLOT_CAPACITY; cells past the capacity in the dense overflow region, the
+    708-parcel cell past the LDS rows of a workgroup), and without it (PMPIC_SLOT_CAPACITY=0,
+    every cell dense, the layout of grids too large for the table).  Same bars as the steps
+    above."""
+    if slots != "auto":
+        monkeypatch.setenv("PMPIC_SLOT_CAPACITY", "0" if slots == "dense" else slots)
     shape = (12, 10, 8)
     ic = crowded_case(*shape)
     g64 = make(shape, 0, ic)
