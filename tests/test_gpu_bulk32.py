@@ -3,8 +3,8 @@ plus three FP32 offsets, FP32 parcel fields, FP32 par2grid sums and interpolatio
 
 It is checked against the plain FP64 GPU form on the same inputs; the FP64 form is itself pinned
 to the oracle by test_gpu_parity.py.  Bars (written per test): the parcel transfer round trip to
-FP32 rounding of the offsets; grids and velocities within 1e-5 relative (the north-star
-velocity bar for the FP32-bulk form) after the steps run; identical split/mix counts on cases
+FP32 rounding of the offsets; grids and velocities within 1e-5 relative (the FP32-bulk
+form's velocity bar) after the steps run; identical split/mix counts on cases
 where no parcel sits within FP32 rounding of a threshold; volume conserved to FP32 rounding.
 """
 import numpy as np

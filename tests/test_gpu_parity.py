@@ -4,7 +4,7 @@ Bars (written per test):
 * operations whose order the GPU keeps from the reference (cell location, grid2par, the
   pair-swap derivatives, the filter, the tridiagonal solves): bit-identical;
 * par2grid (sum order differs) and the FFTs (algorithm differs): 1e-12 relative to the field's
-  maximum, the north-star bar for the plain FP64 form;
+  maximum, the bar for the plain FP64 form;
 * whole timesteps: identical parcel counts and 1e-10 relative field agreement over the steps
   run (chaotic growth of 1e-16 differences stays far below that at these sizes).
 """
@@ -179,7 +179,7 @@ def test_steps_match_oracle(oracle_mod, shape, kw):
 
 
 def test_twenty_steps_counts_match_oracle(oracle_mod):
-    """The north-star count bar: identical parcel counts after split/merge for the first 20
+    """The count bar: identical parcel counts after split/merge for the first 20
     steps (and the split/removed/added counts of every step), with the fields still within
     1e-9 relative at the end."""
     o, g, _ = make_pair(oracle_mod, (12, 10, 8), vort=3.0, shrink=0.03, thin=0.08, near_split=0.04)
