@@ -1,0 +1,107 @@
+"""GPU checks at the benchmark's full sizes, through properties that hold at any size.
+
+The oracle finishes only small grids in seconds, so at 128x128x128 cells with 27 parcels per
+cell (56.6 M parcels) and at 256x256x128 cells with 8 parcels per cell (67.1 M parcels) the
+checks are size-independent properties of the path:
+
+* par2grid of a field of ones is exactly one at every node: the weight sum and the data sum
+  are the same additions (every node of the jittered lattice has weight);
+* grid2par of a field linear in z returns the linear function at each parcel's z to rounding
+  (trilinear interpolation is exact for linear fields, parcel_interpolation.F90:309-321);
+* whole steps lose and duplicate no parcel: the count follows the split/removed/added counts
+  and the sums of vol, vol*b and vol*h are unchanged to rounding (advection moves parcels
+  without changing vol, b or h; splitting halves volumes exactly; mixing conserves them);
+* the FP32-bulk form stays within 1e-5 relative of the FP64 form on the velocity and vorticity
+  grids, with identical parcel counts (the FP32-bulk form's bar), on both workloads; the second
+  runs the binning's 16-slot capacity at full size.
+"""
+import numpy as np
+import pytest
+
+from tests.cases import rel_err
+
+pytestmark = pytest.mark.gpu
+
+SEED = 0x504D504943000000 + 1
+
+
+def _thermal(n, nz, ppcd):
+    from pmpic_amd.config import fortran_real
+    from pmpic_amd.initial import thermal
+    lx, lz = fortran_real("6.28318531"), fortran_real("3.14159265")
+    return thermal(n, n, nz, lx, lx, lz, ppcd, seed=SEED), lx, lz
+
+
+def _ctx(ic, n, nz, lx, lz, precision):
+    from pmpic_amd.lib import Context
+    g = Context(n, n, nz, 0.0, lx, 0.0, lx, 0.0, lz, 2 * len(ic["x"]), dtm=0.01, dtmax=0.01,
+                timestep_prefactor=0.5, precision=precision)
+    g.set_parcels(ic)
+    return g
+
+
+@pytest.fixture(scope="module")
+def thermal128():
+    return _thermal(128, 128, 3)
+
+
+def test_par2grid_of_ones_is_one(thermal128):
+    ic, lx, lz = thermal128
+    g = _ctx(ic, 128, 128, lx, lz, 0)
+    grid = g.par2grid(np.ones(len(ic["x"])))
+    g.close()
+    assert grid.shape == (132, 132, 129)
+    assert np.all(grid == 1.0)
+
+
+def test_grid2par_of_linear_z_field(thermal128):
+    ic, lx, lz = thermal128
+    g = _ctx(ic, 128, 128, lx, lz, 0)
+    a, b = 0.75, -1.25
+    zk = np.arange(129) * (lz / 128)
+    grid = np.broadcast_to(a + b * zk, (132, 132, 129)).copy()
+    vals = g.grid2par(grid)
+    z = g.get_parcels(["z"])["z"]
+    g.close()
+    assert np.max(np.abs(vals - (a + b * z))) <= 1e-12 * (abs(a) + abs(b) * lz)
+
+
+def _sums_and_count_after(g, nsteps):
+    s0 = g.conservation_sums()
+    n0 = g.n
+    for _ in range(nsteps):
+        nb = g.n
+        g.step(1)
+        split, removed, added = g.last_counts()
+        assert g.n == nb + split - removed + added
+    return s0, g.conservation_sums(), n0
+
+
+def test_steps_conserve_parcels_fp64(thermal128):
+    ic, lx, lz = thermal128
+    g = _ctx(ic, 128, 128, lx, lz, 0)
+    s0, s1, _ = _sums_and_count_after(g, 2)
+    g.close()
+    for k in range(3):  # vol, vol*b, vol*h
+        assert abs(s1[k] - s0[k]) <= 1e-12 * abs(s0[k]), k
+
+
+@pytest.mark.parametrize("workload", ["128x128x128 27ppc", "256x256x128 8ppc"])
+def test_fp32_bulk_close_to_fp64(thermal128, workload):
+    if workload.startswith("128"):
+        (ic, lx, lz), n, nz = thermal128, 128, 128
+    else:
+        (ic, lx, lz), n, nz = _thermal(256, 128, 2), 256, 128
+    g64 = _ctx(ic, n, nz, lx, lz, 0)
+    g32 = _ctx(ic, n, nz, lx, lz, 1)
+    for step in range(2):
+        g64.step(1)
+        g32.step(1)
+        assert g32.n == g64.n and g32.last_counts() == g64.last_counts(), step
+    s64, s32 = g64.conservation_sums(), g32.conservation_sums()
+    for name in ("u", "v", "w", "p", "q", "r"):
+        assert rel_err(g32.grid(name), g64.grid(name)) < 1e-5, name
+    g64.close()
+    g32.close()
+    # the FP32-bulk form holds vol in FP32: its volume sum within FP32 rounding of the FP64 form's
+    assert abs(s32[0] - s64[0]) <= 1e-6 * abs(s64[0])
