@@ -584,10 +584,10 @@ __global__ void __launch_bounds__(ATPB) k32_cell_accum(Bulk32 b, int64_t cap, Ge
     }
   }
 #pragma unroll
-  for (int f = 0; f < NF; f++) partial[(int64_t)(q * NF + f) * ncell + c] = acc[f];
+  for (int f = 0; f < NF; f++) partial[((int64_t)q * ncell + c) * NF + f] = acc[f];
   if constexpr (MIX) {
 #pragma unroll
-    for (int f = 0; f < NF; f++) partial2[(int64_t)(q * NF + f) * ncell + c] = res[f];
+    for (int f = 0; f < NF; f++) partial2[((int64_t)q * ncell + c) * NF + f] = res[f];
     if (q == 0) npercell[c] = kept;
   }
 }
@@ -624,7 +624,7 @@ __global__ void k32_node_sum(Geometry g, const float *__restrict__ partial, int6
     ok[q] = ck >= 0 && ck < g.ncz;
     const int64_t c = ok[q] ? ((int64_t)ck * g.ny + cj) * g.nx + ci : 0;
 #pragma unroll
-    for (int f = 0; f <= NV; f++) v[q][f] = partial[(int64_t)(q * (NV + 1) + f) * ncell + c];
+    for (int f = 0; f <= NV; f++) v[q][f] = partial[((int64_t)q * ncell + c) * (NV + 1) + f];
   }
   float s[NV + 1];
 #pragma unroll
