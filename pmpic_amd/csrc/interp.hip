@@ -325,10 +325,10 @@ __global__ void __launch_bounds__(256) k_cell_accum_lds(Geometry g, const double
     }
   }
 #pragma unroll
-  for (int f = 0; f < NF; f++) partial[(int64_t)(q * NF + f) * ncell + c] = acc[f];
+  for (int f = 0; f < NF; f++) partial[((int64_t)q * ncell + c) * NF + f] = acc[f];
   if constexpr (MIX) {
 #pragma unroll
-    for (int f = 0; f < NF; f++) partial2[(int64_t)(q * NF + f) * ncell + c] = res[f];
+    for (int f = 0; f < NF; f++) partial2[((int64_t)q * ncell + c) * NF + f] = res[f];
     if (q == 0) npercell[c] = kept;
   }
 }
@@ -363,7 +363,7 @@ __global__ void k_node_sum(Geometry g, const double *__restrict__ partial, int64
     ok[q] = ck >= 0 && ck < g.ncz;
     const int64_t c = ok[q] ? ((int64_t)ck * g.ny + cj) * g.nx + ci : 0;
 #pragma unroll
-    for (int f = 0; f <= NV; f++) v[q][f] = partial[(int64_t)(q * (NV + 1) + f) * ncell + c];
+    for (int f = 0; f <= NV; f++) v[q][f] = partial[((int64_t)q * ncell + c) * (NV + 1) + f];
   }
   double s[NV + 1];
 #pragma unroll
