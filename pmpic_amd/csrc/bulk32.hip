@@ -353,7 +353,8 @@ __global__ void __launch_bounds__(ATPB) k32_cell_accum(Bulk32 b, int64_t cap, Ge
                                                        const int32_t *__restrict__ sorted,
                                                        int64_t ncell, float *__restrict__ partial,
                                                        float *__restrict__ partial2,
-                                                       int32_t *__restrict__ npercell) {
+                                                       int32_t *__restrict__ npercell,
+                                                       uint8_t *__restrict__ tileres) {
   constexpr bool MIX = (MODE == A32_MIX);
   constexpr int NV = MIX ? 5 : 4;
   constexpr int NF = NV + 1;
@@ -443,7 +444,7 @@ __global__ void __launch_bounds__(ATPB) k32_cell_accum(Bulk32 b, int64_t cap, Ge
   float acc[NF], res[NF];
 #pragma unroll
   for (int f = 0; f < NF; f++) { acc[f] = 0.f; res[f] = 0.f; }
-  int kept = 0;
+  int kept = 0, nres = 0;
   bool big = false;
   int first = 0;
   while (first < ntile) {
@@ -533,6 +534,7 @@ __global__ void __launch_bounds__(ATPB) k32_cell_accum(Bulk32 b, int64_t cap, Ge
         for (int f = 0; f < NV; f++) acc[f + 1] += w * vals[f];
         kept++;
       } else {
+        nres++;
         res[0] += w;
 #pragma unroll
         for (int f = 0; f < NV; f++) res[f + 1] += w * vals[f];
@@ -542,9 +544,9 @@ __global__ void __launch_bounds__(ATPB) k32_cell_accum(Bulk32 b, int64_t cap, Ge
     __syncthreads();
     first = last;
   }
-  if (cl >= ntile) return;
+  const bool live = cl < ntile;
   const int64_t c = c0 + cl;
-  if (big) {
+  if (live && big) {
     // more parcels than LDS rows: always in the overflow region, sorted by parcel there
     const int32_t s0 = sost[cl], s1 = s0 + scnt[cl];
     for (int32_t s = s0; s < s1; s++) {
@@ -577,17 +579,28 @@ __global__ void __launch_bounds__(ATPB) k32_cell_accum(Bulk32 b, int64_t cap, Ge
         for (int f = 0; f < NV; f++) acc[f + 1] += w * vals[f];
         kept++;
       } else {
+        nres++;
         res[0] += w;
 #pragma unroll
         for (int f = 0; f < NV; f++) res[f + 1] += w * vals[f];
       }
     }
   }
+  // mixing: a tile without residual parcels writes no residual partials; tileres tells the
+  // residual node sums which tiles have them
+  bool anyres = false;
+  if constexpr (MIX) {
+    anyres = __syncthreads_or(live && nres > 0);
+    if (tid == 0) tileres[blockIdx.x] = anyres ? 1 : 0;
+  }
+  if (!live) return;
 #pragma unroll
   for (int f = 0; f < NF; f++) partial[((int64_t)q * ncell + c) * NF + f] = acc[f];
   if constexpr (MIX) {
+    if (anyres) {
 #pragma unroll
-    for (int f = 0; f < NF; f++) partial2[((int64_t)q * ncell + c) * NF + f] = res[f];
+      for (int f = 0; f < NF; f++) partial2[((int64_t)q * ncell + c) * NF + f] = res[f];
+    }
     if (q == 0) npercell[c] = kept;
   }
 }
@@ -600,9 +613,11 @@ struct Out32 {
   T *p[6];
 };
 
+// tileres (may be null): per accumulation tile, whether its cells' partials were written; the
+// cells of a tile without them contribute zero
 template <int NV, bool NORM, bool DBL, typename T>
 __global__ void k32_node_sum(Geometry g, const float *__restrict__ partial, int64_t ncell,
-                             Out32<T> out) {
+                             Out32<T> out, const uint8_t *__restrict__ tileres) {
   const int64_t nnode = (int64_t)g.nzn * g.ny * g.nx;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nnode) return;
@@ -623,8 +638,10 @@ __global__ void k32_node_sum(Geometry g, const float *__restrict__ partial, int6
     const int ck = k - bz;
     ok[q] = ck >= 0 && ck < g.ncz;
     const int64_t c = ok[q] ? ((int64_t)ck * g.ny + cj) * g.nx + ci : 0;
+    if (ok[q] && tileres && !tileres[c / TILE32]) ok[q] = false;
 #pragma unroll
-    for (int f = 0; f <= NV; f++) v[q][f] = partial[((int64_t)q * ncell + c) * (NV + 1) + f];
+    for (int f = 0; f <= NV; f++)
+      v[q][f] = ok[q] ? partial[((int64_t)q * ncell + c) * (NV + 1) + f] : 0.f;
   }
   float s[NV + 1];
 #pragma unroll
@@ -1159,16 +1176,16 @@ int b32_par2grid(Ctx *c) {
     PMPIC_TRY(launch_timed(c, T_ACCUM, k32_cell_accum<A32_P2G, CAPC_LOW, ROWS_LOW>,
                            dim3(cdiv(c->ncell, TILE32)), dim3(ATPB), 0, c->b32, c->cap, c->g,
                            zmap_of(c->g), (float)c->g.vmin, c->cell_count, c->cell_start,
-                           c->sorted, c->ncell, partial32(c), nullptr, nullptr));
+                           c->sorted, c->ncell, partial32(c), nullptr, nullptr, nullptr));
   else
     PMPIC_TRY(launch_timed(c, T_ACCUM, k32_cell_accum<A32_P2G, CAPC, ROWS32>,
                            dim3(cdiv(c->ncell, TILE32)), dim3(ATPB), 0, c->b32, c->cap, c->g,
                            zmap_of(c->g), (float)c->g.vmin, c->cell_count, c->cell_start,
-                           c->sorted, c->ncell, partial32(c), nullptr, nullptr));
+                           c->sorted, c->ncell, partial32(c), nullptr, nullptr, nullptr));
   Out32<double> out = {{c->grid[GP], c->grid[GQ], c->grid[GR], c->grid[GB], nullptr, nullptr}};
   const int64_t nnode = (int64_t)c->g.nzn * c->g.ny * c->g.nx;
   hipLaunchKernelGGL((k32_node_sum<4, true, true, double>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0,
-                     c->stream, c->g, partial32(c), c->ncell, out);
+                     c->stream, c->g, partial32(c), c->ncell, out, nullptr);
   PMPIC_CHECK(hipGetLastError());
   return PMPIC_OK;
 }
@@ -1240,19 +1257,20 @@ int b32_mix(Ctx *c) {
     hipLaunchKernelGGL((k32_cell_accum<A32_MIX, CAPC_LOW, ROWS_LOW>), dim3(cdiv(c->ncell, TILE32)),
                        dim3(ATPB), 0, c->stream, b, c->cap, g, zmap_of(g), vmin, c->cell_count,
                        c->cell_start, c->sorted, c->ncell, partial32(c), partial32b(c),
-                       c->npercell);
+                       c->npercell, c->cellflag);
   else
     hipLaunchKernelGGL((k32_cell_accum<A32_MIX, CAPC, ROWS32>), dim3(cdiv(c->ncell, TILE32)),
                        dim3(ATPB), 0, c->stream, b, c->cap, g, zmap_of(g), vmin, c->cell_count,
                        c->cell_start, c->sorted, c->ncell, partial32(c), partial32b(c),
-                       c->npercell);
+                       c->npercell, c->cellflag);
   const int64_t nnode = (int64_t)g.nzn * g.ny * g.nx;
   Out32<float> kept = {{b.mgrid[0], b.mgrid[1], b.mgrid[2], b.mgrid[3], b.mgrid[4], b.mgrid[5]}};
   Out32<float> res = {{b.mgrid[6], b.mgrid[7], b.mgrid[8], b.mgrid[9], b.mgrid[10], b.mgrid[11]}};
   hipLaunchKernelGGL((k32_node_sum<5, false, true, float>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0,
-                     c->stream, g, partial32(c), c->ncell, kept);
+                     c->stream, g, partial32(c), c->ncell, kept, nullptr);
+  // cellflag holds the per-tile residual flags until the new-parcel flags reuse it below
   hipLaunchKernelGGL((k32_node_sum<5, false, false, float>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0,
-                     c->stream, g, partial32b(c), c->ncell, res);
+                     c->stream, g, partial32b(c), c->ncell, res, c->cellflag);
   // the reference builds the kept grids in the model's vol, b, hg, p, q, r grids
   // (parcel_mixing.F90:131), which is what a grid export shows after mixing
   const GridId kept_ids[6] = {GVOL, GB, GHG, GP, GQ, GR};
