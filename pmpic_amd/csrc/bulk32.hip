@@ -929,7 +929,7 @@ __global__ void k32_mix_new(Geometry g, Bulk32 b, int64_t cap, int64_t norig,
   portion_cell[((int64_t)k * g.ny + j) * g.nx + i] = 2 * vmin / v;
 }
 
-__global__ void k32_mix_residual(Geometry g, Mix32 mg, const uint8_t *__restrict__ cellflag,
+__global__ void k32_mix_residual(Geometry g, Mix32 mg, const int32_t *__restrict__ npercell,
                                  const float *__restrict__ portion_cell) {
   const int64_t nnode = (int64_t)g.nzn * g.ny * g.nx;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -947,9 +947,11 @@ __global__ void k32_mix_residual(Geometry g, Mix32 mg, const uint8_t *__restrict
     for (int bb = 0; bb < 2; bb++)
       for (int kc = k - 1; kc <= k; kc++) {
         if (kc < 0 || kc >= g.ncz) continue;
-        const int64_t ord = ((int64_t)ci[a] * g.ny + cj[bb]) * g.ncz + kc;
-        if (!cellflag[ord]) continue;
-        const float portion = portion_cell[((int64_t)kc * g.ny + cj[bb]) * g.nx + ci[a]];
+        // a cell that received a new parcel: the test of the new-cell flags (npercell < 3),
+        // read in cell order so consecutive nodes read consecutive cells
+        const int64_t cn = ((int64_t)kc * g.ny + cj[bb]) * g.nx + ci[a];
+        if (npercell[cn] >= 3) continue;
+        const float portion = portion_cell[cn];
 #pragma unroll
         for (int f = 0; f < 6; f++) r[f] = r[f] - portion * gv[f];
       }
@@ -1293,7 +1295,7 @@ int b32_mix(Ctx *c) {
     hipLaunchKernelGGL(k32_mix_new, dim3(cdiv(nadd, TPB)), dim3(TPB), 0, c->stream, g, b, c->cap,
                        norig, c->sorted, nadd, c->list_a, Z, c->flag, mg, vmin, portion, c->dsc);
   hipLaunchKernelGGL(k32_mix_residual, dim3(cdiv(nnode, TPB)), dim3(TPB), 0, c->stream, g, mg,
-                     c->cellflag, portion);
+                     c->npercell, portion);
   if (norig > 0)
     hipLaunchKernelGGL(k32_mix_absorb, dim3(cdiv(norig, TPB)), dim3(TPB), 0, c->stream, g, b,
                        c->cap, norig, c->flag, mg);

@@ -473,7 +473,7 @@ __global__ void k_mix_new(Geometry g, double *__restrict__ par, int64_t cap, int
 
 // residual minus the share of every adjacent new parcel, visiting cells in the reference's
 // loop order, then the z-boundary doubling (parcel_mixing.F90:202-207, 221-240)
-__global__ void k_mix_residual(Geometry g, MixGrids mg, const uint8_t *__restrict__ cellflag,
+__global__ void k_mix_residual(Geometry g, MixGrids mg, const int32_t *__restrict__ npercell,
                                const double *__restrict__ portion_cell) {
   const int64_t nnode = (int64_t)g.nzn * g.ny * g.nx;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -492,9 +492,11 @@ __global__ void k_mix_residual(Geometry g, MixGrids mg, const uint8_t *__restric
     for (int b = 0; b < 2; b++)
       for (int kc = k - 1; kc <= k; kc++) {
         if (kc < 0 || kc >= g.ncz) continue;
-        const int64_t ord = ((int64_t)ci[a] * g.ny + cj[b]) * g.ncz + kc;
-        if (!cellflag[ord]) continue;
-        const double portion = portion_cell[((int64_t)kc * g.ny + cj[b]) * g.nx + ci[a]];
+        // a cell that received a new parcel: the test of the new-cell flags (npercell < 3),
+        // read in cell order so consecutive nodes read consecutive cells
+        const int64_t cn = ((int64_t)kc * g.ny + cj[b]) * g.nx + ci[a];
+        if (npercell[cn] >= 3) continue;
+        const double portion = portion_cell[cn];
 #pragma unroll
         for (int f = 0; f < 6; f++) r[f] = r[f] - portion * gv[f];
       }
@@ -707,7 +709,7 @@ int parcel_mix(Ctx *c) {
                        c->dsc);
   const int64_t nnode = (int64_t)g.nzn * g.ny * g.nx;
   hipLaunchKernelGGL(k_mix_residual, dim3(cdiv(nnode, TPB)), dim3(TPB), 0, c->stream, g, mg,
-                     c->cellflag, portion);
+                     c->npercell, portion);
   if (norig > 0)
     hipLaunchKernelGGL(k_mix_absorb, dim3(cdiv(norig, TPB)), dim3(TPB), 0, c->stream, g, c->par,
                        c->cap, norig, c->flag, mg);
