@@ -634,7 +634,16 @@ int pmpic_get_grid(pmpic_ctx *ctx, int32_t id, double *grid) {
   if (!ctx || !grid || id < 0 || id >= NGRIDS) return PMPIC_ERR_ARG;
   Ctx *c = &ctx->c;
   std::vector<double> h;
-  PMPIC_TRY(download(c, h, c->grid[id], c->gsz));
+  if (c->precision == 1 && id >= GDP && id <= GDR) {
+    // the FP32-bulk form keeps the tendencies only in the FP32 grids its gathers read
+    std::vector<float> hf(c->gsz);
+    PMPIC_CHECK(hipMemcpyAsync(hf.data(), c->b32.grid[3 + (id - GDP)], sizeof(float) * c->gsz,
+                               hipMemcpyDeviceToHost, c->stream));
+    PMPIC_CHECK(hipStreamSynchronize(c->stream));
+    h.assign(hf.begin(), hf.end());
+  } else {
+    PMPIC_TRY(download(c, h, c->grid[id], c->gsz));
+  }
   grid_from_device_layout(c->g, h, grid);
   return PMPIC_OK;
 }

@@ -641,7 +641,11 @@ __global__ void k_tendency(int64_t nnode, const double *__restrict__ P,
   dr = dr + qq * dwdy[q];
   dr = dr + r * (-dudx[q]);
   dr = dr + r * (-dvdy[q]);
-  DP[q] = dp; DQ[q] = dq; DR[q] = dr;
+  if (DP) {
+    DP[q] = dp;
+    DQ[q] = dq;
+    DR[q] = dr;
+  }
   if (DP32) {
     DP32[q] = (float)dp;
     DQ32[q] = (float)dq;
@@ -935,7 +939,8 @@ int vort2vel_spectral(Ctx *c, int stage) {
 }
 
 // vorticity_tendency timestep_callback (vorticity_tendency.F90:147-265) after par2grid of btot
-// into grid[GB]; tendencies into grid[GDP..GDR]
+// into grid[GB]; tendencies into grid[GDP..GDR] (plain FP64 form) or only into the FP32 grids
+// the FP32-bulk gathers read
 int vorticity_tendency_spectral(Ctx *c, int stage) {
   (void)stage;
   const Geometry &g = c->g;
@@ -954,11 +959,12 @@ int vorticity_tendency_spectral(Ctx *c, int stage) {
     PMPIC_TRY(fft_backward(c, 8, src, dst, nullptr, op));
   }
   const int64_t nnode = (int64_t)g.nzn * g.ny * g.nx;
-  const bool b32 = c->precision == 1;  // FP32 copies of the tendencies for the FP32-bulk gathers
+  const bool b32 = c->precision == 1;  // the FP32-bulk form: FP32 tendencies only
   hipLaunchKernelGGL(k_tendency, dim3(cdiv(nnode, TPB)), dim3(TPB), 0, c->stream, nnode,
                      c->grid[GP], c->grid[GQ], c->grid[GR], c->gtmp[0], c->gtmp[1], c->gtmp[2],
-                     c->gtmp[3], c->gtmp[4], c->gtmp[5], c->gtmp[6], c->gtmp[7], c->grid[GDP],
-                     c->grid[GDQ], c->grid[GDR], b32 ? c->b32.grid[3] : nullptr,
+                     c->gtmp[3], c->gtmp[4], c->gtmp[5], c->gtmp[6], c->gtmp[7],
+                     b32 ? nullptr : c->grid[GDP], b32 ? nullptr : c->grid[GDQ],
+                     b32 ? nullptr : c->grid[GDR], b32 ? c->b32.grid[3] : nullptr,
                      b32 ? c->b32.grid[4] : nullptr, b32 ? c->b32.grid[5] : nullptr);
   PMPIC_CHECK(hipGetLastError());
   return PMPIC_OK;
