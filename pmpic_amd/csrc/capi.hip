@@ -634,10 +634,13 @@ int pmpic_get_grid(pmpic_ctx *ctx, int32_t id, double *grid) {
   if (!ctx || !grid || id < 0 || id >= NGRIDS) return PMPIC_ERR_ARG;
   Ctx *c = &ctx->c;
   std::vector<double> h;
-  if (c->precision == 1 && id >= GDP && id <= GDR) {
-    // the FP32-bulk form keeps the tendencies only in the FP32 grids its gathers read
+  const bool vel = id >= GU && id <= GW, tend = id >= GDP && id <= GDR;
+  if (c->precision == 1 && (vel || tend)) {
+    // the FP32-bulk form keeps u, v, w and the tendencies only in the FP32 grids its gathers
+    // read (b32.grid: U, V, W, DP, DQ, DR)
     std::vector<float> hf(c->gsz);
-    PMPIC_CHECK(hipMemcpyAsync(hf.data(), c->b32.grid[3 + (id - GDP)], sizeof(float) * c->gsz,
+    const float *src32 = c->b32.grid[vel ? id - GU : 3 + (id - GDP)];
+    PMPIC_CHECK(hipMemcpyAsync(hf.data(), src32, sizeof(float) * c->gsz,
                                hipMemcpyDeviceToHost, c->stream));
     PMPIC_CHECK(hipStreamSynchronize(c->stream));
     h.assign(hf.begin(), hf.end());

@@ -390,14 +390,17 @@ __global__ void __launch_bounds__(FFT_TPB) k_cols_c2r(Plan p, const double2 *__r
   __syncthreads();
   double2 *R = lds_fft<true>(A, B, T, p, nbb);
   const double dn = 1.0 / (double)ny;
-  double *out = dst + (int64_t)k * ny * nx;
+  // either output may be absent: the FP32-bulk form keeps some fields only in FP32
+  double *out = dst ? dst + (int64_t)k * ny * nx : nullptr;
   float *o32 = fp.out32[f] ? fp.out32[f] + (int64_t)k * ny * nx : nullptr;
   for (int e = threadIdx.x; e < nbb * ny; e += FFT_TPB) {
     const int y = e / nbb, t = e - y * nbb;
     const int xa = x0 + 2 * t, xb = xa + 1;
     const double2 v = R[t * ny + y];
-    out[(int64_t)y * nx + xa] = v.x * dn;
-    if (xb < nx) out[(int64_t)y * nx + xb] = v.y * dn;
+    if (out) {
+      out[(int64_t)y * nx + xa] = v.x * dn;
+      if (xb < nx) out[(int64_t)y * nx + xb] = v.y * dn;
+    }
     if (o32) {
       o32[(int64_t)y * nx + xa] = (float)(v.x * dn);
       if (xb < nx) o32[(int64_t)y * nx + xb] = (float)(v.y * dn);

@@ -656,8 +656,10 @@ __global__ void k_tendency(int64_t nnode, const double *__restrict__ P,
 // maxima of u^2, v^2, w^2 and p^2+q^2+r^2 over the interior (vort2vel.F90:331-337,
 // vorticity_tendency.F90:279-282).  All values are non-negative, so their bit patterns order
 // like the values and an integer atomicMax gives a deterministic result.
-__global__ void k_max4(int64_t nnode, const double *__restrict__ U, const double *__restrict__ V,
-                       const double *__restrict__ W, const double *__restrict__ P,
+// TU: the velocity grids' type (double, or float for the FP32-bulk form's velocity grids)
+template <typename TU>
+__global__ void k_max4(int64_t nnode, const TU *__restrict__ U, const TU *__restrict__ V,
+                       const TU *__restrict__ W, const double *__restrict__ P,
                        const double *__restrict__ Q, const double *__restrict__ R,
                        DevScalars *sc) {
   __shared__ double sm[4][TPB];
@@ -668,7 +670,7 @@ __global__ void k_max4(int64_t nnode, const double *__restrict__ U, const double
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t q0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q0 < nnode;
        q0 += MPT * stride) {
-    double u[MPT], v[MPT], w[MPT], pp[MPT], qq[MPT], rr[MPT];
+    double u[MPT], v[MPT], w[MPT], pp[MPT], qq[MPT], rr[MPT];  // velocities widened exactly
 #pragma unroll
     for (int k = 0; k < MPT; k++) {
       const int64_t q = min(q0 + k * stride, nnode - 1);
@@ -926,8 +928,10 @@ int vort2vel_spectral(Ctx *c, int stage) {
                      c->sv, c->sw, c->kx, c->ky, c->filt);
   {
     const double *src[6] = {c->sd, c->se, c->sf, c->sder[2], c->sder[3], c->sder[4]};
-    double *dst[6] = {c->grid[GP], c->grid[GQ], c->grid[GR], c->grid[GU], c->grid[GV],
-                      c->grid[GW]};
+    // the FP32-bulk form keeps u, v, w only as the FP32 grids its gathers read
+    const bool b32 = c->precision == 1;
+    double *dst[6] = {c->grid[GP], c->grid[GQ], c->grid[GR], b32 ? nullptr : c->grid[GU],
+                      b32 ? nullptr : c->grid[GV], b32 ? nullptr : c->grid[GW]};
     // the FP32-bulk form gathers u, v, w from FP32 copies written by the same pass
     float *dst32[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     if (c->precision == 1)
@@ -976,8 +980,13 @@ int grid_max_reduce(Ctx *c) {
   hipLaunchKernelGGL(k_zero_max, dim3(1), dim3(1), 0, c->stream, c->dsc);
   int64_t nb = cdiv(nnode, TPB);
   if (nb > 1024) nb = 1024;
-  hipLaunchKernelGGL(k_max4, dim3(nb), dim3(TPB), 0, c->stream, nnode, c->grid[GU], c->grid[GV],
-                     c->grid[GW], c->grid[GP], c->grid[GQ], c->grid[GR], c->dsc);
+  if (c->precision == 1)  // the FP32-bulk form's velocity grids are FP32 only
+    hipLaunchKernelGGL(k_max4<float>, dim3(nb), dim3(TPB), 0, c->stream, nnode, c->b32.grid[0],
+                       c->b32.grid[1], c->b32.grid[2], c->grid[GP], c->grid[GQ], c->grid[GR],
+                       c->dsc);
+  else
+    hipLaunchKernelGGL(k_max4<double>, dim3(nb), dim3(TPB), 0, c->stream, nnode, c->grid[GU],
+                       c->grid[GV], c->grid[GW], c->grid[GP], c->grid[GQ], c->grid[GR], c->dsc);
   hipLaunchKernelGGL(k_select_dt, dim3(1), dim3(1), 0, c->stream, g, c->dsc);
   PMPIC_CHECK(hipGetLastError());
   return PMPIC_OK;
