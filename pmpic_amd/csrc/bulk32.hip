@@ -1083,6 +1083,7 @@ float *partial32b(Ctx *c) { return reinterpret_cast<float *>(c->partial2); }
 
 int b32_alloc(Ctx *c) {
   Bulk32 &b = c->b32;
+  for (int f = 0; f < 6; f++) b.kept_pending[f] = false;
   PMPIC_TRY(dalloc32(&b.f, (int64_t)NF32 * c->cap));
   PMPIC_TRY(dalloc32(&b.cell, c->cap));
   PMPIC_TRY(dalloc32(&b.cello, c->cap));
@@ -1173,6 +1174,9 @@ int b32_get_parcels(Ctx *c, double *const *fields) {
 
 int b32_par2grid(Ctx *c) {
   const int64_t n = c->n;
+  // this par2grid writes grid[GP..GR] and grid[GB]: the kept p, q, r, b are superseded
+  c->b32.kept_pending[1] = c->b32.kept_pending[3] = false;
+  c->b32.kept_pending[4] = c->b32.kept_pending[5] = false;
   PMPIC_TRY(bin32(c, n));
   if (c->b32.capc == CAPC_LOW)
     PMPIC_TRY(launch_timed(c, T_ACCUM, k32_cell_accum<A32_P2G, CAPC_LOW, ROWS_LOW>,
@@ -1274,11 +1278,9 @@ int b32_mix(Ctx *c) {
   hipLaunchKernelGGL((k32_node_sum<5, false, false, float>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0,
                      c->stream, g, partial32b(c), c->ncell, res, c->cellflag);
   // the reference builds the kept grids in the model's vol, b, hg, p, q, r grids
-  // (parcel_mixing.F90:131), which is what a grid export shows after mixing
-  const GridId kept_ids[6] = {GVOL, GB, GHG, GP, GQ, GR};
-  for (int f = 0; f < 6; f++)
-    hipLaunchKernelGGL(k32_grid_to_f64, dim3(cdiv(nnode, TPB)), dim3(TPB), 0, c->stream,
-                       b.mgrid[f], c->grid[kept_ids[f]], nnode);
+  // (parcel_mixing.F90:131), which is what a grid export shows after mixing; nothing in the
+  // step reads them there, so b32_export_grid widens them when one is exported
+  for (int f = 0; f < 6; f++) b.kept_pending[f] = true;
   if (norig > 0)
     hipLaunchKernelGGL(k32_keep_flags, dim3(cdiv(norig, TPB * KPT)), dim3(TPB), 0, c->stream, b.f,
                        c->cap, norig, vmin, c->flag);
@@ -1342,6 +1344,22 @@ int b32_mix(Ctx *c) {
     }
   }
   c->n = np;
+  return PMPIC_OK;
+}
+
+// the FP64 grid `id` as a grid export sees it in the FP32-bulk form: a kept grid of the last
+// mixing still in mgrid is widened into it first
+int b32_export_grid(Ctx *c, int id) {
+  const GridId kept_ids[6] = {GVOL, GB, GHG, GP, GQ, GR};
+  Bulk32 &b = c->b32;
+  const int64_t nnode = (int64_t)c->g.nzn * c->g.ny * c->g.nx;
+  for (int f = 0; f < 6; f++)
+    if (kept_ids[f] == id && b.kept_pending[f]) {
+      hipLaunchKernelGGL(k32_grid_to_f64, dim3(cdiv(nnode, TPB)), dim3(TPB), 0, c->stream,
+                         b.mgrid[f], c->grid[id], nnode);
+      PMPIC_CHECK(hipGetLastError());
+      b.kept_pending[f] = false;
+    }
   return PMPIC_OK;
 }
 

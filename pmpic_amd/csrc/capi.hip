@@ -645,6 +645,7 @@ int pmpic_get_grid(pmpic_ctx *ctx, int32_t id, double *grid) {
     PMPIC_CHECK(hipStreamSynchronize(c->stream));
     h.assign(hf.begin(), hf.end());
   } else {
+    if (c->precision == 1) PMPIC_TRY(b32_export_grid(c, id));
     PMPIC_TRY(download(c, h, c->grid[id], c->gsz));
   }
   grid_from_device_layout(c->g, h, grid);

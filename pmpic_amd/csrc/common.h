@@ -51,6 +51,9 @@ struct Bulk32 {
   int capc;          // capacity of the current binning (0, 16 or 32)
   int capmax;        // entries per cell allocated (0 or 32)
   int capfix;        // capacity forced by PMPIC_SLOT_CAPACITY (16 or 32), or 0 for the mean rule
+  // kept grid k (vol, b, hg, p, q, r) of the last mixing is current in mgrid[k] and not yet
+  // widened into the FP64 grid a grid export reads
+  bool kept_pending[6];
 };
 
 // grid ids exported through the C-ABI (writebinary.F90:287-296 order, then the tendencies)
@@ -290,6 +293,7 @@ int b32_rk_stage(Ctx *c, int stage, int stretch);
 int b32_split(Ctx *c);
 int b32_mix(Ctx *c);
 int b32_conservation_sums(Ctx *c, double out6[6]);
+int b32_export_grid(Ctx *c, int id);  // brings grid[id] up to date for a grid export
 int scan_exclusive_i32(Ctx *c, const int32_t *in, int32_t *out, int64_t n);
 int sync_scalars(Ctx *c);
 
