@@ -31,11 +31,6 @@ __global__ void k_fill_i32(int32_t *a, int64_t n, int32_t v) {
   if (i < n) a[i] = v;
 }
 
-__global__ void k_fill_f64(double *a, int64_t n, double v) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) a[i] = v;
-}
-
 // cell of every parcel and its slot in the cell's bucket.  Parcels that are neighbours in slot
 // order usually share a cell, so lanes are grouped into runs of equal cell and one lane per run
 // takes the whole run's slots with a single integer atomic.  Each wavefront walks BIN_SEQ
