@@ -310,7 +310,7 @@ def main():
         copy_calibrate(args.copy_calibrate)
         return 0
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+        print(json.dumps({"error": "the whole domain runs on one GPU; --gpus must be 1",
                           "n_gpus": args.gpus}))
         print("bench.py: only --gpus 1 is supported (the whole domain is on one MI355X)",
               file=sys.stderr)
