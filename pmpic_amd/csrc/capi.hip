@@ -198,7 +198,7 @@ int free_all(Ctx *c) {
   hipFree(c->dz_den); hipFree(c->dz_cp);
   hipFree(c->fbuf); hipFree(c->ibuf);
   hipFree(c->cbuf); hipFree(c->tw_x); hipFree(c->tw_y);
-  hipFree(c->scan_tmp); hipFree(c->blk); hipFree(c->dsc);
+  hipFree(c->scan_tmp); hipFree(c->blk); hipFree(c->maxblk); hipFree(c->dsc);
   for (int w = 0; w < NTIMERS; w++)
     for (int e = 0; e < 64; e++) {
       if (c->ev[w][e][0]) hipEventDestroy(c->ev[w][e][0]);
@@ -287,6 +287,7 @@ int create(const pmpic_config *cf, Ctx *c) {
   PMPIC_TRY(dalloc(&c->cbuf, 8 * c->csz));
   c->nblk_max = cdiv(big, 256) + 1;
   PMPIC_TRY(dalloc(&c->blk, 20 * c->nblk_max + 64));
+  PMPIC_TRY(dalloc(&c->maxblk, 4 * 1024));
   PMPIC_TRY(dalloc(&c->dsc, 1));
   PMPIC_CHECK(hipHostMalloc(&c->pin, 256));
   PMPIC_CHECK(hipEventCreateWithFlags(&c->sync_ev, hipEventDisableTiming));

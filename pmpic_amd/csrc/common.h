@@ -137,6 +137,7 @@ struct Ctx {
   void *scan_tmp;
   size_t scan_tmp_bytes;
   int32_t *blk;                       // per-block counters (10 * nblk_max + 16)
+  double *maxblk;                     // per-block maxima of the timestep reduction (4 * MAXBLK)
   int64_t nblk_max;
   DevScalars *dsc;                    // device scalars
   DevScalars hsc;                     // host mirror

@@ -661,7 +661,7 @@ template <typename TU>
 __global__ void k_max4(int64_t nnode, const TU *__restrict__ U, const TU *__restrict__ V,
                        const TU *__restrict__ W, const double *__restrict__ P,
                        const double *__restrict__ Q, const double *__restrict__ R,
-                       DevScalars *sc) {
+                       double *__restrict__ blkmax) {
   __shared__ double sm[4][TPB];
   double m[4] = {0, 0, 0, 0};
   // MPT nodes per pass, their loads issued together; an index past the end is clamped to the
@@ -693,15 +693,31 @@ __global__ void k_max4(int64_t nnode, const TU *__restrict__ U, const TU *__rest
       for (int f = 0; f < 4; f++) sm[f][threadIdx.x] = fmax(sm[f][threadIdx.x], sm[f][threadIdx.x + s]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    unsigned long long *t = (unsigned long long *)&sc->umax2;
-    for (int f = 0; f < 4; f++)
-      atomicMax(t + f, (unsigned long long)__double_as_longlong(sm[f][0]));
-  }
+  // one value per block and quantity; k_max4_final reduces them (no contended atomics)
+  if (threadIdx.x == 0)
+    for (int f = 0; f < 4; f++) blkmax[(int64_t)f * gridDim.x + blockIdx.x] = sm[f][0];
 }
 
-__global__ void k_zero_max(DevScalars *sc) {
-  sc->umax2 = 0; sc->vmax2 = 0; sc->wmax2 = 0; sc->omax2 = 0;
+// the maxima over the nb blocks of k_max4 (one workgroup); a maximum is exact in any order
+__global__ void k_max4_final(int nb, const double *__restrict__ blkmax, DevScalars *sc) {
+  __shared__ double sm[4][TPB];
+  double m[4] = {0, 0, 0, 0};
+  for (int i = threadIdx.x; i < nb; i += blockDim.x)
+#pragma unroll
+    for (int f = 0; f < 4; f++) m[f] = fmax(m[f], blkmax[(int64_t)f * nb + i]);
+  for (int f = 0; f < 4; f++) sm[f][threadIdx.x] = m[f];
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int f = 0; f < 4; f++) sm[f][threadIdx.x] = fmax(sm[f][threadIdx.x], sm[f][threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    sc->umax2 = sm[0][0];
+    sc->vmax2 = sm[1][0];
+    sc->wmax2 = sm[2][0];
+    sc->omax2 = sm[3][0];
+  }
 }
 
 // timestep selection at RK stage 1: vort2vel.F90:329-381 then vorticity_tendency.F90:277-321
@@ -977,16 +993,17 @@ int vorticity_tendency_spectral(Ctx *c, int stage) {
 int grid_max_reduce(Ctx *c) {
   const Geometry &g = c->g;
   const int64_t nnode = (int64_t)g.nzn * g.ny * g.nx;
-  hipLaunchKernelGGL(k_zero_max, dim3(1), dim3(1), 0, c->stream, c->dsc);
   int64_t nb = cdiv(nnode, TPB);
-  if (nb > 1024) nb = 1024;
+  if (nb > 1024) nb = 1024;  // c->maxblk holds 4 * 1024 block maxima
   if (c->precision == 1)  // the FP32-bulk form's velocity grids are FP32 only
     hipLaunchKernelGGL(k_max4<float>, dim3(nb), dim3(TPB), 0, c->stream, nnode, c->b32.grid[0],
                        c->b32.grid[1], c->b32.grid[2], c->grid[GP], c->grid[GQ], c->grid[GR],
-                       c->dsc);
+                       c->maxblk);
   else
     hipLaunchKernelGGL(k_max4<double>, dim3(nb), dim3(TPB), 0, c->stream, nnode, c->grid[GU],
-                       c->grid[GV], c->grid[GW], c->grid[GP], c->grid[GQ], c->grid[GR], c->dsc);
+                       c->grid[GV], c->grid[GW], c->grid[GP], c->grid[GQ], c->grid[GR],
+                       c->maxblk);
+  hipLaunchKernelGGL(k_max4_final, dim3(1), dim3(TPB), 0, c->stream, (int)nb, c->maxblk, c->dsc);
   hipLaunchKernelGGL(k_select_dt, dim3(1), dim3(1), 0, c->stream, g, c->dsc);
   PMPIC_CHECK(hipGetLastError());
   return PMPIC_OK;
