@@ -654,8 +654,8 @@ __global__ void k_tendency(int64_t nnode, const double *__restrict__ P,
 }
 
 // maxima of u^2, v^2, w^2 and p^2+q^2+r^2 over the interior (vort2vel.F90:331-337,
-// vorticity_tendency.F90:279-282).  All values are non-negative, so their bit patterns order
-// like the values and an integer atomicMax gives a deterministic result.
+// vorticity_tendency.F90:279-282), per block; a maximum is exact in any order, so the result
+// is deterministic.
 // TU: the velocity grids' type (double, or float for the FP32-bulk form's velocity grids)
 template <typename TU>
 __global__ void k_max4(int64_t nnode, const TU *__restrict__ U, const TU *__restrict__ V,
