@@ -14,6 +14,9 @@ checks are size-independent properties of the path:
 * the FP32-bulk form stays within 1e-5 relative of the FP64 form on the velocity and vorticity
   grids, with identical parcel counts (the FP32-bulk form's bar), on both workloads; the second
   runs the binning's 16-slot capacity at full size.
+
+The reference's own CPU configuration (32x32x32 cells, 27 parcels per cell) is held to the
+oracle directly.
 """
 import numpy as np
 import pytest
@@ -43,6 +46,28 @@ def _ctx(ic, n, nz, lx, lz, precision):
 @pytest.fixture(scope="module")
 def thermal128():
     return _thermal(128, 128, 3)
+
+
+def test_cpu_configuration_matches_oracle(oracle_mod):
+    """The reference's CPU configuration, 32x32x32 cells with 27 parcels per cell (884,736
+    parcels), small enough for the oracle: three whole steps of the FP64 form against it, with
+    identical counts, parcels within 1e-10 and velocity grids within 1e-11 (relative to each
+    field's maximum)."""
+    (ic, lx, lz) = _thermal(32, 32, 3)
+    o = oracle_mod.Oracle(32, 32, 32, 0.0, lx, 0.0, lx, 0.0, lz, 2 * len(ic["x"]), 0.01, 0.01,
+                          0.5)
+    o.set_parcels(ic)
+    g = _ctx(ic, 32, 32, lx, lz, 0)
+    for step in range(3):
+        o.step()
+        g.step(1)
+        assert g.n == o.n, step
+    gp, op = g.get_parcels(), o.get_parcels()
+    for k in ("x", "y", "z", "p", "q", "r", "vol", "b", "h"):
+        assert rel_err(gp[k], op[k]) < 1e-10, k
+    for name in ("u", "v", "w"):
+        assert rel_err(g.grid(name)[2:-2, 2:-2, :], o.grid(name)[2:-2, 2:-2, :]) < 1e-11, name
+    g.close()
 
 
 def test_par2grid_of_ones_is_one(thermal128):
