@@ -22,7 +22,7 @@ def build_fortran(verbose=False):
     fdir = os.path.join(HERE, "fortran")
     if not os.path.exists(os.path.join(fdir, "Makefile")) or shutil.which("amdflang") is None:
         return None
-    out = subprocess.run(["make", "-s", "-C", fdir], capture_output=not verbose, text=True)
+    out = subprocess.run(["make", "-s", "-B", "-C", fdir], capture_output=not verbose, text=True)
     if out.returncode != 0:
         raise RuntimeError("Fortran shim build failed:\n" + (out.stdout or "") + (out.stderr or ""))
     return fdir
