@@ -27,6 +27,7 @@
 #include <algorithm>
 #include <math.h>
 #include <stdlib.h>
+#include <string.h>
 #include <vector>
 
 // The FP32-bulk form is not held to the reference's operation order (its bar is the velocity
@@ -1042,6 +1043,7 @@ int bin32(Ctx *c, int64_t n) {
   // cell (fewer empty entries to read), else 32
   if (b.capmax == 0) b.capc = 0;
   else if (b.capfix) b.capc = b.capfix;
+  else if (b.captuned) b.capc = b.captuned;
   else b.capc = (n <= (int64_t)LOW_PPC * c->ncell) ? CAPC_LOW : CAPC;
   hipLaunchKernelGGL(k32_bin_reset, dim3(cdiv(nc1, TPB)), dim3(TPB), 0, c->stream, c->cell_count,
                      nc1, c->dsc);
@@ -1088,15 +1090,23 @@ int b32_alloc(Ctx *c) {
   PMPIC_TRY(dalloc32(&b.cell, c->cap));
   PMPIC_TRY(dalloc32(&b.cello, c->cap));
   PMPIC_TRY(dalloc32(&b.tag, c->cap));
+  // a parcel created by mixing takes no tag (parcel_mixing.F90 copies tags only when it
+  // backfills, :377): a slot past every earlier count reads 0, as in the FP64 form's arrays
+  PMPIC_CHECK(hipMemset(b.tag, 0, sizeof(double) * (size_t)c->cap));
   for (int i = 0; i < 6; i++) PMPIC_TRY(dalloc32(&b.grid[i], c->gsz));
   for (int i = 0; i < 12; i++) PMPIC_TRY(dalloc32(&b.mgrid[i], c->gsz));
   // 32-bit byte offsets into the slot table.  PMPIC_SLOT_CAPACITY in the environment: 0 drops
   // the table (every cell then binned through the dense region, as grids beyond that range
-  // are); 16 or 32 fixes the capacity instead of choosing it by the mean parcels per cell
+  // are); 16 or 32 fixes the capacity instead of choosing it by the mean parcels per cell;
+  // "tune" lets the first par2grid time the accumulation at both and keep the faster
   b.capmax = (c->ncell * CAPC < ((int64_t)1 << 30)) ? CAPC : 0;
   b.capfix = 0;
+  b.captune = false;
+  b.captuned = 0;
   const char *env = getenv("PMPIC_SLOT_CAPACITY");
-  if (env) {
+  if (env && strcmp(env, "tune") == 0) {
+    b.captune = b.capmax > 0;
+  } else if (env) {
     const int v = atoi(env);
     if (v == 0) b.capmax = 0;
     else if (v == CAPC_LOW || v == CAPC) b.capfix = v;
@@ -1172,22 +1182,64 @@ int b32_get_parcels(Ctx *c, double *const *fields) {
   return PMPIC_OK;
 }
 
+static int p2g_accum(Ctx *c) {
+  if (c->b32.capc == CAPC_LOW)
+    return launch_timed(c, T_ACCUM, k32_cell_accum<A32_P2G, CAPC_LOW, ROWS_LOW>,
+                        dim3(cdiv(c->ncell, TILE32)), dim3(ATPB), 0, c->b32, c->cap, c->g,
+                        zmap_of(c->g), (float)c->g.vmin, c->cell_count, c->cell_start, c->sorted,
+                        c->ncell, partial32(c), nullptr, nullptr, nullptr);
+  return launch_timed(c, T_ACCUM, k32_cell_accum<A32_P2G, CAPC, ROWS32>,
+                      dim3(cdiv(c->ncell, TILE32)), dim3(ATPB), 0, c->b32, c->cap, c->g,
+                      zmap_of(c->g), (float)c->g.vmin, c->cell_count, c->cell_start, c->sorted,
+                      c->ncell, partial32(c), nullptr, nullptr, nullptr);
+}
+
+// The capacity tuning: bin at 16 and at 32 slots, time the accumulation of each (its second
+// launch, after one to warm the caches) and keep the faster for every later binning of this
+// context.  Both capacities sum each cell's parcels in ascending parcel order, so the choice
+// changes no result; the partials of the last run are this par2grid's.
+static int tune_capacity(Ctx *c, int64_t n) {
+  Bulk32 &b = c->b32;
+  const int caps[2] = {CAPC_LOW, CAPC};
+  float ms[2] = {0.f, 0.f};
+  hipEvent_t e0, e1;
+  PMPIC_CHECK(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    hipEventDestroy(e0);
+    return PMPIC_ERR_HIP;
+  }
+  int rc = PMPIC_OK;
+  for (int t = 0; t < 2 && rc == PMPIC_OK; t++) {
+    b.capfix = caps[t];
+    rc = bin32(c, n);
+    if (rc == PMPIC_OK) rc = p2g_accum(c);
+    if (rc == PMPIC_OK && hipEventRecord(e0, c->stream) != hipSuccess) rc = PMPIC_ERR_HIP;
+    if (rc == PMPIC_OK) rc = p2g_accum(c);
+    if (rc == PMPIC_OK && (hipEventRecord(e1, c->stream) != hipSuccess ||
+                           hipEventSynchronize(e1) != hipSuccess ||
+                           hipEventElapsedTime(&ms[t], e0, e1) != hipSuccess))
+      rc = PMPIC_ERR_HIP;
+  }
+  b.capfix = 0;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc != PMPIC_OK) return rc;
+  b.captuned = (ms[0] < ms[1]) ? caps[0] : caps[1];
+  b.captune = false;
+  return PMPIC_OK;
+}
+
 int b32_par2grid(Ctx *c) {
   const int64_t n = c->n;
   // this par2grid writes grid[GP..GR] and grid[GB]: the kept p, q, r, b are superseded
   c->b32.kept_pending[1] = c->b32.kept_pending[3] = false;
   c->b32.kept_pending[4] = c->b32.kept_pending[5] = false;
-  PMPIC_TRY(bin32(c, n));
-  if (c->b32.capc == CAPC_LOW)
-    PMPIC_TRY(launch_timed(c, T_ACCUM, k32_cell_accum<A32_P2G, CAPC_LOW, ROWS_LOW>,
-                           dim3(cdiv(c->ncell, TILE32)), dim3(ATPB), 0, c->b32, c->cap, c->g,
-                           zmap_of(c->g), (float)c->g.vmin, c->cell_count, c->cell_start,
-                           c->sorted, c->ncell, partial32(c), nullptr, nullptr, nullptr));
-  else
-    PMPIC_TRY(launch_timed(c, T_ACCUM, k32_cell_accum<A32_P2G, CAPC, ROWS32>,
-                           dim3(cdiv(c->ncell, TILE32)), dim3(ATPB), 0, c->b32, c->cap, c->g,
-                           zmap_of(c->g), (float)c->g.vmin, c->cell_count, c->cell_start,
-                           c->sorted, c->ncell, partial32(c), nullptr, nullptr, nullptr));
+  if (c->b32.captune && n > 0) {
+    PMPIC_TRY(tune_capacity(c, n));
+  } else {
+    PMPIC_TRY(bin32(c, n));
+    PMPIC_TRY(p2g_accum(c));
+  }
   Out32<double> out = {{c->grid[GP], c->grid[GQ], c->grid[GR], c->grid[GB], nullptr, nullptr}};
   const int64_t nnode = (int64_t)c->g.nzn * c->g.ny * c->g.nx;
   hipLaunchKernelGGL((k32_node_sum<4, true, true, double>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0,

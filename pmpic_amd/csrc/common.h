@@ -51,6 +51,8 @@ struct Bulk32 {
   int capc;          // capacity of the current binning (0, 16 or 32)
   int capmax;        // entries per cell allocated (0 or 32)
   int capfix;        // capacity forced by PMPIC_SLOT_CAPACITY (16 or 32), or 0 for the mean rule
+  bool captune;      // PMPIC_SLOT_CAPACITY=tune and the first par2grid has not timed both yet
+  int captuned;      // the capacity that timing picked (16 or 32), or 0 while none has
   // kept grid k (vol, b, hg, p, q, r) of the last mixing is current in mgrid[k] and not yet
   // widened into the FP64 grid a grid export reads
   bool kept_pending[6];

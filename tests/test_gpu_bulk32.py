@@ -164,3 +164,30 @@ def test_timers_kernels_only_and_all(precision):
     t = g.timer_table()
     assert t["par2grid_cells"][1] == 4 and t["vort2vel"][1] == 4 and t["RK4_all_steps"][1] == 4
     assert t["vort2vel"][0] > 0
+
+
+@pytest.mark.parametrize("ppcd", [2, 3])
+def test_slot_capacity_changes_no_result(monkeypatch, ppcd):
+    """8 and 27 parcels per cell, two steps with the slot capacity fixed at 16, fixed at 32 and
+    tuned (PMPIC_SLOT_CAPACITY=tune: the first par2grid times the accumulation at both and keeps
+    the faster).  Every cell's parcels are summed in ascending parcel order whatever the
+    capacity, so counts, parcels and grids are bit-identical across the three."""
+    shape = (12, 10, 8)
+    ic = thermal_case(*shape, ppcd, vort=1.0, thin=0.1)
+    names = ("u", "v", "w", "p", "q", "r", "b")
+    out = {}
+    for mode in ("16", "32", "tune"):
+        monkeypatch.setenv("PMPIC_SLOT_CAPACITY", mode)
+        g = make(shape, 1, ic)
+        for _ in range(2):
+            g.step()
+        out[mode] = (g.n, g.last_counts(), g.get_parcels(), {k: g.grid(k) for k in names})
+        g.close()
+    n16, c16, p16, g16 = out["16"]
+    for mode in ("32", "tune"):
+        n, cnt, p, gr = out[mode]
+        assert n == n16 and cnt == c16, mode
+        for k in p16:
+            assert np.array_equal(p[k], p16[k]), (mode, k)
+        for k in names:
+            assert np.array_equal(gr[k], g16[k]), (mode, k)
