@@ -40,6 +40,7 @@ def lib():
         L.orc_create.restype = vp
         L.orc_create.argtypes = [i, i, i, d, d, d, d, d, d, i64, d, d, d]
         L.orc_destroy.argtypes = [vp]
+        L.orc_set_threads.argtypes = [i]
         for name in ("orc_field", "orc_grid", "orc_spec"):
             getattr(L, name).restype = ctypes.POINTER(ctypes.c_double)
             getattr(L, name).argtypes = [vp, i]
@@ -90,6 +91,11 @@ def lib():
 
 def _p(a):
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's later calls; its results do not depend on the count."""
+    lib().orc_set_threads(int(n))
 
 
 class Oracle:

@@ -23,6 +23,7 @@
 
 #include <complex.h>
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -43,6 +44,13 @@ struct orc {
   int32_t *is, *js, *ks;
   double *delxs, *delys, *delzs;
   int *index;   /* haloswap.F90:21 */
+  /* x-slabs of halo'd node columns for the threaded scatters: slab t owns node x indices
+   * [slab_lo[t], slab_lo[t+1]) and lists, in ascending parcel order, the parcels whose cell
+   * column is slab_lo[t]-1 .. slab_lo[t+1]-1 (those that deposit into its nodes) */
+  int nslab;
+  int *slab_lo;
+  int64_t *slab_n, *slab_cap;
+  int64_t **slab_list;
   int *keep;    /* parcel_mixing.F90:29 */
   /* grids */
   size_t gsz, ssz;
@@ -69,6 +77,8 @@ struct orc {
 #define NPC(o, k, j, i) ((((size_t)(i)-1) * (size_t)((o)->nyh - 1) + ((size_t)(j)-1)) * (size_t)((o)->nzn - 1) + ((size_t)(k)-1))
 
 static double *dalloc(size_t n) { return (double *)calloc(n ? n : 1, sizeof(double)); }
+static const double complex *twiddles(int n);
+static double complex *make_twiddles(int n);
 
 /* ffte_mod.F90:137-161 */
 static int check_factors(int n) {
@@ -163,7 +173,26 @@ orc_t *orc_create(int nx, int ny, int nz_cells, double xmin, double xmax, double
   o->dtm = dtm; o->dtmax = dtmax; o->prefactor = timestep_prefactor;
   o->rksteps = 4;    /* rk4_integrator.F90:51 */
   o->rkstep = 1;     /* rk4_integrator.F90:69 */
+  /* node-column slabs: one per thread, at least two node columns each */
+  {
+    int t = omp_get_max_threads();
+    if (t > o->nxh / 2) t = o->nxh / 2;
+    if (t < 1) t = 1;
+    o->nslab = t;
+    o->slab_lo = (int *)calloc((size_t)t + 1, sizeof(int));
+    o->slab_n = (int64_t *)calloc((size_t)t, sizeof(int64_t));
+    o->slab_cap = (int64_t *)calloc((size_t)t, sizeof(int64_t));
+    o->slab_list = (int64_t **)calloc((size_t)t, sizeof(int64_t *));
+    for (int s = 0; s <= t; s++) o->slab_lo[s] = 1 + (int)(((int64_t)o->nxh * s) / t);
+  }
+  /* the transform lengths' twiddle tables, built here so threads only read them */
+  twiddles(nx);
+  twiddles(ny);
   return o;
+}
+
+void orc_set_threads(int nthreads) {
+  if (nthreads > 0) omp_set_num_threads(nthreads);
 }
 
 void orc_destroy(orc_t *o) {
@@ -182,6 +211,8 @@ void orc_destroy(orc_t *o) {
   free(o->vres); free(o->bres); free(o->hres); free(o->pres); free(o->qres); free(o->rres);
   free(o->npercell);
   free(o->kx); free(o->ky); free(o->k2); free(o->filt);
+  for (int s = 0; s < o->nslab; s++) free(o->slab_list[s]);
+  free(o->slab_list); free(o->slab_n); free(o->slab_cap); free(o->slab_lo);
   free(o);
 }
 
@@ -211,26 +242,70 @@ void orc_geometry(const orc_t *o, double out[16]) {
 /* parcel_interpolation_mod                                                                  */
 /* ---------------------------------------------------------------------------------------- */
 
-/* cache_parcel_interp_weights, parcel_interpolation.F90:186-262 */
+/* the slab lists of the current cache (see struct orc): each thread scans the cached cell
+ * columns and keeps, in ascending order, the parcels that deposit into its slab's nodes */
+static void build_slabs(orc_t *o) {
+  const int64_t np = o->np;
+#pragma omp parallel num_threads(o->nslab)
+  {
+    for (int t = omp_get_thread_num(); t < o->nslab; t += omp_get_num_threads()) {
+      const int lo = o->slab_lo[t] - 1, hi = o->slab_lo[t + 1] - 1;  /* cell columns lo..hi */
+      int64_t cnt = 0;
+      for (int64_t n = 0; n < np; n++) cnt += (o->is[n] >= lo && o->is[n] <= hi);
+      if (cnt > o->slab_cap[t]) {
+        free(o->slab_list[t]);
+        o->slab_cap[t] = cnt + cnt / 8 + 16;
+        o->slab_list[t] = (int64_t *)malloc(sizeof(int64_t) * (size_t)o->slab_cap[t]);
+      }
+      int64_t m = 0;
+      for (int64_t n = 0; n < np; n++)
+        if (o->is[n] >= lo && o->is[n] <= hi) o->slab_list[t][m++] = n;
+      o->slab_n[t] = m;
+    }
+  }
+}
+
+/* cache_parcel_interp_weights, parcel_interpolation.F90:186-262.  Parcels are independent;
+ * the status returned is that of the first failing parcel, as the sequential loop's. */
 int orc_cache_weights(orc_t *o) {
   const double *X = o->f[ORC_X], *Y = o->f[ORC_Y], *Z = o->f[ORC_Z];
-  for (int64_t n = 0; n < o->np; n++) {
-    double xp = X[n], yp = Y[n], zp = Z[n];
-    int i = (int)floor((xp - o->xmin_h) / o->dx) + 1;
-    int j = (int)floor((yp - o->ymin_h) / o->dy) + 1;
-    int k = (int)floor((zp - o->zmin_h) / o->dz) + 1;
-    if (i < 1 || i >= o->nxh) return PMPIC_ERR_OUT_OF_BOX_X;
-    if (j < 1 || j >= o->nyh) return PMPIC_ERR_OUT_OF_BOX_Y;
-    if (k < 1 || k >= o->nzn) return PMPIC_ERR_OUT_OF_BOX_Z;
-    double delx = ((xp - o->xmin_h) - (double)(i - 1) * o->dx) / o->dx;
-    double dely = ((yp - o->ymin_h) - (double)(j - 1) * o->dy) / o->dy;
-    double delz = ((zp - o->zmin_h) - (double)(k - 1) * o->dz) / o->dz;
-    if (delx > 1. || delx < 0) return PMPIC_ERR_DELTA;
-    if (dely > 1. || dely < 0) return PMPIC_ERR_DELTA;
-    if (delz > 1. || delz < 0) return PMPIC_ERR_DELTA;
-    o->is[n] = i; o->js[n] = j; o->ks[n] = k;
-    o->delxs[n] = delx; o->delys[n] = dely; o->delzs[n] = delz;
+  int64_t bad = INT64_MAX;
+  int badst = PMPIC_OK;
+#pragma omp parallel
+  {
+    int64_t mybad = INT64_MAX;
+    int myst = PMPIC_OK;
+#pragma omp for schedule(static)
+    for (int64_t n = 0; n < o->np; n++) {
+      double xp = X[n], yp = Y[n], zp = Z[n];
+      int i = (int)floor((xp - o->xmin_h) / o->dx) + 1;
+      int j = (int)floor((yp - o->ymin_h) / o->dy) + 1;
+      int k = (int)floor((zp - o->zmin_h) / o->dz) + 1;
+      int st = PMPIC_OK;
+      if (i < 1 || i >= o->nxh) st = PMPIC_ERR_OUT_OF_BOX_X;
+      else if (j < 1 || j >= o->nyh) st = PMPIC_ERR_OUT_OF_BOX_Y;
+      else if (k < 1 || k >= o->nzn) st = PMPIC_ERR_OUT_OF_BOX_Z;
+      double delx = 0, dely = 0, delz = 0;
+      if (st == PMPIC_OK) {
+        delx = ((xp - o->xmin_h) - (double)(i - 1) * o->dx) / o->dx;
+        dely = ((yp - o->ymin_h) - (double)(j - 1) * o->dy) / o->dy;
+        delz = ((zp - o->zmin_h) - (double)(k - 1) * o->dz) / o->dz;
+        if (delx > 1. || delx < 0) st = PMPIC_ERR_DELTA;
+        else if (dely > 1. || dely < 0) st = PMPIC_ERR_DELTA;
+        else if (delz > 1. || delz < 0) st = PMPIC_ERR_DELTA;
+      }
+      if (st != PMPIC_OK) {
+        if (n < mybad) { mybad = n; myst = st; }
+        continue;
+      }
+      o->is[n] = i; o->js[n] = j; o->ks[n] = k;
+      o->delxs[n] = delx; o->delys[n] = dely; o->delzs[n] = delz;
+    }
+#pragma omp critical(orc_cache)
+    if (mybad < bad) { bad = mybad; badst = myst; }
   }
+  if (badst != PMPIC_OK) return badst;
+  build_slabs(o);
   return PMPIC_OK;
 }
 
@@ -265,6 +340,7 @@ void orc_fill(orc_t *o, double *A) {
 /* grid2par, parcel_interpolation.F90:267-332 */
 int orc_grid2par(orc_t *o, double *grid, double *var) {
   orc_fill(o, grid);
+#pragma omp parallel for schedule(static)
   for (int64_t n = 0; n < o->np; n++) {
     double delx = o->delxs[n], dely = o->delys[n], delz = o->delzs[n];
     int i = o->is[n], j = o->js[n], k = o->ks[n];
@@ -288,35 +364,47 @@ int orc_grid2par(orc_t *o, double *grid, double *var) {
 }
 
 /* par2grid, parcel_interpolation.F90:336-481 (the z-level doubling at :458-459 is overwritten
- * by the division at :463-469, so it has no effect) */
+ * by the division at :463-469, so it has no effect).  Each thread owns a slab of node columns
+ * and walks its slab list in ascending parcel order, adding only into its own nodes: every
+ * node receives the same terms in the same order as in the sequential loop. */
 int orc_par2grid(orc_t *o, const double *var, double *grid) {
   double *W = o->wgt, *D = o->dat;
   memset(W, 0, o->gsz * sizeof(double));
   memset(D, 0, o->gsz * sizeof(double));
   const double *VOL = o->f[ORC_VOL];
-  for (int64_t n = 0; n < o->np; n++) {
-    double delx = o->delxs[n], dely = o->delys[n], delz = o->delzs[n];
-    int i = o->is[n], j = o->js[n], k = o->ks[n];
-    double v = VOL[n], w = var[n];
-    double w000 = (1 - delz) * (1 - dely) * (1 - delx) * v;
-    D[GI(o, k, j, i)] += w000 * w; W[GI(o, k, j, i)] += w000;
-    double w001 = (1 - delz) * (1 - dely) * (delx)*v;
-    D[GI(o, k, j, i + 1)] += w001 * w; W[GI(o, k, j, i + 1)] += w001;
-    double w010 = (1 - delz) * (dely) * (1 - delx) * v;
-    D[GI(o, k, j + 1, i)] += w010 * w; W[GI(o, k, j + 1, i)] += w010;
-    double w011 = (1 - delz) * (dely) * (delx)*v;
-    D[GI(o, k, j + 1, i + 1)] += w011 * w; W[GI(o, k, j + 1, i + 1)] += w011;
-    double w100 = (delz) * (1 - dely) * (1 - delx) * v;
-    D[GI(o, k + 1, j, i)] += w100 * w; W[GI(o, k + 1, j, i)] += w100;
-    double w101 = (delz) * (1 - dely) * (delx)*v;
-    D[GI(o, k + 1, j, i + 1)] += w101 * w; W[GI(o, k + 1, j, i + 1)] += w101;
-    double w110 = (delz) * (dely) * (1 - delx) * v;
-    D[GI(o, k + 1, j + 1, i)] += w110 * w; W[GI(o, k + 1, j + 1, i)] += w110;
-    double w111 = (delz) * (dely) * (delx)*v;
-    D[GI(o, k + 1, j + 1, i + 1)] += w111 * w; W[GI(o, k + 1, j + 1, i + 1)] += w111;
+#pragma omp parallel num_threads(o->nslab)
+  {
+    for (int t = omp_get_thread_num(); t < o->nslab; t += omp_get_num_threads()) {
+      const int lo = o->slab_lo[t], hi = o->slab_lo[t + 1];  /* owned node columns [lo, hi) */
+      const int64_t *list = o->slab_list[t];
+      for (int64_t m = 0; m < o->slab_n[t]; m++) {
+        const int64_t n = list[m];
+        double delx = o->delxs[n], dely = o->delys[n], delz = o->delzs[n];
+        int i = o->is[n], j = o->js[n], k = o->ks[n];
+        double v = VOL[n], w = var[n];
+        const int own0 = (i >= lo && i < hi), own1 = (i + 1 >= lo && i + 1 < hi);
+        double w000 = (1 - delz) * (1 - dely) * (1 - delx) * v;
+        if (own0) { D[GI(o, k, j, i)] += w000 * w; W[GI(o, k, j, i)] += w000; }
+        double w001 = (1 - delz) * (1 - dely) * (delx)*v;
+        if (own1) { D[GI(o, k, j, i + 1)] += w001 * w; W[GI(o, k, j, i + 1)] += w001; }
+        double w010 = (1 - delz) * (dely) * (1 - delx) * v;
+        if (own0) { D[GI(o, k, j + 1, i)] += w010 * w; W[GI(o, k, j + 1, i)] += w010; }
+        double w011 = (1 - delz) * (dely) * (delx)*v;
+        if (own1) { D[GI(o, k, j + 1, i + 1)] += w011 * w; W[GI(o, k, j + 1, i + 1)] += w011; }
+        double w100 = (delz) * (1 - dely) * (1 - delx) * v;
+        if (own0) { D[GI(o, k + 1, j, i)] += w100 * w; W[GI(o, k + 1, j, i)] += w100; }
+        double w101 = (delz) * (1 - dely) * (delx)*v;
+        if (own1) { D[GI(o, k + 1, j, i + 1)] += w101 * w; W[GI(o, k + 1, j, i + 1)] += w101; }
+        double w110 = (delz) * (dely) * (1 - delx) * v;
+        if (own0) { D[GI(o, k + 1, j + 1, i)] += w110 * w; W[GI(o, k + 1, j + 1, i)] += w110; }
+        double w111 = (delz) * (dely) * (delx)*v;
+        if (own1) { D[GI(o, k + 1, j + 1, i + 1)] += w111 * w; W[GI(o, k + 1, j + 1, i + 1)] += w111; }
+      }
+    }
   }
   orc_fold(o, W);
   orc_fold(o, D);
+#pragma omp parallel for schedule(static)
   for (size_t c = 0; c < o->gsz; c++) {
     if (W[c] == 0.) W[c] = 1.;
     grid[c] = D[c] / W[c];
@@ -332,15 +420,30 @@ int orc_par2grid(orc_t *o, const double *var, double *grid) {
 
 /* twiddle table W_N^e = exp(-2 pi i e/N), e = 0..N-1, cached per transform length */
 typedef struct { int n; double complex *w; } tw_t;
-static tw_t tw_cache[4];
+#define TW_SLOTS 64
+static tw_t tw_cache[TW_SLOTS];
 
+/* tables are built under a lock and never freed, so a pointer handed out stays valid while
+ * other threads transform */
 static const double complex *twiddles(int n) {
-  for (int c = 0; c < 4; c++)
-    if (tw_cache[c].n == n) return tw_cache[c].w;
-  int slot = 0;
-  for (int c = 0; c < 4; c++)
-    if (tw_cache[c].n == 0) { slot = c; break; }
-  if (tw_cache[slot].n != 0) { free(tw_cache[slot].w); slot = 0; }
+  const double complex *found = NULL;
+#pragma omp critical(orc_twiddles)
+  {
+    for (int c = 0; c < TW_SLOTS && !found; c++)
+      if (tw_cache[c].n == n) found = tw_cache[c].w;
+    if (!found) {
+      int slot = -1;
+      for (int c = 0; c < TW_SLOTS; c++)
+        if (tw_cache[c].n == 0) { slot = c; break; }
+      double complex *w = make_twiddles(n);
+      if (slot >= 0) { tw_cache[slot].n = n; tw_cache[slot].w = w; }
+      found = w;  /* past TW_SLOTS lengths a table is not cached (and not freed) */
+    }
+  }
+  return found;
+}
+
+static double complex *make_twiddles(int n) {
   const double twopi = 8.0 * atan(1.0);
   double complex *w = (double complex *)malloc(sizeof(double complex) * n);
   for (int e = 0; e < n; e++) {
@@ -348,8 +451,6 @@ static const double complex *twiddles(int n) {
     double ang = -twopi * (double)e / (double)n;
     w[e] = cos(ang) + I * sin(ang);
   }
-  tw_cache[slot].n = n;
-  tw_cache[slot].w = w;
   return w;
 }
 
@@ -372,77 +473,120 @@ static void cfft_level(int n, const double complex *in, int stride, double compl
   }
 }
 
-static void cfft_forward(int n, double complex *a) {
-  const double complex *tw = twiddles(n);
-  double complex *in = (double complex *)malloc(sizeof(double complex) * n);
-  double complex *scr = (double complex *)malloc(sizeof(double complex) * n);
-  memcpy(in, a, sizeof(double complex) * n);
-  cfft_level(n, in, 1, a, scr, tw, 1);
-  free(in); free(scr);
+/* work buffers of one thread's transforms of length n */
+typedef struct { int n; const double complex *tw; double complex *a, *in, *scr; } fftws_t;
+
+static void fftws_init(fftws_t *w, int n) {
+  w->n = n;
+  w->tw = twiddles(n);
+  w->a = (double complex *)malloc(sizeof(double complex) * n);
+  w->in = (double complex *)malloc(sizeof(double complex) * n);
+  w->scr = (double complex *)malloc(sizeof(double complex) * n);
+}
+
+static void fftws_free(fftws_t *w) { free(w->a); free(w->in); free(w->scr); }
+
+/* forward transform of w->a in place */
+static void cfft_forward_ws(fftws_t *w) {
+  memcpy(w->in, w->a, sizeof(double complex) * w->n);
+  cfft_level(w->n, w->in, 1, w->a, w->scr, w->tw, 1);
 }
 
 /* ffte_r2c, ffte_mod.F90:66-89 */
-void orc_r2c(int n, const double *in, double *out) {
-  double complex *a = (double complex *)malloc(sizeof(double complex) * n);
-  for (int i = 0; i < n; i++) a[i] = in[i];
-  cfft_forward(n, a);
-  for (int i = 0; i <= n / 2; i++) { out[2 * i] = creal(a[i]); out[2 * i + 1] = cimag(a[i]); }
-  free(a);
+static void r2c_ws(fftws_t *w, const double *in, double *out) {
+  const int n = w->n;
+  for (int i = 0; i < n; i++) w->a[i] = in[i];
+  cfft_forward_ws(w);
+  for (int i = 0; i <= n / 2; i++) { out[2 * i] = creal(w->a[i]); out[2 * i + 1] = cimag(w->a[i]); }
 }
 
 /* ffte_c2r, ffte_mod.F90:97-124 with ZFFT1D(IOPT=+1), vzfft1d.f:41-69 */
-void orc_c2r(int n, const double *in, double *out) {
-  double complex *a = (double complex *)malloc(sizeof(double complex) * n);
+static void c2r_ws(fftws_t *w, const double *in, double *out) {
+  const int n = w->n;
+  double complex *a = w->a;
   for (int i = 0; i <= n / 2; i++) a[i] = in[2 * i] + I * in[2 * i + 1];
   for (int i = n / 2 + 1; i < n; i++) a[i] = conj(a[n - i]);
   for (int i = 0; i < n; i++) a[i] = conj(a[i]);
-  cfft_forward(n, a);
+  cfft_forward_ws(w);
   double dn = 1.0 / (double)n;
   for (int i = 0; i < n; i++) out[i] = creal(conj(a[i]) * dn);
-  free(a);
+}
+
+void orc_r2c(int n, const double *in, double *out) {
+  fftws_t w;
+  fftws_init(&w, n);
+  r2c_ws(&w, in, out);
+  fftws_free(&w);
+}
+
+void orc_c2r(int n, const double *in, double *out) {
+  fftws_t w;
+  fftws_init(&w, n);
+  c2r_ws(&w, in, out);
+  fftws_free(&w);
 }
 
 /* perform_forward_3dfft, pencilfft.F90:151-178 with transpose_and_forward_fft_in_y (:329-342)
  * and _in_x (:372-385): r2c along y of the interior, (Re,Im) interleaved (:838-864), then r2c
- * along x of each of those real y-rows. */
+ * along x of each of those real y-rows.  Rows are independent; threads take whole rows. */
 void orc_fft_forward(orc_t *o, const double *grid, double *spec) {
   const int nx = o->ncx, ny = o->ncy, nz = o->nzn, sy = o->sy, sx = o->sx;
-  double *rowy = dalloc(ny), *outy = dalloc(sy), *rowx = dalloc(nx), *outx = dalloc(sx);
   double *rb = dalloc((size_t)nz * sy * nx); /* (k, jj, x) */
-  for (int k = 1; k <= nz; k++)
-    for (int x = 0; x < nx; x++) {
+#pragma omp parallel
+  {
+    double *rowy = dalloc(ny), *outy = dalloc(sy), *rowx = dalloc(nx), *outx = dalloc(sx);
+    fftws_t wy, wx;
+    fftws_init(&wy, ny);
+    fftws_init(&wx, nx);
+#pragma omp for schedule(static)
+    for (int64_t kx = 0; kx < (int64_t)nz * nx; kx++) {
+      const int k = (int)(kx / nx) + 1, x = (int)(kx % nx);
       for (int y = 0; y < ny; y++) rowy[y] = grid[GI(o, k, y + 3, x + 3)];
-      orc_r2c(ny, rowy, outy);
+      r2c_ws(&wy, rowy, outy);
       for (int jj = 0; jj < sy; jj++) rb[((size_t)(k - 1) * sy + jj) * nx + x] = outy[jj];
     }
-  for (int k = 1; k <= nz; k++)
-    for (int jj = 0; jj < sy; jj++) {
+#pragma omp for schedule(static)
+    for (int64_t kj = 0; kj < (int64_t)nz * sy; kj++) {
+      const int k = (int)(kj / sy) + 1, jj = (int)(kj % sy);
       for (int x = 0; x < nx; x++) rowx[x] = rb[((size_t)(k - 1) * sy + jj) * nx + x];
-      orc_r2c(nx, rowx, outx);
+      r2c_ws(&wx, rowx, outx);
       for (int ii = 0; ii < sx; ii++) spec[SI(o, k, jj + 1, ii + 1)] = outx[ii];
     }
-  free(rowy); free(outy); free(rowx); free(outx); free(rb);
+    fftws_free(&wy); fftws_free(&wx);
+    free(rowy); free(outy); free(rowx); free(outx);
+  }
+  free(rb);
 }
 
 /* perform_backwards_3dfft, pencilfft.F90:187-212: c2r along x of each (Re,Im)-interleaved
  * row (:351-364), then c2r along y (:394-408), written to the interior of grid. */
 void orc_fft_backward(orc_t *o, const double *spec, double *grid) {
   const int nx = o->ncx, ny = o->ncy, nz = o->nzn, sy = o->sy, sx = o->sx;
-  double *inx = dalloc(sx), *rowx = dalloc(nx), *iny = dalloc(sy), *rowy = dalloc(ny);
   double *rb = dalloc((size_t)nz * sy * nx);
-  for (int k = 1; k <= nz; k++)
-    for (int jj = 0; jj < sy; jj++) {
+#pragma omp parallel
+  {
+    double *inx = dalloc(sx), *rowx = dalloc(nx), *iny = dalloc(sy), *rowy = dalloc(ny);
+    fftws_t wy, wx;
+    fftws_init(&wy, ny);
+    fftws_init(&wx, nx);
+#pragma omp for schedule(static)
+    for (int64_t kj = 0; kj < (int64_t)nz * sy; kj++) {
+      const int k = (int)(kj / sy) + 1, jj = (int)(kj % sy);
       for (int ii = 0; ii < sx; ii++) inx[ii] = spec[SI(o, k, jj + 1, ii + 1)];
-      orc_c2r(nx, inx, rowx);
+      c2r_ws(&wx, inx, rowx);
       for (int x = 0; x < nx; x++) rb[((size_t)(k - 1) * sy + jj) * nx + x] = rowx[x];
     }
-  for (int k = 1; k <= nz; k++)
-    for (int x = 0; x < nx; x++) {
+#pragma omp for schedule(static)
+    for (int64_t kx = 0; kx < (int64_t)nz * nx; kx++) {
+      const int k = (int)(kx / nx) + 1, x = (int)(kx % nx);
       for (int jj = 0; jj < sy; jj++) iny[jj] = rb[((size_t)(k - 1) * sy + jj) * nx + x];
-      orc_c2r(ny, iny, rowy);
+      c2r_ws(&wy, iny, rowy);
       for (int y = 0; y < ny; y++) grid[GI(o, k, y + 3, x + 3)] = rowy[y];
     }
-  free(inx); free(rowx); free(iny); free(rowy); free(rb);
+    fftws_free(&wy); fftws_free(&wx);
+    free(inx); free(rowx); free(iny); free(rowy);
+  }
+  free(rb);
 }
 
 /* ---------------------------------------------------------------------------------------- */
@@ -456,6 +600,7 @@ double orc_wavenumber(orc_t *o, int which, int jj, int ii) {
 
 /* spectral_filter, fft_ops.F90:176-197 */
 void orc_filter(orc_t *o, const double *in, double *out) {
+#pragma omp parallel for schedule(static)
   for (int i = 1; i <= o->sx; i++)
     for (int j = 1; j <= o->sy; j++) {
       double fl = o->filt[CI(o, j, i)];
@@ -465,12 +610,14 @@ void orc_filter(orc_t *o, const double *in, double *out) {
 
 /* diffx, fft_ops.F90:204-302 (one rank: no pair split across processes) */
 void orc_diffx(orc_t *o, const double *in, double *out) {
+#pragma omp parallel for schedule(static)
   for (int i = 1; i <= o->sx; i += 2)
     for (int j = 1; j <= o->sy; j++)
       for (int k = 1; k <= o->nzn; k++) {
         out[SI(o, k, j, i)] = -1. * in[SI(o, k, j, i + 1)];
         out[SI(o, k, j, i + 1)] = in[SI(o, k, j, i)];
       }
+#pragma omp parallel for schedule(static)
   for (int i = 1; i <= o->sx; i++)
     for (int j = 1; j <= o->sy; j++) {
       double kx = o->kx[CI(o, j, i)];
@@ -480,12 +627,14 @@ void orc_diffx(orc_t *o, const double *in, double *out) {
 
 /* diffy, fft_ops.F90:305-401 */
 void orc_diffy(orc_t *o, const double *in, double *out) {
+#pragma omp parallel for schedule(static)
   for (int i = 1; i <= o->sx; i++)
     for (int j = 1; j <= o->sy; j += 2)
       for (int k = 1; k <= o->nzn; k++) {
         out[SI(o, k, j, i)] = -1. * in[SI(o, k, j + 1, i)];
         out[SI(o, k, j + 1, i)] = in[SI(o, k, j, i)];
       }
+#pragma omp parallel for schedule(static)
   for (int i = 1; i <= o->sx; i++)
     for (int j = 1; j <= o->sy; j++) {
       double ky = o->ky[CI(o, j, i)];
@@ -510,7 +659,9 @@ int orc_tridiagonal(int n, const double *a, const double *b, const double *c, co
   return PMPIC_OK;
 }
 
-/* diffz, fft_ops.F90:419-483 (literals 1./6., 2./3., 1./3. are default-real FP32) */
+/* diffz, fft_ops.F90:419-483 (literals 1./6., 2./3., 1./3. are default-real FP32).  Columns
+ * are independent: threads take x-pair columns; a failing solve reports the first failing
+ * column in the sequential (i, j) order. */
 int orc_diffz(orc_t *o, const double *f, double *dfdz, const double *bot, const double *top) {
   const int nz = o->nzn;
   const double a0 = 0., b0 = 1., c0 = 0.;
@@ -518,68 +669,95 @@ int orc_diffz(orc_t *o, const double *f, double *dfdz, const double *bot, const 
   const double an = 0., bn = 1., cn = 0.;
   const double al = (double)(1.f / 3.f), cl = (double)(1.f / 3.f);
   const double dz = o->dz;
-  double *a = dalloc(nz), *b = dalloc(nz), *c = dalloc(nz), *d = dalloc(nz), *col = dalloc(nz);
   int bt = (bot != NULL && top != NULL);
-  a[0] = a0; b[0] = b0; c[0] = c0;
-  for (int k = 2; k <= nz - 1; k++) { a[k - 1] = a1; b[k - 1] = b1; c[k - 1] = c1; }
-  a[nz - 1] = an; b[nz - 1] = bn; c[nz - 1] = cn;
-  if (bt) { c[0] = cl; a[nz - 1] = al; }
-  int st = PMPIC_OK;
-  for (int i = 1; i <= o->sx && st == PMPIC_OK; i++)
-    for (int j = 1; j <= o->sy; j++) {
-      if (bt) {
-        d[0] = f[SI(o, 2, j, i)] / dz - a1 * bot[CI(o, j, i)] * dz;
-        d[nz - 1] = -f[SI(o, nz - 1, j, i)] / dz + a1 * top[CI(o, j, i)] * dz;
-      } else {
-        d[0] = 0.0; d[nz - 1] = 0.0;
+  int64_t bad = INT64_MAX;
+  int badst = PMPIC_OK;
+#pragma omp parallel
+  {
+    double *a = dalloc(nz), *b = dalloc(nz), *c = dalloc(nz), *d = dalloc(nz), *col = dalloc(nz);
+    a[0] = a0; b[0] = b0; c[0] = c0;
+    for (int k = 2; k <= nz - 1; k++) { a[k - 1] = a1; b[k - 1] = b1; c[k - 1] = c1; }
+    a[nz - 1] = an; b[nz - 1] = bn; c[nz - 1] = cn;
+    if (bt) { c[0] = cl; a[nz - 1] = al; }
+    int64_t mybad = INT64_MAX;
+    int myst = PMPIC_OK;
+#pragma omp for schedule(static)
+    for (int i = 1; i <= o->sx; i++)
+      for (int j = 1; j <= o->sy; j++) {
+        if (bt) {
+          d[0] = f[SI(o, 2, j, i)] / dz - a1 * bot[CI(o, j, i)] * dz;
+          d[nz - 1] = -f[SI(o, nz - 1, j, i)] / dz + a1 * top[CI(o, j, i)] * dz;
+        } else {
+          d[0] = 0.0; d[nz - 1] = 0.0;
+        }
+        for (int k = 2; k <= nz - 1; k++) d[k - 1] = (f[SI(o, k + 1, j, i)] - f[SI(o, k - 1, j, i)]) / 2. / dz;
+        int st = orc_tridiagonal(nz, a, b, c, d, col);
+        if (st) {
+          int64_t key = (int64_t)i * o->sy + j;
+          if (key < mybad) { mybad = key; myst = st; }
+          continue;
+        }
+        for (int k = 1; k <= nz; k++) dfdz[SI(o, k, j, i)] = col[k - 1];
       }
-      for (int k = 2; k <= nz - 1; k++) d[k - 1] = (f[SI(o, k + 1, j, i)] - f[SI(o, k - 1, j, i)]) / 2. / dz;
-      st = orc_tridiagonal(nz, a, b, c, d, col);
-      if (st) break;
-      for (int k = 1; k <= nz; k++) dfdz[SI(o, k, j, i)] = col[k - 1];
-    }
-  free(a); free(b); free(c); free(d); free(col);
-  return st;
+#pragma omp critical(orc_diffz)
+    if (mybad < bad) { bad = mybad; badst = myst; }
+    free(a); free(b); free(c); free(d); free(col);
+  }
+  return badst;
 }
 
-/* laplinv, fft_ops.F90:505-601; columns with k2 == 0 are left unchanged (:567) */
+/* laplinv, fft_ops.F90:505-601; columns with k2 == 0 are left unchanged (:567).  Threads take
+ * x-pair columns, as in orc_diffz. */
 int orc_laplinv(orc_t *o, double *f, int df0) {
   const int nz = o->nzn;
   const double f16 = (double)(1.f / 6.f), f13 = (double)(1.f / 3.f);
   const double f112 = (double)(1.f / 12.f), f56 = (double)(5.f / 6.f);
   const double dz = o->dz;
   const double i2dz2 = 2. / dz / dz, idz2 = 1. / dz / dz;
-  double *a = dalloc(nz), *b = dalloc(nz), *c = dalloc(nz), *d = dalloc(nz), *col = dalloc(nz);
-  int st = PMPIC_OK;
-  for (int i = 1; i <= o->sx && st == PMPIC_OK; i++)
-    for (int j = 1; j <= o->sy; j++) {
-      double k2 = o->k2[CI(o, j, i)];
-      if (k2 == 0) continue;
-      if (!df0) {
-        a[0] = 0.; b[0] = 1.; c[0] = 0.; d[0] = 0.;
-        a[nz - 1] = 0.; b[nz - 1] = 1.; c[nz - 1] = 0.; d[nz - 1] = 0.;
-      } else {
-        a[0] = 0.;
-        b[0] = -idz2 - f13 * k2;
-        c[0] = idz2 - f16 * k2;
-        d[0] = f13 * f[SI(o, 1, j, i)] + f16 * f[SI(o, 2, j, i)];
-        a[nz - 1] = idz2 - f16 * k2;
-        b[nz - 1] = -idz2 - f13 * k2;
-        c[nz - 1] = 0.;
-        d[nz - 1] = f13 * f[SI(o, nz, j, i)] + f16 * f[SI(o, nz - 1, j, i)];
+  int64_t bad = INT64_MAX;
+  int badst = PMPIC_OK;
+#pragma omp parallel
+  {
+    double *a = dalloc(nz), *b = dalloc(nz), *c = dalloc(nz), *d = dalloc(nz), *col = dalloc(nz);
+    int64_t mybad = INT64_MAX;
+    int myst = PMPIC_OK;
+#pragma omp for schedule(static)
+    for (int i = 1; i <= o->sx; i++)
+      for (int j = 1; j <= o->sy; j++) {
+        double k2 = o->k2[CI(o, j, i)];
+        if (k2 == 0) continue;
+        if (!df0) {
+          a[0] = 0.; b[0] = 1.; c[0] = 0.; d[0] = 0.;
+          a[nz - 1] = 0.; b[nz - 1] = 1.; c[nz - 1] = 0.; d[nz - 1] = 0.;
+        } else {
+          a[0] = 0.;
+          b[0] = -idz2 - f13 * k2;
+          c[0] = idz2 - f16 * k2;
+          d[0] = f13 * f[SI(o, 1, j, i)] + f16 * f[SI(o, 2, j, i)];
+          a[nz - 1] = idz2 - f16 * k2;
+          b[nz - 1] = -idz2 - f13 * k2;
+          c[nz - 1] = 0.;
+          d[nz - 1] = f13 * f[SI(o, nz, j, i)] + f16 * f[SI(o, nz - 1, j, i)];
+        }
+        for (int k = 2; k <= nz - 1; k++) {
+          a[k - 1] = idz2 - f112 * k2;
+          b[k - 1] = -i2dz2 - f56 * k2;
+          c[k - 1] = idz2 - f112 * k2;
+          d[k - 1] = f112 * f[SI(o, k - 1, j, i)] + f56 * f[SI(o, k, j, i)] + f112 * f[SI(o, k + 1, j, i)];
+        }
+        int st = orc_tridiagonal(nz, a, b, c, d, col);
+        if (st) {
+          int64_t key = (int64_t)i * o->sy + j;
+          if (key < mybad) { mybad = key; myst = st; }
+          continue;
+        }
+        for (int k = 1; k <= nz; k++) f[SI(o, k, j, i)] = col[k - 1];
       }
-      for (int k = 2; k <= nz - 1; k++) {
-        a[k - 1] = idz2 - f112 * k2;
-        b[k - 1] = -i2dz2 - f56 * k2;
-        c[k - 1] = idz2 - f112 * k2;
-        d[k - 1] = f112 * f[SI(o, k - 1, j, i)] + f56 * f[SI(o, k, j, i)] + f112 * f[SI(o, k + 1, j, i)];
-      }
-      st = orc_tridiagonal(nz, a, b, c, d, col);
-      if (st) break;
-      for (int k = 1; k <= nz; k++) f[SI(o, k, j, i)] = col[k - 1];
-    }
-  free(a); free(b); free(c); free(d); free(col);
-  return st;
+#pragma omp critical(orc_laplinv)
+    if (mybad < bad) { bad = mybad; badst = myst; }
+    free(a); free(b); free(c); free(d); free(col);
+  }
+  return badst;
 }
 
 /* ---------------------------------------------------------------------------------------- */
@@ -611,12 +789,14 @@ int orc_vort2vel(orc_t *o) {
 
   orc_diffx(o, a, d);
   orc_diffy(o, b, e);
+#pragma omp parallel for schedule(static)
   for (int i = 1; i <= sx; i++)
     for (int j = 1; j <= sy; j++) {
       f[SI(o, 1, j, i)] = hdzi * (4. * c[SI(o, 2, j, i)] - c[SI(o, 3, j, i)] - 3. * c[SI(o, 1, j, i)]);
       for (int k = 2; k <= nz - 1; k++) f[SI(o, k, j, i)] = (c[SI(o, k + 1, j, i)] - c[SI(o, k - 1, j, i)]) * hdzi;
       f[SI(o, nz, j, i)] = hdzi * (c[SI(o, nz - 2, j, i)] + 3. * c[SI(o, nz, j, i)] - 4. * c[SI(o, nz - 1, j, i)]);
     }
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < o->ssz; q++) f[q] = d[q] + e[q] + f[q];
   for (int k = 1; k <= nz; k++) f[SI(o, k, 1, 1)] = 0.0;
 
@@ -624,12 +804,16 @@ int orc_vort2vel(orc_t *o) {
   orc_filter(o, f, f);
 
   orc_diffx(o, f, d);
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < o->ssz; q++) a[q] = a[q] - d[q];
   orc_diffy(o, f, d);
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < o->ssz; q++) b[q] = b[q] - d[q];
   if ((st = orc_diffz(o, f, d, NULL, NULL))) goto done;
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < o->ssz; q++) c[q] = c[q] - d[q];
 
+#pragma omp parallel for schedule(static)
   for (int i = 1; i <= sx; i++)
     for (int j = 1; j <= sy; j++) c[SI(o, 1, j, i)] = 0.;
 
@@ -637,6 +821,7 @@ int orc_vort2vel(orc_t *o) {
   orc_filter(o, b, e);
   orc_filter(o, c, f);
 
+#pragma omp parallel for schedule(static)
   for (int i = 1; i <= sx; i++)
     for (int j = 1; j <= sy; j++) {
       atop[CI(o, j, i)] = a[SI(o, nz, j, i)];
@@ -673,6 +858,7 @@ int orc_vort2vel(orc_t *o) {
   /* u = db/dz - dc/dy */
   if ((st = orc_diffz(o, b, e, bbot, btop))) goto done;
   orc_diffy(o, c, d);
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < o->ssz; q++) f[q] = e[q] - d[q];
   for (int k = 1; k <= nz; k++) f[SI(o, k, 1, 1)] = ubar[k - 1];
   orc_filter(o, f, o->s[ORC_SU]);
@@ -681,6 +867,7 @@ int orc_vort2vel(orc_t *o) {
   /* v = dc/dx - da/dz */
   if ((st = orc_diffz(o, a, e, abot, atop))) goto done;
   orc_diffx(o, c, d);
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < o->ssz; q++) f[q] = d[q] - e[q];
   for (int k = 1; k <= nz; k++) f[SI(o, k, 1, 1)] = vbar[k - 1];
   orc_filter(o, f, o->s[ORC_SV]);
@@ -689,6 +876,7 @@ int orc_vort2vel(orc_t *o) {
   /* w = da/dy - db/dx */
   orc_diffx(o, b, d);
   orc_diffy(o, a, e);
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < o->ssz; q++) f[q] = e[q] - d[q];
   orc_filter(o, f, o->s[ORC_SW]);
   orc_fft_backward(o, f, o->g[ORC_GW]);
@@ -698,15 +886,17 @@ int orc_vort2vel(orc_t *o) {
   orc_grid2par(o, o->g[ORC_GW], o->f[ORC_DZDT]);
 
   if (o->it_v2v % o->rksteps == 0) {
+    /* maxval of the squares: the maximum does not depend on the order it is taken in */
     double umax = 0, vmax = 0, wmax = 0;
-    int first = 1;
+#pragma omp parallel for schedule(static) reduction(max : umax, vmax, wmax)
     for (int i = 3; i <= o->ncx + 2; i++)
       for (int j = 3; j <= o->ncy + 2; j++)
         for (int k = 1; k <= nz; k++) {
           double u = o->g[ORC_GU][GI(o, k, j, i)], v = o->g[ORC_GV][GI(o, k, j, i)], w = o->g[ORC_GW][GI(o, k, j, i)];
           double u2 = u * u, v2 = v * v, w2 = w * w;
-          if (first) { umax = u2; vmax = v2; wmax = w2; first = 0; }
-          else { if (u2 > umax) umax = u2; if (v2 > vmax) vmax = v2; if (w2 > wmax) wmax = w2; }
+          if (u2 > umax) umax = u2;
+          if (v2 > vmax) vmax = v2;
+          if (w2 > wmax) wmax = w2;
         }
     umax = sqrt(umax); vmax = sqrt(vmax); wmax = sqrt(wmax);
     const double tiny = (double)1.E-15f;
@@ -733,6 +923,7 @@ int orc_vorticity_tendency(orc_t *o) {
   double *DP = o->g[ORC_GDP], *DQ = o->g[ORC_GDQ], *DR = o->g[ORC_GDR];
   double *bt = o->f[ORC_BTOT];
   const double *B = o->f[ORC_B], *H = o->f[ORC_H], *Z = o->f[ORC_Z];
+#pragma omp parallel for schedule(static)
   for (int64_t n = 0; n < o->np; n++) {
     double hl = H[n] - exp(-Z[n]);
     bt[n] = B[n] + 12.5 * (hl > 0.0 ? hl : 0.0);
@@ -744,16 +935,19 @@ int orc_vorticity_tendency(orc_t *o) {
   orc_fft_backward(o, o->as, DP);
   orc_diffx(o, o->s[ORC_SU], o->as);
   orc_fft_backward(o, o->as, o->dudx);
+#pragma omp parallel for schedule(static)
   for (int i = 3; i <= o->ncx + 2; i++)
     for (int j = 3; j <= o->ncy + 2; j++)
       for (int k = 1; k <= o->nzn; k++) { size_t q = GI(o, k, j, i); DP[q] = DP[q] + P[q] * o->dudx[q]; }
   orc_diffy(o, o->s[ORC_SU], o->as);
   orc_fft_backward(o, o->as, o->ap);
+#pragma omp parallel for schedule(static)
   for (int i = 3; i <= o->ncx + 2; i++)
     for (int j = 3; j <= o->ncy + 2; j++)
       for (int k = 1; k <= o->nzn; k++) { size_t q = GI(o, k, j, i); DP[q] = DP[q] + Q[q] * o->ap[q]; }
   orc_diffx(o, o->s[ORC_SW], o->as);
   orc_fft_backward(o, o->as, o->dwdx);
+#pragma omp parallel for schedule(static)
   for (int i = 3; i <= o->ncx + 2; i++)
     for (int j = 3; j <= o->ncy + 2; j++)
       for (int k = 1; k <= o->nzn; k++) { size_t q = GI(o, k, j, i); DP[q] = DP[q] + R[q] * (Q[q] + o->dwdx[q]); }
@@ -762,20 +956,24 @@ int orc_vorticity_tendency(orc_t *o) {
   orc_fft_backward(o, o->as, DQ);
   orc_diffx(o, o->s[ORC_SV], o->as);
   orc_fft_backward(o, o->as, o->ap);
+#pragma omp parallel for schedule(static)
   for (int i = 3; i <= o->ncx + 2; i++)
     for (int j = 3; j <= o->ncy + 2; j++)
       for (int k = 1; k <= o->nzn; k++) { size_t q = GI(o, k, j, i); DQ[q] = -DQ[q] + P[q] * o->ap[q]; }
   orc_diffy(o, o->s[ORC_SV], o->as);
   orc_fft_backward(o, o->as, o->dvdy);
+#pragma omp parallel for schedule(static)
   for (int i = 3; i <= o->ncx + 2; i++)
     for (int j = 3; j <= o->ncy + 2; j++)
       for (int k = 1; k <= o->nzn; k++) { size_t q = GI(o, k, j, i); DQ[q] = DQ[q] + Q[q] * o->dvdy[q]; }
   orc_diffy(o, o->s[ORC_SW], o->as);
   orc_fft_backward(o, o->as, o->dwdy);
+#pragma omp parallel for schedule(static)
   for (int i = 3; i <= o->ncx + 2; i++)
     for (int j = 3; j <= o->ncy + 2; j++)
       for (int k = 1; k <= o->nzn; k++) { size_t q = GI(o, k, j, i); DQ[q] = DQ[q] + R[q] * (-P[q] + o->dwdy[q]); }
 
+#pragma omp parallel for schedule(static)
   for (int i = 3; i <= o->ncx + 2; i++)
     for (int j = 3; j <= o->ncy + 2; j++)
       for (int k = 1; k <= o->nzn; k++) {
@@ -792,13 +990,13 @@ int orc_vorticity_tendency(orc_t *o) {
 
   if (o->it_vt % o->rksteps == 0) {
     double omax = 0;
-    int first = 1;
+#pragma omp parallel for schedule(static) reduction(max : omax)
     for (int i = 3; i <= o->ncx + 2; i++)
       for (int j = 3; j <= o->ncy + 2; j++)
         for (int k = 1; k <= o->nzn; k++) {
           size_t q = GI(o, k, j, i);
           double w2 = P[q] * P[q] + Q[q] * Q[q] + R[q] * R[q];
-          if (first) { omax = w2; first = 0; } else if (w2 > omax) omax = w2;
+          if (w2 > omax) omax = w2;
         }
     omax = sqrt(omax);
     double dtmax;
@@ -816,54 +1014,56 @@ int orc_vorticity_tendency(orc_t *o) {
 int orc_parcel_haloswap(orc_t *o) {
   const int64_t n0 = o->np;
   double *X = o->f[ORC_X], *Y = o->f[ORC_Y], *Z = o->f[ORC_Z];
+  const double xmin = o->hs_xmin, xmax = o->hs_xmax, ymin = o->hs_ymin, ymax = o->hs_ymax;
+  int64_t total = 0;
+  /* reflection, then each parcel's direction (the eight tests are mutually exclusive, so
+   * testing them per parcel marks what the direction-by-direction sweeps mark) */
+#pragma omp parallel for schedule(static) reduction(+ : total)
   for (int64_t i = 0; i < n0; i++) {
     double z = Z[i];
     if (z < o->zmin_h) Z[i] = o->zmin_h + (o->zmin_h - z);
     else if (z > o->zmax_h) Z[i] = o->zmax_h - (z - o->zmax_h);
+    double x = X[i], y = Y[i];
+    int dir = 0;
+    if ((y >= ymax) && (x >= xmin && x < xmax)) dir = 1;          /* N */
+    else if (x >= xmax && y >= ymax) dir = 2;                     /* NE */
+    else if ((x >= xmax) && (y >= ymin && y < ymax)) dir = 3;     /* E */
+    else if (x >= xmax && y < ymin) dir = 4;                      /* SE */
+    else if ((y < ymin) && (x >= xmin && x < xmax)) dir = 5;      /* S */
+    else if (x < xmin && y < ymin) dir = 6;                       /* SW */
+    else if ((x < xmin) && (y >= ymin && y < ymax)) dir = 7;      /* W */
+    else if (x < xmin && y >= ymax) dir = 8;                      /* NW */
+    o->index[i] = dir;
+    total += (dir != 0);
   }
-  const double xmin = o->hs_xmin, xmax = o->hs_xmax, ymin = o->hs_ymin, ymax = o->hs_ymax;
-  int64_t nsend[9] = {0};
-  for (int64_t i = 0; i < n0; i++) o->index[i] = 0;
-  for (int dir = 1; dir <= 8; dir++)
-    for (int64_t i = 0; i < n0; i++) {
-      double x = X[i], y = Y[i];
-      int hit = 0;
-      switch (dir) {
-        case 1: hit = (y >= ymax) && (x >= xmin && x < xmax); break;       /* N */
-        case 2: hit = (x >= xmax && y >= ymax); break;                      /* NE */
-        case 3: hit = (x >= xmax) && (y >= ymin && y < ymax); break;        /* E */
-        case 4: hit = (x >= xmax && y < ymin); break;                       /* SE */
-        case 5: hit = (y < ymin) && (x >= xmin && x < xmax); break;         /* S */
-        case 6: hit = (x < xmin && y < ymin); break;                        /* SW */
-        case 7: hit = (x < xmin) && (y >= ymin && y < ymax); break;         /* W */
-        case 8: hit = (x < xmin && y >= ymax); break;                       /* NW */
-      }
-      if (hit) { nsend[dir]++; o->index[i] = dir; }
-    }
-  int64_t total = 0;
-  for (int d = 1; d <= 8; d++) total += nsend[d];
   if (total == 0) return PMPIC_OK;
-  /* pack: per direction, ascending slot order, with the periodic shift */
+  /* the crossing parcels in (direction, ascending slot) order */
+  int64_t nsend[9] = {0};
+  for (int64_t i = 0; i < n0; i++) nsend[o->index[i]]++;
+  int64_t off[9];
+  off[1] = 0;
+  for (int d = 1; d < 8; d++) off[d + 1] = off[d] + nsend[d];
+  int64_t *src = (int64_t *)malloc(sizeof(int64_t) * total);
+  for (int64_t i = 0; i < n0; i++)
+    if (o->index[i] != 0) src[off[o->index[i]]++] = i;
+  /* pack, with the periodic shift */
   const int NF = ORC_BTOT; /* 29 fields travel: 17 properties + 12 RK arrays */
   double *buf = dalloc((size_t)total * NF);
-  int64_t *src = (int64_t *)malloc(sizeof(int64_t) * total);
-  int64_t t = 0;
-  for (int d = 1; d <= 8; d++) {
+#pragma omp parallel for schedule(static)
+  for (int64_t t = 0; t < total; t++) {
+    const int64_t i = src[t];
+    const int d = o->index[i];
     double xs = 0, ys = 0;
     if (d == 3 || d == 2 || d == 4) xs = o->xshift_e;
     else if (d == 7 || d == 8 || d == 6) xs = o->xshift_w;
     if (d == 1 || d == 2 || d == 8) ys = o->yshift_n;
     else if (d == 5 || d == 6 || d == 4) ys = o->yshift_s;
-    for (int64_t i = 0; i < n0; i++)
-      if (o->index[i] == d) {
-        for (int fld = 0; fld < NF; fld++) {
-          double v = o->f[fld][i];
-          if (fld == ORC_X || fld == ORC_XO || fld == ORC_XF) v = v + xs;
-          if (fld == ORC_Y || fld == ORC_YO || fld == ORC_YF) v = v + ys;
-          buf[(size_t)t * NF + fld] = v;
-        }
-        src[t++] = i;
-      }
+    for (int fld = 0; fld < NF; fld++) {
+      double v = o->f[fld][i];
+      if (fld == ORC_X || fld == ORC_XO || fld == ORC_XF) v = v + xs;
+      if (fld == ORC_Y || fld == ORC_YO || fld == ORC_YF) v = v + ys;
+      buf[(size_t)t * NF + fld] = v;
+    }
   }
   /* unpack (haloswap.F90:699-716): free slot = beyond n_initial or index != 0 */
   int64_t istart = 0;
@@ -886,9 +1086,11 @@ int orc_rk4_integrator(orc_t *o) {
   double *Yo[6] = {o->f[ORC_XO], o->f[ORC_YO], o->f[ORC_ZO], o->f[ORC_PO], o->f[ORC_QO], o->f[ORC_RO]};
   double *Yf[6] = {o->f[ORC_XF], o->f[ORC_YF], o->f[ORC_ZF], o->f[ORC_PF], o->f[ORC_QF], o->f[ORC_RF]};
   double *K[6] = {o->f[ORC_DXDT], o->f[ORC_DYDT], o->f[ORC_DZDT], o->f[ORC_DPDT], o->f[ORC_DQDT], o->f[ORC_DRDT]};
-  for (int v = 0; v < 6; v++)
-    for (int64_t i = 0; i < n; i++) {
-      switch (o->rkstep) {
+  const int stage = o->rkstep;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; i++)
+    for (int v = 0; v < 6; v++) {
+      switch (stage) {
         case 1:
           Yo[v][i] = Yv[v][i];
           Yf[v][i] = Yo[v][i] + dt6 * K[v][i];
@@ -926,11 +1128,18 @@ int orc_parcel_splitting(orc_t *o) {
     double *P = o->f[ORC_P], *Q = o->f[ORC_Q], *R = o->f[ORC_R];
     double *S = o->f[ORC_STRETCH], *V = o->f[ORC_VOL];
     o->last_split = 0;
+    /* the stretch increments touch only parcel i, so they are taken for every parcel first;
+     * the splits then run in ascending parcel order, appending children as the loop does */
+    int *hit = o->keep;
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < norig; i++) {
       double ds = P[i] * o->f[ORC_DPDT][i] + Q[i] * o->f[ORC_DQDT][i] + R[i] * o->f[ORC_DRDT][i];
       ds = pow(fabs(ds), third);
       S[i] = S[i] + ds * dt;
-      if (S[i] > stretchmax) {
+      hit[i] = S[i] > stretchmax;
+    }
+    for (int64_t i = 0; i < norig; i++) {
+      if (hit[i]) {
         double r = V[i] / 4. / pi;
         r = pow(r, third);
         double absw = P[i] * P[i] + Q[i] * Q[i] + R[i] * R[i];
@@ -987,30 +1196,51 @@ int orc_parcel_mixing(orc_t *o) {
     for (int g = 0; g < 6; g++) { memset(Gs[g], 0, o->gsz * sizeof(double)); memset(Rs[g], 0, o->gsz * sizeof(double)); }
     int64_t nremove = 0;
     const double *PB = o->f[ORC_B], *PH = o->f[ORC_H], *PP = o->f[ORC_P], *PQ = o->f[ORC_Q], *PR = o->f[ORC_R];
+#pragma omp parallel for schedule(static) reduction(+ : nremove)
     for (int64_t n = 0; n < norig; n++) {
-      int i = o->is[n], j = o->js[n], k = o->ks[n];
-      double delx = o->delxs[n], dely = o->delys[n], delz = o->delzs[n];
-      double v = o->f[ORC_VOL][n];
-      double w[8];
-      w[0] = (1 - delz) * (1 - dely) * (1 - delx) * v;
-      w[1] = (1 - delz) * (1 - dely) * (delx)*v;
-      w[2] = (1 - delz) * (dely) * (1 - delx) * v;
-      w[3] = (1 - delz) * (dely) * (delx)*v;
-      w[4] = (delz) * (1 - dely) * (1 - delx) * v;
-      w[5] = (delz) * (1 - dely) * (delx)*v;
-      w[6] = (delz) * (dely) * (1 - delx) * v;
-      w[7] = (delz) * (dely) * (delx)*v;
-      size_t q[8] = {GI(o, k, j, i), GI(o, k, j, i + 1), GI(o, k, j + 1, i), GI(o, k, j + 1, i + 1),
-                     GI(o, k + 1, j, i), GI(o, k + 1, j, i + 1), GI(o, k + 1, j + 1, i), GI(o, k + 1, j + 1, i + 1)};
-      double **T;
-      if (o->f[ORC_VOL][n] < vmin) { keep[n] = 0; nremove++; T = Rs; }
-      else { keep[n] = 1; o->npercell[NPC(o, k, j, i)] += 1; T = Gs; }
-      for (int c = 0; c < 8; c++) T[0][q[c]] = T[0][q[c]] + w[c];
-      for (int c = 0; c < 8; c++) T[1][q[c]] = T[1][q[c]] + w[c] * PB[n];
-      for (int c = 0; c < 8; c++) T[2][q[c]] = T[2][q[c]] + w[c] * PH[n];
-      for (int c = 0; c < 8; c++) T[3][q[c]] = T[3][q[c]] + w[c] * PP[n];
-      for (int c = 0; c < 8; c++) T[4][q[c]] = T[4][q[c]] + w[c] * PQ[n];
-      for (int c = 0; c < 8; c++) T[5][q[c]] = T[5][q[c]] + w[c] * PR[n];
+      keep[n] = (o->f[ORC_VOL][n] < vmin) ? 0 : 1;
+      nremove += (keep[n] == 0);
+    }
+    /* the scatter, by node-column slab as in orc_par2grid: each node gets its terms in
+     * ascending parcel order; a slab counts the kept parcels of the cells in its own columns */
+#pragma omp parallel num_threads(o->nslab)
+    {
+      for (int t = omp_get_thread_num(); t < o->nslab; t += omp_get_num_threads()) {
+        const int lo = o->slab_lo[t], hi = o->slab_lo[t + 1];
+        const int64_t *list = o->slab_list[t];
+        for (int64_t mm = 0; mm < o->slab_n[t]; mm++) {
+          const int64_t n = list[mm];
+          if (n >= norig) continue;
+          int i = o->is[n], j = o->js[n], k = o->ks[n];
+          double delx = o->delxs[n], dely = o->delys[n], delz = o->delzs[n];
+          double v = o->f[ORC_VOL][n];
+          double w[8];
+          w[0] = (1 - delz) * (1 - dely) * (1 - delx) * v;
+          w[1] = (1 - delz) * (1 - dely) * (delx)*v;
+          w[2] = (1 - delz) * (dely) * (1 - delx) * v;
+          w[3] = (1 - delz) * (dely) * (delx)*v;
+          w[4] = (delz) * (1 - dely) * (1 - delx) * v;
+          w[5] = (delz) * (1 - dely) * (delx)*v;
+          w[6] = (delz) * (dely) * (1 - delx) * v;
+          w[7] = (delz) * (dely) * (delx)*v;
+          size_t q[8] = {GI(o, k, j, i), GI(o, k, j, i + 1), GI(o, k, j + 1, i), GI(o, k, j + 1, i + 1),
+                         GI(o, k + 1, j, i), GI(o, k + 1, j, i + 1), GI(o, k + 1, j + 1, i), GI(o, k + 1, j + 1, i + 1)};
+          /* corners 0, 2, 4, 6 are in node column i, the others in i + 1 */
+          const int own[2] = {i >= lo && i < hi, i + 1 >= lo && i + 1 < hi};
+          double **T;
+          if (keep[n] == 0) T = Rs;
+          else {
+            if (own[0]) o->npercell[NPC(o, k, j, i)] += 1;
+            T = Gs;
+          }
+          const double val[5] = {PB[n], PH[n], PP[n], PQ[n], PR[n]};
+          for (int c = 0; c < 8; c++)
+            if (own[c & 1]) T[0][q[c]] = T[0][q[c]] + w[c];
+          for (int g = 1; g < 6; g++)
+            for (int c = 0; c < 8; c++)
+              if (own[c & 1]) T[g][q[c]] = T[g][q[c]] + w[c] * val[g - 1];
+        }
+      }
     }
     for (int g = 0; g < 6; g++) { orc_fold(o, Gs[g]); orc_fill(o, Gs[g]); }
     for (int g = 0; g < 6; g++)
@@ -1072,6 +1302,7 @@ int orc_parcel_mixing(orc_t *o) {
         }
     double *PV = o->f[ORC_VOL];
     double *PBw = o->f[ORC_B], *PHw = o->f[ORC_H], *PPw = o->f[ORC_P], *PQw = o->f[ORC_Q], *PRw = o->f[ORC_R];
+#pragma omp parallel for schedule(static)
     for (int64_t m = 0; m < norig; m++) {
       if (keep[m] != 1) continue;
       int i = o->is[m], j = o->js[m], k = o->ks[m];

@@ -43,6 +43,10 @@ orc_t *orc_create(int nx, int ny, int nz_cells, double xmin, double xmax, double
                   double ymax, double zmin, double zmax, int64_t maxparcels, double dtm,
                   double dtmax, double timestep_prefactor);
 void orc_destroy(orc_t *o);
+/* OpenMP threads for later calls (0 or less: unchanged).  Results do not depend on the count:
+ * every threaded loop either touches disjoint data or adds into each grid node in the
+ * sequential parcel order. */
+void orc_set_threads(int nthreads);
 
 double *orc_field(orc_t *o, int id);     /* capacity-sized parcel array */
 double *orc_grid(orc_t *o, int id);      /* halo'd grid */

@@ -15,6 +15,8 @@ sign, index or coefficient, a transposed layout) fails it:
 * haloswap against an independent model of the one-rank send-to-self unpack;
 * splitting against its closed form; mixing against volume-integral conservation.
 """
+import os
+
 import numpy as np
 import pytest
 import scipy.linalg
@@ -533,6 +535,35 @@ def test_step_is_deterministic(oracle_mod):
         res.append(o.get_parcels())
     for k in res[0]:
         np.testing.assert_array_equal(res[0][k], res[1][k])
+
+
+def test_threads_change_no_result(oracle_mod):
+    """The threaded oracle on 1, 2 and 5 threads: identical parcels, grids and counts over three
+    steps with splits, removals, additions and haloswap crossings (each grid node receives its
+    terms in ascending parcel order whatever the thread count)."""
+    nx, ny, nz = 20, 15, 8
+    ic = thermal_case(nx, ny, nz, 3, seed=5, vort=3.0, shrink=0.03, thin=0.05, near_split=0.03)
+    res = []
+    try:
+        for thr in (1, 2, 5):
+            oracle_mod.set_threads(thr)
+            o = make(oracle_mod, nx, ny, nz, cap=3 * len(ic["x"]))
+            o.set_parcels(ic)
+            counts = []
+            for _ in range(3):
+                o.step()
+                counts.append(o.last_counts())
+            res.append((o.get_parcels(), {g: o.grid(g).copy() for g in oracle_mod.GRIDS},
+                        counts, o.dtm))
+    finally:
+        oracle_mod.set_threads(os.cpu_count() or 1)
+    assert all(c > 0 for c in res[0][2][0]), res[0][2]  # splits, removals and additions ran
+    for p, g, counts, dtm in res[1:]:
+        assert counts == res[0][2] and dtm == res[0][3]
+        for k in p:
+            np.testing.assert_array_equal(p[k], res[0][0][k], err_msg=k)
+        for k in g:
+            np.testing.assert_array_equal(g[k], res[0][1][k], err_msg=k)
 
 
 @pytest.mark.parametrize("vort,dtmax", [(4.0, 10.0), (0.2, 10.0), (4.0, 0.01)])
