@@ -1,18 +1,21 @@
 #!/usr/bin/env python3
 """Throughput of the PMPIC per-timestep parcel-in-cell path on one MI355X.
 
-Workload: the moist thermal on a 128x128x128 grid (2pi x 2pi x pi box, periodic x/y, rigid z)
-with 27 parcels per cell on a jittered 3x3x3 lattice (56,623,104 parcels), RK4 with dt = 0.01,
-splitting and mixing on.  A step is one full timestep (the integrator group run for the four RK
-stages: par2grid, vort2vel, vorticity_tendency, grid2par + RK update, haloswap, and at stage 4
-splitting and mixing) with the parcels resident in HBM.
+Workload (the precision form's configuration, the one the metric is quoted on): the moist
+thermal on a 256x256x128 grid (2pi x 2pi x pi box, periodic x/y, rigid z) with 8 parcels per
+cell on a jittered 2x2x2 lattice (67,108,864 parcels), RK4 with dt = 0.01, splitting and mixing
+on.  A step is one full timestep (the integrator group run for the four RK stages: par2grid,
+vort2vel, vorticity_tendency, grid2par + RK update, haloswap, and at stage 4 splitting and
+mixing) with the parcels resident in HBM.
 
 Both numeric forms run on the same initial condition: the FP32-bulk form (positions as cell
 index + FP32 offsets, FP32 parcel arithmetic; `value`) and the plain FP64 form.  The velocity
-error of the FP32-bulk form against the FP64 form is measured after --err-steps steps on the
---err-n grid.  Prints one JSON line: parcel-steps/s of each form, the roofline of each form's
-dominant kernel and of the other parcel<->grid kernel (HIP events on the library's stream around
-every launch in the timed region) and a CPU baseline (the C restatement of the same path, single-threaded, on a bounded sample).
+error of the FP32-bulk form against the FP64 form is measured after --err-steps steps of both
+from that same state.  Prints one JSON line: parcel-steps/s of each form, the roofline of each
+form's dominant kernel and of the other parcel<->grid kernel (HIP events taken by every launch
+in the timed region), the HBM ceilings measured in the same run (device copy, STREAM triad) and
+a CPU baseline: the threaded C FP64 restatement of the same path on all host cores for one step
+of the same workload.
 """
 import argparse
 import json
@@ -47,16 +50,17 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=128, help="cells per horizontal direction")
+    ap.add_argument("--n", type=int, default=256, help="cells per horizontal direction")
     ap.add_argument("--nz", type=int, default=128)
-    ap.add_argument("--ppc", type=int, default=3, help="parcels per cell per direction")
-    ap.add_argument("--cpu-steps", type=int, default=30)
+    ap.add_argument("--ppc", type=int, default=2, help="parcels per cell per direction")
+    ap.add_argument("--cpu-steps", type=int, default=1,
+                    help="steps of the workload the CPU baseline times")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--forms", default="fp32,fp64",
                     help="numeric forms to time: fp32 (FP32-bulk), fp64 (plain), or both")
     ap.add_argument("--err-steps", type=int, default=100,
-                    help="steps of the FP32-bulk vs FP64 velocity-error run (0 = skip)")
-    ap.add_argument("--err-n", type=int, default=64, help="grid of the error run (n x n x n)")
+                    help="steps of the FP32-bulk vs FP64 velocity-error run on the workload "
+                         "(0 = skip)")
     ap.add_argument("--ensemble", type=int, default=0, metavar="M",
                     help="instead of the single thermal: M FP32-bulk thermals on grids cycling "
                          "32^3, 48^3, 64^3, 96^3 (8 parcels/cell, radius U(0.5,1), amplitude "
@@ -114,37 +118,46 @@ def copy_bandwidth(mib=1024, reps=5):
             "what": f"hipMemcpy device-to-device of {mib} MiB, best of {reps}"}
 
 
-def cpu_baseline(steps):
-    """The oracle (C FP64 restatement, one thread) on the 32^3, 27 parcels/cell thermal."""
+def cpu_baseline(ic, n, nz, steps):
+    """The oracle (C FP64 restatement of the same path, OpenMP over every host core, results
+    independent of the thread count) timed for `steps` whole steps of the bench's own workload
+    from its initial condition."""
     from oracle import oracle
-    from pmpic_amd.config import fortran_real
-    from pmpic_amd.initial import thermal
     oracle.build()
-    lx, lz = fortran_real("6.28318531"), fortran_real("3.14159265")
-    ic = thermal(32, 32, 32, lx, lx, lz, 3, seed=0x504D504943000000)
-    o = oracle.Oracle(32, 32, 32, 0.0, lx, 0.0, lx, 0.0, lz, 2 * len(ic["x"]), 0.01, 0.01, 0.5)
-    o.set_parcels(ic)
+    lx, lz = ic["_lx"], ic["_lz"]
+    npar = len(ic["x"])
+    o = oracle.Oracle(n, n, nz, 0.0, lx, 0.0, lx, 0.0, lz, npar + npar // 4 + 1024, 0.01, 0.01,
+                      0.5)
+    o.set_parcels({k: v for k, v in ic.items() if not k.startswith("_")})
+    threads = oracle.max_threads()
     total = 0
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for s in range(steps):
         o.step()
         total += o.n
+        print(f"bench: cpu baseline step {s + 1} done", file=sys.stderr, flush=True)
     dt = time.perf_counter() - t0
-    return {"value": total / dt, "unit": "parcel-steps/s", "cores": 1, "kind": "port",
-            "sample": f"C FP64 restatement of the same path, 32x32x32 cells, 27 parcels/cell "
-                      f"({len(ic['x'])} parcels), {steps} steps, single thread, {dt:.1f} s"}
+    del o
+    return {"value": total / dt, "unit": "parcel-steps/s", "cores": threads, "kind": "port",
+            "nproc": os.cpu_count(),
+            "sample": f"C FP64 restatement of the same path (oracle/, OpenMP, {threads} threads), "
+                      f"{steps} step(s) of the bench workload: {n}x{n}x{nz} cells, {npar} "
+                      f"parcels, {dt:.1f} s"}
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_thermal128_27ppc.txt")
+def traffic_file(n, nz, ppcd):
+    return os.path.join(ROOT, "profiles",
+                        f"hbm_traffic_thermal{n}x{n}x{nz}_{ppcd ** 3}ppc.txt")
 
 
 def measured_traffic(form, family, n, nz, ppcd):
-    """HBM bytes per launch of a kernel family from the committed rocprofv3 counter summary
-    (FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel calibration); None when the summary
-    does not cover this workload."""
-    if (n, nz, ppcd) != (128, 128, 3) or not os.path.exists(TRAFFIC_FILE):
+    """HBM bytes per launch of a kernel family from the committed rocprofv3 counter summary of
+    this workload (FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel calibration); None when
+    no summary covers the workload."""
+    path = traffic_file(n, nz, ppcd)
+    if not os.path.exists(path):
         return None
-    for line in open(TRAFFIC_FILE):
+    for line in open(path):
         parts = line.split()
         if len(parts) == 7 and parts[0] == "traffic" and parts[1] == form and parts[2] == family:
             return float(parts[6])
@@ -220,31 +233,28 @@ def time_form(precision, ic, args, n, nz):
     }
 
 
-def velocity_error(args):
-    """Both forms from the same state for args.err_steps steps on an err_n^3 grid; the FP32-bulk
+def velocity_error(args, ic, n, nz):
+    """Both forms from the bench's initial condition for args.err_steps steps; the FP32-bulk
     form's maximum velocity error relative to the FP64 form's largest velocity component."""
     import numpy as np
-    from pmpic_amd.config import fortran_real
-    from pmpic_amd.initial import thermal
     from pmpic_amd.lib import Context
-    n = args.err_n
-    lx, lz = fortran_real("6.28318531"), fortran_real("3.14159265")
-    ic = thermal(n, n, n, lx, lx, lz, args.ppc, seed=0x504D504943000000 + 1)
+    lx, lz = ic["_lx"], ic["_lz"]
     grids = {}
     counts = {}
     for precision in (1, 0):
-        ctx = Context(n, n, n, 0.0, lx, 0.0, lx, 0.0, lz, 2 * len(ic["x"]), dtm=0.01,
+        ctx = Context(n, n, nz, 0.0, lx, 0.0, lx, 0.0, lz, 2 * len(ic["x"]), dtm=0.01,
                       dtmax=0.01, timestep_prefactor=0.5, precision=precision)
         ctx.set_parcels(ic)
-        ctx.step(args.err_steps)
+        for s in range(0, args.err_steps, 25):
+            ctx.step(min(25, args.err_steps - s))
+            print(f"bench: error run precision={precision} at step "
+                  f"{min(s + 25, args.err_steps)}", file=sys.stderr, flush=True)
         grids[precision] = [ctx.grid(k) for k in ("u", "v", "w")]
         counts[precision] = ctx.n
         ctx.close()
-        print(f"bench: error run precision={precision} done ({args.err_steps} steps)",
-              file=sys.stderr, flush=True)
     scale = max(float(np.max(np.abs(g))) for g in grids[0])
     err = max(float(np.max(np.abs(a - b))) for a, b in zip(grids[1], grids[0])) / scale
-    return {"rel_velocity_error": err, "steps": args.err_steps, "grid": [n, n, n],
+    return {"rel_velocity_error": err, "steps": args.err_steps, "grid": [n, n, nz],
             "parcels_fp32": counts[1], "parcels_fp64": counts[0],
             "ok": bool(err <= 1e-5)}
 
@@ -329,8 +339,7 @@ def main():
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
     res = {}
     for f in forms:
-        res[f] = time_form(1 if f == "fp32" else 0, {k: v for k, v in ic.items()}, args, n, nz)
-    del ic
+        res[f] = time_form(1 if f == "fp32" else 0, ic, args, n, nz)
     head = res.get("fp32") or res["fp64"]
     form = "FP32-bulk" if "fp32" in res else "plain FP64"
     out = {
@@ -360,14 +369,23 @@ def main():
                        "timers": res["fp64"]["timers"]}
     if "fp32" in res and "fp64" in res:
         out["fp32_speedup_vs_fp64"] = res["fp32"]["value"] / res["fp64"]["value"]
-    # the measured copy bandwidth: the practical ceiling beside the 8 TB/s peak
+    # the measured ceilings beside the 8 TB/s peak: the runtime's device copy and a STREAM triad
     out["hbm_copy"] = copy_bandwidth()
-    for r in (out["roofline"], out["roofline_other"]):
+    from pmpic_amd.lib import hbm_triad
+    tms, tgbps = hbm_triad(1 << 30, 5)
+    out["hbm_triad"] = {"GBps": tgbps, "ms": tms, "bytes": 3 << 30,
+                        "what": "STREAM triad a = b + s*c over three 1 GiB FP64 arrays "
+                                "(pmpic_hbm_triad), best of 5"}
+    rl = [out["roofline"], out["roofline_other"]]
+    if "fp64" in out:
+        rl += [out["fp64"]["roofline"], out["fp64"]["roofline_other"]]
+    for r in rl:
         r["frac_of_copy"] = r["achieved"] / out["hbm_copy"]["GBps"]
+        r["frac_of_triad"] = r["achieved"] / tgbps
     if args.err_steps > 0 and "fp32" in res:
-        out["fp32_error"] = velocity_error(args)
+        out["fp32_error"] = velocity_error(args, ic, n, nz)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(ic, n, nz, args.cpu_steps)
     print(json.dumps(out))
     return 0
 

@@ -43,7 +43,10 @@ typedef struct pmpic_config {
   int64_t max_parcels;                /* "max_parcels" */
   double dtm, dtmax;                  /* "dtm", "dtmax" */
   double timestep_prefactor;          /* "timestep_prefactor" */
-  int32_t precision;                  /* 0 = FP64 (the only form built) */
+  int32_t precision;                  /* 0 = plain FP64 (the reference's arithmetic, held to the
+                                         oracle); 1 = FP32-bulk (cell index + FP32 offsets, FP32
+                                         parcel fields, interpolation and RK4; FP64 spectral
+                                         inversion, timestep and conservation sums) */
   int32_t splitting_enabled;          /* "parcel_splitting_enabled" */
   int32_t mixing_enabled;             /* "parcel_mixing_enabled" */
   int32_t device;                     /* HIP device ordinal */
@@ -99,8 +102,14 @@ int pmpic_grid2par(pmpic_ctx *ctx, const double *grid, double *var);
 int pmpic_parcel_haloswap(pmpic_ctx *ctx);
 
 /* grids for writebinary (writebinary.F90:287-296): id 0 u,1 v,2 w,3 p,4 q,5 r,6 b,7 hg,
- * 8 hgliq, 9 vol, 10 dp, 11 dq, 12 dr; spectral id 0 u_s, 1 v_s, 2 w_s */
+ * 8 hgliq, 9 vol, 10 dp, 11 dq, 12 dr; spectral id 0 u_s, 1 v_s, 2 w_s.  A grid holds what the
+ * reference's state grid of that name holds at this point: p, q, r and b (total buoyancy) from
+ * vort2vel / vorticity_tendency, the kept vol, b, hg, p, q, r after parcel_mixing, and hgliq (0
+ * until then) plus writebinary's b, hg, p, q, r after pmpic_writebinary_grids. */
 int pmpic_get_grid(pmpic_ctx *ctx, int32_t id, double *grid);
+/* the grid computation of write_grids_to_file (writebinary.F90:217-233): p, q, r, b and hg by
+ * par2grid of the current parcels' p, q, r, b, h; hgliq = max(0, hg - exp(-z)); b += 12.5 hgliq */
+int pmpic_writebinary_grids(pmpic_ctx *ctx);
 int pmpic_get_spectral(pmpic_ctx *ctx, int32_t id, double *spec);
 
 /* standalone semi-spectral operators on host arrays (pencilfft.F90:151-212, fft_ops.F90) */
@@ -131,6 +140,11 @@ int pmpic_last_counts(pmpic_ctx *ctx, int64_t out3[3]);
  * timer's region name (the reference's timer names for the components, timer.F90:247-406),
  * NULL past the last. */
 int pmpic_timing_enable(pmpic_ctx *ctx, int32_t enable);
+/* HBM bandwidth probe (no reference counterpart; a measurement aid): a STREAM-style triad
+ * a = b + s*c over three device arrays of bytes_per_array bytes on `device`, the best of `reps`
+ * timed launches; gbps counts 3 * bytes_per_array per launch.  Allocates and frees its arrays. */
+int pmpic_hbm_triad(int32_t device, int64_t bytes_per_array, int32_t reps, double *best_ms,
+                    double *gbps);
 int pmpic_timing_read(pmpic_ctx *ctx, int32_t which, double *total_ms, int64_t *launches);
 const char *pmpic_timing_name(int32_t which);
 

@@ -41,6 +41,8 @@ def lib():
         L.orc_create.argtypes = [i, i, i, d, d, d, d, d, d, i64, d, d, d]
         L.orc_destroy.argtypes = [vp]
         L.orc_set_threads.argtypes = [i]
+        L.orc_max_threads.restype = i
+        L.orc_max_threads.argtypes = []
         for name in ("orc_field", "orc_grid", "orc_spec"):
             getattr(L, name).restype = ctypes.POINTER(ctypes.c_double)
             getattr(L, name).argtypes = [vp, i]
@@ -63,7 +65,7 @@ def lib():
         L.orc_last_counts.argtypes = [vp, i]
         for name in ("orc_cache_weights", "orc_vort2vel", "orc_vorticity_tendency",
                      "orc_rk4_integrator", "orc_parcel_splitting", "orc_parcel_mixing",
-                     "orc_parcel_haloswap", "orc_step"):
+                     "orc_parcel_haloswap", "orc_step", "orc_writebinary_grids"):
             getattr(L, name).restype = i
             getattr(L, name).argtypes = [vp]
         L.orc_par2grid.restype = i
@@ -96,6 +98,10 @@ def _p(a):
 def set_threads(n):
     """OpenMP threads of the oracle's later calls; its results do not depend on the count."""
     lib().orc_set_threads(int(n))
+
+
+def max_threads():
+    return int(lib().orc_max_threads())
 
 
 class Oracle:
@@ -276,6 +282,9 @@ class Oracle:
 
     def parcel_haloswap(self):
         self._chk(self.L.orc_parcel_haloswap(self.h), "parcel_haloswap")
+
+    def writebinary_grids(self):
+        self._chk(self.L.orc_writebinary_grids(self.h), "writebinary_grids")
 
     def step(self):
         st = self.L.orc_step(self.h)

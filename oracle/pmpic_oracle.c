@@ -195,6 +195,8 @@ void orc_set_threads(int nthreads) {
   if (nthreads > 0) omp_set_num_threads(nthreads);
 }
 
+int orc_max_threads(void) { return omp_get_max_threads(); }
+
 void orc_destroy(orc_t *o) {
   if (!o) return;
   free(o->xc); free(o->yc); free(o->zc);
@@ -1353,6 +1355,29 @@ int orc_parcel_mixing(orc_t *o) {
 out:
   o->it_mix++;
   return st;
+}
+
+/* write_grids_to_file's grid computation, writebinary.F90:217-233: par2grid of b, p, q, r and
+ * h, then over every element hgliq = max(0., hg - exp(-z_coords(k))) and b = b + 12.5*hgliq */
+int orc_writebinary_grids(orc_t *o) {
+  int st;
+  if ((st = orc_cache_weights(o))) return st;
+  orc_par2grid(o, o->f[ORC_B], o->g[ORC_GB]);
+  orc_par2grid(o, o->f[ORC_P], o->g[ORC_GP]);
+  orc_par2grid(o, o->f[ORC_Q], o->g[ORC_GQ]);
+  orc_par2grid(o, o->f[ORC_R], o->g[ORC_GR]);
+  orc_par2grid(o, o->f[ORC_H], o->g[ORC_GHG]);
+  double *B = o->g[ORC_GB], *HG = o->g[ORC_GHG], *HL = o->g[ORC_GHGLIQ];
+#pragma omp parallel for schedule(static)
+  for (int i = 1; i <= o->nxh; i++)
+    for (int j = 1; j <= o->nyh; j++)
+      for (int k = 1; k <= o->nzn; k++) {
+        size_t q = GI(o, k, j, i);
+        double hl = HG[q] - exp(-o->zc[k - 1]);
+        HL[q] = hl > 0. ? hl : 0.;
+        B[q] = B[q] + 12.5 * HL[q];
+      }
+  return PMPIC_OK;
 }
 
 /* one timestep: the integrator group (global_config:43) run rksteps times

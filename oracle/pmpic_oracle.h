@@ -47,6 +47,7 @@ void orc_destroy(orc_t *o);
  * every threaded loop either touches disjoint data or adds into each grid node in the
  * sequential parcel order. */
 void orc_set_threads(int nthreads);
+int orc_max_threads(void);  /* threads a parallel region of the next call uses */
 
 double *orc_field(orc_t *o, int id);     /* capacity-sized parcel array */
 double *orc_grid(orc_t *o, int id);      /* halo'd grid */
@@ -89,7 +90,9 @@ int orc_rk4_integrator(orc_t *o);
 int orc_parcel_splitting(orc_t *o);
 int orc_parcel_mixing(orc_t *o);
 int orc_parcel_haloswap(orc_t *o);
-int orc_step(orc_t *o);  /* one timestep: 4 passes of the integrator group, then time += dtm */
+int orc_step(orc_t *o);
+/* the grids writebinary dumps (p, q, r, b, hg, hgliq) from the current parcels */
+int orc_writebinary_grids(orc_t *o);  /* one timestep: 4 passes of the integrator group, then time += dtm */
 int64_t orc_last_counts(const orc_t *o, int which); /* 0 split, 1 removed, 2 added */
 
 #ifdef __cplusplus

@@ -335,7 +335,9 @@ __global__ void k32_sort_big(const int32_t *__restrict__ start, int64_t ncell, i
 // loaded together with the cell counts (their addresses do not depend on the counts), the
 // parcels' values are staged in LDS in dense rows, cell after cell, and thread q of a cell sums
 // corner q over the cell's parcels in ascending parcel order, so the sums are deterministic.
-enum Acc32 { A32_P2G = 0, A32_MIX = 1 };
+// P2G: p, q, r and the total buoyancy (vort2vel + vorticity_tendency); MIX: kept / residual
+// vol, b, h, p, q, r (parcel_mixing); DUMP: b, h, p, q, r (writebinary's grids)
+enum Acc32 { A32_P2G = 0, A32_MIX = 1, A32_DUMP = 2 };
 constexpr int TILE32 = 16;
 constexpr int ROWS32 = 512;     // LDS rows with the full capacity
 constexpr int ROWS_LOW = 256;   // LDS rows with CAPC_LOW
@@ -357,13 +359,14 @@ __global__ void __launch_bounds__(ATPB) k32_cell_accum(Bulk32 b, int64_t cap, Ge
                                                        int32_t *__restrict__ npercell,
                                                        uint8_t *__restrict__ tileres) {
   constexpr bool MIX = (MODE == A32_MIX);
-  constexpr int NV = MIX ? 5 : 4;
+  constexpr bool FIVE = (MODE != A32_P2G);  // five values b, h, p, q, r
+  constexpr int NV = FIVE ? 5 : 4;
   constexpr int NF = NV + 1;
   constexpr int SPT = TILE32 * CAP / ATPB;  // slot entries per thread
   constexpr int RPT = ROWS / ATPB;         // dense rows per thread
   __shared__ float4 sgeo[ROWS];  // ox, oy, oz, vol
   __shared__ float4 sva[ROWS];   // values 0..3
-  __shared__ float svb[MIX ? ROWS : 1];
+  __shared__ float svb[FIVE ? ROWS : 1];
   __shared__ int32_t sidx[ROWS];
   __shared__ int16_t sperm[ROWS];
   __shared__ uint8_t skeep[MIX ? ROWS : 1];
@@ -417,7 +420,7 @@ __global__ void __launch_bounds__(ATPB) k32_cell_accum(Bulk32 b, int64_t cap, Ge
   auto fetch = [&](uint32_t pp, float4 &geo, float4 &va, float &vb) {
     geo = make_float4(ldnt(fld(B_OX), pp), ldnt(fld(B_OY), pp), ldnt(fld(B_OZ), pp),
                       ldnt(fld(B_VOL), pp));
-    if constexpr (!MIX) {
+    if constexpr (!FIVE) {
       va = make_float4(ldnt(fld(B_P), pp), ldnt(fld(B_Q), pp), ldnt(fld(B_R), pp),
                        ldnt(fld(B_B), pp));
       vb = ldnt(fld(B_H), pp);  // humidity, for the total buoyancy below
@@ -429,14 +432,14 @@ __global__ void __launch_bounds__(ATPB) k32_cell_accum(Bulk32 b, int64_t cap, Ge
   };
   // kz: the z level of the parcel's cell (known from the tile, so the cell is not read)
   auto put = [&](int r, int32_t pp, float4 geo, float4 v, float vb, float kz) {
-    if constexpr (!MIX) {
+    if constexpr (!FIVE) {
       // total buoyancy (vorticity_tendency.F90:121-126)
       const float z = zm.zbot + (kz + geo.z) * zm.dz;
       const float hl = vb - expf(-z);
       v.w = v.w + 12.5f * (hl > 0.f ? hl : 0.f);
     } else {
       svb[r] = vb;
-      skeep[r] = (geo.w < vmin) ? 0 : 1;
+      if constexpr (MIX) skeep[r] = (geo.w < vmin) ? 0 : 1;
     }
     sgeo[r] = geo;
     sva[r] = v;
@@ -528,7 +531,7 @@ __global__ void __launch_bounds__(ATPB) k32_cell_accum(Bulk32 b, int64_t cap, Ge
       const float fz = fmaf(sz, gr.z, tz);
       const float w = fz * fy * fx * gr.w;
       float vals[5] = {v.x, v.y, v.z, v.w, 0.f};
-      if constexpr (MIX) vals[4] = svb[r];
+      if constexpr (FIVE) vals[4] = svb[r];
       if (!MIX || skeep[r]) {
         acc[0] += w;
 #pragma unroll
@@ -555,7 +558,7 @@ __global__ void __launch_bounds__(ATPB) k32_cell_accum(Bulk32 b, int64_t cap, Ge
       const float ox = F[B_OX * cap + pp], oy = F[B_OY * cap + pp], oz = F[B_OZ * cap + pp];
       const float vol = F[B_VOL * cap + pp];
       float vals[5];
-      if constexpr (!MIX) {
+      if constexpr (!FIVE) {
         const float z = zm.zbot + (skz[cl] + oz) * zm.dz;
         const float hl = F[B_H * cap + pp] - expf(-z);
         vals[0] = F[B_P * cap + pp];
@@ -1090,8 +1093,10 @@ int b32_alloc(Ctx *c) {
   PMPIC_TRY(dalloc32(&b.cell, c->cap));
   PMPIC_TRY(dalloc32(&b.cello, c->cap));
   PMPIC_TRY(dalloc32(&b.tag, c->cap));
-  // a parcel created by mixing takes no tag (parcel_mixing.F90 copies tags only when it
-  // backfills, :377): a slot past every earlier count reads 0, as in the FP64 form's arrays
+  // a parcel created by mixing takes neither a tag nor tendencies (parcel_mixing.F90 copies
+  // tags only when it backfills, :377, and never writes dxdt..drdt): a slot past every earlier
+  // count reads 0 for them, as in the FP64 form's arrays, which are cleared at allocation too
+  PMPIC_CHECK(hipMemset(b.f, 0, sizeof(float) * (size_t)NF32 * (size_t)c->cap));
   PMPIC_CHECK(hipMemset(b.tag, 0, sizeof(double) * (size_t)c->cap));
   for (int i = 0; i < 6; i++) PMPIC_TRY(dalloc32(&b.grid[i], c->gsz));
   for (int i = 0; i < 12; i++) PMPIC_TRY(dalloc32(&b.mgrid[i], c->gsz));
@@ -1396,6 +1401,32 @@ int b32_mix(Ctx *c) {
     }
   }
   c->n = np;
+  return PMPIC_OK;
+}
+
+// writebinary's grids (writebinary.F90:217-228): b, hg, p, q, r by par2grid of the FP32
+// parcels' b, h, p, q, r, FP32 sums normalised into the FP64 grids
+int b32_dump_grids(Ctx *c) {
+  Bulk32 &b = c->b32;
+  const Geometry &g = c->g;
+  PMPIC_TRY(bin32(c, c->n));
+  if (b.capc == CAPC_LOW)
+    hipLaunchKernelGGL((k32_cell_accum<A32_DUMP, CAPC_LOW, ROWS_LOW>), dim3(cdiv(c->ncell, TILE32)),
+                       dim3(ATPB), 0, c->stream, b, c->cap, g, zmap_of(g), (float)g.vmin,
+                       c->cell_count, c->cell_start, c->sorted, c->ncell, partial32(c), nullptr,
+                       nullptr, nullptr);
+  else
+    hipLaunchKernelGGL((k32_cell_accum<A32_DUMP, CAPC, ROWS32>), dim3(cdiv(c->ncell, TILE32)),
+                       dim3(ATPB), 0, c->stream, b, c->cap, g, zmap_of(g), (float)g.vmin,
+                       c->cell_count, c->cell_start, c->sorted, c->ncell, partial32(c), nullptr,
+                       nullptr, nullptr);
+  Out32<double> out = {{c->grid[GB], c->grid[GHG], c->grid[GP], c->grid[GQ], c->grid[GR], nullptr}};
+  const int64_t nnode = (int64_t)g.nzn * g.ny * g.nx;
+  hipLaunchKernelGGL((k32_node_sum<5, true, true, double>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0,
+                     c->stream, g, partial32(c), c->ncell, out, nullptr);
+  PMPIC_CHECK(hipGetLastError());
+  // these grids now hold writebinary's values, not the kept grids of the last mixing
+  for (int f = 1; f < 6; f++) b.kept_pending[f] = false;
   return PMPIC_OK;
 }
 

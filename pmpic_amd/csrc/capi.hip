@@ -653,6 +653,15 @@ int pmpic_get_grid(pmpic_ctx *ctx, int32_t id, double *grid) {
   return PMPIC_OK;
 }
 
+int pmpic_writebinary_grids(pmpic_ctx *ctx) {
+  if (!ctx) return PMPIC_ERR_ARG;
+  Ctx *c = &ctx->c;
+  if (c->precision == 1) PMPIC_TRY(b32_dump_grids(c));
+  else PMPIC_TRY(dump_grids_f64(c, c->n));
+  PMPIC_TRY(hgliq_grid(c));
+  return take_status(c);
+}
+
 int pmpic_get_spectral(pmpic_ctx *ctx, int32_t id, double *spec) {
   if (!ctx || !spec || id < 0 || id > 2) return PMPIC_ERR_ARG;
   Ctx *c = &ctx->c;

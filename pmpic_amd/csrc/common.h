@@ -226,13 +226,30 @@ __device__ __forceinline__ int64_t gidx(const Geometry &g, int k, int j, int i) 
   return ((int64_t)k * g.ny + j) * g.nx + i;
 }
 
+// Buffers of one FP64 binning + single-field par2grid: the context's own, or caller-provided
+// memory (pmpic_par2grid_noalloc).  cell, rank, sorted: n entries; count, start: ncell + 1;
+// partial: 16 * ncell doubles; scan_tmp (scan_bytes) the exclusive scan's temporary storage,
+// or null to use the context's (grown on demand).
+struct BinWork {
+  int32_t *cell, *rank, *count, *start, *sorted;
+  double *partial;
+  void *scan_tmp;
+  size_t scan_bytes;
+};
+
 // ---- launchers implemented in the translation units ----
 // interp.hip
 int bin_parcels(Ctx *c, int64_t n);
+int bin_parcels_ws(Ctx *c, int64_t n, const BinWork &w);
+int par2grid_single_ws(Ctx *c, int64_t n, const double *d_var, double *d_grid, const BinWork &w);
+size_t scan_temp_bytes(Ctx *c, int64_t n);  // temporary storage of an exclusive scan of n
+int to_ref_layout(Ctx *c, const double *d_interior, double *d_ref);
 int par2grid_fused(Ctx *c, int64_t n);  // P,Q,R,B grids from p,q,r,btot
 int par2grid_single(Ctx *c, int64_t n, const double *d_var, double *d_grid);
 int grid2par_single(Ctx *c, int64_t n, const double *d_grid, double *d_var);
 int mixing_grids(Ctx *c, int64_t n);
+int dump_grids_f64(Ctx *c, int64_t n);  // writebinary's b, hg, p, q, r grids
+int hgliq_grid(Ctx *c);                 // hgliq and b += 12.5 hgliq (writebinary.F90:224-233)
 // fft.hip
 int fft_setup(Ctx *c);
 int fft_forward(Ctx *c, int nf, const double *const *src, double *const *dst);
@@ -297,6 +314,7 @@ int b32_split(Ctx *c);
 int b32_mix(Ctx *c);
 int b32_conservation_sums(Ctx *c, double out6[6]);
 int b32_export_grid(Ctx *c, int id);  // brings grid[id] up to date for a grid export
+int b32_dump_grids(Ctx *c);             // writebinary's b, hg, p, q, r grids
 int scan_exclusive_i32(Ctx *c, const int32_t *in, int32_t *out, int64_t n);
 int sync_scalars(Ctx *c);
 

@@ -119,7 +119,7 @@ __global__ void k_bin_sort_big(const int32_t *__restrict__ start, int64_t ncell,
 }
 
 // value sources for the per-cell accumulation
-enum AccMode { ACC_P2G = 0, ACC_SINGLE = 1, ACC_MIX_KEEP = 2, ACC_MIX_RES = 3 };
+enum AccMode { ACC_P2G = 0, ACC_SINGLE = 1, ACC_MIX_KEEP = 2, ACC_MIX_RES = 3, ACC_DUMP = 4 };
 
 // Per-cell corner partial sums.
 // A workgroup owns TILE consecutive cells.  Their buckets are consecutive in the bucket array, so
@@ -416,6 +416,14 @@ __global__ void k_grid2par(Geometry g, const double *__restrict__ par, int64_t c
 
 }  // namespace
 
+size_t scan_temp_bytes(Ctx *c, int64_t n) {
+  size_t need = 0;
+  if (hipcub::DeviceScan::ExclusiveSum(nullptr, need, (const int32_t *)nullptr, (int32_t *)nullptr,
+                                       (int)n, c->stream) != hipSuccess)
+    return 0;
+  return need;
+}
+
 int scan_exclusive_i32(Ctx *c, const int32_t *in, int32_t *out, int64_t n) {
   size_t need = 0;
   PMPIC_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, need, in, out, (int)n, c->stream));
@@ -430,24 +438,45 @@ int scan_exclusive_i32(Ctx *c, const int32_t *in, int32_t *out, int64_t n) {
   return PMPIC_OK;
 }
 
-int bin_parcels(Ctx *c, int64_t n) {
+// the context's own binning buffers (the scan's temporary storage is grown on demand)
+static BinWork ctx_work(Ctx *c) {
+  BinWork w;
+  w.cell = c->cell;
+  w.rank = c->rank;
+  w.count = c->cell_count;
+  w.start = c->cell_start;
+  w.sorted = c->sorted;
+  w.partial = c->partial;
+  w.scan_tmp = nullptr;  // c->scan_tmp, grown by scan_exclusive_i32
+  w.scan_bytes = 0;
+  return w;
+}
+
+int bin_parcels_ws(Ctx *c, int64_t n, const BinWork &w) {
   const Geometry &g = c->g;
   const int64_t nc1 = c->ncell + 1;
-  hipLaunchKernelGGL(k_fill_i32, dim3(cdiv(nc1, TPB)), dim3(TPB), 0, c->stream, c->cell_count,
-                     nc1, 0);
+  hipLaunchKernelGGL(k_fill_i32, dim3(cdiv(nc1, TPB)), dim3(TPB), 0, c->stream, w.count, nc1, 0);
   if (n > 0)
     hipLaunchKernelGGL(k_bin_count, dim3(cdiv(n, (int64_t)TPB * BIN_SEQ)), dim3(TPB), 0, c->stream, g,
                        c->par + FX * c->cap, c->par + FY * c->cap, c->par + FZ * c->cap, n,
-                       c->cell, c->rank, c->cell_count, c->dsc);
-  PMPIC_TRY(scan_exclusive_i32(c, c->cell_count, c->cell_start, nc1));
+                       w.cell, w.rank, w.count, c->dsc);
+  if (w.scan_tmp) {
+    size_t have = w.scan_bytes;
+    PMPIC_CHECK(hipcub::DeviceScan::ExclusiveSum(w.scan_tmp, have, w.count, w.start, (int)nc1,
+                                                 c->stream));
+  } else {
+    PMPIC_TRY(scan_exclusive_i32(c, w.count, w.start, nc1));
+  }
   if (n > 0)
-    hipLaunchKernelGGL(k_bin_scatter, dim3(cdiv(n, TPB)), dim3(TPB), 0, c->stream, c->cell,
-                       c->rank, c->cell_start, n, c->sorted);
-  hipLaunchKernelGGL(k_bin_sort_big, dim3(cdiv(c->ncell, TPB)), dim3(TPB), 0, c->stream,
-                     c->cell_start, c->ncell, ROWS_MIX, c->sorted);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(cdiv(n, TPB)), dim3(TPB), 0, c->stream, w.cell, w.rank,
+                       w.start, n, w.sorted);
+  hipLaunchKernelGGL(k_bin_sort_big, dim3(cdiv(c->ncell, TPB)), dim3(TPB), 0, c->stream, w.start,
+                     c->ncell, ROWS_MIX, w.sorted);
   PMPIC_CHECK(hipGetLastError());
   return PMPIC_OK;
 }
+
+int bin_parcels(Ctx *c, int64_t n) { return bin_parcels_ws(c, n, ctx_work(c)); }
 
 template <int MODE, int NV, int ROWS>
 static void launch_accum_lds(Ctx *c, const double *var, double *partial, double *partial2,
@@ -472,14 +501,82 @@ int par2grid_fused(Ctx *c, int64_t n) {
   return PMPIC_OK;
 }
 
-int par2grid_single(Ctx *c, int64_t n, const double *d_var, double *d_grid) {
-  PMPIC_TRY(bin_parcels(c, n));
+int par2grid_single_ws(Ctx *c, int64_t n, const double *d_var, double *d_grid,
+                       const BinWork &w) {
+  PMPIC_TRY(bin_parcels_ws(c, n, w));
   const int64_t ncell = c->ncell;
-  launch_accum_lds<ACC_SINGLE, 1, ROWS_P2G>(c, d_var, c->partial, nullptr, nullptr);
+  hipLaunchKernelGGL((k_cell_accum_lds<ACC_SINGLE, 1, ROWS_P2G>), dim3(cdiv(ncell, TILE)),
+                     dim3(TILE * 8), 0, c->stream, c->g, c->par, c->cap, d_var, w.start, w.sorted,
+                     ncell, w.partial, nullptr, nullptr);
   OutTable table = {{d_grid, nullptr, nullptr, nullptr, nullptr, nullptr}};
   const int64_t nnode = (int64_t)c->g.nzn * c->g.ny * c->g.nx;
   hipLaunchKernelGGL((k_node_sum<1, true>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0, c->stream, c->g,
-                     c->partial, ncell, table);
+                     w.partial, ncell, table);
+  PMPIC_CHECK(hipGetLastError());
+  return PMPIC_OK;
+}
+
+int par2grid_single(Ctx *c, int64_t n, const double *d_var, double *d_grid) {
+  return par2grid_single_ws(c, n, d_var, d_grid, ctx_work(c));
+}
+
+// interior [nzn][ny][nx] -> the reference layout data(z, y, x) with x/y halo 2 holding the
+// periodic copies of the interior (the layout pmpic_par2grid returns)
+__global__ void k_to_ref_layout(Geometry g, const double *__restrict__ in, double *__restrict__ out) {
+  const int nxh = g.nx + 4, nyh = g.ny + 4;
+  const int64_t total = (int64_t)nxh * nyh * g.nzn;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int k = (int)(t % g.nzn);
+  const int jh = (int)((t / g.nzn) % nyh);
+  const int ih = (int)(t / ((int64_t)g.nzn * nyh));
+  int i = ih - 2, j = jh - 2;
+  if (i < 0) i += g.nx;
+  else if (i >= g.nx) i -= g.nx;
+  if (j < 0) j += g.ny;
+  else if (j >= g.ny) j -= g.ny;
+  out[t] = in[gidx(g, k, j, i)];
+}
+
+int to_ref_layout(Ctx *c, const double *d_interior, double *d_ref) {
+  const int64_t total = (int64_t)(c->g.nx + 4) * (c->g.ny + 4) * c->g.nzn;
+  hipLaunchKernelGGL(k_to_ref_layout, dim3(cdiv(total, TPB)), dim3(TPB), 0, c->stream, c->g,
+                     d_interior, d_ref);
+  PMPIC_CHECK(hipGetLastError());
+  return PMPIC_OK;
+}
+
+// writebinary's grids (writebinary.F90:217-233): b, hg, p, q, r by par2grid of the parcels'
+// b, h, p, q, r (one binning, the five sums of each node in parcel order as five par2grid
+// calls would take them), then hgliq = max(0, hg - exp(-z)) and b += 12.5 hgliq per node
+__global__ void k_hgliq(Geometry g, double *__restrict__ B, const double *__restrict__ HG,
+                        double *__restrict__ HGLIQ) {
+  const int64_t nnode = (int64_t)g.nzn * g.ny * g.nx;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nnode) return;
+  const int k = (int)(t / ((int64_t)g.nx * g.ny));
+  const double z = g.zmin_h + (double)k * g.dz + g.zbot;  // z_coords(k+1), setup as in create
+  const double hl = HG[t] - exp(-z);
+  const double liq = hl > 0. ? hl : 0.;
+  HGLIQ[t] = liq;
+  B[t] = B[t] + 12.5 * liq;
+}
+
+int dump_grids_f64(Ctx *c, int64_t n) {
+  PMPIC_TRY(bin_parcels(c, n));
+  launch_accum_lds<ACC_DUMP, 5, ROWS_MIX>(c, nullptr, c->partial, nullptr, nullptr);
+  OutTable out = {{c->grid[GB], c->grid[GHG], c->grid[GP], c->grid[GQ], c->grid[GR], nullptr}};
+  const int64_t nnode = (int64_t)c->g.nzn * c->g.ny * c->g.nx;
+  hipLaunchKernelGGL((k_node_sum<5, true>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0, c->stream, c->g,
+                     c->partial, c->ncell, out);
+  PMPIC_CHECK(hipGetLastError());
+  return PMPIC_OK;
+}
+
+int hgliq_grid(Ctx *c) {
+  const int64_t nnode = (int64_t)c->g.nzn * c->g.ny * c->g.nx;
+  hipLaunchKernelGGL(k_hgliq, dim3(cdiv(nnode, TPB)), dim3(TPB), 0, c->stream, c->g, c->grid[GB],
+                     c->grid[GHG], c->grid[GHGLIQ]);
   PMPIC_CHECK(hipGetLastError());
   return PMPIC_OK;
 }

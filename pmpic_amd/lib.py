@@ -36,7 +36,8 @@ EXPORTS = [
     "pmpic_parcel_haloswap", "pmpic_get_grid", "pmpic_get_spectral", "pmpic_fft_forward",
     "pmpic_fft_backward", "pmpic_diffx", "pmpic_diffy", "pmpic_spectral_filter", "pmpic_diffz",
     "pmpic_laplinv", "pmpic_conservation_sums", "pmpic_last_counts", "pmpic_timing_enable",
-    "pmpic_timing_read", "pmpic_step_concurrent", "pmpic_timing_name",
+    "pmpic_timing_read", "pmpic_step_concurrent", "pmpic_timing_name", "pmpic_hbm_triad",
+    "pmpic_writebinary_grids",
 ]
 
 
@@ -84,7 +85,8 @@ def load():
         L.pmpic_get_state.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(d), ctypes.POINTER(d)]
         L.pmpic_set_time.argtypes = [vp, d, d]
         for name in ("pmpic_vort2vel", "pmpic_vorticity_tendency", "pmpic_rk4_integrator",
-                     "pmpic_parcel_splitting", "pmpic_parcel_mixing", "pmpic_parcel_haloswap"):
+                     "pmpic_parcel_splitting", "pmpic_parcel_mixing", "pmpic_parcel_haloswap",
+                     "pmpic_writebinary_grids"):
             getattr(L, name).argtypes = [vp]
         L.pmpic_step.argtypes = [vp, i32]
         L.pmpic_step_concurrent.argtypes = [pp, i32, i32]
@@ -106,6 +108,7 @@ def load():
         L.pmpic_timing_read.argtypes = [vp, i32, ctypes.POINTER(d), ctypes.POINTER(i64)]
         L.pmpic_timing_name.restype = ctypes.c_char_p
         L.pmpic_timing_name.argtypes = [i32]
+        L.pmpic_hbm_triad.argtypes = [i32, i64, i32, ctypes.POINTER(d), ctypes.POINTER(d)]
         _lib = L
     return _lib
 
@@ -138,6 +141,16 @@ def step_concurrent(contexts, nsteps=1):
     st = L.pmpic_step_concurrent(arr, len(contexts), int(nsteps))
     if st != 0:
         raise PmpicError(st, "pmpic_step_concurrent")
+
+
+def hbm_triad(bytes_per_array=1 << 30, reps=5, device=0):
+    """STREAM-style triad bandwidth of the device (pmpic_hbm_triad): (best ms, GB/s)."""
+    ms, gbps = ctypes.c_double(), ctypes.c_double()
+    st = load().pmpic_hbm_triad(int(device), int(bytes_per_array), int(reps), ctypes.byref(ms),
+                                ctypes.byref(gbps))
+    if st != 0:
+        raise PmpicError(st, "pmpic_hbm_triad")
+    return ms.value, gbps.value
 
 
 class Context:
@@ -283,6 +296,10 @@ class Context:
         out = np.zeros(self.gshape)
         self._chk(self.L.pmpic_get_grid(self.h, GRID_IDS[name], _p(out)), "pmpic_get_grid")
         return out
+
+    def writebinary_grids(self):
+        """Compute the grids writebinary dumps (p, q, r, b, hg, hgliq) from the current parcels."""
+        self._chk(self.L.pmpic_writebinary_grids(self.h), "pmpic_writebinary_grids")
 
     def spectral(self, name):
         out = np.zeros(self.sshape)

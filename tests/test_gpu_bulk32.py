@@ -166,14 +166,20 @@ def test_timers_kernels_only_and_all(precision):
     assert t["vort2vel"][0] > 0
 
 
-@pytest.mark.parametrize("ppcd", [2, 3])
-def test_slot_capacity_changes_no_result(monkeypatch, ppcd):
-    """8 and 27 parcels per cell, two steps with the slot capacity fixed at 16, fixed at 32 and
-    tuned (PMPIC_SLOT_CAPACITY=tune: the first par2grid times the accumulation at both and keeps
-    the faster).  Every cell's parcels are summed in ascending parcel order whatever the
-    capacity, so counts, parcels and grids are bit-identical across the three."""
+@pytest.mark.parametrize("case", ["8ppc", "27ppc", "crowded300"])
+def test_slot_capacity_changes_no_result(monkeypatch, case):
+    """8 and 27 parcels per cell, and a crowded case with cells of 32, 33 and 300 parcels (300 is
+    past the 256 LDS rows of the 16-slot accumulation, so that capacity sums the cell on its
+    serial big-cell path while 32 slots sum it in LDS): two steps with the slot capacity fixed
+    at 16, fixed at 32 and tuned (PMPIC_SLOT_CAPACITY=tune: the first par2grid times the
+    accumulation at both and keeps the faster).  Every cell's parcels are summed in ascending
+    parcel order with the same expressions whatever the capacity and path, so counts, parcels
+    and grids are bit-identical across the three."""
     shape = (12, 10, 8)
-    ic = thermal_case(*shape, ppcd, vort=1.0, thin=0.1)
+    if case == "crowded300":
+        ic = crowded_case(*shape, extra=(24, 25, 292))
+    else:
+        ic = thermal_case(*shape, 2 if case == "8ppc" else 3, vort=1.0, thin=0.1)
     names = ("u", "v", "w", "p", "q", "r", "b")
     out = {}
     for mode in ("16", "32", "tune"):
@@ -191,3 +197,17 @@ def test_slot_capacity_changes_no_result(monkeypatch, ppcd):
             assert np.array_equal(p[k], p16[k]), (mode, k)
         for k in names:
             assert np.array_equal(gr[k], g16[k]), (mode, k)
+
+
+def test_writebinary_grids_close_to_fp64():
+    """writebinary's grids (p, q, r, b + 12.5 hgliq, hg, hgliq) from the FP32 parcels after two
+    steps, within 1e-5 of the FP64 form's (which test_gpu_parity.py holds to the oracle)."""
+    shape = (16, 12, 8)
+    ic = thermal_case(*shape, 2, vort=1.0)
+    g64 = make(shape, 0, ic)
+    g32 = make(shape, 1, ic)
+    for g in (g64, g32):
+        g.step(2)
+        g.writebinary_grids()
+    for name in ("p", "q", "r", "b", "hg", "hgliq"):
+        assert rel_err(g32.grid(name), g64.grid(name)) < 1e-5, name

@@ -71,6 +71,22 @@ def test_par2grid(oracle_mod, shape):
         assert rel_err(interior(got), interior(want)) < 1e-12
 
 
+@pytest.mark.parametrize("shape", SHAPES[:2])
+def test_writebinary_grids(oracle_mod, shape):
+    """The grids writebinary dumps (writebinary.F90:217-233) after a step: p, q, r, hg by
+    par2grid, hgliq = max(0, hg - exp(-z)), b = par2grid(b) + 12.5 hgliq; 1e-12 relative (the
+    par2grid bar); hgliq must not be all zero here (the thermal's humidity exceeds exp(-z))."""
+    o, g, _ = make_pair(oracle_mod, shape, vort=0.5)
+    o.step()
+    g.step()
+    o.writebinary_grids()
+    g.writebinary_grids()
+    for name in ("p", "q", "r", "b", "hg", "hgliq"):
+        want = interior(o.grid(name))
+        assert rel_err(interior(g.grid(name)), want) < 1e-12, name
+    assert np.max(interior(o.grid("hgliq"))) > 0
+
+
 def test_par2grid_and_steps_crowded_cells(oracle_mod):
     """Cells holding 32, 33, 44 and 708 parcels: the binning's per-cell capacity edges and a
     cell with more parcels than one workgroup stages in LDS."""
