@@ -35,3 +35,13 @@ def oracle_mod():
     from oracle import oracle
     oracle.build()
     return oracle
+
+
+@pytest.fixture
+def bar(record_property):
+    """check(name, err, limit): records the achieved error as a test property (it lands in
+    pytest's --junitxml report) and asserts it is below the limit."""
+    def check(name, err, limit):
+        record_property(name, f"{err:.3e} (bar {limit:.0e})")
+        assert err < limit, (name, err, limit)
+    return check

@@ -15,8 +15,10 @@ checks are size-independent properties of the path:
   grids, with identical parcel counts (the FP32-bulk form's bar), on both workloads; the second
   runs the binning's 16-slot capacity at full size.
 
-The reference's own CPU configuration (32x32x32 cells, 27 parcels per cell) is held to the
-oracle directly.
+Two configurations are also held to the oracle directly, at the FP64 bar (1e-12 relative to
+each field's maximum, identical counts): the reference's own CPU configuration (32x32x32 cells,
+27 parcels per cell) over three steps, and the FP64 GPU configuration (128x128x128 cells, 27
+parcels per cell) over one step, the oracle threaded over the host cores.
 """
 import numpy as np
 import pytest
@@ -48,26 +50,42 @@ def thermal128():
     return _thermal(128, 128, 3)
 
 
-def test_cpu_configuration_matches_oracle(oracle_mod):
-    """The reference's CPU configuration, 32x32x32 cells with 27 parcels per cell (884,736
-    parcels), small enough for the oracle: three whole steps of the FP64 form against it, with
-    identical counts, parcels within 1e-10 and velocity grids within 1e-11 (relative to each
-    field's maximum)."""
-    (ic, lx, lz) = _thermal(32, 32, 3)
-    o = oracle_mod.Oracle(32, 32, 32, 0.0, lx, 0.0, lx, 0.0, lz, 2 * len(ic["x"]), 0.01, 0.01,
+def _against_oracle(oracle_mod, bar, n, nz, ppcd, steps):
+    (ic, lx, lz) = _thermal(n, nz, ppcd)
+    o = oracle_mod.Oracle(n, n, nz, 0.0, lx, 0.0, lx, 0.0, lz, 2 * len(ic["x"]), 0.01, 0.01,
                           0.5)
     o.set_parcels(ic)
-    g = _ctx(ic, 32, 32, lx, lz, 0)
-    for step in range(3):
+    g = _ctx(ic, n, nz, lx, lz, 0)
+    for step in range(steps):
         o.step()
         g.step(1)
         assert g.n == o.n, step
+        assert g.last_counts() == o.last_counts(), step
     gp, op = g.get_parcels(), o.get_parcels()
     for k in ("x", "y", "z", "p", "q", "r", "vol", "b", "h"):
-        assert rel_err(gp[k], op[k]) < 1e-10, k
-    for name in ("u", "v", "w"):
-        assert rel_err(g.grid(name)[2:-2, 2:-2, :], o.grid(name)[2:-2, 2:-2, :]) < 1e-11, name
+        bar(k, rel_err(gp[k], op[k]), 1e-12)
+    # stretch adds |p dpdt + q dqdt + r drdt|^(1/3) dt (parcel_splitting.F90:71): where the dot
+    # product cancels to near zero, the cube root's infinite slope turns a last-bit difference
+    # dx of the tendencies into (dx)^(1/3), so stretch alone gets a looser bar
+    bar("stretch", rel_err(gp["stretch"], op["stretch"]), 1e-10)
+    for name in ("u", "v", "w", "p", "q", "r"):
+        bar("grid_" + name, rel_err(g.grid(name)[2:-2, 2:-2, :], o.grid(name)[2:-2, 2:-2, :]),
+            1e-12)
     g.close()
+
+
+def test_cpu_configuration_matches_oracle(oracle_mod, bar):
+    """The reference's CPU configuration, 32x32x32 cells with 27 parcels per cell (884,736
+    parcels), small enough for the oracle: three whole steps of the FP64 form against it, with
+    identical counts, and parcels and grids within 1e-12 relative to each field's maximum."""
+    _against_oracle(oracle_mod, bar, 32, 32, 3, 3)
+
+
+def test_gpu_configuration_one_step_matches_oracle(oracle_mod, bar):
+    """The FP64 GPU configuration, 128x128x128 cells with 27 parcels per cell (56,623,104
+    parcels): one whole step against the oracle (threaded over the host cores), identical
+    counts, parcels and grids within 1e-12."""
+    _against_oracle(oracle_mod, bar, 128, 128, 3, 1)
 
 
 def test_par2grid_of_ones_is_one(thermal128):
