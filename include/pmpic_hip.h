@@ -98,6 +98,20 @@ int pmpic_cache_parcel_interp_weights(pmpic_ctx *ctx, int32_t *is, int32_t *js, 
                                       double *delxs, double *delys, double *delzs);
 int pmpic_par2grid(pmpic_ctx *ctx, const double *var, double *grid);
 int pmpic_grid2par(pmpic_ctx *ctx, const double *grid, double *var);
+
+/* pmpic_par2grid on device arrays, allocating nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  var: the context's current parcel count of FP64
+ * values in device memory of the context's device; grid: (nz+1) x (ny+4) x (nx+4) FP64 in device
+ * memory, written in pmpic_par2grid's layout with the same values, bit for bit.  The working
+ * memory is one static device array of PMPIC_NOALLOC_SCRATCH_BYTES per GPU, held until the
+ * process ends, so the sizes are capped: at most PMPIC_NOALLOC_MAX_PARCELS parcels and
+ * PMPIC_NOALLOC_MAX_CELLS cells, else PMPIC_ERR_ARG.  Calls that overlap on one GPU (from two
+ * threads or two contexts' streams) race on that array.  Plain FP64 form only. */
+#define PMPIC_NOALLOC_MAX_PARCELS (1 << 20)
+#define PMPIC_NOALLOC_MAX_CELLS (1 << 15)
+#define PMPIC_NOALLOC_SCRATCH_BYTES 17826304
+int pmpic_par2grid_noalloc(pmpic_ctx *ctx, const double *var, double *grid);
 /* parcel_haloswap (haloswap.F90:109-289) on the current device parcels */
 int pmpic_parcel_haloswap(pmpic_ctx *ctx);
 

@@ -12,10 +12,6 @@
 
 using namespace pmpic;
 
-struct pmpic_ctx {
-  Ctx c;
-};
-
 namespace pmpic {
 
 int timing_begin(Ctx *c, int which) {
@@ -44,6 +40,16 @@ int timing_harvest(Ctx *c) {
     c->nev[w] = 0;
   }
   return PMPIC_OK;
+}
+
+// status raised by kernels since the last check; clears it
+int take_status(Ctx *c) {
+  int st = sync_scalars(c);
+  if (st != PMPIC_OK) {
+    int zero = 0;
+    hipMemcpy(&c->dsc->status, &zero, sizeof(int), hipMemcpyHostToDevice);
+  }
+  return st;
 }
 
 }  // namespace pmpic
@@ -159,15 +165,6 @@ int download(Ctx *c, std::vector<double> &h, const double *src, size_t n) {
   return PMPIC_OK;
 }
 
-// status raised by kernels since the last check; clears it
-int take_status(Ctx *c) {
-  int st = sync_scalars(c);
-  if (st != PMPIC_OK) {
-    int zero = 0;
-    hipMemcpy(&c->dsc->status, &zero, sizeof(int), hipMemcpyHostToDevice);
-  }
-  return st;
-}
 
 int set_dev_scalars(Ctx *c) {
   c->hsc.dtm = c->dtm;

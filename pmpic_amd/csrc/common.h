@@ -317,5 +317,12 @@ int b32_export_grid(Ctx *c, int id);  // brings grid[id] up to date for a grid e
 int b32_dump_grids(Ctx *c);             // writebinary's b, hg, p, q, r grids
 int scan_exclusive_i32(Ctx *c, const int32_t *in, int32_t *out, int64_t n);
 int sync_scalars(Ctx *c);
+// capi.hip: status raised by kernels since the last check, cleared once read
+int take_status(Ctx *c);
 
 }  // namespace pmpic
+
+// the opaque context handle of the C ABI
+struct pmpic_ctx {
+  pmpic::Ctx c;
+};

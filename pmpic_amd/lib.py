@@ -18,6 +18,10 @@ GRID_IDS = {"u": 0, "v": 1, "w": 2, "p": 3, "q": 4, "r": 5, "b": 6, "hg": 7, "hg
             "vol": 9, "dp": 10, "dq": 11, "dr": 12}
 SPEC_IDS = {"u_s": 0, "v_s": 1, "w_s": 2}
 
+# size caps of pmpic_par2grid_noalloc (its static scratch area)
+PMPIC_NOALLOC_MAX_PARCELS = 1 << 20
+PMPIC_NOALLOC_MAX_CELLS = 1 << 15
+
 STATUS = {
     0: "PMPIC_OK", 1: "PMPIC_ERR_OUT_OF_BOX_X", 2: "PMPIC_ERR_OUT_OF_BOX_Y",
     3: "PMPIC_ERR_OUT_OF_BOX_Z", 4: "PMPIC_ERR_DELTA", 5: "PMPIC_ERR_MAXPARCELS",
@@ -37,7 +41,7 @@ EXPORTS = [
     "pmpic_fft_backward", "pmpic_diffx", "pmpic_diffy", "pmpic_spectral_filter", "pmpic_diffz",
     "pmpic_laplinv", "pmpic_conservation_sums", "pmpic_last_counts", "pmpic_timing_enable",
     "pmpic_timing_read", "pmpic_step_concurrent", "pmpic_timing_name", "pmpic_hbm_triad",
-    "pmpic_writebinary_grids",
+    "pmpic_writebinary_grids", "pmpic_par2grid_noalloc",
 ]
 
 
@@ -93,6 +97,7 @@ def load():
         L.pmpic_cache_parcel_interp_weights.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.pmpic_par2grid.argtypes = [vp, vp, vp]
         L.pmpic_grid2par.argtypes = [vp, vp, vp]
+        L.pmpic_par2grid_noalloc.argtypes = [vp, vp, vp]
         L.pmpic_get_grid.argtypes = [vp, i32, vp]
         L.pmpic_get_spectral.argtypes = [vp, i32, vp]
         L.pmpic_fft_forward.argtypes = [vp, vp, vp]
