@@ -10,8 +10,10 @@ mixing) with the parcels resident in HBM.
 
 Both numeric forms run on the same initial condition: the FP32-bulk form (positions as cell
 index + FP32 offsets, FP32 parcel arithmetic; `value`) and the plain FP64 form.  The velocity
-error of the FP32-bulk form against the FP64 form is measured after --err-steps steps of both
-from that same state.  Prints one JSON line: parcel-steps/s of each form, the roofline of each
+error of the FP32-bulk form against the FP64 form is measured over --err-steps steps of both,
+stepped side by side from that same state: relative L2 error of the u, v, w grids (the bar,
+1e-5), the maximum-norm error beside it, and the step at which the two forms' parcel counts
+first differ.  Prints one JSON line: parcel-steps/s of each form, the roofline of each
 form's dominant kernel and of the other parcel<->grid kernel (HIP events taken by every launch
 in the timed region), the HBM ceilings measured in the same run (device copy, STREAM triad) and
 a CPU baseline: the threaded C FP64 restatement of the same path on all host cores for one step
@@ -234,29 +236,56 @@ def time_form(precision, ic, args, n, nz):
 
 
 def velocity_error(args, ic, n, nz):
-    """Both forms from the bench's initial condition for args.err_steps steps; the FP32-bulk
-    form's maximum velocity error relative to the FP64 form's largest velocity component."""
+    """Both forms from the bench's initial condition, stepped side by side for args.err_steps
+    steps.  The FP32-bulk form's velocity error relative to the FP64 form over the u, v, w grids,
+    in the L2 norm (||u32 - u64|| / ||u64||, the bar) and in the maximum norm (max |u32 - u64| /
+    max |u64|), at checkpoints, with the parcel counts of both forms and the first step at which
+    they differ (a split or mixing decision taken differently at its threshold)."""
     import numpy as np
     from pmpic_amd.lib import Context
     lx, lz = ic["_lx"], ic["_lz"]
-    grids = {}
-    counts = {}
+    ctx = {}
     for precision in (1, 0):
-        ctx = Context(n, n, nz, 0.0, lx, 0.0, lx, 0.0, lz, 2 * len(ic["x"]), dtm=0.01,
-                      dtmax=0.01, timestep_prefactor=0.5, precision=precision)
-        ctx.set_parcels(ic)
-        for s in range(0, args.err_steps, 25):
-            ctx.step(min(25, args.err_steps - s))
-            print(f"bench: error run precision={precision} at step "
-                  f"{min(s + 25, args.err_steps)}", file=sys.stderr, flush=True)
-        grids[precision] = [ctx.grid(k) for k in ("u", "v", "w")]
-        counts[precision] = ctx.n
-        ctx.close()
-    scale = max(float(np.max(np.abs(g))) for g in grids[0])
-    err = max(float(np.max(np.abs(a - b))) for a, b in zip(grids[1], grids[0])) / scale
-    return {"rel_velocity_error": err, "steps": args.err_steps, "grid": [n, n, nz],
-            "parcels_fp32": counts[1], "parcels_fp64": counts[0],
-            "ok": bool(err <= 1e-5)}
+        c = Context(n, n, nz, 0.0, lx, 0.0, lx, 0.0, lz, 2 * len(ic["x"]), dtm=0.01,
+                    dtmax=0.01, timestep_prefactor=0.5, precision=precision)
+        c.set_parcels(ic)
+        ctx[precision] = c
+
+    def errors():
+        g32 = [ctx[1].grid(k)[2:-2, 2:-2, :] for k in ("u", "v", "w")]
+        g64 = [ctx[0].grid(k)[2:-2, 2:-2, :] for k in ("u", "v", "w")]
+        l2 = np.sqrt(sum(float(np.sum((a - b) ** 2)) for a, b in zip(g32, g64)) /
+                     sum(float(np.sum(b ** 2)) for b in g64))
+        mx = (max(float(np.max(np.abs(a - b))) for a, b in zip(g32, g64)) /
+              max(float(np.max(np.abs(b))) for b in g64))
+        return l2, mx
+
+    history = []
+    first_diff = None
+    checkpoints = {1, 10, 25, 50, 75, args.err_steps}
+    for s in range(1, args.err_steps + 1):
+        ctx[1].step(1)
+        ctx[0].step(1)
+        differ = ctx[1].n != ctx[0].n
+        if differ and first_diff is None:
+            first_diff = s
+            l2, mx = errors()
+            history.append([s, l2, mx, ctx[1].n, ctx[0].n])
+        if s in checkpoints:
+            l2, mx = errors()
+            history.append([s, l2, mx, ctx[1].n, ctx[0].n])
+            print(f"bench: error run at step {s}: L2 {l2:.3e}, max {mx:.3e}, parcels "
+                  f"{ctx[1].n} / {ctx[0].n}", file=sys.stderr, flush=True)
+    l2, mx = history[-1][1], history[-1][2]
+    out = {"rel_velocity_error_l2": l2, "rel_velocity_error_max": mx, "steps": args.err_steps,
+           "grid": [n, n, nz], "parcels_fp32": ctx[1].n, "parcels_fp64": ctx[0].n,
+           "first_step_counts_differ": first_diff,
+           "history": {"columns": ["step", "l2", "max", "parcels_fp32", "parcels_fp64"],
+                       "rows": history},
+           "ok": bool(l2 <= 1e-5)}
+    for c in ctx.values():
+        c.close()
+    return out
 
 
 def ensemble(args):
