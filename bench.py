@@ -55,6 +55,10 @@ def parse():
     ap.add_argument("--n", type=int, default=256, help="cells per horizontal direction")
     ap.add_argument("--nz", type=int, default=128)
     ap.add_argument("--ppc", type=int, default=2, help="parcels per cell per direction")
+    ap.add_argument("--n2", type=float, default=1.0,
+                    help="buoyancy frequency squared of the stable layer (4 for the large run)")
+    ap.add_argument("--cap-factor", type=float, default=2.0,
+                    help="parcel capacity as a multiple of the initial parcel count")
     ap.add_argument("--cpu-steps", type=int, default=1,
                     help="steps of the workload the CPU baseline times")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -171,9 +175,10 @@ def time_form(precision, ic, args, n, nz):
     from pmpic_amd.lib import Context
     lx, lz = ic["_lx"], ic["_lz"]
     npar = len(ic["x"])
-    ctx = Context(n, n, nz, 0.0, lx, 0.0, lx, 0.0, lz, 2 * npar, dtm=0.01, dtmax=0.01,
-                  timestep_prefactor=0.5, precision=precision)
+    ctx = Context(n, n, nz, 0.0, lx, 0.0, lx, 0.0, lz, int(args.cap_factor * npar), dtm=0.01,
+                  dtmax=0.01, timestep_prefactor=0.5, precision=precision)
     ctx.set_parcels(ic)
+    sums0 = ctx.conservation_sums()
     tag = "fp32-bulk" if precision else "fp64"
     for s in range(args.warmup):
         ctx.step(1)
@@ -190,6 +195,11 @@ def time_form(precision, ic, args, n, nz):
     print(f"bench[{tag}]: {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
     p2g_ms, p2g_calls = ctx.timing_read(0)
     rk_ms, rk_calls = ctx.timing_read(1)
+    # parcel count and conservation drift over the warm-up and timed steps
+    sums1 = ctx.conservation_sums()
+    drift = {"steps": args.warmup + args.steps, "parcels_start": npar, "parcels_end": ctx.n,
+             "rel_drift": {k: (sums1[i] - sums0[i]) / sums0[i]
+                           for i, k in enumerate(("vol", "vol_b", "vol_h"))}}
     # the component timers (reference region names) from two further, untimed steps
     ctx.timing_enable(True)
     ctx.step(2)
@@ -232,6 +242,7 @@ def time_form(precision, ic, args, n, nz):
         "roofline": roof(dominant),
         "roofline_other": roof(other),
         "timers": timers,
+        "conservation": drift,
     }
 
 
@@ -360,7 +371,7 @@ def main():
     n, nz, ppcd = args.n, args.nz, args.ppc
     lx, lz = fortran_real("6.28318531"), fortran_real("3.14159265")
     t_ic = time.perf_counter()
-    ic = thermal(n, n, nz, lx, lx, lz, ppcd, seed=0x504D504943000000 + 1)
+    ic = thermal(n, n, nz, lx, lx, lz, ppcd, seed=0x504D504943000000 + 1, n2=args.n2)
     ic["_lx"], ic["_lz"] = lx, lz
     npar = len(ic["x"])
     print(f"bench: initial condition {npar} parcels in {time.perf_counter() - t_ic:.1f} s",
@@ -385,17 +396,20 @@ def main():
         "dtype": "f32" if "fp32" in res else "f64",
         "data": "synthetic (seeded jittered-lattice moist thermal)",
         "config": {"workload": f"moist thermal {n}x{n}x{nz} cells, {ppcd ** 3} parcels/cell "
-                               f"({npar} parcels), {form}, dt=0.01, split/mix on",
-                   "parcels": npar, "grid": [n, n, nz], "form": form},
+                               f"({npar} parcels), N2={args.n2:g}, {form}, dt=0.01, "
+                               f"split/mix on",
+                   "parcels": npar, "grid": [n, n, nz], "n2": args.n2, "form": form},
         "roofline": head["roofline"],
         "roofline_other": head["roofline_other"],
         "timers": head["timers"],
+        "conservation": head["conservation"],
     }
     if "fp64" in res:
         out["fp64"] = {"value": res["fp64"]["value"], "ms_per_step": res["fp64"]["ms_per_step"],
                        "roofline": res["fp64"]["roofline"],
                        "roofline_other": res["fp64"]["roofline_other"],
-                       "timers": res["fp64"]["timers"]}
+                       "timers": res["fp64"]["timers"],
+                       "conservation": res["fp64"]["conservation"]}
     if "fp32" in res and "fp64" in res:
         out["fp32_speedup_vs_fp64"] = res["fp32"]["value"] / res["fp64"]["value"]
     # the measured ceilings beside the 8 TB/s peak: the runtime's device copy and a STREAM triad

@@ -231,9 +231,12 @@ __global__ void k32_bin_reset(int32_t *__restrict__ count, int64_t nc1, DevScala
   if (i == 0) dsc->novf = 0;
 }
 
+// capc = 0 (no slot table): every parcel's rank within its cell goes to rankv instead, and
+// k32_dense_scatter lays all cells out densely after the scan
 __global__ void k32_bin_slot(Geometry g, const int32_t *__restrict__ cellv, int64_t n, int capc,
                              int32_t *__restrict__ slot, int2 *__restrict__ ovf,
-                             int32_t *__restrict__ count, DevScalars *dsc) {
+                             int32_t *__restrict__ rankv, int32_t *__restrict__ count,
+                             DevScalars *dsc) {
   // parcel indices fit 32 bits (capacity < 2^30)
   const int lane = threadIdx.x & 63;
   const int32_t w0 = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64 * BIN_SEQ;
@@ -259,6 +262,16 @@ __global__ void k32_bin_slot(Geometry g, const int32_t *__restrict__ cellv, int6
     const int next = above ? (__ffsll(above) - 1) : 64;
     base[it] = (head && c >= 0) ? atomicAdd(&count[c], next - hl) : 0;
     ch[it] = (int)((uint32_t)c << 6) | hl;
+  }
+  if (capc == 0) {
+#pragma unroll
+    for (int it = 0; it < BIN_SEQ; it++) {
+      const int32_t p = w0 + it * 64 + lane;
+      const int hl = ch[it] & 63;
+      const int32_t rk = __shfl(base[it], hl) + (lane - hl);
+      if (p < n32) st(rankv, (uint32_t)p, rk);
+    }
+    return;
   }
   bool anyover = false;
 #pragma unroll
@@ -313,6 +326,16 @@ __global__ void k32_ovf_scatter(Geometry g, const int32_t *__restrict__ cellv,
   }
 }
 
+// without a slot table: parcel p at its cell's dense offset plus its rank
+__global__ void k32_dense_scatter(Geometry g, const int32_t *__restrict__ cellv,
+                                  const int32_t *__restrict__ rankv, int64_t n,
+                                  const int32_t *__restrict__ start, int32_t *__restrict__ sorted) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int32_t c = linear_cell(g, ld(cellv, (uint32_t)p));
+  st(sorted, (uint32_t)(start[c] + ld(rankv, (uint32_t)p)), (int32_t)p);
+}
+
 __global__ void k32_sort_big(const int32_t *__restrict__ start, int64_t ncell, int rows,
                              int32_t *__restrict__ sorted) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -348,9 +371,35 @@ struct Zmap {
   float zbot, dz;  // z of a parcel = zbot + (k + oz) dz
 };
 
+// One parcel's weights at the two x-corners (bx = 0, 1) of the corner pair (by, bz), in the
+// form's one order of operations (every accumulation kernel and path uses it, so a cell's sums
+// do not depend on which of them ran): the y and z factors are s * offset + t, the offset for an
+// upper corner and 1 - offset for a lower one.
+__device__ __forceinline__ void pair_weights(float ox, float oy, float oz, float vol, float sy,
+                                             float ty, float sz, float tz, float &w0, float &w1) {
+  const float fy = fmaf(sy, oy, ty);
+  const float fz = fmaf(sz, oz, tz);
+  const float wv = (fz * fy) * vol;
+  w1 = wv * ox;
+  w0 = wv * (1.f - ox);
+}
+
+// acc[0] += w, acc[1 + f] += w * v[f], fused
+template <int NV>
+__device__ __forceinline__ void add_parcel(float *acc, float w, const float *v) {
+  acc[0] += w;
+#pragma unroll
+  for (int f = 0; f < NV; f++) acc[f + 1] = fmaf(w, v[f], acc[f + 1]);
+}
+
 // CAP: the binning's slot capacity (any value when capc = 0); ROWS: LDS rows of a pass
+// The kernel is bound by memory latency, so waves in flight set its speed: with the 16-entry
+// table the LDS rows allow 16 workgroups (32 waves) per CU, and holding the registers to 72
+// allows 7 waves per SIMD (28 per CU) instead of 5 (4-5 % faster at 8 parcels per cell).  The
+// 32-entry form is bound by its LDS rows (8 workgroups per CU) instead.
 template <int MODE, int CAP, int ROWS>
-__global__ void __launch_bounds__(ATPB) k32_cell_accum(Bulk32 b, int64_t cap, Geometry g, Zmap zm,
+__global__ void __launch_bounds__(ATPB)
+__attribute__((amdgpu_waves_per_eu(CAP == CAPC_LOW ? 7 : 1))) k32_cell_accum(Bulk32 b, int64_t cap, Geometry g, Zmap zm,
                                                        float vmin, const int32_t *__restrict__ count,
                                                        const int32_t *__restrict__ ostart,
                                                        const int32_t *__restrict__ sorted,
@@ -380,10 +429,10 @@ __global__ void __launch_bounds__(ATPB) k32_cell_accum(Bulk32 b, int64_t cap, Ge
   const int q = tid & 7;
   const int bx = q & 1, by = (q >> 1) & 1, bz = (q >> 2) & 1;
   // corner factor f = s * offset + t: offset for the upper corner, 1 - offset for the lower
-  const float sx = bx ? 1.f : -1.f, tx = bx ? 0.f : 1.f;
   const float sy = by ? 1.f : -1.f, ty = by ? 0.f : 1.f;
   const float sz = bz ? 1.f : -1.f, tz = bz ? 0.f : 1.f;
-  const int64_t c0 = (int64_t)blockIdx.x * TILE32;
+  const uint32_t tile = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t c0 = (int64_t)tile * TILE32;
   const int ntile = (int)min((int64_t)TILE32, ncell - c0);
   const int capc = b.capc;
   int32_t sl[SPT];
@@ -526,22 +575,17 @@ __global__ void __launch_bounds__(ATPB) k32_cell_accum(Bulk32 b, int64_t cap, Ge
       const int r = sperm[a];
       const float4 gr = sgeo[r];
       const float4 v = sva[r];
-      const float fx = fmaf(sx, gr.x, tx);
-      const float fy = fmaf(sy, gr.y, ty);
-      const float fz = fmaf(sz, gr.z, tz);
-      const float w = fz * fy * fx * gr.w;
+      float w0, w1;
+      pair_weights(gr.x, gr.y, gr.z, gr.w, sy, ty, sz, tz, w0, w1);
+      const float w = bx ? w1 : w0;
       float vals[5] = {v.x, v.y, v.z, v.w, 0.f};
       if constexpr (FIVE) vals[4] = svb[r];
       if (!MIX || skeep[r]) {
-        acc[0] += w;
-#pragma unroll
-        for (int f = 0; f < NV; f++) acc[f + 1] += w * vals[f];
+        add_parcel<NV>(acc, w, vals);
         kept++;
       } else {
         nres++;
-        res[0] += w;
-#pragma unroll
-        for (int f = 0; f < NV; f++) res[f + 1] += w * vals[f];
+        add_parcel<NV>(res, w, vals);
       }
     }
     if (fits == false && cl == first) big = true;
@@ -573,20 +617,15 @@ __global__ void __launch_bounds__(ATPB) k32_cell_accum(Bulk32 b, int64_t cap, Ge
         vals[3] = F[B_Q * cap + pp];
         vals[4] = F[B_R * cap + pp];
       }
-      const float fx = bx ? ox : 1 - ox;
-      const float fy = by ? oy : 1 - oy;
-      const float fz = bz ? oz : 1 - oz;
-      const float w = fz * fy * fx * vol;
+      float w0, w1;
+      pair_weights(ox, oy, oz, vol, sy, ty, sz, tz, w0, w1);
+      const float w = bx ? w1 : w0;
       if (!MIX || !(vol < vmin)) {
-        acc[0] += w;
-#pragma unroll
-        for (int f = 0; f < NV; f++) acc[f + 1] += w * vals[f];
+        add_parcel<NV>(acc, w, vals);
         kept++;
       } else {
         nres++;
-        res[0] += w;
-#pragma unroll
-        for (int f = 0; f < NV; f++) res[f + 1] += w * vals[f];
+        add_parcel<NV>(res, w, vals);
       }
     }
   }
@@ -595,7 +634,7 @@ __global__ void __launch_bounds__(ATPB) k32_cell_accum(Bulk32 b, int64_t cap, Ge
   bool anyres = false;
   if constexpr (MIX) {
     anyres = __syncthreads_or(live && nres > 0);
-    if (tid == 0) tileres[blockIdx.x] = anyres ? 1 : 0;
+    if (tid == 0) tileres[tile] = anyres ? 1 : 0;
   }
   if (!live) return;
 #pragma unroll
@@ -1048,11 +1087,13 @@ int bin32(Ctx *c, int64_t n) {
   else if (b.capfix) b.capc = b.capfix;
   else if (b.captuned) b.capc = b.captuned;
   else b.capc = (n <= (int64_t)LOW_PPC * c->ncell) ? CAPC_LOW : CAPC;
+  if (b.capc > b.capmax) b.capc = b.capmax;
   hipLaunchKernelGGL(k32_bin_reset, dim3(cdiv(nc1, TPB)), dim3(TPB), 0, c->stream, c->cell_count,
                      nc1, c->dsc);
   if (n > 0)
     hipLaunchKernelGGL(k32_bin_slot, dim3(cdiv(n, (int64_t)TPB * BIN_SEQ)), dim3(TPB), 0,
-                       c->stream, c->g, b.cell, n, b.capc, b.slot, b.ovf, c->cell_count, c->dsc);
+                       c->stream, c->g, b.cell, n, b.capc, b.slot, b.ovf, c->rank,
+                       c->cell_count, c->dsc);
   // dense-region offsets of the cells past the slot capacity
   auto it = rocprim::make_transform_iterator(static_cast<const int32_t *>(c->cell_count),
                                              OvfCount{b.capc});
@@ -1068,7 +1109,10 @@ int bin32(Ctx *c, int64_t n) {
   size_t have = c->scan_tmp_bytes;
   PMPIC_CHECK(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp, have, it, c->cell_start, (int)nc1,
                                                c->stream));
-  if (n > 0) {
+  if (n > 0 && b.capc == 0) {
+    hipLaunchKernelGGL(k32_dense_scatter, dim3(cdiv(n, TPB)), dim3(TPB), 0, c->stream, c->g,
+                       b.cell, c->rank, n, c->cell_start, c->sorted);
+  } else if (n > 0) {
     const int64_t nb = std::min<int64_t>(cdiv(n, TPB), 1024);
     hipLaunchKernelGGL(k32_ovf_scatter, dim3(nb), dim3(TPB), 0, c->stream, c->g, b.cell, b.ovf,
                        c->dsc, c->cell_start, b.slot, b.capc, c->sorted);
@@ -1104,13 +1148,16 @@ int b32_alloc(Ctx *c) {
   // the table (every cell then binned through the dense region, as grids beyond that range
   // are); 16 or 32 fixes the capacity instead of choosing it by the mean parcels per cell;
   // "tune" lets the first par2grid time the accumulation at both and keep the faster
-  b.capmax = (c->ncell * CAPC < ((int64_t)1 << 30)) ? CAPC : 0;
+  // (a table of 16 entries per cell reaches 2^30 entries at 512 x 512 x 256 cells)
+  b.capmax = (c->ncell * CAPC <= ((int64_t)1 << 30))       ? CAPC
+             : (c->ncell * CAPC_LOW <= ((int64_t)1 << 30)) ? CAPC_LOW
+                                                            : 0;
   b.capfix = 0;
   b.captune = false;
   b.captuned = 0;
   const char *env = getenv("PMPIC_SLOT_CAPACITY");
   if (env && strcmp(env, "tune") == 0) {
-    b.captune = b.capmax > 0;
+    b.captune = b.capmax == CAPC;
   } else if (env) {
     const int v = atoi(env);
     if (v == 0) b.capmax = 0;
@@ -1118,7 +1165,7 @@ int b32_alloc(Ctx *c) {
   }
   b.capc = b.capmax;
   b.slot = nullptr;
-  if (b.capmax > 0) PMPIC_TRY(dalloc32(&b.slot, c->ncell * CAPC));
+  if (b.capmax > 0) PMPIC_TRY(dalloc32(&b.slot, c->ncell * b.capmax));
   PMPIC_TRY(dalloc32(&b.ovf, c->cap));
   return PMPIC_OK;
 }
@@ -1136,6 +1183,7 @@ void b32_free(Ctx *c) {
 }
 
 int b32_set_parcels(Ctx *c, int64_t n, const double *const *fields) {
+  PMPIC_TRY(ensure_fbuf(c));
   Bulk32 &b = c->b32;
   const dim3 gb(cdiv(n > 0 ? n : 1, TPB)), tb(TPB);
   for (int a = 0; a < 3; a++) {
@@ -1163,6 +1211,7 @@ int b32_set_parcels(Ctx *c, int64_t n, const double *const *fields) {
 }
 
 int b32_get_parcels(Ctx *c, double *const *fields) {
+  PMPIC_TRY(ensure_fbuf(c));
   Bulk32 &b = c->b32;
   const int64_t n = c->n;
   if (n == 0) return PMPIC_OK;

@@ -42,6 +42,16 @@ int timing_harvest(Ctx *c) {
   return PMPIC_OK;
 }
 
+// the device staging buffer of host-array transfers, allocated at its first use: a parcel field
+// or a parcel array followed by a grid (max(cap, gsz) + max(gsz, ssz) doubles)
+int ensure_fbuf(Ctx *c) {
+  if (c->fbuf) return PMPIC_OK;
+  const int64_t a = c->cap > c->gsz ? c->cap : c->gsz;
+  const int64_t b = c->gsz > c->ssz ? c->gsz : c->ssz;
+  PMPIC_CHECK(hipMalloc((void **)&c->fbuf, sizeof(double) * (size_t)(a + b)));
+  return PMPIC_OK;
+}
+
 // status raised by kernels since the last check; clears it
 int take_status(Ctx *c) {
   int st = sync_scalars(c);
@@ -193,7 +203,7 @@ int free_all(Ctx *c) {
   hipFree(c->ubar); hipFree(c->vbar);
   hipFree(c->kx); hipFree(c->ky); hipFree(c->k2); hipFree(c->filt); hipFree(c->coords);
   hipFree(c->dz_den); hipFree(c->dz_cp);
-  hipFree(c->fbuf); hipFree(c->ibuf);
+  hipFree(c->fbuf);
   hipFree(c->cbuf); hipFree(c->tw_x); hipFree(c->tw_y);
   hipFree(c->scan_tmp); hipFree(c->blk); hipFree(c->maxblk); hipFree(c->dsc);
   for (int w = 0; w < NTIMERS; w++)
@@ -251,8 +261,10 @@ int create(const pmpic_config *cf, Ctx *c) {
   PMPIC_TRY(dalloc(&c->list_b, c->cap));
   PMPIC_TRY(dalloc(&c->cell_count, c->ncell + 1));
   PMPIC_TRY(dalloc(&c->cell_start, c->ncell + 1));
-  PMPIC_TRY(dalloc(&c->partial, 48 * c->ncell));
-  PMPIC_TRY(dalloc(&c->partial2, 48 * c->ncell));
+  // corner partial sums: 8 corners x up to 6 values per cell, FP64 or FP32 by the form
+  const int64_t pdoubles = (c->precision == 1 ? 24 : 48) * c->ncell;
+  PMPIC_TRY(dalloc(&c->partial, pdoubles));
+  if (c->precision == 1) PMPIC_TRY(dalloc(&c->partial2, pdoubles));
   PMPIC_TRY(dalloc(&c->npercell, c->ncell));
   PMPIC_TRY(dalloc(&c->cellflag, c->ncell));
   for (int i = 0; i < NGRIDS; i++) PMPIC_TRY(dalloc(&c->grid[i], c->gsz));
@@ -277,10 +289,6 @@ int create(const pmpic_config *cf, Ctx *c) {
   PMPIC_TRY(dalloc(&c->coords, (int64_t)coords.size()));
   PMPIC_CHECK(hipMemcpy(c->coords, coords.data(), sizeof(double) * coords.size(),
                         hipMemcpyHostToDevice));
-  int64_t fb = c->gsz > c->ssz ? c->gsz : c->ssz;
-  if (c->cap > fb) fb = c->cap;
-  PMPIC_TRY(dalloc(&c->fbuf, 2 * fb));
-  PMPIC_TRY(dalloc(&c->ibuf, c->cap));
   PMPIC_TRY(dalloc(&c->cbuf, 8 * c->csz));
   c->nblk_max = cdiv(big, 256) + 1;
   PMPIC_TRY(dalloc(&c->blk, 20 * c->nblk_max + 64));
@@ -296,6 +304,18 @@ int create(const pmpic_config *cf, Ctx *c) {
   PMPIC_TRY(fft_setup(c));
   PMPIC_TRY(spectral_setup(c));
   PMPIC_TRY(set_dev_scalars(c));
+  // The FP64 form's residual partials of mixing get their own buffer when the device still has
+  // room for it (and a margin for the haloswap staging and scan storage allocated later);
+  // otherwise, or with PMPIC_LEAN_MIXING=1, mixing sums the kept and the residual parcels in two
+  // passes over one buffer, with the same sums.
+  if (c->precision == 0) {
+    const char *lean = getenv("PMPIC_LEAN_MIXING");
+    size_t fr = 0, tot = 0;
+    PMPIC_CHECK(hipMemGetInfo(&fr, &tot));
+    const size_t want = sizeof(double) * (size_t)pdoubles;
+    if (!(lean && lean[0] == '1') && fr > want + ((size_t)8 << 30))
+      PMPIC_TRY(dalloc(&c->partial2, pdoubles));
+  }
   return PMPIC_OK;
 }
 
@@ -600,6 +620,7 @@ int pmpic_cache_parcel_interp_weights(pmpic_ctx *ctx, int32_t *is, int32_t *js, 
 int pmpic_par2grid(pmpic_ctx *ctx, const double *var, double *grid) {
   if (!ctx || !var || !grid || ctx->c.precision != 0) return PMPIC_ERR_ARG;
   Ctx *c = &ctx->c;
+  PMPIC_TRY(ensure_fbuf(c));
   if (c->n > 0)
     PMPIC_CHECK(hipMemcpyAsync(c->fbuf, var, sizeof(double) * c->n, hipMemcpyHostToDevice,
                                c->stream));
@@ -615,6 +636,7 @@ int pmpic_par2grid(pmpic_ctx *ctx, const double *var, double *grid) {
 int pmpic_grid2par(pmpic_ctx *ctx, const double *grid, double *var) {
   if (!ctx || !var || !grid || ctx->c.precision != 0) return PMPIC_ERR_ARG;
   Ctx *c = &ctx->c;
+  PMPIC_TRY(ensure_fbuf(c));
   std::vector<double> h;
   grid_to_device_layout(c->g, grid, h);
   double *dg = c->fbuf + (c->cap > c->gsz ? c->cap : c->gsz);

@@ -113,8 +113,9 @@ struct Ctx {
   // cells
   int64_t ncell;
   int32_t *cell_count, *cell_start;   // ncell + 1
-  double *partial;                    // 48 * ncell corner partial sums
-  double *partial2;                   // 48 * ncell (mixing residual pass)
+  double *partial;                    // corner partial sums: 48 * ncell doubles (FP32 form: 24)
+  double *partial2;                   // mixing residual partials, same size; may be null in the
+                                      // FP64 form (then mixing takes two passes over partial)
   int32_t *npercell;                  // ncell
   uint8_t *cellflag;                  // ncell
   // grids (interior [nzn][ny][nx])
@@ -131,8 +132,7 @@ struct Ctx {
   // Thomas elimination: diffz's denominators and c' [bt][k] (the same for every column)
   double *dz_den, *dz_cp;
   double *coords;                     // x_coords (nx+4), y_coords (ny+4), z_coords (nzn)
-  double *fbuf;                       // host-array staging for standalone ops (max(gsz, ssz, cap))
-  int32_t *ibuf;                      // cap int32 staging
+  double *fbuf;                       // host-array staging, allocated at first use (ensure_fbuf)
   double2 *cbuf;                      // complex FFT intermediates, 8 fields
   double2 *tw_x, *tw_y;               // twiddles W_n^e, e in [0,n)
   // scan scratch
@@ -167,6 +167,26 @@ struct Ctx {
   } while (0)
 
 static inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// XCD-aware block order.  MI355X dispatches consecutive workgroups round-robin over its 8
+// XCDs, each with its own L2.  This maps the workgroup to the piece of work it takes so that
+// every XCD gets one contiguous run of pieces: neighbouring pieces, which share cache lines
+// (the ends of parcel runs, gathered grid nodes, partial-sum lines), meet in the same L2.  A
+// bijection of [0, nb).
+#ifndef PMPIC_XCD_SWIZZLE
+#define PMPIC_XCD_SWIZZLE 1
+#endif
+constexpr uint32_t NXCD = 8;
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+#if PMPIC_XCD_SWIZZLE
+  const uint32_t per = nb / NXCD, rem = nb % NXCD;
+  const uint32_t x = b % NXCD, i = b / NXCD;
+  return x * per + (x < rem ? x : rem) + i;
+#else
+  (void)nb;
+  return b;
+#endif
+}
 
 __device__ __forceinline__ void raise_status(DevScalars *s, int code) {
   atomicCAS(&s->status, 0, code);
@@ -319,6 +339,7 @@ int scan_exclusive_i32(Ctx *c, const int32_t *in, int32_t *out, int64_t n);
 int sync_scalars(Ctx *c);
 // capi.hip: status raised by kernels since the last check, cleared once read
 int take_status(Ctx *c);
+int ensure_fbuf(Ctx *c);  // allocates the host-transfer staging buffer at its first use
 
 }  // namespace pmpic
 

@@ -319,12 +319,17 @@ __global__ void __launch_bounds__(256) k_cell_accum_lds(Geometry g, const double
       }
     }
   }
+  // MIX: either output may be null (the two-pass form of mixing_grids); the sums are the same
+  if (!MIX || partial) {
 #pragma unroll
-  for (int f = 0; f < NF; f++) partial[((int64_t)q * ncell + c) * NF + f] = acc[f];
+    for (int f = 0; f < NF; f++) partial[((int64_t)q * ncell + c) * NF + f] = acc[f];
+  }
   if constexpr (MIX) {
+    if (partial2) {
 #pragma unroll
-    for (int f = 0; f < NF; f++) partial2[((int64_t)q * ncell + c) * NF + f] = res[f];
-    if (q == 0) npercell[c] = kept;
+      for (int f = 0; f < NF; f++) partial2[((int64_t)q * ncell + c) * NF + f] = res[f];
+    }
+    if (npercell && q == 0) npercell[c] = kept;
   }
 }
 
@@ -593,18 +598,29 @@ int grid2par_single(Ctx *c, int64_t n, const double *d_grid, double *d_var) {
 // the fold/fill of :133-138): grid[GVOL], GB, GHG, GP, GQ, GR hold the kept grids with the
 // z-boundary doubling of :141-152; kres[0..5] hold the residual grids before the new-parcel
 // correction of :202-207 and before their own doubling (:229-240).
+// Without a residual buffer (partial2 null: a context too large for both) the kept and the
+// residual sums take one pass each over `partial`, with the same sums.
 int mixing_grids(Ctx *c, int64_t n) {
   PMPIC_TRY(bin_parcels(c, n));
   const int64_t ncell = c->ncell;
-  launch_accum_lds<ACC_MIX_KEEP, 5, ROWS_MIX>(c, nullptr, c->partial, c->partial2, c->npercell);
   OutTable kept = {{c->grid[GVOL], c->grid[GB], c->grid[GHG], c->grid[GP], c->grid[GQ],
                     c->grid[GR]}};
   OutTable res = {{c->kres[0], c->kres[1], c->kres[2], c->kres[3], c->kres[4], c->kres[5]}};
   const int64_t nnode = (int64_t)c->g.nzn * c->g.ny * c->g.nx;
-  hipLaunchKernelGGL((k_node_sum<5, false>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0, c->stream,
-                     c->g, c->partial, ncell, kept);
-  hipLaunchKernelGGL((k_node_sum<5, false, false>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0,
-                     c->stream, c->g, c->partial2, ncell, res);
+  if (c->partial2) {
+    launch_accum_lds<ACC_MIX_KEEP, 5, ROWS_MIX>(c, nullptr, c->partial, c->partial2, c->npercell);
+    hipLaunchKernelGGL((k_node_sum<5, false>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0, c->stream,
+                       c->g, c->partial, ncell, kept);
+    hipLaunchKernelGGL((k_node_sum<5, false, false>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0,
+                       c->stream, c->g, c->partial2, ncell, res);
+  } else {
+    launch_accum_lds<ACC_MIX_KEEP, 5, ROWS_MIX>(c, nullptr, c->partial, nullptr, c->npercell);
+    hipLaunchKernelGGL((k_node_sum<5, false>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0, c->stream,
+                       c->g, c->partial, ncell, kept);
+    launch_accum_lds<ACC_MIX_KEEP, 5, ROWS_MIX>(c, nullptr, nullptr, c->partial, nullptr);
+    hipLaunchKernelGGL((k_node_sum<5, false, false>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0,
+                       c->stream, c->g, c->partial, ncell, res);
+  }
   PMPIC_CHECK(hipGetLastError());
   return PMPIC_OK;
 }

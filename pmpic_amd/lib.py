@@ -196,7 +196,15 @@ class Context:
     # ---- state ----
     def set_parcels(self, fields):
         n = len(fields["x"])
-        arrs = [np.ascontiguousarray(fields.get(k, np.zeros(n)), dtype=np.float64) for k in PARCEL_FIELDS]
+        zero = None  # one shared array of zeros for every field not given
+        arrs = []
+        for k in PARCEL_FIELDS:
+            if k in fields:
+                arrs.append(np.ascontiguousarray(fields[k], dtype=np.float64))
+            else:
+                if zero is None:
+                    zero = np.zeros(n)
+                arrs.append(zero)
         self._chk(self.L.pmpic_set_parcels(self.h, n, _ptrs(arrs)), "pmpic_set_parcels")
 
     def get_parcels(self, names=None):

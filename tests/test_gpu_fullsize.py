@@ -148,3 +148,38 @@ def test_fp32_bulk_close_to_fp64(thermal128, workload):
     g32.close()
     # the FP32-bulk form holds vol in FP32: its volume sum within FP32 rounding of the FP64 form's
     assert abs(s32[0] - s64[0]) <= 1e-6 * abs(s64[0])
+
+
+def test_large_run_512x512x256_strong_stratification():
+    """The large single run: 512x512x256 cells with 8 parcels per cell (536,870,912 parcels) and
+    N^2 = 4, capacity 1.25 N_p.  The FP32-bulk form (every cell in the dense binning: a slot
+    table of 32 entries per cell would need offsets past 32 bits) and then the FP64 form (about
+    250 GB of HBM) run two steps each from the same state: the count follows the
+    split/removed/added counts, the FP64 sums
+    of vol, vol*b and vol*h drift by at most 1e-12 relative (the FP32-bulk form's by its FP32
+    rounding, 1e-6), the counts agree and u, v, w agree to 1e-5 (relative L2)."""
+    from pmpic_amd.config import fortran_real
+    from pmpic_amd.initial import thermal
+    lx, lz = fortran_real("6.28318531"), fortran_real("3.14159265")
+    n, nz = 512, 256
+    ic = thermal(n, n, nz, lx, lx, lz, 2, seed=SEED + 4, n2=4.0)
+    npar = len(ic["x"])
+    assert npar == 536870912
+    grids, counts = {}, {}
+    for precision in (1, 0):
+        from pmpic_amd.lib import Context
+        g = Context(n, n, nz, 0.0, lx, 0.0, lx, 0.0, lz, npar + npar // 4, dtm=0.01, dtmax=0.01,
+                    timestep_prefactor=0.5, precision=precision)
+        g.set_parcels(ic)
+        s0, s1, _ = _sums_and_count_after(g, 2)
+        tol = 1e-6 if precision else 1e-12
+        for k in range(3):  # vol, vol*b, vol*h
+            assert abs(s1[k] - s0[k]) <= tol * abs(s0[k]), (precision, k)
+        counts[precision] = (g.n, g.last_counts())
+        grids[precision] = [g.grid(k)[2:-2, 2:-2, :].copy() for k in ("u", "v", "w")]
+        g.close()
+    assert counts[1] == counts[0]
+    num = sum(float(np.sum((a - b) ** 2)) for a, b in zip(grids[1], grids[0]))
+    den = sum(float(np.sum(b ** 2)) for b in grids[0])
+    assert np.sqrt(num / den) < 1e-5
+

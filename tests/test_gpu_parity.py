@@ -257,3 +257,38 @@ def test_adaptive_timestep_steps_match_oracle(oracle_mod, vort, bar):
     gp, op = g.get_parcels(), o.get_parcels()
     for k in ("x", "y", "z", "p", "q", "r", "vol", "b"):
         bar(k, rel_err(gp[k], op[k]), 1e-12)
+
+
+def test_lean_mixing_bitwise(oracle_mod, monkeypatch):
+    """A context without room for mixing's residual partial buffer (forced here with
+    PMPIC_LEAN_MIXING=1) sums the kept and the residual parcels in two passes over one buffer:
+    every parcel field bit-identical to the one-pass form over steps with splitting and mixing,
+    and both equal to the oracle's counts."""
+    from pmpic_amd.lib import Context
+    shape = (12, 10, 8)
+    ic = thermal_case(*shape, 2, vort=3.0, shrink=0.05, thin=0.1, near_split=0.05)
+    b = box(*shape)
+    cap = 3 * len(ic["x"]) + 64
+    ctx = {}
+    for lean in ("0", "1"):
+        monkeypatch.setenv("PMPIC_LEAN_MIXING", lean)
+        ctx[lean] = Context(*shape, b["xmin"], b["xmax"], b["ymin"], b["ymax"], b["zmin"],
+                            b["zmax"], cap)
+        ctx[lean].set_parcels(ic)
+    o = oracle_mod.Oracle(*shape, b["xmin"], b["xmax"], b["ymin"], b["ymax"], b["zmin"],
+                          b["zmax"], cap, 0.01, 0.01, 0.5)
+    o.set_parcels(ic)
+    mixed = 0
+    for step in range(3):
+        o.step()
+        for g in ctx.values():
+            g.step()
+            assert g.n == o.n and g.last_counts() == o.last_counts(), step
+        mixed += o.last_counts()[1] + o.last_counts()[2]
+    assert mixed > 0
+    a, c = ctx["0"].get_parcels(), ctx["1"].get_parcels()
+    for k in a:
+        np.testing.assert_array_equal(a[k], c[k])
+    for name in ("vol", "b", "hg", "p", "q", "r"):
+        np.testing.assert_array_equal(ctx["0"].grid(name), ctx["1"].grid(name))
+
