@@ -69,8 +69,9 @@ def parse():
                          "(0 = skip)")
     ap.add_argument("--ensemble", type=int, default=0, metavar="M",
                     help="instead of the single thermal: M FP32-bulk thermals on grids cycling "
-                         "32^3, 48^3, 64^3, 96^3 (8 parcels/cell, radius U(0.5,1), amplitude "
-                         "U(0.5,2), own seeds), timed stepped together and one after another")
+                         "32^3, 48^3, 64^3, 96^3 (8 or 27 parcels/cell, radius U(0.5,1), "
+                         "amplitude U(0.5,2), own seeds), timed stepped together and one after "
+                         "another")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (the reference kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -300,35 +301,25 @@ def velocity_error(args, ic, n, nz):
 
 
 def ensemble(args):
-    """Batched-ensemble figure: M members stepped concurrently (pmpic_step_concurrent) and the
-    same members stepped one after another, from the same initial states."""
-    import numpy as np
-    from pmpic_amd.config import fortran_real
-    from pmpic_amd.initial import thermal
-    from pmpic_amd.lib import Context, step_concurrent
-    lx, lz = fortran_real("6.28318531"), fortran_real("3.14159265")
-    rng = np.random.default_rng(0x504D5049)
-    sizes = [32, 48, 64, 96]
-    specs = []
-    for m in range(args.ensemble):
-        n = sizes[m % len(sizes)]
-        specs.append((n, float(rng.uniform(0.5, 1.0)), float(rng.uniform(0.5, 2.0))))
-    ics = [thermal(n, n, n, lx, lx, lz, 2, seed=0x504D504943000000 + 3 + m, radius=r,
-                   amplitude=a) for m, (n, r, a) in enumerate(specs)]
+    """Batched-ensemble figure: M FP32-bulk members (pmpic_amd/ensemble.py: grids cycling 32^3,
+    48^3, 64^3, 96^3 cells, 8 or 27 parcels per cell, own radius, amplitude and seed) stepped
+    concurrently (pmpic_step_concurrent) and the same members stepped one after another, from the
+    same initial states."""
+    from pmpic_amd import ensemble as ens
+    from pmpic_amd.lib import step_concurrent
+    specs = ens.member_specs(args.ensemble)
+    ics = [ens.member_parcels(s) for s in specs]
     total = sum(len(ic["x"]) for ic in ics)
     print(f"bench: ensemble of {len(ics)} members, {total} parcels", file=sys.stderr, flush=True)
     out = {}
     for mode in ("concurrent", "sequential"):
-        ctxs = []
-        for (n, _, _), ic in zip(specs, ics):
-            c = Context(n, n, n, 0.0, lx, 0.0, lx, 0.0, lz, 2 * len(ic["x"]), precision=1)
-            c.set_parcels(ic)
-            ctxs.append(c)
+        ctxs = [ens.member_context(s, ic) for s, ic in zip(specs, ics)]
         if mode == "concurrent":
             step_concurrent(ctxs, args.warmup)
         else:
             for c in ctxs:
                 c.step(args.warmup)
+        n0 = sum(c.n for c in ctxs)
         t0 = time.perf_counter()
         if mode == "concurrent":
             step_concurrent(ctxs, args.steps)
@@ -336,7 +327,8 @@ def ensemble(args):
             for c in ctxs:
                 c.step(args.steps)
         el = time.perf_counter() - t0
-        out[mode] = total * args.steps / el
+        n1 = sum(c.n for c in ctxs)
+        out[mode] = 0.5 * (n0 + n1) * args.steps / el
         print(f"bench: ensemble {mode}: {args.steps} steps in {el:.3f} s", file=sys.stderr,
               flush=True)
         for c in ctxs:
@@ -345,8 +337,9 @@ def ensemble(args):
         "metric": "ensemble parcel-steps/s (FP32-bulk)", "value": out["concurrent"],
         "unit": "parcel-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "higher_is_better": True, "dtype": "f32", "data": "synthetic (seeded moist thermals)",
-        "config": {"workload": f"{len(ics)} thermals on grids cycling 32^3..96^3, 8 parcels/cell",
-                   "members": [s[0] for s in specs], "parcels": total},
+        "config": {"workload": f"{len(ics)} thermals on grids cycling 32^3..96^3, 8 or 27 "
+                               f"parcels/cell",
+                   "members": [[s[0], s[1] ** 3] for s in specs], "parcels": total},
         "sequential_value": out["sequential"],
         "concurrency_speedup": out["concurrent"] / out["sequential"]}))
     return 0

@@ -588,6 +588,206 @@ __global__ void __launch_bounds__(64) k_diffz_reg(Geometry g, ColFields cf,
   for (int k = 0; k < NZ; k++) o[(int64_t)k * plane] = F[k];
 }
 
+// Long columns (more z nodes than the instances above): Q lanes per column, lane q holding
+// levels [q L, q L + L) of the column in registers, for any column length nz <= Q L chosen at
+// run time.  The sweeps run lane group after lane group as in k_laplinv_q4, the recurrence
+// state crossing by lane shuffles, every operation the one of laplinv_column / k_diffz_reg in
+// the same order, so results are bit-identical to them.
+template <int Q, int L>
+__global__ void __launch_bounds__(64) k_laplinv_qn(Geometry g, ColFields cf,
+                                                   const double *__restrict__ k2a,
+                                                   DevScalars *sc) {
+  constexpr int CPW = 64 / Q;  // columns per wavefront
+  const int NZ = g.nzn;
+  const int lane = threadIdx.x;
+  const int q = lane / CPW;
+  const int64_t plane = (int64_t)g.sy * g.sx;
+  const int64_t col = (int64_t)blockIdx.x * CPW + (lane % CPW);
+  if (col >= plane) return;  // all Q lanes of a column leave together
+  const double k2 = k2a[col];
+  if (k2 == 0) return;
+  const int df0 = cf.df0[blockIdx.y];
+  double *f = cf.out[blockIdx.y] + col;
+  double F[L], CPl[L];
+#pragma unroll
+  for (int i = 0; i < L; i++) {
+    F[i] = 0.0;
+    CPl[i] = 0.0;
+  }
+#pragma unroll
+  for (int i = 0; i < L; i++)
+    if (q * L + i < NZ) F[i] = f[(int64_t)(q * L + i) * plane];
+  const double f16 = (double)(1.f / 6.f), f13 = (double)(1.f / 3.f);
+  const double f112 = (double)(1.f / 12.f), f56 = (double)(5.f / 6.f);
+  const double dz = g.dz;
+  const double i2dz2 = 2. / dz / dz, idz2 = 1. / dz / dz;
+  const double ai = idz2 - f112 * k2, bi = -i2dz2 - f56 * k2, ci = idz2 - f112 * k2;
+  const double nxt0 = __shfl(F[0], lane + CPW);  // the original of the next group's first level
+  const int PTOP = (NZ - 1) / L;                 // group holding level NZ - 1
+  double denominator = 0.0, fprev = 0.0, cprev = 0.0, f0 = 0.0, fp = 0.0;
+  bool bad = false;
+  for (int p = 0; p <= PTOP; p++) {
+    if (p > 0) {
+      denominator = __shfl(denominator, lane - CPW);
+      fprev = __shfl(fprev, lane - CPW);
+      cprev = __shfl(cprev, lane - CPW);
+      f0 = __shfl(f0, lane - CPW);
+      fp = __shfl(fp, lane - CPW);
+    }
+    if (q == p) {
+#pragma unroll
+      for (int i = 0; i < L; i++) {
+        const int j = p * L + i;
+        if (j >= NZ) break;
+        const double onext = (i + 1 < L) ? F[i + 1 < L ? i + 1 : 0] : nxt0;
+        if (j == 0) {
+          f0 = F[0];
+          fp = onext;
+          double b1, c1, d1;
+          if (!df0) { b1 = 1.; c1 = 0.; d1 = 0.; }
+          else { b1 = -idz2 - f13 * k2; c1 = idz2 - f16 * k2; d1 = f13 * f0 + f16 * fp; }
+          denominator = b1;
+          fprev = d1 / denominator;
+          F[0] = fprev;
+          cprev = c1;
+          continue;
+        }
+        const double fm = f0;
+        f0 = fp;
+        fp = (j + 1 < NZ) ? onext : 0.0;
+        double a, b, c, d;
+        if (j + 1 < NZ) {
+          a = ai; b = bi; c = ci;
+          d = f112 * fm + f56 * f0 + f112 * fp;
+        } else if (!df0) {
+          a = 0.; b = 1.; c = 0.; d = 0.;
+        } else {
+          a = idz2 - f16 * k2; b = -idz2 - f13 * k2; c = 0.;
+          d = f13 * f0 + f16 * fm;
+        }
+        const double cpk = cprev / denominator;
+        CPl[i] = cpk;
+        denominator = b - a * cpk;
+        bad |= (denominator == 0);
+        fprev = (d - a * fprev) / (denominator);
+        F[i] = fprev;
+        cprev = c;
+      }
+    }
+  }
+  const double cpn = __shfl(CPl[0], lane + CPW);
+  double fnext = fprev;
+  for (int p = PTOP; p >= 0; p--) {
+    if (p < PTOP) fnext = __shfl(fnext, lane + CPW);
+    if (q == p) {
+#pragma unroll
+      for (int i = L - 1; i >= 0; i--) {
+        const int j = p * L + i;
+        if (j >= NZ - 1) continue;
+        const double cpj1 = (i + 1 < L) ? CPl[i + 1 < L ? i + 1 : 0] : cpn;
+        const double v = F[i] - cpj1 * fnext;
+        F[i] = v;
+        fnext = v;
+      }
+    }
+  }
+  if (bad) raise_status(sc, PMPIC_ERR_ZERO_PIVOT);
+#pragma unroll
+  for (int i = 0; i < L; i++)
+    if (q * L + i < NZ) f[(int64_t)(q * L + i) * plane] = F[i];
+}
+
+template <int Q, int L>
+__global__ void __launch_bounds__(64) k_diffz_qn(Geometry g, ColFields cf,
+                                                 const double *__restrict__ tden,
+                                                 const double *__restrict__ tcp,
+                                                 DevScalars *sc) {
+  constexpr int CPW = 64 / Q;
+  __shared__ double den[Q * L], cpt[Q * L];
+  const int NZ = g.nzn;
+  const int lane = threadIdx.x;
+  const int q = lane / CPW;
+  const int64_t plane = (int64_t)g.sy * g.sx;
+  const int64_t col = (int64_t)blockIdx.x * CPW + (lane % CPW);
+  const int fi = blockIdx.y;
+  const bool bt = cf.bot[fi] != nullptr && cf.top[fi] != nullptr;
+  for (int k = threadIdx.x; k < NZ; k += blockDim.x) {
+    den[k] = tden[(bt ? NZ : 0) + k];
+    cpt[k] = tcp[(bt ? NZ : 0) + k];
+  }
+  __syncthreads();
+  if (col >= plane) return;  // all Q lanes of a column leave together
+  const double *fin = cf.in[fi] + col;
+  double *o = cf.out[fi] + col;
+  double F[L];
+#pragma unroll
+  for (int i = 0; i < L; i++) F[i] = (q * L + i < NZ) ? fin[(int64_t)(q * L + i) * plane] : 0.0;
+  const double bot = bt ? cf.bot[fi][col] : 0.0, top = bt ? cf.top[fi][col] : 0.0;
+  const double a1 = (double)(1.f / 6.f), al = (double)(1.f / 3.f);
+  const double dz = g.dz;
+  const double nxt0 = __shfl(F[0], lane + CPW);  // the original of the next group's first level
+  const int PTOP = (NZ - 1) / L;
+  double fprev = 0.0, fkm2 = 0.0, fkm1 = 0.0;  // originals f(k-2), f(k-1) of the next row
+  bool bad = false;
+  for (int p = 0; p <= PTOP; p++) {
+    if (p > 0) {
+      fprev = __shfl(fprev, lane - CPW);
+      fkm2 = __shfl(fkm2, lane - CPW);
+      fkm1 = __shfl(fkm1, lane - CPW);
+    }
+    if (q == p) {
+#pragma unroll
+      for (int i = 0; i < L; i++) {
+        const int j = p * L + i;  // level written: row j + 1 of the reference
+        if (j >= NZ) break;
+        const double onext = (i + 1 < L) ? F[i + 1 < L ? i + 1 : 0] : nxt0;  // original j + 1
+        if (j == 0) {
+          fkm2 = F[0];
+          fkm1 = onext;
+          const double d0 = bt ? fkm1 / dz - a1 * bot * dz : 0.0;
+          bad |= (den[0] == 0);
+          fprev = d0 / den[0];
+          F[0] = fprev;
+          continue;
+        }
+        double a, d, fk = 0.0;
+        if (j + 1 < NZ) {
+          a = a1;
+          fk = onext;
+          d = (fk - fkm2) / 2. / dz;
+        } else {
+          a = bt ? al : 0.;
+          d = bt ? -fkm2 / dz + a1 * top * dz : 0.0;
+        }
+        const double dn = den[j];
+        bad |= (dn == 0);
+        fprev = (d - a * fprev) / (dn);
+        F[i] = fprev;
+        fkm2 = fkm1;
+        fkm1 = fk;
+      }
+    }
+  }
+  double fnext = fprev;
+  for (int p = PTOP; p >= 0; p--) {
+    if (p < PTOP) fnext = __shfl(fnext, lane + CPW);
+    if (q == p) {
+#pragma unroll
+      for (int i = L - 1; i >= 0; i--) {
+        const int j = p * L + i;
+        if (j >= NZ - 1) continue;
+        const double v = F[i] - cpt[j + 1] * fnext;
+        F[i] = v;
+        fnext = v;
+      }
+    }
+  }
+  if (bad) raise_status(sc, PMPIC_ERR_ZERO_PIVOT);
+#pragma unroll
+  for (int i = 0; i < L; i++)
+    if (q * L + i < NZ) o[(int64_t)(q * L + i) * plane] = F[i];
+}
+
 // u, v, w spectra (vort2vel.F90:251-320): fu = dzb - diffy(c), column (1,1) = ubar;
 // fv = diffx(c) - dza, column (1,1) = vbar; fw = diffy(a) - diffx(b); u_s.. = filtered
 __global__ void k_velocity_spectra(Geometry g, const double *__restrict__ a,
@@ -807,6 +1007,8 @@ static int lds_cols(const Geometry &g) {
 
 // column lengths (z nodes) with a register-resident solver instance
 #define PMPIC_REG_COLUMNS(X) X(9) X(17) X(33) X(65) X(129)
+// longer columns, up to LONG_Q * LONG_L z nodes: LONG_Q lanes per column, LONG_L levels each
+constexpr int LONG_Q = 8, LONG_L = 33;
 
 int op_diffz_n(Ctx *c, int nf, const double *const *in, double *const *out,
                const double *const *bot, const double *const *top) {
@@ -831,6 +1033,12 @@ int op_diffz_n(Ctx *c, int nf, const double *const *in, double *const *out,
 #undef PMPIC_DZ_CASE
     default:
       break;
+  }
+  if (c->g.nzn > 129 && c->g.nzn <= LONG_Q * LONG_L) {
+    hipLaunchKernelGGL((k_diffz_qn<LONG_Q, LONG_L>), dim3(cdiv(plane, 64 / LONG_Q), nf), dim3(64),
+                       0, c->stream, c->g, rf, c->dz_den, c->dz_cp, c->dsc);
+    PMPIC_CHECK(hipGetLastError());
+    return PMPIC_OK;
   }
   if (nc > 0) {
     ColFields cf{};
@@ -879,6 +1087,12 @@ int op_laplinv_n(Ctx *c, int nf, double *const *f, const int *df0) {
 #undef PMPIC_LAP_CASE
     default:
       break;
+  }
+  if (c->g.nzn > 129 && c->g.nzn <= LONG_Q * LONG_L) {
+    hipLaunchKernelGGL((k_laplinv_qn<LONG_Q, LONG_L>), dim3(cdiv(plane, 64 / LONG_Q), nf),
+                       dim3(64), 0, c->stream, c->g, rf, c->k2, c->dsc);
+    PMPIC_CHECK(hipGetLastError());
+    return PMPIC_OK;
   }
   if (nc > 0) {
     ColFields cf{};

@@ -1,0 +1,52 @@
+"""The batched-ensemble workload: many independent moist thermals on uneven grids.
+
+Member m of an ensemble of M takes the grid sizes[m % 4] cubed from (32, 48, 64, 96) cells, 8
+parcels per cell for the first half of each run of eight members and 27 for the second (a
+jittered 2x2x2 or 3x3x3 lattice), a thermal radius drawn from U(0.5, 1.0) and a buoyancy
+amplitude from U(0.5, 2.0) (numpy's PCG64 seeded with 0x504D5049, two draws per member in
+member order), and its own SplitMix64 seed 0x504D504943000000 + 3 + m for the lattice jitter and
+noise.  The box is the thermal's 2pi x 2pi x pi with the reals parsed the reference's way.
+"""
+import numpy as np
+
+from .config import fortran_real
+from .initial import SEED_BASE, thermal
+
+SIZES = (32, 48, 64, 96)
+
+
+def member_specs(count):
+    """[(n, ppc_dir, radius, amplitude, seed)] for an ensemble of `count` members."""
+    rng = np.random.default_rng(0x504D5049)
+    out = []
+    for m in range(count):
+        n = SIZES[m % len(SIZES)]
+        ppcd = 2 if (m // len(SIZES)) % 2 == 0 else 3
+        radius = float(rng.uniform(0.5, 1.0))
+        amplitude = float(rng.uniform(0.5, 2.0))
+        out.append((n, ppcd, radius, amplitude, SEED_BASE + 3 + m))
+    return out
+
+
+def box():
+    return fortran_real("6.28318531"), fortran_real("3.14159265")
+
+
+def member_parcels(spec):
+    """The initial parcels of one member."""
+    n, ppcd, radius, amplitude, seed = spec
+    lx, lz = box()
+    return thermal(n, n, n, lx, lx, lz, ppcd, seed=seed, radius=radius, amplitude=amplitude)
+
+
+def member_context(spec, ic=None, cap_factor=2.0):
+    """An FP32-bulk context holding member `spec`'s initial parcels."""
+    from .lib import Context
+    n = spec[0]
+    lx, lz = box()
+    if ic is None:
+        ic = member_parcels(spec)
+    c = Context(n, n, n, 0.0, lx, 0.0, lx, 0.0, lz, int(cap_factor * len(ic["x"])) + 64,
+                precision=1)
+    c.set_parcels(ic)
+    return c

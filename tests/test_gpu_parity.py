@@ -212,10 +212,12 @@ def test_twenty_steps_counts_match_oracle(oracle_mod, bar):
         bar("grid_" + name, rel_err(interior(g.grid(name)), interior(o.grid(name))), 1e-12)
 
 
-@pytest.mark.parametrize("shape", [(8, 6, 16), (8, 6, 32), (6, 8, 64), (6, 4, 128), (4, 6, 40)])
+@pytest.mark.parametrize("shape", [(8, 6, 16), (8, 6, 32), (6, 8, 64), (6, 4, 128), (4, 6, 40),
+                                   (4, 4, 200), (4, 6, 256), (6, 4, 263)])
 def test_column_solvers_bitwise_all_forms(oracle_mod, shape):
     """diffz and laplinv bit-identical to the oracle for column lengths served by the
-    register-resident solvers (17, 33, 65, 129 z nodes) and by the LDS form (41)."""
+    register-resident solvers (17, 33, 65, 129 z nodes), by the LDS form (41) and by the
+    eight-lanes-per-column form of long columns (201, 257 and its longest, 264)."""
     from pmpic_amd.lib import Context
     nx, ny, nz = shape
     b = box(nx, ny, nz)
