@@ -45,8 +45,9 @@ typedef struct pmpic_config {
   double timestep_prefactor;          /* "timestep_prefactor" */
   int32_t precision;                  /* 0 = plain FP64 (the reference's arithmetic, held to the
                                          oracle); 1 = FP32-bulk (cell index + FP32 offsets, FP32
-                                         parcel fields, interpolation and RK4; FP64 spectral
-                                         inversion, timestep and conservation sums) */
+                                         parcel fields, interpolation and RK4, FP32 semi-spectral
+                                         transforms and spectral arrays; FP64 mean flow, boundary
+                                         planes, column solves, timestep and conservation sums) */
   int32_t splitting_enabled;          /* "parcel_splitting_enabled" */
   int32_t mixing_enabled;             /* "parcel_mixing_enabled" */
   int32_t device;                     /* HIP device ordinal */

@@ -204,7 +204,7 @@ int free_all(Ctx *c) {
   hipFree(c->kx); hipFree(c->ky); hipFree(c->k2); hipFree(c->filt); hipFree(c->coords);
   hipFree(c->dz_den); hipFree(c->dz_cp);
   hipFree(c->fbuf);
-  hipFree(c->cbuf); hipFree(c->tw_x); hipFree(c->tw_y);
+  hipFree(c->cbuf); hipFree(c->tw_x); hipFree(c->tw_y); hipFree(c->tw_x32); hipFree(c->tw_y32);
   hipFree(c->scan_tmp); hipFree(c->blk); hipFree(c->maxblk); hipFree(c->dsc);
   for (int w = 0; w < NTIMERS; w++)
     for (int e = 0; e < 64; e++) {
@@ -686,7 +686,16 @@ int pmpic_get_spectral(pmpic_ctx *ctx, int32_t id, double *spec) {
   Ctx *c = &ctx->c;
   double *src = id == 0 ? c->su : id == 1 ? c->sv : c->sw;
   std::vector<double> h;
-  PMPIC_TRY(download(c, h, src, c->ssz));
+  if (c->precision == 1) {
+    // the FP32-bulk form's spectral arrays hold FP32 values; widened exactly
+    std::vector<float> h32((size_t)c->ssz);
+    PMPIC_CHECK(hipMemcpyAsync(h32.data(), src, sizeof(float) * (size_t)c->ssz,
+                               hipMemcpyDeviceToHost, c->stream));
+    PMPIC_CHECK(hipStreamSynchronize(c->stream));
+    h.assign(h32.begin(), h32.end());
+  } else {
+    PMPIC_TRY(download(c, h, src, c->ssz));
+  }
   spec_from_device_layout(c->g, h, spec);
   return PMPIC_OK;
 }

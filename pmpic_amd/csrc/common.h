@@ -135,6 +135,7 @@ struct Ctx {
   double *fbuf;                       // host-array staging, allocated at first use (ensure_fbuf)
   double2 *cbuf;                      // complex FFT intermediates, 8 fields
   double2 *tw_x, *tw_y;               // twiddles W_n^e, e in [0,n)
+  float2 *tw_x32, *tw_y32;            // the same rounded to FP32
   // scan scratch
   void *scan_tmp;
   size_t scan_tmp_bytes;
@@ -279,6 +280,12 @@ int fft_forward(Ctx *c, int nf, const double *const *src, double *const *dst);
 enum SpecOp { SPEC_PLAIN = 0, SPEC_DX = 1, SPEC_DY = 2 };
 int fft_backward(Ctx *c, int nf, const double *const *src, double *const *dst,
                  float *const *dst32 = nullptr, const int *deriv = nullptr);
+// the same transforms with spectral arrays (and the transforms' arithmetic) of type S
+template <typename S>
+int fft_forward_t(Ctx *c, int nf, const double *const *src, S *const *dst);
+template <typename S>
+int fft_backward_t(Ctx *c, int nf, const S *const *src, double *const *dst, float *const *dst32,
+                   const int *deriv);
 // spectral.hip
 int spectral_setup(Ctx *c);
 int vort2vel_spectral(Ctx *c, int stage);

@@ -33,77 +33,101 @@ struct Plan {
 };
 
 constexpr int FFT_TPB = 256;
-constexpr int LDS_CPLX = 1024;  // complex elements per LDS buffer (16 KiB in FP64)
 constexpr int MAXF = 8;
 
+// The transforms run in the precision S of the spectral arrays: FP64 (double2 complex
+// elements), or FP32 (float2) in the FP32-bulk form, whose LDS buffers then hold twice as many
+// elements, so a column pass covers twice as many adjacent columns.  Physical grids are FP64
+// in both.
+template <typename S> struct Cx;
+template <> struct Cx<double> { using T = double2; };
+template <> struct Cx<float> { using T = float2; };
+template <typename S> using cx_t = typename Cx<S>::T;
+template <typename C> struct Re;
+template <> struct Re<double2> { using T = double; };
+template <> struct Re<float2> { using T = float; };
+template <typename S>
+__host__ __device__ constexpr int lds_cplx() { return 16384 / (int)sizeof(cx_t<S>); }  // 16 KiB
+
+template <typename C, typename S>
+__device__ __forceinline__ C mkc(S a, S b) {
+  C r;
+  r.x = a;
+  r.y = b;
+  return r;
+}
+
 struct FieldPtrs {
-  const double *in[MAXF];
-  double *out[MAXF];
-  float *out32[MAXF];  // optional FP32 copy of a real output (the FP32-bulk form's gathers)
-  int op[MAXF];        // SpecOp applied to the input of the inverse (k_rows_inv)
+  const void *in[MAXF];  // forward: FP64 physical grids; inverse: spectral arrays of type S
+  void *out[MAXF];       // forward: spectral arrays of type S; inverse: FP64 physical grids
+  float *out32[MAXF];    // optional FP32 copy of a real output (the FP32-bulk form's gathers)
+  int op[MAXF];          // SpecOp applied to the input of the inverse (k_rows_inv)
   const double *kx, *ky;  // wavenumber planes for SPEC_DX / SPEC_DY
 };
 
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) {
-  return make_double2(a.x + b.x, a.y + b.y);
+template <typename C>
+__device__ __forceinline__ C cmul(C a, C b) {
+  C r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
 }
 
 // DFT of length R on registers: v[o] <- sum_q v[q] W^(o q), W = exp(-2 pi i / R) for the
 // forward transform and its conjugate for the inverse (INV); products with +-1 and +-i are
-// exact swaps and sign changes.
-template <int R, bool INV>
-__device__ __forceinline__ void dft(double2 *v) {
+// exact swaps and sign changes.  C is the complex type, S its real type.
+template <int R, bool INV, typename C>
+__device__ __forceinline__ void dft(C *v) {
+  using S = typename Re<C>::T;
   if constexpr (R == 2) {
-    const double2 a = v[0], b = v[1];
-    v[0] = make_double2(a.x + b.x, a.y + b.y);
-    v[1] = make_double2(a.x - b.x, a.y - b.y);
+    const C a = v[0], b = v[1];
+    v[0] = mkc<C, S>(a.x + b.x, a.y + b.y);
+    v[1] = mkc<C, S>(a.x - b.x, a.y - b.y);
   } else if constexpr (R == 4) {
-    const double2 s02 = make_double2(v[0].x + v[2].x, v[0].y + v[2].y);
-    const double2 d02 = make_double2(v[0].x - v[2].x, v[0].y - v[2].y);
-    const double2 s13 = make_double2(v[1].x + v[3].x, v[1].y + v[3].y);
-    const double2 d13 = make_double2(v[1].x - v[3].x, v[1].y - v[3].y);
+    const C s02 = mkc<C, S>(v[0].x + v[2].x, v[0].y + v[2].y);
+    const C d02 = mkc<C, S>(v[0].x - v[2].x, v[0].y - v[2].y);
+    const C s13 = mkc<C, S>(v[1].x + v[3].x, v[1].y + v[3].y);
+    const C d13 = mkc<C, S>(v[1].x - v[3].x, v[1].y - v[3].y);
     // forward: y1 = d02 - i d13, y3 = d02 + i d13 (inverse swaps them)
-    const double2 mi = make_double2(d13.y, -d13.x);  // -i * d13
-    v[0] = make_double2(s02.x + s13.x, s02.y + s13.y);
-    v[2] = make_double2(s02.x - s13.x, s02.y - s13.y);
+    const C mi = mkc<C, S>(d13.y, -d13.x);  // -i * d13
+    v[0] = mkc<C, S>(s02.x + s13.x, s02.y + s13.y);
+    v[2] = mkc<C, S>(s02.x - s13.x, s02.y - s13.y);
     if (!INV) {
-      v[1] = make_double2(d02.x + mi.x, d02.y + mi.y);
-      v[3] = make_double2(d02.x - mi.x, d02.y - mi.y);
+      v[1] = mkc<C, S>(d02.x + mi.x, d02.y + mi.y);
+      v[3] = mkc<C, S>(d02.x - mi.x, d02.y - mi.y);
     } else {
-      v[1] = make_double2(d02.x - mi.x, d02.y - mi.y);
-      v[3] = make_double2(d02.x + mi.x, d02.y + mi.y);
+      v[1] = mkc<C, S>(d02.x - mi.x, d02.y - mi.y);
+      v[3] = mkc<C, S>(d02.x + mi.x, d02.y + mi.y);
     }
   } else if constexpr (R == 3) {
-    constexpr double c = -0.5, s = INV ? -0.86602540378443864676 : 0.86602540378443864676;
-    const double2 t = make_double2(v[1].x + v[2].x, v[1].y + v[2].y);
-    const double2 d = make_double2(v[1].x - v[2].x, v[1].y - v[2].y);
-    const double2 a = make_double2(v[0].x + c * t.x, v[0].y + c * t.y);
-    v[0] = make_double2(v[0].x + t.x, v[0].y + t.y);
+    constexpr S c = (S)-0.5;
+    constexpr S s = INV ? (S)-0.86602540378443864676 : (S)0.86602540378443864676;
+    const C t = mkc<C, S>(v[1].x + v[2].x, v[1].y + v[2].y);
+    const C d = mkc<C, S>(v[1].x - v[2].x, v[1].y - v[2].y);
+    const C a = mkc<C, S>(v[0].x + c * t.x, v[0].y + c * t.y);
+    v[0] = mkc<C, S>(v[0].x + t.x, v[0].y + t.y);
     // y1 = a - i s d, y2 = a + i s d;  -i (x + i y) = y - i x
-    v[1] = make_double2(a.x + s * d.y, a.y - s * d.x);
-    v[2] = make_double2(a.x - s * d.y, a.y + s * d.x);
+    v[1] = mkc<C, S>(a.x + s * d.y, a.y - s * d.x);
+    v[2] = mkc<C, S>(a.x - s * d.y, a.y + s * d.x);
   } else {
     static_assert(R == 5, "radix 2, 3, 4 or 5");
-    constexpr double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
-    constexpr double s1 = INV ? -0.95105651629515357212 : 0.95105651629515357212;
-    constexpr double s2 = INV ? -0.58778525229247312917 : 0.58778525229247312917;
-    const double2 t1 = make_double2(v[1].x + v[4].x, v[1].y + v[4].y);
-    const double2 t2 = make_double2(v[2].x + v[3].x, v[2].y + v[3].y);
-    const double2 t3 = make_double2(v[1].x - v[4].x, v[1].y - v[4].y);
-    const double2 t4 = make_double2(v[2].x - v[3].x, v[2].y - v[3].y);
-    const double2 a1 = make_double2(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
-    const double2 a2 = make_double2(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
-    const double2 b1 = make_double2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
-    const double2 b2 = make_double2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
-    v[0] = make_double2(v[0].x + t1.x + t2.x, v[0].y + t1.y + t2.y);
+    constexpr S c1 = (S)0.30901699437494742410, c2 = (S)-0.80901699437494742410;
+    constexpr S s1 = INV ? (S)-0.95105651629515357212 : (S)0.95105651629515357212;
+    constexpr S s2 = INV ? (S)-0.58778525229247312917 : (S)0.58778525229247312917;
+    const C t1 = mkc<C, S>(v[1].x + v[4].x, v[1].y + v[4].y);
+    const C t2 = mkc<C, S>(v[2].x + v[3].x, v[2].y + v[3].y);
+    const C t3 = mkc<C, S>(v[1].x - v[4].x, v[1].y - v[4].y);
+    const C t4 = mkc<C, S>(v[2].x - v[3].x, v[2].y - v[3].y);
+    const C a1 = mkc<C, S>(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
+    const C a2 = mkc<C, S>(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
+    const C b1 = mkc<C, S>(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
+    const C b2 = mkc<C, S>(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+    v[0] = mkc<C, S>(v[0].x + t1.x + t2.x, v[0].y + t1.y + t2.y);
     // y1 = a1 - i b1, y4 = a1 + i b1, y2 = a2 - i b2, y3 = a2 + i b2
-    v[1] = make_double2(a1.x + b1.y, a1.y - b1.x);
-    v[4] = make_double2(a1.x - b1.y, a1.y + b1.x);
-    v[2] = make_double2(a2.x + b2.y, a2.y - b2.x);
-    v[3] = make_double2(a2.x - b2.y, a2.y + b2.x);
+    v[1] = mkc<C, S>(a1.x + b1.y, a1.y - b1.x);
+    v[4] = mkc<C, S>(a1.x - b1.y, a1.y + b1.x);
+    v[2] = mkc<C, S>(a2.x + b2.y, a2.y - b2.x);
+    v[3] = mkc<C, S>(a2.x - b2.y, a2.y + b2.x);
   }
 }
 
@@ -111,32 +135,32 @@ __device__ __forceinline__ void dft(double2 *v) {
 // the radices already applied: item (t, i) takes x[i + q n/R], q < R, twiddled by
 // W_n^(q k n/(Ns R)) with k = i mod Ns, and writes the R-point DFT to
 // y[(i / Ns) Ns R + k + o Ns].  tw[e] = W_n^e staged in LDS (conjugated for the inverse).
-template <int R, bool INV>
-__device__ __forceinline__ void stockham_stage(const double2 *x, double2 *y, const double2 *tw,
-                                               int n, int Ns, int nb) {
+template <int R, bool INV, typename C>
+__device__ __forceinline__ void stockham_stage(const C *x, C *y, const C *tw, int n, int Ns,
+                                               int nb) {
   const int nq = n / R;
   const int tstep = n / (Ns * R);
   const int tot = nb * nq;
   for (int bi = threadIdx.x; bi < tot; bi += FFT_TPB) {
     const int t = bi / nq, i = bi - t * nq;
     const int j = i / Ns, k = i - j * Ns;
-    const double2 *xs = x + t * n + i;
-    double2 v[R];
+    const C *xs = x + t * n + i;
+    C v[R];
 #pragma unroll
     for (int q = 0; q < R; q++) {
-      const double2 e = xs[q * nq];
+      const C e = xs[q * nq];
       v[q] = (q == 0 || k == 0) ? e : cmul(e, tw[q * k * tstep]);
     }
     dft<R, INV>(v);
-    double2 *ys = y + t * n + j * Ns * R + k;
+    C *ys = y + t * n + j * Ns * R + k;
 #pragma unroll
     for (int o = 0; o < R; o++) ys[o * Ns] = v[o];
   }
 }
 
 // nb transforms of length p.n stored t-major in `a`; result in the returned buffer.
-template <bool INV>
-__device__ double2 *lds_fft(double2 *a, double2 *b, const double2 *tw, const Plan &p, int nb) {
+template <bool INV, typename C>
+__device__ C *lds_fft(C *a, C *b, const C *tw, const Plan &p, int nb) {
   int Ns = 1;
   for (int s = 0; s < p.nrad; s++) {
     const int r = (int)((p.code >> (4 * s)) & 15u);
@@ -145,7 +169,7 @@ __device__ double2 *lds_fft(double2 *a, double2 *b, const double2 *tw, const Pla
     else if (r == 3) stockham_stage<3, INV>(a, b, tw, p.n, Ns, nb);
     else stockham_stage<5, INV>(a, b, tw, p.n, Ns, nb);
     __syncthreads();
-    double2 *sw = a;
+    C *sw = a;
     a = b;
     b = sw;
     Ns *= r;
@@ -155,39 +179,44 @@ __device__ double2 *lds_fft(double2 *a, double2 *b, const double2 *tw, const Pla
 
 // Loads into LDS are issued as a batch (each thread's elements loaded into registers, then
 // stored), so a workgroup waits on memory once per pass rather than once per element.
-constexpr int EPT = LDS_CPLX / FFT_TPB;  // elements per thread of a full LDS buffer
-constexpr int TPT = 512 / FFT_TPB;       // twiddles per thread (n <= 512)
+template <typename S>
+__host__ __device__ constexpr int ept() { return lds_cplx<S>() / FFT_TPB; }  // per thread
+constexpr int TPT = 512 / FFT_TPB;  // twiddles per thread (n <= 512)
 
-__device__ __forceinline__ void load_twiddles(double2 (&w)[TPT], const double2 *tw, int n) {
+template <typename C>
+__device__ __forceinline__ void load_twiddles(C (&w)[TPT], const C *tw, int n) {
 #pragma unroll
   for (int i = 0; i < TPT; i++) {
     const int e = threadIdx.x + i * FFT_TPB;
     if (e < n) w[i] = tw[e];
   }
 }
-__device__ __forceinline__ void put_twiddles(double2 *tws, const double2 (&w)[TPT], int n,
-                                             bool conj_) {
+template <typename C>
+__device__ __forceinline__ void put_twiddles(C *tws, const C (&w)[TPT], int n, bool conj_) {
 #pragma unroll
   for (int i = 0; i < TPT; i++) {
     const int e = threadIdx.x + i * FFT_TPB;
-    if (e < n) tws[e] = conj_ ? make_double2(w[i].x, -w[i].y) : w[i];
+    if (e < n) tws[e] = conj_ ? mkc<C>(w[i].x, -w[i].y) : w[i];
   }
 }
 
 // forward pass 1: r2c along x of every row (k, y) -> C1[k][y][l], l <= nx/2.  Two real rows
 // a, b share one complex transform Z = FFT(a + i b): A[l] = (Z[l] + conj Z[n-l]) / 2 and
 // B[l] = (Z[l] - conj Z[n-l]) / (2i).
-__global__ void __launch_bounds__(FFT_TPB) k_rows_r2c(Plan p, const double2 *__restrict__ tw,
-                                                      FieldPtrs fp, double2 *__restrict__ c1,
+template <typename S>
+__global__ void __launch_bounds__(FFT_TPB) k_rows_r2c(Plan p, const cx_t<S> *__restrict__ tw,
+                                                      FieldPtrs fp, cx_t<S> *__restrict__ c1,
                                                       int64_t c1_field, int nrows, int nb) {
-  __shared__ double2 A[LDS_CPLX], B[LDS_CPLX], T[512];
+  using C = cx_t<S>;
+  constexpr int EPT = ept<S>();
+  __shared__ C A[lds_cplx<S>()], B[lds_cplx<S>()], T[512];
   const int f = blockIdx.y;
-  const double *src = fp.in[f];
-  double2 *dst = c1 + (int64_t)f * c1_field;
+  const double *src = static_cast<const double *>(fp.in[f]);
+  C *dst = c1 + (int64_t)f * c1_field;
   const int n = p.n, nl = n / 2 + 1;
   const int r0 = blockIdx.x * 2 * nb;                  // first row of this workgroup
   const int nbb = min(nb, (nrows - r0 + 1) / 2);       // transforms (row pairs)
-  double2 w[TPT], v[EPT];
+  C w[TPT], v[EPT];
   load_twiddles(w, tw, n);
 #pragma unroll
   for (int i = 0; i < EPT; i++) {
@@ -195,8 +224,8 @@ __global__ void __launch_bounds__(FFT_TPB) k_rows_r2c(Plan p, const double2 *__r
     if (e < nbb * n) {
       const int t = e / n, x = e - t * n;
       const int ra = r0 + 2 * t, rb = ra + 1;
-      v[i].x = src[(int64_t)ra * n + x];
-      v[i].y = (rb < nrows) ? src[(int64_t)rb * n + x] : 0.0;
+      v[i].x = (S)src[(int64_t)ra * n + x];
+      v[i].y = (rb < nrows) ? (S)src[(int64_t)rb * n + x] : (S)0;
     }
   }
   put_twiddles(T, w, n, false);
@@ -206,34 +235,37 @@ __global__ void __launch_bounds__(FFT_TPB) k_rows_r2c(Plan p, const double2 *__r
     if (e < nbb * n) A[e] = v[i];
   }
   __syncthreads();
-  double2 *R = lds_fft<false>(A, B, T, p, nbb);
+  C *R = lds_fft<false>(A, B, T, p, nbb);
   for (int e = threadIdx.x; e < nbb * nl; e += FFT_TPB) {
     const int t = e / nl, l = e - t * nl;
     const int ra = r0 + 2 * t, rb = ra + 1;
-    const double2 z = R[t * n + l];
-    const double2 w = R[t * n + (l == 0 ? 0 : n - l)];  // Z[n-l]
-    dst[(int64_t)ra * nl + l] = make_double2((z.x + w.x) / 2, (z.y - w.y) / 2);
-    if (rb < nrows) dst[(int64_t)rb * nl + l] = make_double2((z.y + w.y) / 2, (w.x - z.x) / 2);
+    const C z = R[t * n + l];
+    const C w = R[t * n + (l == 0 ? 0 : n - l)];  // Z[n-l]
+    dst[(int64_t)ra * nl + l] = mkc<C, S>((z.x + w.x) / 2, (z.y - w.y) / 2);
+    if (rb < nrows) dst[(int64_t)rb * nl + l] = mkc<C, S>((z.y + w.y) / 2, (w.x - z.x) / 2);
   }
 }
 
 // forward pass 2: complex FFT along y of nb adjacent columns l of one z level, then the
 // (Re,Im)-pair split into the semi-spectral layout
-__global__ void __launch_bounds__(FFT_TPB) k_cols_fwd(Plan p, const double2 *__restrict__ tw,
-                                                      FieldPtrs fp, const double2 *__restrict__ c1,
+template <typename S>
+__global__ void __launch_bounds__(FFT_TPB) k_cols_fwd(Plan p, const cx_t<S> *__restrict__ tw,
+                                                      FieldPtrs fp, const cx_t<S> *__restrict__ c1,
                                                       int64_t c1_field, int nl, int nb, int sx,
                                                       int sy) {
-  __shared__ double2 A[LDS_CPLX], B[LDS_CPLX], T[512];
+  using C = cx_t<S>;
+  constexpr int EPT = ept<S>();
+  __shared__ C A[lds_cplx<S>()], B[lds_cplx<S>()], T[512];
   const int f = blockIdx.z;
-  const double2 *src = c1 + (int64_t)f * c1_field;
-  double *dst = fp.out[f];
+  const C *src = c1 + (int64_t)f * c1_field;
+  S *dst = static_cast<S *>(fp.out[f]);
   const int ny = p.n;
   const int k = blockIdx.y;
   const int l0 = blockIdx.x * nb;
   const int nbb = min(nb, nl - l0);
-  const double2 *lev = src + (int64_t)k * ny * nl;
-  double2 w[TPT];
-  double vx[EPT], vy[EPT];
+  const C *lev = src + (int64_t)k * ny * nl;
+  C w[TPT];
+  S vx[EPT], vy[EPT];
   int at[EPT];
   load_twiddles(w, tw, ny);
 #pragma unroll
@@ -241,10 +273,10 @@ __global__ void __launch_bounds__(FFT_TPB) k_cols_fwd(Plan p, const double2 *__r
     const int e = threadIdx.x + i * FFT_TPB;
     const int y = e / nbb, t = e - y * nbb;
     at[i] = (e < nbb * ny) ? t * ny + y : -1;
-    vx[i] = 0.0;
-    vy[i] = 0.0;
+    vx[i] = 0;
+    vy[i] = 0;
     if (at[i] >= 0) {
-      const double2 u = lev[(int64_t)y * nl + l0 + t];
+      const C u = lev[(int64_t)y * nl + l0 + t];
       vx[i] = u.x;
       vy[i] = u.y;
     }
@@ -252,18 +284,18 @@ __global__ void __launch_bounds__(FFT_TPB) k_cols_fwd(Plan p, const double2 *__r
   put_twiddles(T, w, ny, false);
 #pragma unroll
   for (int i = 0; i < EPT; i++)
-    if (at[i] >= 0) A[at[i]] = make_double2(vx[i], vy[i]);
+    if (at[i] >= 0) A[at[i]] = mkc<C, S>(vx[i], vy[i]);
   __syncthreads();
-  double2 *R = lds_fft<false>(A, B, T, p, nbb);
+  C *R = lds_fft<false>(A, B, T, p, nbb);
   const int nm = ny / 2 + 1;
-  double *out = dst + (int64_t)k * sy * sx;
+  S *out = dst + (int64_t)k * sy * sx;
   for (int e = threadIdx.x; e < nbb * nm; e += FFT_TPB) {
     int m = e / nbb, t = e - m * nbb;
     int l = l0 + t;
-    double2 g = R[t * ny + m];
-    double2 h = R[t * ny + ((ny - m) % ny)];
-    double2 s = make_double2(g.x + h.x, g.y + h.y);
-    double2 d = make_double2(g.x - h.x, g.y - h.y);
+    C g = R[t * ny + m];
+    C h = R[t * ny + ((ny - m) % ny)];
+    C s = mkc<C, S>(g.x + h.x, g.y + h.y);
+    C d = mkc<C, S>(g.x - h.x, g.y - h.y);
     out[(int64_t)(2 * m) * sx + 2 * l] = s.x / 2;
     out[(int64_t)(2 * m) * sx + 2 * l + 1] = s.y / 2;
     out[(int64_t)(2 * m + 1) * sx + 2 * l] = d.y / 2;
@@ -274,21 +306,24 @@ __global__ void __launch_bounds__(FFT_TPB) k_cols_fwd(Plan p, const double2 *__r
 // inverse pass 1: per (k, m) build H = A_m + i B_m over l in [0, nx) from the two real
 // x-spectra (imaginary parts of the DC and Nyquist bins dropped as c2r drops them), inverse
 // FFT along x, scale 1/nx -> C2[k][m][x]
-__global__ void __launch_bounds__(FFT_TPB) k_rows_inv(Plan p, const double2 *__restrict__ tw,
-                                                      FieldPtrs fp, double2 *__restrict__ c2,
+template <typename S>
+__global__ void __launch_bounds__(FFT_TPB) k_rows_inv(Plan p, const cx_t<S> *__restrict__ tw,
+                                                      FieldPtrs fp, cx_t<S> *__restrict__ c2,
                                                       int64_t c2_field, int nrows, int nm, int nb,
                                                       int sx) {
-  __shared__ double2 A[LDS_CPLX], B[LDS_CPLX], T[512];
+  using C = cx_t<S>;
+  constexpr int EPT = ept<S>();
+  __shared__ C A[lds_cplx<S>()], B[lds_cplx<S>()], T[512];
   const int f = blockIdx.y;
-  const double *src = fp.in[f];
+  const S *src = static_cast<const S *>(fp.in[f]);
   const int op = fp.op[f];
-  double2 *dst = c2 + (int64_t)f * c2_field;
+  C *dst = c2 + (int64_t)f * c2_field;
   const int nx = p.n;
   const int r0 = blockIdx.x * nb;
   const int nbb = min(nb, nrows - r0);
   const int half = nx / 2;
   const bool even = (nx % 2) == 0;
-  double2 w[TPT], va[EPT], vb[EPT];
+  C w[TPT], va[EPT], vb[EPT];
   load_twiddles(w, tw, nx);
 #pragma unroll
   for (int i = 0; i < EPT; i++) {
@@ -297,24 +332,25 @@ __global__ void __launch_bounds__(FFT_TPB) k_rows_inv(Plan p, const double2 *__r
       const int t = e / nx, l = e - t * nx;
       const int row = r0 + t;  // row = k * nm + m
       const int k = row / nm, m = row - k * nm;
-      const double *sa = src + ((int64_t)k * (2 * nm) + 2 * m) * sx;
+      const S *sa = src + ((int64_t)k * (2 * nm) + 2 * m) * sx;
       const int ll = (l <= half) ? l : nx - l;
-      va[i] = *reinterpret_cast<const double2 *>(sa + 2 * ll);       // (ar, ai)
-      vb[i] = *reinterpret_cast<const double2 *>(sa + sx + 2 * ll);  // (br, bi)
+      va[i] = *reinterpret_cast<const C *>(sa + 2 * ll);       // (ar, ai)
+      vb[i] = *reinterpret_cast<const C *>(sa + sx + 2 * ll);  // (br, bi)
       if (op != SPEC_PLAIN) {
         // the derivative of spectral rows 2m (a) and 2m+1 (b), element for element as the
         // standalone diffx / diffy compute it (fft_ops.F90:266-269, :366-369); the
         // wavenumber planes are indexed by (row, x) like the spectral plane
         const double *kp = (op == SPEC_DX ? fp.kx : fp.ky) + (int64_t)(2 * m) * sx + 2 * ll;
-        const double2 ka = *reinterpret_cast<const double2 *>(kp);
-        const double2 kb = *reinterpret_cast<const double2 *>(kp + sx);
-        const double2 a0 = va[i], b0 = vb[i];
+        const double2 kad = *reinterpret_cast<const double2 *>(kp);
+        const double2 kbd = *reinterpret_cast<const double2 *>(kp + sx);
+        const S kax = (S)kad.x, kay = (S)kad.y, kbx = (S)kbd.x, kby = (S)kbd.y;
+        const C a0 = va[i], b0 = vb[i];
         if (op == SPEC_DX) {  // pair swap along x within each row
-          va[i] = make_double2((-1. * a0.y) * ka.x, a0.x * ka.y);
-          vb[i] = make_double2((-1. * b0.y) * kb.x, b0.x * kb.y);
+          va[i] = mkc<C, S>(((S)-1. * a0.y) * kax, a0.x * kay);
+          vb[i] = mkc<C, S>(((S)-1. * b0.y) * kbx, b0.x * kby);
         } else {  // pair swap along y between the two rows
-          va[i] = make_double2((-1. * b0.x) * ka.x, (-1. * b0.y) * ka.y);
-          vb[i] = make_double2(a0.x * kb.x, a0.y * kb.y);
+          va[i] = mkc<C, S>(((S)-1. * b0.x) * kax, ((S)-1. * b0.y) * kay);
+          vb[i] = mkc<C, S>(a0.x * kbx, a0.y * kby);
         }
       }
     }
@@ -326,42 +362,45 @@ __global__ void __launch_bounds__(FFT_TPB) k_rows_inv(Plan p, const double2 *__r
     if (e < nbb * nx) {
       const int t = e / nx, l = e - t * nx;
       const int ll = (l <= half) ? l : nx - l;
-      const double ar = va[i].x, br = vb[i].x;
-      double ai = va[i].y, bi = vb[i].y;
-      if (ll == 0 || (even && ll == half)) { ai = 0.0; bi = 0.0; }
-      double2 h;
-      if (l <= half) h = make_double2(ar - bi, ai + br);
-      else h = make_double2(ar + bi, -ai + br);
+      const S ar = va[i].x, br = vb[i].x;
+      S ai = va[i].y, bi = vb[i].y;
+      if (ll == 0 || (even && ll == half)) { ai = 0; bi = 0; }
+      C h;
+      if (l <= half) h = mkc<C, S>(ar - bi, ai + br);
+      else h = mkc<C, S>(ar + bi, -ai + br);
       A[t * nx + l] = h;
     }
   }
   __syncthreads();
-  double2 *R = lds_fft<true>(A, B, T, p, nbb);
-  const double dn = 1.0 / (double)nx;
+  C *R = lds_fft<true>(A, B, T, p, nbb);
+  const S dn = (S)1 / (S)nx;
   for (int e = threadIdx.x; e < nbb * nx; e += FFT_TPB) {
     int t = e / nx, x = e - t * nx;
-    double2 v = R[t * nx + x];
-    dst[(int64_t)(r0 + t) * nx + x] = make_double2(v.x * dn, v.y * dn);
+    C v = R[t * nx + x];
+    dst[(int64_t)(r0 + t) * nx + x] = mkc<C, S>(v.x * dn, v.y * dn);
   }
 }
 
 // inverse pass 2: c2r along y for 2 nb adjacent x columns of one z level.  The Hermitian
 // spectra of two columns a, b share one complex inverse transform of H_a + i H_b, whose real
 // and imaginary parts are the two real columns.
-__global__ void __launch_bounds__(FFT_TPB) k_cols_c2r(Plan p, const double2 *__restrict__ tw,
-                                                      FieldPtrs fp, const double2 *__restrict__ c2,
+template <typename S>
+__global__ void __launch_bounds__(FFT_TPB) k_cols_c2r(Plan p, const cx_t<S> *__restrict__ tw,
+                                                      FieldPtrs fp, const cx_t<S> *__restrict__ c2,
                                                       int64_t c2_field, int nx, int nb) {
-  __shared__ double2 A[LDS_CPLX], B[LDS_CPLX], T[512];
+  using C = cx_t<S>;
+  constexpr int EPT = ept<S>();
+  __shared__ C A[lds_cplx<S>()], B[lds_cplx<S>()], T[512];
   const int f = blockIdx.z;
-  const double2 *src = c2 + (int64_t)f * c2_field;
-  double *dst = fp.out[f];
+  const C *src = c2 + (int64_t)f * c2_field;
+  double *dst = static_cast<double *>(fp.out[f]);
   const int ny = p.n, nm = ny / 2 + 1;
   const int k = blockIdx.y;
   const int x0 = blockIdx.x * 2 * nb;
   const int nbb = min(nb, (nx - x0 + 1) / 2);
-  const double2 *lev = src + (int64_t)k * nm * nx;
+  const C *lev = src + (int64_t)k * nm * nx;
   const bool even = (ny % 2) == 0;
-  double2 w[TPT], va[EPT], vb[EPT];
+  C w[TPT], va[EPT], vb[EPT];
   load_twiddles(w, tw, ny);
 #pragma unroll
   for (int i = 0; i < EPT; i++) {
@@ -371,7 +410,7 @@ __global__ void __launch_bounds__(FFT_TPB) k_cols_c2r(Plan p, const double2 *__r
       const int xa = x0 + 2 * t, xb = xa + 1;
       const int m = (y < nm) ? y : ny - y;
       va[i] = lev[(int64_t)m * nx + xa];
-      vb[i] = (xb < nx) ? lev[(int64_t)m * nx + xb] : make_double2(0.0, 0.0);
+      vb[i] = (xb < nx) ? lev[(int64_t)m * nx + xb] : mkc<C, S>(0, 0);
     }
   }
   put_twiddles(T, w, ny, true);
@@ -381,22 +420,22 @@ __global__ void __launch_bounds__(FFT_TPB) k_cols_c2r(Plan p, const double2 *__r
     if (e < nbb * ny) {
       const int y = e / nbb, t = e - y * nbb;
       const int m = (y < nm) ? y : ny - y;
-      double2 wa = va[i], wb = vb[i];
-      if (m == 0 || (even && m == ny / 2)) { wa.y = 0.0; wb.y = 0.0; }
+      C wa = va[i], wb = vb[i];
+      if (m == 0 || (even && m == ny / 2)) { wa.y = 0; wb.y = 0; }
       if (y >= nm) { wa.y = -wa.y; wb.y = -wb.y; }
-      A[t * ny + y] = make_double2(wa.x - wb.y, wa.y + wb.x);
+      A[t * ny + y] = mkc<C, S>(wa.x - wb.y, wa.y + wb.x);
     }
   }
   __syncthreads();
-  double2 *R = lds_fft<true>(A, B, T, p, nbb);
-  const double dn = 1.0 / (double)ny;
+  C *R = lds_fft<true>(A, B, T, p, nbb);
+  const S dn = (S)1 / (S)ny;
   // either output may be absent: the FP32-bulk form keeps some fields only in FP32
   double *out = dst ? dst + (int64_t)k * ny * nx : nullptr;
   float *o32 = fp.out32[f] ? fp.out32[f] + (int64_t)k * ny * nx : nullptr;
   for (int e = threadIdx.x; e < nbb * ny; e += FFT_TPB) {
     const int y = e / nbb, t = e - y * nbb;
     const int xa = x0 + 2 * t, xb = xa + 1;
-    const double2 v = R[t * ny + y];
+    const C v = R[t * ny + y];
     if (out) {
       out[(int64_t)y * nx + xa] = v.x * dn;
       if (xb < nx) out[(int64_t)y * nx + xb] = v.y * dn;
@@ -424,38 +463,51 @@ Plan make_plan(int n) {
   return p;
 }
 
-int upload_twiddles(int n, double2 **dptr) {
+// W_n^e for e in [0, n), in FP64 and rounded to FP32
+int upload_twiddles(int n, double2 **dptr, float2 **dptr32) {
   std::vector<double2> h(n);
+  std::vector<float2> h32(n);
   for (int e = 0; e < n; e++) {
     // quarter turns are exact so that purely real or imaginary inputs stay so
     if ((4L * e) % n == 0) {
       const int q = (int)((4L * e) / n);
       const double re[4] = {1.0, 0.0, -1.0, 0.0}, im[4] = {0.0, -1.0, 0.0, 1.0};
       h[e] = make_double2(re[q], im[q]);
-      continue;
+    } else {
+      long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)e / (long double)n;
+      h[e] = make_double2((double)cosl(ang), (double)sinl(ang));
     }
-    long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)e / (long double)n;
-    h[e] = make_double2((double)cosl(ang), (double)sinl(ang));
+    h32[e] = make_float2((float)h[e].x, (float)h[e].y);
   }
   PMPIC_CHECK(hipMalloc(dptr, sizeof(double2) * n));
   PMPIC_CHECK(hipMemcpy(*dptr, h.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
+  PMPIC_CHECK(hipMalloc(dptr32, sizeof(float2) * n));
+  PMPIC_CHECK(hipMemcpy(*dptr32, h32.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
   return PMPIC_OK;
 }
+
+template <typename S> const cx_t<S> *twiddles_x(const Ctx *c);
+template <> const double2 *twiddles_x<double>(const Ctx *c) { return c->tw_x; }
+template <> const float2 *twiddles_x<float>(const Ctx *c) { return c->tw_x32; }
+template <typename S> const cx_t<S> *twiddles_y(const Ctx *c);
+template <> const double2 *twiddles_y<double>(const Ctx *c) { return c->tw_y; }
+template <> const float2 *twiddles_y<float>(const Ctx *c) { return c->tw_y32; }
 
 }  // namespace
 
 int fft_setup(Ctx *c) {
   if (c->g.nx > 512 || c->g.ny > 512) return PMPIC_ERR_FFT_SIZE;
   if (make_plan(c->g.nx).nrad < 0 || make_plan(c->g.ny).nrad < 0) return PMPIC_ERR_FFT_SIZE;
-  PMPIC_TRY(upload_twiddles(c->g.nx, &c->tw_x));
-  PMPIC_TRY(upload_twiddles(c->g.ny, &c->tw_y));
+  PMPIC_TRY(upload_twiddles(c->g.nx, &c->tw_x, &c->tw_x32));
+  PMPIC_TRY(upload_twiddles(c->g.ny, &c->tw_y, &c->tw_y32));
   return PMPIC_OK;
 }
 
 static int64_t c1_field_size(const Geometry &g) { return (int64_t)g.nzn * g.ny * (g.nx / 2 + 1); }
 static int64_t c2_field_size(const Geometry &g) { return (int64_t)g.nzn * (g.ny / 2 + 1) * g.nx; }
 
-int fft_forward(Ctx *c, int nf, const double *const *src, double *const *dst) {
+template <typename S>
+int fft_forward_t(Ctx *c, int nf, const double *const *src, S *const *dst) {
   const Geometry &g = c->g;
   if (nf < 1 || nf > MAXF) return PMPIC_ERR_ARG;
   FieldPtrs fp{};
@@ -463,20 +515,22 @@ int fft_forward(Ctx *c, int nf, const double *const *src, double *const *dst) {
   Plan px = make_plan(g.nx), py = make_plan(g.ny);
   const int64_t c1f = c1_field_size(g);
   const int nrows = g.nzn * g.ny;
-  const int nbx = LDS_CPLX / g.nx;  // transforms per workgroup, two rows each
-  hipLaunchKernelGGL(k_rows_r2c, dim3(cdiv(nrows, 2 * nbx), nf), dim3(FFT_TPB), 0, c->stream, px,
-                     c->tw_x, fp, c->cbuf, c1f, nrows, nbx);
+  cx_t<S> *cb = reinterpret_cast<cx_t<S> *>(c->cbuf);
+  const int nbx = lds_cplx<S>() / g.nx;  // transforms per workgroup, two rows each
+  hipLaunchKernelGGL(k_rows_r2c<S>, dim3(cdiv(nrows, 2 * nbx), nf), dim3(FFT_TPB), 0, c->stream,
+                     px, twiddles_x<S>(c), fp, cb, c1f, nrows, nbx);
   const int nl = g.nx / 2 + 1;
-  int nby = LDS_CPLX / g.ny;
+  int nby = lds_cplx<S>() / g.ny;
   if (nby > 32) nby = 32;
-  hipLaunchKernelGGL(k_cols_fwd, dim3(cdiv(nl, nby), g.nzn, nf), dim3(FFT_TPB), 0, c->stream, py,
-                     c->tw_y, fp, c->cbuf, c1f, nl, nby, g.sx, g.sy);
+  hipLaunchKernelGGL(k_cols_fwd<S>, dim3(cdiv(nl, nby), g.nzn, nf), dim3(FFT_TPB), 0, c->stream,
+                     py, twiddles_y<S>(c), fp, cb, c1f, nl, nby, g.sx, g.sy);
   PMPIC_CHECK(hipGetLastError());
   return PMPIC_OK;
 }
 
-int fft_backward(Ctx *c, int nf, const double *const *src, double *const *dst,
-                 float *const *dst32, const int *deriv) {
+template <typename S>
+int fft_backward_t(Ctx *c, int nf, const S *const *src, double *const *dst, float *const *dst32,
+                   const int *deriv) {
   const Geometry &g = c->g;
   if (nf < 1 || nf > MAXF) return PMPIC_ERR_ARG;
   FieldPtrs fp{};
@@ -493,15 +547,32 @@ int fft_backward(Ctx *c, int nf, const double *const *src, double *const *dst,
   const int64_t c2f = c2_field_size(g);
   const int nm = g.ny / 2 + 1;
   const int nrows = g.nzn * nm;
-  const int nbx = LDS_CPLX / g.nx;
-  hipLaunchKernelGGL(k_rows_inv, dim3(cdiv(nrows, nbx), nf), dim3(FFT_TPB), 0, c->stream, px,
-                     c->tw_x, fp, c->cbuf, c2f, nrows, nm, nbx, g.sx);
-  int nby = LDS_CPLX / g.ny;  // transforms per workgroup, two x columns each
+  cx_t<S> *cb = reinterpret_cast<cx_t<S> *>(c->cbuf);
+  const int nbx = lds_cplx<S>() / g.nx;
+  hipLaunchKernelGGL(k_rows_inv<S>, dim3(cdiv(nrows, nbx), nf), dim3(FFT_TPB), 0, c->stream, px,
+                     twiddles_x<S>(c), fp, cb, c2f, nrows, nm, nbx, g.sx);
+  int nby = lds_cplx<S>() / g.ny;  // transforms per workgroup, two x columns each
   if (nby > 32) nby = 32;
-  hipLaunchKernelGGL(k_cols_c2r, dim3(cdiv(g.nx, 2 * nby), g.nzn, nf), dim3(FFT_TPB), 0,
-                     c->stream, py, c->tw_y, fp, c->cbuf, c2f, g.nx, nby);
+  hipLaunchKernelGGL(k_cols_c2r<S>, dim3(cdiv(g.nx, 2 * nby), g.nzn, nf), dim3(FFT_TPB), 0,
+                     c->stream, py, twiddles_y<S>(c), fp, cb, c2f, g.nx, nby);
   PMPIC_CHECK(hipGetLastError());
   return PMPIC_OK;
+}
+
+template int fft_forward_t<double>(Ctx *, int, const double *const *, double *const *);
+template int fft_forward_t<float>(Ctx *, int, const double *const *, float *const *);
+template int fft_backward_t<double>(Ctx *, int, const double *const *, double *const *,
+                                    float *const *, const int *);
+template int fft_backward_t<float>(Ctx *, int, const float *const *, double *const *,
+                                   float *const *, const int *);
+
+int fft_forward(Ctx *c, int nf, const double *const *src, double *const *dst) {
+  return fft_forward_t<double>(c, nf, src, dst);
+}
+
+int fft_backward(Ctx *c, int nf, const double *const *src, double *const *dst,
+                 float *const *dst32, const int *deriv) {
+  return fft_backward_t<double>(c, nf, src, dst, dst32, deriv);
 }
 
 }  // namespace pmpic

@@ -34,15 +34,20 @@ __device__ __forceinline__ SpecDims sdims(const Geometry &g) {
   return d;
 }
 
+// The spectral arrays are of type S: FP64, or FP32 in the FP32-bulk form, whose spectral
+// kernels then read and write FP32 values and compute in FP64 between.
+//
 // diffx of element q (fft_ops.F90:266-269, :295): pair swap along x, times kx
-__device__ __forceinline__ double dx_at(const double *in, const double *kx, int64_t q, int ii,
+template <typename S>
+__device__ __forceinline__ double dx_at(const S *in, const double *kx, int64_t q, int ii,
                                         int64_t col) {
-  return ((ii & 1) == 0 ? (-1. * in[q + 1]) : in[q - 1]) * kx[col];
+  return ((ii & 1) == 0 ? (-1. * (double)in[q + 1]) : (double)in[q - 1]) * kx[col];
 }
 // diffy of element q (fft_ops.F90:366-369, :394)
-__device__ __forceinline__ double dy_at(const double *in, const double *ky, int64_t q, int jj,
+template <typename S>
+__device__ __forceinline__ double dy_at(const S *in, const double *ky, int64_t q, int jj,
                                         int sx, int64_t col) {
-  return ((jj & 1) == 0 ? (-1. * in[q + sx]) : in[q - sx]) * ky[col];
+  return ((jj & 1) == 0 ? (-1. * (double)in[q + sx]) : (double)in[q - sx]) * ky[col];
 }
 
 __global__ void k_diffx(Geometry g, const double *__restrict__ in, double *__restrict__ out,
@@ -65,20 +70,23 @@ __global__ void k_diffy(Geometry g, const double *__restrict__ in, double *__res
   out[q] = dy_at(in, ky, q, jj, g.sx, col);
 }
 
-__global__ void k_filter(Geometry g, const double *__restrict__ in, double *__restrict__ out,
+template <typename S>
+__global__ void k_filter(Geometry g, const S *__restrict__ in, S *__restrict__ out,
                          const double *__restrict__ filt) {
   const int64_t ssz = (int64_t)g.nzn * g.sy * g.sx;
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= ssz) return;
-  out[q] = in[q] * filt[q % ((int64_t)g.sy * g.sx)];
+  out[q] = (S)((double)in[q] * filt[q % ((int64_t)g.sy * g.sx)]);
 }
 
 // vort2vel.F90:158-174: f = (diffx(a) + diffy(b)) + dc/dz (explicit 2nd order, one-sided at
 // the ends), then the constant column zeroed
-__global__ void k_divergence(Geometry g, const double *__restrict__ a, const double *__restrict__ b,
-                             const double *__restrict__ c, double *__restrict__ f,
+template <typename S>
+__global__ void k_divergence(Geometry g, const S *__restrict__ a, const S *__restrict__ b,
+                             const S *__restrict__ cs, S *__restrict__ f,
                              const double *__restrict__ kx, const double *__restrict__ ky,
                              double hdzi) {
+  auto c = [&](int64_t i) { return (double)cs[i]; };
   const int64_t plane = (int64_t)g.sy * g.sx;
   const int64_t ssz = (int64_t)g.nzn * plane;
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -91,19 +99,20 @@ __global__ void k_divergence(Geometry g, const double *__restrict__ a, const dou
   const double e = dy_at(b, ky, q, jj, g.sx, col);
   double fz;
   if (k == 0)
-    fz = hdzi * (4. * c[q + plane] - c[q + 2 * plane] - 3. * c[q]);
+    fz = hdzi * (4. * c(q + plane) - c(q + 2 * plane) - 3. * c(q));
   else if (k == nz - 1)
-    fz = hdzi * (c[q - 2 * plane] + 3. * c[q] - 4. * c[q - plane]);
+    fz = hdzi * (c(q - 2 * plane) + 3. * c(q) - 4. * c(q - plane));
   else
-    fz = (c[q + plane] - c[q - plane]) * hdzi;
+    fz = (c(q + plane) - c(q - plane)) * hdzi;
   double r = d + e + fz;
   if (col == 0) r = 0.0;
-  f[q] = r;
+  f[q] = (S)r;
 }
 
 // vort2vel.F90:184-194: a -= diffx(f); b -= diffy(f)
-__global__ void k_correct_ab(Geometry g, double *__restrict__ a, double *__restrict__ b,
-                             const double *__restrict__ f, const double *__restrict__ kx,
+template <typename S>
+__global__ void k_correct_ab(Geometry g, S *__restrict__ a, S *__restrict__ b,
+                             const S *__restrict__ f, const double *__restrict__ kx,
                              const double *__restrict__ ky) {
   const int64_t plane = (int64_t)g.sy * g.sx;
   const int64_t ssz = (int64_t)g.nzn * plane;
@@ -111,16 +120,17 @@ __global__ void k_correct_ab(Geometry g, double *__restrict__ a, double *__restr
   if (q >= ssz) return;
   const int64_t col = q % plane;
   const int ii = (int)(q % g.sx), jj = (int)(col / g.sx);
-  a[q] = a[q] - dx_at(f, kx, q, ii, col);
-  b[q] = b[q] - dy_at(f, ky, q, jj, g.sx, col);
+  a[q] = (S)((double)a[q] - dx_at(f, kx, q, ii, col));
+  b[q] = (S)((double)b[q] - dy_at(f, ky, q, jj, g.sx, col));
 }
 
 // vort2vel.F90:197-218: c -= dz(f); c(level 1) = 0; d,e,f = filtered a,b,c; cache the top and
 // bottom levels of the unfiltered a, b
-__global__ void k_correct_c_filter(Geometry g, const double *__restrict__ a,
-                                   const double *__restrict__ b, double *__restrict__ c,
-                                   const double *__restrict__ dzf, double *__restrict__ d,
-                                   double *__restrict__ e, double *__restrict__ f,
+template <typename S>
+__global__ void k_correct_c_filter(Geometry g, const S *__restrict__ a,
+                                   const S *__restrict__ b, S *__restrict__ c,
+                                   const S *__restrict__ dzf, S *__restrict__ d,
+                                   S *__restrict__ e, S *__restrict__ f,
                                    const double *__restrict__ filt, double *__restrict__ atop,
                                    double *__restrict__ abot, double *__restrict__ btop,
                                    double *__restrict__ bbot) {
@@ -130,19 +140,21 @@ __global__ void k_correct_c_filter(Geometry g, const double *__restrict__ a,
   if (q >= ssz) return;
   const int64_t col = q % plane;
   const int k = (int)(q / plane);
-  double cv = c[q] - dzf[q];
+  double cv = (double)c[q] - (double)dzf[q];
   if (k == 0) cv = 0.;
-  c[q] = cv;
+  c[q] = (S)cv;
   const double fl = filt[col];
-  d[q] = a[q] * fl;
-  e[q] = b[q] * fl;
-  f[q] = cv * fl;
-  if (k == g.nzn - 1) { atop[col] = a[q]; btop[col] = b[q]; }
-  if (k == 0) { abot[col] = a[q]; bbot[col] = b[q]; }
+  const double av = a[q], bv = b[q];
+  d[q] = (S)(av * fl);
+  e[q] = (S)(bv * fl);
+  f[q] = (S)((double)(S)cv * fl);
+  if (k == g.nzn - 1) { atop[col] = av; btop[col] = bv; }
+  if (k == 0) { abot[col] = av; bbot[col] = bv; }
 }
 
 // mean flow, vort2vel.F90:230-241 (sequential over z; 1./nz is a default-real quotient)
-__global__ void k_mean_flow(Geometry g, const double *__restrict__ d, const double *__restrict__ e,
+template <typename S>
+__global__ void k_mean_flow(Geometry g, const S *__restrict__ d, const S *__restrict__ e,
                             double *__restrict__ ubar, double *__restrict__ vbar) {
   // the (1,1) columns are staged in LDS by the whole block; one thread then runs the
   // reference's sequential recurrences and sums on them, and the block writes the result
@@ -182,7 +194,8 @@ __global__ void k_mean_flow(Geometry g, const double *__restrict__ d, const doub
 // Column stage-in with many loads in flight: a chunk of LVL levels is read into registers
 // (independent loads, issued back to back) before any is written to the LDS slice.
 constexpr int LVL = 16;
-__device__ __forceinline__ void column_to_lds(const double *__restrict__ f, int64_t plane, int nz,
+template <typename S>
+__device__ __forceinline__ void column_to_lds(const S *__restrict__ f, int64_t plane, int nz,
                                               double *F, int nc) {
   for (int k0 = 0; k0 < nz; k0 += LVL) {
     double t[LVL];
@@ -211,8 +224,8 @@ struct Col {
 // laplinv (fft_ops.F90:505-601) of one column, in place.  The right-hand side of row k reads
 // the original f(k-1), f(k), f(k+1); a sliding window keeps the originals while the forward
 // sweep overwrites f with d'.  Returns PMPIC_ERR_ZERO_PIVOT on a zero pivot.
-template <typename C>
-__device__ __forceinline__ int laplinv_column(const Geometry &g, C F, C CP, double k2, int df0) {
+template <typename CF, typename CC>
+__device__ __forceinline__ int laplinv_column(const Geometry &g, CF F, CC CP, double k2, int df0) {
   const int nz = g.nzn;
   const double f16 = (double)(1.f / 6.f), f13 = (double)(1.f / 3.f);
   const double f112 = (double)(1.f / 12.f), f56 = (double)(5.f / 6.f);
@@ -264,9 +277,9 @@ __device__ __forceinline__ int laplinv_column(const Geometry &g, C F, C CP, doub
 
 // diffz (fft_ops.F90:419-483) of one column; O may alias F (the original f(k-2), f(k-1) are
 // kept in registers, and f(k) is read before d'(k-1) is written).
-template <typename CI, typename CO>
-__device__ __forceinline__ int diffz_column(const Geometry &g, CI F, CO O, CO CP, bool bt, double bot,
-                                            double top) {
+template <typename CI, typename CO, typename CC>
+__device__ __forceinline__ int diffz_column(const Geometry &g, CI F, CO O, CC CP, bool bt,
+                                            double bot, double top) {
   const int nz = g.nzn;
   const double a1 = (double)(1.f / 6.f), b1 = (double)(2.f / 3.f), c1 = (double)(1.f / 6.f);
   const double al = (double)(1.f / 3.f), cl = (double)(1.f / 3.f);
@@ -309,29 +322,34 @@ __device__ __forceinline__ int diffz_column(const Geometry &g, CI F, CO O, CO CP
   return PMPIC_OK;
 }
 
-// up to 4 fields per launch (blockIdx.y)
+// up to 4 fields per launch (blockIdx.y); the boundary planes are FP64 in both forms
+template <typename S>
 struct ColFields {
-  const double *in[4];
-  double *out[4];
+  const S *in[4];
+  S *out[4];
   const double *bot[4];
   const double *top[4];
   int df0[4];
 };
 
-__global__ void k_laplinv(Geometry g, ColFields cf, double *__restrict__ cp,
+template <typename S>
+__global__ void k_laplinv(Geometry g, ColFields<S> cf, double *__restrict__ cp,
                           const double *__restrict__ k2a, DevScalars *sc) {
   const int64_t plane = (int64_t)g.sy * g.sx;
   const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= plane) return;
   const double k2 = k2a[col];
   if (k2 == 0) return;
-  Col<double> F{cf.out[blockIdx.y] + col, plane}, CP{cp + (int64_t)blockIdx.y * g.nzn * plane + col, plane};
+  Col<S> F{cf.out[blockIdx.y] + col, plane};
+  Col<double> CP{cp + (int64_t)blockIdx.y * g.nzn * plane + col, plane};
   const int st = laplinv_column(g, F, CP, k2, cf.df0[blockIdx.y]);
   if (st != PMPIC_OK) raise_status(sc, st);
 }
 
-__global__ void __launch_bounds__(64) k_laplinv_lds(Geometry g, ColFields cf, const double *__restrict__ k2a,
-                              DevScalars *sc) {
+template <typename S>
+__global__ void __launch_bounds__(64) k_laplinv_lds(Geometry g, ColFields<S> cf,
+                                                    const double *__restrict__ k2a,
+                                                    DevScalars *sc) {
   extern __shared__ double sh[];
   const int nc = blockDim.x, t = threadIdx.x, nz = g.nzn;
   const int64_t plane = (int64_t)g.sy * g.sx;
@@ -339,7 +357,7 @@ __global__ void __launch_bounds__(64) k_laplinv_lds(Geometry g, ColFields cf, co
   if (col >= plane) return;  // no barriers below: each thread touches only its own slice
   const double k2 = k2a[col];
   if (k2 == 0) return;
-  double *f = cf.out[blockIdx.y] + col;
+  S *f = cf.out[blockIdx.y] + col;
   Col<double> F{sh + t, nc}, CP{sh + (int64_t)nz * nc + t, nc};
   column_to_lds(f, plane, nz, sh + t, nc);
   const int st = laplinv_column(g, F, CP, k2, cf.df0[blockIdx.y]);
@@ -348,23 +366,26 @@ __global__ void __launch_bounds__(64) k_laplinv_lds(Geometry g, ColFields cf, co
     return;
   }
 #pragma unroll 8
-  for (int k = 0; k < nz; k++) f[(int64_t)k * plane] = F[k];
+  for (int k = 0; k < nz; k++) f[(int64_t)k * plane] = (S)F[k];
 }
 
-__global__ void k_diffz(Geometry g, ColFields cf, double *__restrict__ cp, DevScalars *sc) {
+template <typename S>
+__global__ void k_diffz(Geometry g, ColFields<S> cf, double *__restrict__ cp, DevScalars *sc) {
   const int64_t plane = (int64_t)g.sy * g.sx;
   const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= plane) return;
   const int fi = blockIdx.y;
   const bool bt = cf.bot[fi] != nullptr && cf.top[fi] != nullptr;
-  Col<const double> F{cf.in[fi] + col, plane};
-  Col<double> O{cf.out[fi] + col, plane}, CP{cp + (int64_t)fi * g.nzn * plane + col, plane};
+  Col<const S> F{cf.in[fi] + col, plane};
+  Col<S> O{cf.out[fi] + col, plane};
+  Col<double> CP{cp + (int64_t)fi * g.nzn * plane + col, plane};
   const int st = diffz_column(g, F, O, CP, bt, bt ? cf.bot[fi][col] : 0.0,
                               bt ? cf.top[fi][col] : 0.0);
   if (st != PMPIC_OK) raise_status(sc, st);
 }
 
-__global__ void __launch_bounds__(64) k_diffz_lds(Geometry g, ColFields cf, DevScalars *sc) {
+template <typename S>
+__global__ void __launch_bounds__(64) k_diffz_lds(Geometry g, ColFields<S> cf, DevScalars *sc) {
   extern __shared__ double sh[];
   const int nc = blockDim.x, t = threadIdx.x, nz = g.nzn;
   const int64_t plane = (int64_t)g.sy * g.sx;
@@ -372,8 +393,8 @@ __global__ void __launch_bounds__(64) k_diffz_lds(Geometry g, ColFields cf, DevS
   if (col >= plane) return;  // no barriers below: each thread touches only its own slice
   const int fi = blockIdx.y;
   const bool bt = cf.bot[fi] != nullptr && cf.top[fi] != nullptr;
-  const double *f = cf.in[fi] + col;
-  double *o = cf.out[fi] + col;
+  const S *f = cf.in[fi] + col;
+  S *o = cf.out[fi] + col;
   Col<double> F{sh + t, nc}, CP{sh + (int64_t)nz * nc + t, nc};
   column_to_lds(f, plane, nz, sh + t, nc);
   const int st = diffz_column(g, F, F, CP, bt, bt ? cf.bot[fi][col] : 0.0,
@@ -383,7 +404,7 @@ __global__ void __launch_bounds__(64) k_diffz_lds(Geometry g, ColFields cf, DevS
     return;
   }
 #pragma unroll 8
-  for (int k = 0; k < nz; k++) o[(int64_t)k * plane] = F[k];
+  for (int k = 0; k < nz; k++) o[(int64_t)k * plane] = (S)F[k];
 }
 
 // Register-resident column solvers: the column stays in registers (NZ levels, unrolled) and
@@ -422,8 +443,8 @@ __global__ void k_dz_tables(Geometry g, double *__restrict__ den, double *__rest
 // next quarter by a lane shuffle, so no level waits on memory and every operation is the one of
 // laplinv_column in the same order: results are bit-identical.  A zero pivot is flagged and
 // raised after the sweeps (the reference stops there; the column's values are then moot).
-template <int NZ>
-__global__ void __launch_bounds__(64) k_laplinv_q4(Geometry g, ColFields cf,
+template <int NZ, typename S>
+__global__ void __launch_bounds__(64) k_laplinv_q4(Geometry g, ColFields<S> cf,
                                                    const double *__restrict__ k2a,
                                                    DevScalars *sc) {
   constexpr int L = (NZ + 3) / 4;
@@ -435,7 +456,7 @@ __global__ void __launch_bounds__(64) k_laplinv_q4(Geometry g, ColFields cf,
   const double k2 = k2a[col];
   if (k2 == 0) return;
   const int df0 = cf.df0[blockIdx.y];
-  double *f = cf.out[blockIdx.y] + col;
+  S *f = cf.out[blockIdx.y] + col;
   double F[L], CPl[L];
 #pragma unroll
   for (int i = 0; i < L; i++) {
@@ -526,11 +547,11 @@ __global__ void __launch_bounds__(64) k_laplinv_q4(Geometry g, ColFields cf,
   if (bad) raise_status(sc, PMPIC_ERR_ZERO_PIVOT);
 #pragma unroll
   for (int i = 0; i < L; i++)
-    if (q * L + i < NZ) f[(int64_t)(q * L + i) * plane] = F[i];
+    if (q * L + i < NZ) f[(int64_t)(q * L + i) * plane] = (S)F[i];
 }
 
-template <int NZ>
-__global__ void __launch_bounds__(64) k_diffz_reg(Geometry g, ColFields cf,
+template <int NZ, typename S>
+__global__ void __launch_bounds__(64) k_diffz_reg(Geometry g, ColFields<S> cf,
                                                   const double *__restrict__ tden,
                                                   const double *__restrict__ tcp,
                                                   DevScalars *sc) {
@@ -547,8 +568,8 @@ __global__ void __launch_bounds__(64) k_diffz_reg(Geometry g, ColFields cf,
   }
   __syncthreads();
   if (col >= plane) return;
-  const double *fin = cf.in[fi] + col;
-  double *o = cf.out[fi] + col;
+  const S *fin = cf.in[fi] + col;
+  S *o = cf.out[fi] + col;
   double F[NZ];
 #pragma unroll
   for (int k = 0; k < NZ; k++) F[k] = fin[(int64_t)k * plane];
@@ -585,7 +606,7 @@ __global__ void __launch_bounds__(64) k_diffz_reg(Geometry g, ColFields cf,
     fnext = v;
   }
 #pragma unroll
-  for (int k = 0; k < NZ; k++) o[(int64_t)k * plane] = F[k];
+  for (int k = 0; k < NZ; k++) o[(int64_t)k * plane] = (S)F[k];
 }
 
 // Long columns (more z nodes than the instances above): Q lanes per column, lane q holding
@@ -593,8 +614,8 @@ __global__ void __launch_bounds__(64) k_diffz_reg(Geometry g, ColFields cf,
 // run time.  The sweeps run lane group after lane group as in k_laplinv_q4, the recurrence
 // state crossing by lane shuffles, every operation the one of laplinv_column / k_diffz_reg in
 // the same order, so results are bit-identical to them.
-template <int Q, int L>
-__global__ void __launch_bounds__(64) k_laplinv_qn(Geometry g, ColFields cf,
+template <int Q, int L, typename S>
+__global__ void __launch_bounds__(64) k_laplinv_qn(Geometry g, ColFields<S> cf,
                                                    const double *__restrict__ k2a,
                                                    DevScalars *sc) {
   constexpr int CPW = 64 / Q;  // columns per wavefront
@@ -607,7 +628,7 @@ __global__ void __launch_bounds__(64) k_laplinv_qn(Geometry g, ColFields cf,
   const double k2 = k2a[col];
   if (k2 == 0) return;
   const int df0 = cf.df0[blockIdx.y];
-  double *f = cf.out[blockIdx.y] + col;
+  S *f = cf.out[blockIdx.y] + col;
   double F[L], CPl[L];
 #pragma unroll
   for (int i = 0; i < L; i++) {
@@ -694,11 +715,11 @@ __global__ void __launch_bounds__(64) k_laplinv_qn(Geometry g, ColFields cf,
   if (bad) raise_status(sc, PMPIC_ERR_ZERO_PIVOT);
 #pragma unroll
   for (int i = 0; i < L; i++)
-    if (q * L + i < NZ) f[(int64_t)(q * L + i) * plane] = F[i];
+    if (q * L + i < NZ) f[(int64_t)(q * L + i) * plane] = (S)F[i];
 }
 
-template <int Q, int L>
-__global__ void __launch_bounds__(64) k_diffz_qn(Geometry g, ColFields cf,
+template <int Q, int L, typename S>
+__global__ void __launch_bounds__(64) k_diffz_qn(Geometry g, ColFields<S> cf,
                                                  const double *__restrict__ tden,
                                                  const double *__restrict__ tcp,
                                                  DevScalars *sc) {
@@ -717,8 +738,8 @@ __global__ void __launch_bounds__(64) k_diffz_qn(Geometry g, ColFields cf,
   }
   __syncthreads();
   if (col >= plane) return;  // all Q lanes of a column leave together
-  const double *fin = cf.in[fi] + col;
-  double *o = cf.out[fi] + col;
+  const S *fin = cf.in[fi] + col;
+  S *o = cf.out[fi] + col;
   double F[L];
 #pragma unroll
   for (int i = 0; i < L; i++) F[i] = (q * L + i < NZ) ? fin[(int64_t)(q * L + i) * plane] : 0.0;
@@ -785,18 +806,19 @@ __global__ void __launch_bounds__(64) k_diffz_qn(Geometry g, ColFields cf,
   if (bad) raise_status(sc, PMPIC_ERR_ZERO_PIVOT);
 #pragma unroll
   for (int i = 0; i < L; i++)
-    if (q * L + i < NZ) o[(int64_t)(q * L + i) * plane] = F[i];
+    if (q * L + i < NZ) o[(int64_t)(q * L + i) * plane] = (S)F[i];
 }
 
 // u, v, w spectra (vort2vel.F90:251-320): fu = dzb - diffy(c), column (1,1) = ubar;
 // fv = diffx(c) - dza, column (1,1) = vbar; fw = diffy(a) - diffx(b); u_s.. = filtered
-__global__ void k_velocity_spectra(Geometry g, const double *__restrict__ a,
-                                   const double *__restrict__ b, const double *__restrict__ c,
-                                   const double *__restrict__ dzb, const double *__restrict__ dza,
+template <typename S>
+__global__ void k_velocity_spectra(Geometry g, const S *__restrict__ a,
+                                   const S *__restrict__ b, const S *__restrict__ c,
+                                   const S *__restrict__ dzb, const S *__restrict__ dza,
                                    const double *__restrict__ ubar, const double *__restrict__ vbar,
-                                   double *__restrict__ fu, double *__restrict__ fv,
-                                   double *__restrict__ fw, double *__restrict__ us,
-                                   double *__restrict__ vs, double *__restrict__ ws,
+                                   S *__restrict__ fu, S *__restrict__ fv,
+                                   S *__restrict__ fw, S *__restrict__ us,
+                                   S *__restrict__ vs, S *__restrict__ ws,
                                    const double *__restrict__ kx, const double *__restrict__ ky,
                                    const double *__restrict__ filt) {
   const int64_t plane = (int64_t)g.sy * g.sx;
@@ -807,13 +829,13 @@ __global__ void k_velocity_spectra(Geometry g, const double *__restrict__ a,
   const int k = (int)(q / plane);
   const int ii = (int)(q % g.sx), jj = (int)(col / g.sx);
   const double fl = filt[col];
-  double u = dzb[q] - dy_at(c, ky, q, jj, g.sx, col);
+  double u = (double)dzb[q] - dy_at(c, ky, q, jj, g.sx, col);
   if (col == 0) u = ubar[k];
-  double v = dx_at(c, kx, q, ii, col) - dza[q];
+  double v = dx_at(c, kx, q, ii, col) - (double)dza[q];
   if (col == 0) v = vbar[k];
   const double w = dy_at(a, ky, q, jj, g.sx, col) - dx_at(b, kx, q, ii, col);
-  fu[q] = u; fv[q] = v; fw[q] = w;
-  us[q] = u * fl; vs[q] = v * fl; ws[q] = w * fl;
+  fu[q] = (S)u; fv[q] = (S)v; fw[q] = (S)w;
+  us[q] = (S)(u * fl); vs[q] = (S)(v * fl); ws[q] = (S)(w * fl);
 }
 
 // vorticity_tendency.F90:169-265, physical interior, accumulation order as written
@@ -991,12 +1013,6 @@ int op_diffy(Ctx *c, const double *in, double *out) {
   PMPIC_CHECK(hipGetLastError());
   return PMPIC_OK;
 }
-int op_filter(Ctx *c, const double *in, double *out) {
-  hipLaunchKernelGGL(k_filter, dim3(cdiv(ssz_of(c->g), TPB)), dim3(TPB), 0, c->stream, c->g, in,
-                     out, c->filt);
-  PMPIC_CHECK(hipGetLastError());
-  return PMPIC_OK;
-}
 // threads per block of the LDS column kernels: the most columns (64 down to 8) whose two
 // slices (f and cp) fit in 64 KiB of LDS; 0 = columns too long, solve straight from HBM
 static int lds_cols(const Geometry &g) {
@@ -1010,12 +1026,22 @@ static int lds_cols(const Geometry &g) {
 // longer columns, up to LONG_Q * LONG_L z nodes: LONG_Q lanes per column, LONG_L levels each
 constexpr int LONG_Q = 8, LONG_L = 33;
 
-int op_diffz_n(Ctx *c, int nf, const double *const *in, double *const *out,
-               const double *const *bot, const double *const *top) {
+template <typename S>
+int op_filter_t(Ctx *c, const S *in, S *out) {
+  hipLaunchKernelGGL(k_filter<S>, dim3(cdiv(ssz_of(c->g), TPB)), dim3(TPB), 0, c->stream, c->g,
+                     in, out, c->filt);
+  PMPIC_CHECK(hipGetLastError());
+  return PMPIC_OK;
+}
+int op_filter(Ctx *c, const double *in, double *out) { return op_filter_t<double>(c, in, out); }
+
+template <typename S>
+int op_diffz_n(Ctx *c, int nf, const S *const *in, S *const *out, const double *const *bot,
+               const double *const *top) {
   if (nf < 1 || nf > 4) return PMPIC_ERR_ARG;
   const int64_t plane = plane_of(c->g);
   const int nc = lds_cols(c->g);
-  ColFields rf{};
+  ColFields<S> rf{};
   for (int f = 0; f < nf; f++) {
     rf.in[f] = in[f];
     rf.out[f] = out[f];
@@ -1025,7 +1051,7 @@ int op_diffz_n(Ctx *c, int nf, const double *const *in, double *const *out,
   switch (c->g.nzn) {
 #define PMPIC_DZ_CASE(NZ)                                                                     \
   case NZ:                                                                                     \
-    hipLaunchKernelGGL(k_diffz_reg<NZ>, dim3(cdiv(plane, 64), nf), dim3(64), 0, c->stream,   \
+    hipLaunchKernelGGL((k_diffz_reg<NZ, S>), dim3(cdiv(plane, 64), nf), dim3(64), 0, c->stream, \
                        c->g, rf, c->dz_den, c->dz_cp, c->dsc);                                 \
     PMPIC_CHECK(hipGetLastError());                                                            \
     return PMPIC_OK;
@@ -1035,29 +1061,22 @@ int op_diffz_n(Ctx *c, int nf, const double *const *in, double *const *out,
       break;
   }
   if (c->g.nzn > 129 && c->g.nzn <= LONG_Q * LONG_L) {
-    hipLaunchKernelGGL((k_diffz_qn<LONG_Q, LONG_L>), dim3(cdiv(plane, 64 / LONG_Q), nf), dim3(64),
-                       0, c->stream, c->g, rf, c->dz_den, c->dz_cp, c->dsc);
+    hipLaunchKernelGGL((k_diffz_qn<LONG_Q, LONG_L, S>), dim3(cdiv(plane, 64 / LONG_Q), nf),
+                       dim3(64), 0, c->stream, c->g, rf, c->dz_den, c->dz_cp, c->dsc);
     PMPIC_CHECK(hipGetLastError());
     return PMPIC_OK;
   }
   if (nc > 0) {
-    ColFields cf{};
-    for (int f = 0; f < nf; f++) {
-      cf.in[f] = in[f];
-      cf.out[f] = out[f];
-      cf.bot[f] = bot ? bot[f] : nullptr;
-      cf.top[f] = top ? top[f] : nullptr;
-    }
-    hipLaunchKernelGGL(k_diffz_lds, dim3(cdiv(plane, nc), nf), dim3(nc),
-                       (size_t)2 * nc * c->g.nzn * sizeof(double), c->stream, c->g, cf, c->dsc);
+    hipLaunchKernelGGL(k_diffz_lds<S>, dim3(cdiv(plane, nc), nf), dim3(nc),
+                       (size_t)2 * nc * c->g.nzn * sizeof(double), c->stream, c->g, rf, c->dsc);
   } else {
     for (int f = 0; f < nf; f++) {  // one field at a time: sder[7] holds one field's cp
-      ColFields one{};
+      ColFields<S> one{};
       one.in[0] = in[f];
       one.out[0] = out[f];
       one.bot[0] = bot ? bot[f] : nullptr;
       one.top[0] = top ? top[f] : nullptr;
-      hipLaunchKernelGGL(k_diffz, dim3(cdiv(plane, 64), 1), dim3(64), 0, c->stream, c->g, one,
+      hipLaunchKernelGGL(k_diffz<S>, dim3(cdiv(plane, 64), 1), dim3(64), 0, c->stream, c->g, one,
                          c->sder[7], c->dsc);
     }
   }
@@ -1065,13 +1084,15 @@ int op_diffz_n(Ctx *c, int nf, const double *const *in, double *const *out,
   return PMPIC_OK;
 }
 int op_diffz(Ctx *c, const double *in, double *out, const double *bot, const double *top) {
-  return op_diffz_n(c, 1, &in, &out, bot ? &bot : nullptr, top ? &top : nullptr);
+  return op_diffz_n<double>(c, 1, &in, &out, bot ? &bot : nullptr, top ? &top : nullptr);
 }
-int op_laplinv_n(Ctx *c, int nf, double *const *f, const int *df0) {
+
+template <typename S>
+int op_laplinv_n(Ctx *c, int nf, S *const *f, const int *df0) {
   if (nf < 1 || nf > 4) return PMPIC_ERR_ARG;
   const int64_t plane = plane_of(c->g);
   const int nc = lds_cols(c->g);
-  ColFields rf{};
+  ColFields<S> rf{};
   for (int q = 0; q < nf; q++) {
     rf.out[q] = f[q];
     rf.df0[q] = df0[q] ? 1 : 0;
@@ -1079,8 +1100,8 @@ int op_laplinv_n(Ctx *c, int nf, double *const *f, const int *df0) {
   switch (c->g.nzn) {
 #define PMPIC_LAP_CASE(NZ)                                                                    \
   case NZ:                                                                                     \
-    hipLaunchKernelGGL((k_laplinv_q4<NZ>), dim3(cdiv(plane, 16), nf), dim3(64), 0, c->stream,   \
-                       c->g, rf, c->k2, c->dsc);                                                \
+    hipLaunchKernelGGL((k_laplinv_q4<NZ, S>), dim3(cdiv(plane, 16), nf), dim3(64), 0,         \
+                       c->stream, c->g, rf, c->k2, c->dsc);                                    \
     PMPIC_CHECK(hipGetLastError());                                                            \
     return PMPIC_OK;
     PMPIC_REG_COLUMNS(PMPIC_LAP_CASE)
@@ -1089,108 +1110,128 @@ int op_laplinv_n(Ctx *c, int nf, double *const *f, const int *df0) {
       break;
   }
   if (c->g.nzn > 129 && c->g.nzn <= LONG_Q * LONG_L) {
-    hipLaunchKernelGGL((k_laplinv_qn<LONG_Q, LONG_L>), dim3(cdiv(plane, 64 / LONG_Q), nf),
+    hipLaunchKernelGGL((k_laplinv_qn<LONG_Q, LONG_L, S>), dim3(cdiv(plane, 64 / LONG_Q), nf),
                        dim3(64), 0, c->stream, c->g, rf, c->k2, c->dsc);
     PMPIC_CHECK(hipGetLastError());
     return PMPIC_OK;
   }
   if (nc > 0) {
-    ColFields cf{};
-    for (int q = 0; q < nf; q++) {
-      cf.out[q] = f[q];
-      cf.df0[q] = df0[q];
-    }
-    hipLaunchKernelGGL(k_laplinv_lds, dim3(cdiv(plane, nc), nf), dim3(nc),
-                       (size_t)2 * nc * c->g.nzn * sizeof(double), c->stream, c->g, cf, c->k2,
+    hipLaunchKernelGGL(k_laplinv_lds<S>, dim3(cdiv(plane, nc), nf), dim3(nc),
+                       (size_t)2 * nc * c->g.nzn * sizeof(double), c->stream, c->g, rf, c->k2,
                        c->dsc);
   } else {
     for (int q = 0; q < nf; q++) {
-      ColFields one{};
+      ColFields<S> one{};
       one.out[0] = f[q];
       one.df0[0] = df0[q];
-      hipLaunchKernelGGL(k_laplinv, dim3(cdiv(plane, 64), 1), dim3(64), 0, c->stream, c->g, one,
-                         c->sder[7], c->k2, c->dsc);
+      hipLaunchKernelGGL(k_laplinv<S>, dim3(cdiv(plane, 64), 1), dim3(64), 0, c->stream, c->g,
+                         one, c->sder[7], c->k2, c->dsc);
     }
   }
   PMPIC_CHECK(hipGetLastError());
   return PMPIC_OK;
 }
-int op_laplinv(Ctx *c, double *f, int df0) { return op_laplinv_n(c, 1, &f, &df0); }
+int op_laplinv(Ctx *c, double *f, int df0) { return op_laplinv_n<double>(c, 1, &f, &df0); }
+
+namespace {
+// the spectral scratch arrays as arrays of the form's spectral type (the FP32-bulk form uses
+// the first half of each FP64 allocation)
+template <typename S> S *sp(double *a) { return reinterpret_cast<S *>(a); }
+}  // namespace
 
 // vort2vel timestep_callback (vort2vel.F90:134-320) after par2grid of p, q, r into the grids:
 // corrected and filtered vorticity back into grid[GP..GR], velocities into grid[GU..GW], the
-// filtered velocity spectra into su, sv, sw.
-int vort2vel_spectral(Ctx *c, int stage) {
-  (void)stage;
+// filtered velocity spectra into su, sv, sw.  S: the spectral arrays' type (FP32 in the
+// FP32-bulk form: its transforms and spectral arrays are FP32, the mean flow, the boundary
+// planes and the column solvers' arithmetic FP64).
+template <typename S>
+static int vort2vel_spectral_t(Ctx *c) {
   const Geometry &g = c->g;
   const int64_t ssz = ssz_of(g);
   const dim3 eb(cdiv(ssz, TPB)), et(TPB);
+  S *sa = sp<S>(c->sa), *sb = sp<S>(c->sb), *sc = sp<S>(c->sc), *sd = sp<S>(c->sd);
+  S *se = sp<S>(c->se), *sf = sp<S>(c->sf);
+  S *sder[5] = {sp<S>(c->sder[0]), sp<S>(c->sder[1]), sp<S>(c->sder[2]), sp<S>(c->sder[3]),
+                sp<S>(c->sder[4])};
   {
     const double *src[3] = {c->grid[GP], c->grid[GQ], c->grid[GR]};
-    double *dst[3] = {c->sa, c->sb, c->sc};
-    PMPIC_TRY(fft_forward(c, 3, src, dst));
+    S *dst[3] = {sa, sb, sc};
+    PMPIC_TRY(fft_forward_t<S>(c, 3, src, dst));
   }
   const double hdzi = 1. / 2. / g.dz;
-  hipLaunchKernelGGL(k_divergence, eb, et, 0, c->stream, g, c->sa, c->sb, c->sc, c->sf, c->kx,
-                     c->ky, hdzi);
-  PMPIC_TRY(op_laplinv(c, c->sf, 1));
-  PMPIC_TRY(op_filter(c, c->sf, c->sf));
-  hipLaunchKernelGGL(k_correct_ab, eb, et, 0, c->stream, g, c->sa, c->sb, c->sf, c->kx, c->ky);
-  PMPIC_TRY(op_diffz(c, c->sf, c->sd, nullptr, nullptr));
-  hipLaunchKernelGGL(k_correct_c_filter, eb, et, 0, c->stream, g, c->sa, c->sb, c->sc, c->sd,
-                     c->sd, c->se, c->sf, c->filt, c->atop, c->abot, c->btop, c->bbot);
-  hipLaunchKernelGGL(k_mean_flow, dim3(1), dim3(256), (size_t)(4 * g.nzn + 2) * sizeof(double),
-                     c->stream, g, c->sd, c->se, c->ubar, c->vbar);
+  hipLaunchKernelGGL(k_divergence<S>, eb, et, 0, c->stream, g, sa, sb, sc, sf, c->kx, c->ky,
+                     hdzi);
   {
-    double *fl[3] = {c->sa, c->sb, c->sc};
+    const int df0[1] = {1};
+    PMPIC_TRY(op_laplinv_n<S>(c, 1, &sf, df0));
+  }
+  PMPIC_TRY(op_filter_t<S>(c, sf, sf));
+  hipLaunchKernelGGL(k_correct_ab<S>, eb, et, 0, c->stream, g, sa, sb, sf, c->kx, c->ky);
+  {
+    const S *in[1] = {sf};
+    PMPIC_TRY(op_diffz_n<S>(c, 1, in, &sd, nullptr, nullptr));
+  }
+  hipLaunchKernelGGL(k_correct_c_filter<S>, eb, et, 0, c->stream, g, sa, sb, sc, sd, sd, se, sf,
+                     c->filt, c->atop, c->abot, c->btop, c->bbot);
+  hipLaunchKernelGGL(k_mean_flow<S>, dim3(1), dim3(256), (size_t)(4 * g.nzn + 2) * sizeof(double),
+                     c->stream, g, sd, se, c->ubar, c->vbar);
+  {
+    S *fl[3] = {sa, sb, sc};
     const int df0[3] = {0, 0, 1};
-    PMPIC_TRY(op_laplinv_n(c, 3, fl, df0));
+    PMPIC_TRY(op_laplinv_n<S>(c, 3, fl, df0));
   }
   {
     // dz(b) with b's cached top/bottom into sder[0], dz(a) into sder[1]
-    const double *in[2] = {c->sb, c->sa};
-    double *out[2] = {c->sder[0], c->sder[1]};
+    const S *in[2] = {sb, sa};
+    S *out[2] = {sder[0], sder[1]};
     const double *bot[2] = {c->bbot, c->abot}, *top[2] = {c->btop, c->atop};
-    PMPIC_TRY(op_diffz_n(c, 2, in, out, bot, top));
+    PMPIC_TRY(op_diffz_n<S>(c, 2, in, out, bot, top));
   }
-  hipLaunchKernelGGL(k_velocity_spectra, eb, et, 0, c->stream, g, c->sa, c->sb, c->sc, c->sder[0],
-                     c->sder[1], c->ubar, c->vbar, c->sder[2], c->sder[3], c->sder[4], c->su,
-                     c->sv, c->sw, c->kx, c->ky, c->filt);
+  hipLaunchKernelGGL(k_velocity_spectra<S>, eb, et, 0, c->stream, g, sa, sb, sc, sder[0],
+                     sder[1], c->ubar, c->vbar, sder[2], sder[3], sder[4], sp<S>(c->su),
+                     sp<S>(c->sv), sp<S>(c->sw), c->kx, c->ky, c->filt);
   {
-    const double *src[6] = {c->sd, c->se, c->sf, c->sder[2], c->sder[3], c->sder[4]};
+    const S *src[6] = {sd, se, sf, sder[2], sder[3], sder[4]};
     // the FP32-bulk form keeps u, v, w only as the FP32 grids its gathers read
     const bool b32 = c->precision == 1;
     double *dst[6] = {c->grid[GP], c->grid[GQ], c->grid[GR], b32 ? nullptr : c->grid[GU],
                       b32 ? nullptr : c->grid[GV], b32 ? nullptr : c->grid[GW]};
     // the FP32-bulk form gathers u, v, w from FP32 copies written by the same pass
     float *dst32[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-    if (c->precision == 1)
+    if (b32)
       for (int a = 0; a < 3; a++) dst32[3 + a] = c->b32.grid[a];
-    PMPIC_TRY(fft_backward(c, 6, src, dst, dst32));
+    PMPIC_TRY(fft_backward_t<S>(c, 6, src, dst, dst32, nullptr));
   }
   PMPIC_CHECK(hipGetLastError());
   return PMPIC_OK;
 }
 
+int vort2vel_spectral(Ctx *c, int stage) {
+  (void)stage;
+  return c->precision == 1 ? vort2vel_spectral_t<float>(c) : vort2vel_spectral_t<double>(c);
+}
+
 // vorticity_tendency timestep_callback (vorticity_tendency.F90:147-265) after par2grid of btot
 // into grid[GB]; tendencies into grid[GDP..GDR] (plain FP64 form) or only into the FP32 grids
 // the FP32-bulk gathers read
-int vorticity_tendency_spectral(Ctx *c, int stage) {
-  (void)stage;
+template <typename S>
+static int vorticity_tendency_spectral_t(Ctx *c) {
   const Geometry &g = c->g;
+  S *sbs = sp<S>(c->sbs);
   {
     const double *src[1] = {c->grid[GB]};
-    double *dst[1] = {c->sbs};
-    PMPIC_TRY(fft_forward(c, 1, src, dst));
+    S *dst[1] = {sbs};
+    PMPIC_TRY(fft_forward_t<S>(c, 1, src, dst));
   }
   {
     // the eight spectral derivatives of vorticity_tendency.F90:155-232 (db/dy, du/dx, du/dy,
     // dw/dx, db/dx, dv/dx, dv/dy, dw/dy), each taken while its inverse transform reads it
-    const double *src[8] = {c->sbs, c->su, c->su, c->sw, c->sbs, c->sv, c->sv, c->sw};
+    S *su = sp<S>(c->su), *sv = sp<S>(c->sv), *sw = sp<S>(c->sw);
+    const S *src[8] = {sbs, su, su, sw, sbs, sv, sv, sw};
     const int op[8] = {SPEC_DY, SPEC_DX, SPEC_DY, SPEC_DX, SPEC_DX, SPEC_DX, SPEC_DY, SPEC_DY};
     double *dst[8];
     for (int f = 0; f < 8; f++) dst[f] = c->gtmp[f];
-    PMPIC_TRY(fft_backward(c, 8, src, dst, nullptr, op));
+    PMPIC_TRY(fft_backward_t<S>(c, 8, src, dst, nullptr, op));
   }
   const int64_t nnode = (int64_t)g.nzn * g.ny * g.nx;
   const bool b32 = c->precision == 1;  // the FP32-bulk form: FP32 tendencies only
@@ -1202,6 +1243,12 @@ int vorticity_tendency_spectral(Ctx *c, int stage) {
                      b32 ? c->b32.grid[4] : nullptr, b32 ? c->b32.grid[5] : nullptr);
   PMPIC_CHECK(hipGetLastError());
   return PMPIC_OK;
+}
+
+int vorticity_tendency_spectral(Ctx *c, int stage) {
+  (void)stage;
+  return c->precision == 1 ? vorticity_tendency_spectral_t<float>(c)
+                           : vorticity_tendency_spectral_t<double>(c);
 }
 
 int grid_max_reduce(Ctx *c) {
