@@ -161,6 +161,10 @@ int pmpic_timing_enable(pmpic_ctx *ctx, int32_t enable);
 int pmpic_hbm_triad(int32_t device, int64_t bytes_per_array, int32_t reps, double *best_ms,
                     double *gbps);
 int pmpic_timing_read(pmpic_ctx *ctx, int32_t which, double *total_ms, int64_t *launches);
+/* the same timer's statistics for the reference's timer summary (timer.F90:247-303): total,
+ * number of intervals, longest interval and the sum of squared intervals (ms, ms^2) */
+int pmpic_timing_stats(pmpic_ctx *ctx, int32_t which, double *total_ms, int64_t *calls,
+                       double *max_ms, double *sumsq_ms2);
 const char *pmpic_timing_name(int32_t which);
 
 #ifdef __cplusplus

@@ -104,6 +104,7 @@ class RunState:
         self.termination_reason = 0
         self.numparcels = 0
         self.model_start_wtime = 0.0
+        self.iterations = 0  # state.F90:53; restart_num after a restart
 
 
 class TerminationCheck:
@@ -195,14 +196,119 @@ class ModelSynopsis:
             self.dat.close()
 
 
-def run_model(ctx, options, log=print, parcels_dat="parcels.dat", wtime=_time.time):
-    """The timestep loop (monc.F90:211-230) over a library context; returns the RunState."""
+class WriteBinary:
+    """writebinary's timestep callback (components/writebinary/src/writebinary.F90:38-135), run
+    in the start group before the termination check.  Mode "step": the parcels every
+    parcel_step_write_frequency iterations, then the grids every grid_step_write_frequency;
+    mode "time": the grids once the model time reaches the next multiple of
+    grid_time_write_frequency, then the parcels likewise; "none" writes nothing.  The dump
+    number is the iteration count; files go to `directory`."""
+
+    def __init__(self, options, directory=".", log=print):
+        self.mode = str(options.get("writebinary_mode", "none")).strip().strip('"')
+        self.directory = directory
+        self.log = log
+        self.pwritten = self.gwritten = 0
+        self.ppersteps = int(options.get("parcel_step_write_frequency", 0))
+        self.gpersteps = int(options.get("grid_step_write_frequency", 0))
+        self.dtparcels = float(options.get("parcel_time_write_frequency", 0.0))
+        self.dtgrids = float(options.get("grid_time_write_frequency", 0.0))
+        self.tnextparcels = self.tnextgrids = 0.0
+        if self.mode not in ("step", "time", "none"):
+            raise ValueError(f"Error: mode '{self.mode}' not recognised.")
+        log(f" Selected writing mode: {self.mode}")
+
+    def _parcels(self, ctx, num):
+        from .dumpio import dump, parcel_file
+        self.log(f" Writing parcels to '{parcel_file(0, num).replace('00000', '[rank]')}'")
+        dump(ctx, self.directory, num, grids=False)
+        self.pwritten += 1
+
+    def _grids(self, ctx, num):
+        from .dumpio import dump, grid_file
+        self.log(f" Writing grids to '{grid_file(0, num).replace('00000', '[rank]')}'")
+        dump(ctx, self.directory, num, parcels=False)
+        self.gwritten += 1
+
+    def timestep(self, ctx, st):
+        num = st.iterations
+        if self.mode == "step":
+            if self.ppersteps != 0 and num % self.ppersteps == 0:
+                self._parcels(ctx, num)
+            if self.gpersteps != 0 and num % self.gpersteps == 0:
+                self._grids(ctx, num)
+        elif self.mode == "time":
+            if self.dtgrids != 0. and st.time >= self.tnextgrids:
+                self._grids(ctx, num)
+                self.tnextgrids += self.dtgrids
+            if self.dtparcels != 0. and st.time >= self.tnextparcels:
+                self._parcels(ctx, num)
+                self.tnextparcels += self.dtparcels
+
+    def finalise(self):
+        self.log(f" Written {self.pwritten} parcel dumps")
+        self.log(f" Written {self.gwritten} grid dumps")
+
+
+def timer_summary(rows, log=print):
+    """The master rank's timer summary of finalize_timing (timer.F90:247-303).
+
+    rows: [(name, calls, total_s, max_s, sum of squared intervals in s^2)], the first being the
+    run's own timer (``MONC``, monc.F90:155) whose total the percentages are of.  Rows are listed
+    from the longest total to the shortest (the reference's bubble sort: stable for equal
+    totals); a timer's stddev % is sqrt(sum dt^2 / n - mean^2) / mean."""
+    if not rows:
+        return
+    mxtime = rows[0][2]
+    order = list(range(len(rows)))
+    vals = [r[2] for r in rows]
+    swapped = True
+    while swapped:  # bubble sort, descending, as sort_values (timer.F90:412-443)
+        swapped = False
+        for i in range(len(vals) - 1):
+            if vals[i] < vals[i + 1]:
+                vals[i], vals[i + 1] = vals[i + 1], vals[i]
+                order[i], order[i + 1] = order[i + 1], order[i]
+                swapped = True
+    log("")
+    log(" Summary of times for the master rank (sorted from longest to shortest time):")
+    log(" ------------------------------------------------------------------------------ ")
+    log("|     Function       |   #    |  Total   |  % of  |  Mean   |  Max    | stddev |")
+    log("|       Name         | Calls  |  Time    |  Time  |Duration |Duration |    %   |")
+    log("|------------------------------------------------------------------------------|")
+    for n in order:
+        name, calls, tot, mx, sq = rows[n]
+        if calls > 0:
+            mean = tot / calls
+            var = sq / calls - mean * mean
+            stddev = var ** 0.5 if var > 0 else 0.0
+            imbalance = stddev / mean * 100 if mean > 0 else 0.0
+        else:
+            mean = stddev = imbalance = 0.0
+        pct = tot / mxtime * 100 if mxtime > 0 else 0.0
+        log(f"|{name[:20]:<20}|{calls:8d}|{tot:9.3f}s| {pct:6.2f}%|{mean:8.3f}s|{mx:8.3f}s|"
+            f"{imbalance:7.2f}%|")
+    log(" ------------------------------------------------------------------------------  ")
+
+
+def run_model(ctx, options, log=print, parcels_dat="parcels.dat", wtime=_time.time,
+              timers=False, writer=None, iterations=0):
+    """The timestep loop (monc.F90:211-230) over a library context; returns the RunState.
+    With timers, the library's component timers run and the reference's timer summary is
+    logged at the end, under the run's own MONC timer.  `writer` (a WriteBinary) runs first in
+    the start group; `iterations` is the iteration count to start from (restart_num after a
+    restart)."""
     st = RunState(timestep=int(options.get("start_timestep", 1)), time=ctx.time, dtm=ctx.dtm)
+    st.iterations = iterations
     st.model_start_wtime = wtime()
+    if timers:
+        ctx.timing_enable(True)
     term = TerminationCheck(options, log, wtime)
     syn = ModelSynopsis(options, st, log, parcels_dat, wtime)
     while st.continue_timestep:
-        # start group
+        # start group: writebinary, then the termination check (global_config:36)
+        if writer is not None:
+            writer.timestep(ctx, st)
         term(st)
         # integrator group: one whole step of the path
         t_before = ctx.time
@@ -214,9 +320,18 @@ def run_model(ctx, options, log=print, parcels_dat="parcels.dat", wtime=_time.ti
         if st.continue_timestep:
             st.timestep += 1
             st.time = ctx.time
+            st.iterations += 1
         else:
             # the reference does not advance the model time on the final iteration
             ctx.set_time(t_before, ctx.dtm)
             st.time = t_before
     syn.finalise(st)
+    if writer is not None:
+        writer.finalise()
+    if timers:
+        total = wtime() - st.model_start_wtime
+        rows = [("MONC", 1, total, total, total * total)]
+        for name, (t, n, mx, sq) in ctx.timer_stats().items():
+            rows.append((name, n, t / 1e3, mx / 1e3, sq / 1e6))
+        timer_summary(rows, log)
     return st

@@ -36,6 +36,8 @@ int timing_harvest(Ctx *c) {
       PMPIC_CHECK(hipEventElapsedTime(&ms, c->ev[w][e][0], c->ev[w][e][1]));
       c->t_ms[w] += ms;
       c->t_calls[w]++;
+      if (ms > c->t_max[w]) c->t_max[w] = ms;
+      c->t_sq[w] += (double)ms * (double)ms;
     }
     c->nev[w] = 0;
   }
@@ -816,6 +818,8 @@ int pmpic_timing_enable(pmpic_ctx *ctx, int32_t enable) {
   for (int w = 0; w < NTIMERS; w++) {
     c->t_ms[w] = 0;
     c->t_calls[w] = 0;
+    c->t_max[w] = 0;
+    c->t_sq[w] = 0;
   }
   return PMPIC_OK;
 }
@@ -827,6 +831,18 @@ const char *pmpic_timing_name(int32_t which) {
   static const char *names[NTIMERS] = {"par2grid_cells", "grid2par_rk_stage", "vort2vel",
                                        "vort_tend", "RK4_all_steps", "splitting", "mixing"};
   return (which >= 0 && which < NTIMERS) ? names[which] : nullptr;
+}
+
+int pmpic_timing_stats(pmpic_ctx *ctx, int32_t which, double *total_ms, int64_t *calls,
+                       double *max_ms, double *sumsq_ms2) {
+  if (!ctx || which < 0 || which >= NTIMERS) return PMPIC_ERR_ARG;
+  Ctx *c = &ctx->c;
+  PMPIC_TRY(timing_harvest(c));
+  if (total_ms) *total_ms = c->t_ms[which];
+  if (calls) *calls = c->t_calls[which];
+  if (max_ms) *max_ms = c->t_max[which];
+  if (sumsq_ms2) *sumsq_ms2 = c->t_sq[which];
+  return PMPIC_OK;
 }
 
 int pmpic_timing_read(pmpic_ctx *ctx, int32_t which, double *total_ms, int64_t *launches) {

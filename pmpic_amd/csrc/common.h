@@ -153,6 +153,8 @@ struct Ctx {
   int nev[NTIMERS];
   double t_ms[NTIMERS];
   int64_t t_calls[NTIMERS];
+  double t_max[NTIMERS];  // longest single interval (ms)
+  double t_sq[NTIMERS];   // sum of squared intervals (ms^2), for the standard deviation
 };
 
 #define PMPIC_CHECK(call)                                   \

@@ -41,7 +41,7 @@ EXPORTS = [
     "pmpic_fft_backward", "pmpic_diffx", "pmpic_diffy", "pmpic_spectral_filter", "pmpic_diffz",
     "pmpic_laplinv", "pmpic_conservation_sums", "pmpic_last_counts", "pmpic_timing_enable",
     "pmpic_timing_read", "pmpic_step_concurrent", "pmpic_timing_name", "pmpic_hbm_triad",
-    "pmpic_writebinary_grids", "pmpic_par2grid_noalloc",
+    "pmpic_writebinary_grids", "pmpic_par2grid_noalloc", "pmpic_timing_stats",
 ]
 
 
@@ -111,6 +111,8 @@ def load():
         L.pmpic_last_counts.argtypes = [vp, vp]
         L.pmpic_timing_enable.argtypes = [vp, i32]
         L.pmpic_timing_read.argtypes = [vp, i32, ctypes.POINTER(d), ctypes.POINTER(i64)]
+        L.pmpic_timing_stats.argtypes = [vp, i32, ctypes.POINTER(d), ctypes.POINTER(i64),
+                                         ctypes.POINTER(d), ctypes.POINTER(d)]
         L.pmpic_timing_name.restype = ctypes.c_char_p
         L.pmpic_timing_name.argtypes = [i32]
         L.pmpic_hbm_triad.argtypes = [i32, i64, i32, ctypes.POINTER(d), ctypes.POINTER(d)]
@@ -176,6 +178,7 @@ class Context:
         self.h = h
         self.precision = int(precision)
         self.nx, self.ny, self.nz = nx, ny, nz
+        self.box = (xmin, xmax, ymin, ymax, zmin, zmax)
         self.max_parcels = int(max_parcels)
         self.gshape = (nx + 4, ny + 4, nz + 1)
         self.sshape = ((nx // 2 + 1) * 2, (ny // 2 + 1) * 2, nz + 1)
@@ -376,6 +379,25 @@ class Context:
         self._chk(self.L.pmpic_timing_read(self.h, int(which), ctypes.byref(ms), ctypes.byref(calls)),
                   "pmpic_timing_read")
         return ms.value, calls.value
+
+    def timing_stats(self, which):
+        """(total ms, intervals, longest ms, sum of squared ms) of timer `which`."""
+        t, n, mx, sq = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
+        self._chk(self.L.pmpic_timing_stats(self.h, int(which), ctypes.byref(t), ctypes.byref(n),
+                                            ctypes.byref(mx), ctypes.byref(sq)),
+                  "pmpic_timing_stats")
+        return t.value, n.value, mx.value, sq.value
+
+    def timer_stats(self):
+        """{region name: (total ms, intervals, longest ms, sum of squared ms)} for every timer."""
+        out = {}
+        w = 0
+        while True:
+            name = self.L.pmpic_timing_name(w)
+            if name is None:
+                return out
+            out[name.decode()] = self.timing_stats(w)
+            w += 1
 
     def timer_table(self):
         """{region name: (total ms, count)} for every timer (timing must be enabled)."""

@@ -117,3 +117,50 @@ def test_walltime_parse(text, secs, warn):
                                        (1e-7, 5, "0.00000"), (2.0 / 3.0, 2, "0.66")])
 def test_real_strings(x, dp, want):
     assert control.real_str(x, dp) == want
+
+
+class DumpCtx(FakeCtx):
+    """A stand-in that also answers what a dump reads."""
+
+    nx, ny, nz = 4, 4, 2
+    box = (0.0, 4.0, 0.0, 4.0, 0.0, 2.0)
+
+    def get_parcels(self, names=None):
+        import numpy as np
+        return {"x": np.array([0.5, 1.5]), "y": np.array([0.5, 0.5]), "z": np.array([1.0, 1.0])}
+
+    def writebinary_grids(self):
+        pass
+
+    def grid(self, name):
+        import numpy as np
+        return np.zeros((self.nx + 4, self.ny + 4, self.nz + 1))
+
+
+@pytest.mark.parametrize("mode,opts,parcels,grids", [
+    ("step", {"parcel_step_write_frequency": 2, "grid_step_write_frequency": 3},
+     [0, 2, 4], [0, 3]),
+    ("time", {"parcel_time_write_frequency": 0.5, "grid_time_write_frequency": 0.25},
+     [0, 2, 4], [0, 1, 2, 3, 4]),
+    ("none", {}, [], []),
+])
+def test_writebinary_schedule(tmp_path, mode, opts, parcels, grids):
+    """writebinary.F90:84-135 in the start group: the dump number is the iteration count; step
+    mode writes every n iterations, time mode once the model time reaches the next multiple."""
+    import os
+    logs = []
+    w = control.WriteBinary(dict(opts, writebinary_mode=f'"{mode}"'), str(tmp_path), logs.append)
+    ctx = DumpCtx(dtm=0.25)
+    control.run_model(ctx, {"nn_timesteps": 5, "termination_time": 100.0,
+                            "display_synopsis_frequency": 100}, log=logs.append,
+                      parcels_dat=None, writer=w)
+    files = sorted(os.listdir(tmp_path))
+    assert [f for f in files if f.startswith("parcels_")] == \
+        [f"parcels_00000_{n:05d}.dat" for n in parcels]
+    assert [f for f in files if f.startswith("grids_")] == [f"grids_00000_{n:05d}.dat" for n in grids]
+    assert f" Written {len(parcels)} parcel dumps" in logs
+
+
+def test_writebinary_rejects_unknown_mode():
+    with pytest.raises(ValueError):
+        control.WriteBinary({"writebinary_mode": "often"}, ".", lambda s: None)

@@ -51,3 +51,127 @@ def test_run_model_matches_plain_steps(tmp_path, precision):
     assert any("Model run complete" in s for s in logs)
     a.close()
     b.close()
+
+
+def test_dump_and_restart_resume_the_run(tmp_path):
+    """writebinary's dumps of a stepped context (parcels and grids, dump number 2), the grid dump
+    holding the exported grids; the restart reader's parcels in a fresh context continue the run
+    bit for bit (readfromfile_parcelsetup.F90:26-150: counters restart at zero, the dump holds
+    every parcel property the step reads)."""
+    from pmpic_amd import dumpio
+    shape = (12, 10, 8)
+    ic = thermal_case(*shape, 2, vort=1.0, shrink=0.02)
+    a = _ctx(ic, shape, 0)
+    a.step(2)
+    paths = dumpio.dump(a, str(tmp_path), 2)
+    assert [p.split("/")[-1] for p in paths] == ["parcels_00000_00002.dat",
+                                                 "grids_00000_00002.dat"]
+    t, bounds, dims, grids = dumpio.read_grids(paths[1])
+    assert dims == (12, 10, 9) and t == a.time
+    for k in dumpio.GRID_FIELDS:
+        np.testing.assert_array_equal(grids[k], a.grid(k)[2:-2, 2:-2, :], err_msg=k)
+    assert np.max(grids["hgliq"]) > 0
+    b = box(*shape)
+    time, fields = dumpio.read_restart(str(tmp_path), 2, b["xmin"], b["xmax"], b["ymin"],
+                                       b["ymax"], log=lambda s: None)
+    assert time == a.time
+    c = _ctx(fields, shape, 0)
+    c.set_time(time, a.dtm)
+    a.step(2)
+    c.step(2)
+    assert a.n == c.n and a.time == c.time
+    pa, pc = a.get_parcels(), c.get_parcels()
+    for k in pa:
+        np.testing.assert_array_equal(pa[k], pc[k], err_msg=k)
+    a.close()
+    c.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_timer_summary_of_a_run(precision):
+    shape = (12, 10, 8)
+    a = _ctx(thermal_case(*shape, 2), shape, precision)
+    logs = []
+    control.run_model(a, {"nn_timesteps": 3, "display_synopsis_frequency": 10},
+                      log=logs.append, parcels_dat=None, timers=True)
+    rows = [s for s in logs if s.startswith("|") and s.count("|") == 8 and "Name" not in s
+            and "Function" not in s]
+    names = [r.split("|")[1].strip() for r in rows]
+    assert names[0] == "MONC"
+    for n in ("vort2vel", "vort_tend", "RK4_all_steps", "splitting", "mixing"):
+        assert n in names
+    calls = {r.split("|")[1].strip(): int(r.split("|")[2]) for r in rows}
+    assert calls["vort2vel"] == 12 and calls["mixing"] == 3
+    a.close()
+
+
+MCF = """nx=16
+ny=16
+nz=16
+xmin=0
+xmax=6.28
+ymin=0
+ymax=6.28
+zmin=0
+zmax=6.28
+max_parcels=400000
+lambda=1
+H=0.8
+z_c=2.5
+mu=0.9
+z_d=4.
+z_m=5.
+r_plume=0.8
+e_values=0.3, -0.4 ,0.5
+dtm=0.01
+dtmax=0.01
+timestep_prefactor=0.5
+parcel_splitting_enabled=.true.
+parcel_mixing_enabled=.true.
+writebinary_enabled=.true.
+writebinary_mode="step"
+parcel_step_write_frequency=1
+grid_step_write_frequency=2
+termination_time=1000.
+display_synopsis_frequency=10
+timing_enabled=.true.
+"""
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_driver_plume_run_dumps_and_restart(tmp_path, precision):
+    """A run from a config.mcf: the plume set-up, three steps, writebinary's dumps every step
+    (parcels) and every second step (grids), the timer summary; then a run restarted from dump 2
+    (readfromfile_parcelsetup) begins from that dump's parcels and iteration count."""
+    import os
+    from pmpic_amd import driver, dumpio
+    from pmpic_amd.config import apply_cli, parse_mcf
+    mcf = tmp_path / "plume.mcf"
+    mcf.write_text(MCF)
+    opts = apply_cli(parse_mcf(str(mcf)), ["--plume_parcelsetup_enabled=.true.",
+                                           "--nn_timesteps=3", f"--precision={precision}"])
+    logs = []
+    st, ctx = driver.run(opts, str(tmp_path), log=logs.append)
+    assert st.timestep == 3 and st.iterations == 2
+    files = sorted(os.listdir(tmp_path))
+    assert [f for f in files if f.startswith("parcels_")] == \
+        [f"parcels_00000_{n:05d}.dat" for n in range(3)]
+    assert [f for f in files if f.startswith("grids_")] == ["grids_00000_00000.dat",
+                                                            "grids_00000_00002.dat"]
+    assert any(s.startswith("|MONC") for s in logs)
+    t2, _, fields2 = dumpio.read_parcels(str(tmp_path / "parcels_00000_00002.dat"))
+    ctx.close()
+    opts2 = apply_cli(parse_mcf(str(mcf)), ["--readfromfile_parcelsetup_enabled=.true.",
+                                            "--restart_num=2", "--nn_timesteps=1",
+                                            "--writebinary_enabled=.false.",
+                                            f"--precision={precision}"])
+    fields, time, iterations = driver.initial_parcels(opts2, 16, 16, 16,
+                                                      (0.0, 6.28000021, 0.0, 6.28000021, 0.0,
+                                                       6.28000021), str(tmp_path),
+                                                      log=lambda s: None)
+    assert iterations == 2 and time == t2
+    for k in dumpio.FIELDS:
+        np.testing.assert_array_equal(fields[k], fields2[k])
+    st2, ctx2 = driver.run(opts2, str(tmp_path), log=lambda s: None)
+    assert st2.iterations == 2 and ctx2.n > 0
+    ctx2.close()
