@@ -166,6 +166,12 @@ int pmpic_timing_read(pmpic_ctx *ctx, int32_t which, double *total_ms, int64_t *
 int pmpic_timing_stats(pmpic_ctx *ctx, int32_t which, double *total_ms, int64_t *calls,
                        double *max_ms, double *sumsq_ms2);
 const char *pmpic_timing_name(int32_t which);
+/* HIP-graph use of pmpic_step (no reference counterpart; a diagnostic).  With PMPIC_GRAPH=1 in
+ * the environment at pmpic_create, an FP32-bulk context with timing off launches RK stages 1-3
+ * of each step after its first as one captured graph, captured again when the parcel count
+ * leaves the range a capture covers (the results are the same bits as eager launches; off by
+ * default: it measured no faster).  `captures` graphs captured, `launches` graph launches. */
+int pmpic_graph_stats(pmpic_ctx *ctx, int64_t *captures, int64_t *launches);
 
 #ifdef __cplusplus
 }

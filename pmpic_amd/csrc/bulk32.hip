@@ -240,7 +240,11 @@ __global__ void k32_bin_slot(Geometry g, const int32_t *__restrict__ cellv, int6
   // parcel indices fit 32 bits (capacity < 2^30)
   const int lane = threadIdx.x & 63;
   const int32_t w0 = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64 * BIN_SEQ;
-  const int32_t n32 = (int32_t)n;
+  // the parcel count from device memory: a graph replays this launch, sized for a bound on
+  // the count, whatever the count is then (n is the launch's size only)
+  (void)n;
+  const int32_t n32 = (int32_t)dsc->n;
+  if (n32 == 0 || w0 >= n32) return;
   // all count atomics of the wavefront are issued before any result is used, in group order;
   // ch[it] = cell << 6 | head lane of the parcel's run (the cell part is only used while the
   // slot table exists, ncell * capc < 2^30)
@@ -328,10 +332,10 @@ __global__ void k32_ovf_scatter(Geometry g, const int32_t *__restrict__ cellv,
 
 // without a slot table: parcel p at its cell's dense offset plus its rank
 __global__ void k32_dense_scatter(Geometry g, const int32_t *__restrict__ cellv,
-                                  const int32_t *__restrict__ rankv, int64_t n,
+                                  const int32_t *__restrict__ rankv, const DevScalars *dsc,
                                   const int32_t *__restrict__ start, int32_t *__restrict__ sorted) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
+  if (p >= dsc->n) return;
   const int32_t c = linear_cell(g, ld(cellv, (uint32_t)p));
   st(sorted, (uint32_t)(start[c] + ld(rankv, (uint32_t)p)), (int32_t)p);
 }
@@ -731,7 +735,8 @@ __global__ void __launch_bounds__(TPB) k32_rk_stage(Geometry g, Bulk32 b, int64_
                                                     uint8_t *__restrict__ split_flag,
                                                     DevScalars *sc) {
   const int64_t p64 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p64 >= n) return;
+  (void)n;  // launch size only: the count comes from device memory (see k32_bin_slot)
+  if (p64 >= sc->n) return;
   const uint32_t p = (uint32_t)p64;
   float *F = b.f;
   auto fld = [&](int f) { return F + (size_t)f * cap; };
@@ -1083,16 +1088,14 @@ int bin32(Ctx *c, int64_t n) {
   Bulk32 &b = c->b32;
   // the capacity of this binning: 16 entries while the mean is at most LOW_PPC parcels per
   // cell (fewer empty entries to read), else 32
-  if (b.capmax == 0) b.capc = 0;
-  else if (b.capfix) b.capc = b.capfix;
-  else if (b.captuned) b.capc = b.captuned;
-  else b.capc = (n <= (int64_t)LOW_PPC * c->ncell) ? CAPC_LOW : CAPC;
-  if (b.capc > b.capmax) b.capc = b.capmax;
+  b.capc = b32_choose_capc(c, n);
+  // the launches of a captured graph cover a parcel bound (the kernels read the count)
+  const int64_t ln = c->graph_n ? c->graph_n : n;
   hipLaunchKernelGGL(k32_bin_reset, dim3(cdiv(nc1, TPB)), dim3(TPB), 0, c->stream, c->cell_count,
                      nc1, c->dsc);
-  if (n > 0)
-    hipLaunchKernelGGL(k32_bin_slot, dim3(cdiv(n, (int64_t)TPB * BIN_SEQ)), dim3(TPB), 0,
-                       c->stream, c->g, b.cell, n, b.capc, b.slot, b.ovf, c->rank,
+  if (ln > 0)
+    hipLaunchKernelGGL(k32_bin_slot, dim3(cdiv(ln, (int64_t)TPB * BIN_SEQ)), dim3(TPB), 0,
+                       c->stream, c->g, b.cell, ln, b.capc, b.slot, b.ovf, c->rank,
                        c->cell_count, c->dsc);
   // dense-region offsets of the cells past the slot capacity
   auto it = rocprim::make_transform_iterator(static_cast<const int32_t *>(c->cell_count),
@@ -1109,11 +1112,11 @@ int bin32(Ctx *c, int64_t n) {
   size_t have = c->scan_tmp_bytes;
   PMPIC_CHECK(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp, have, it, c->cell_start, (int)nc1,
                                                c->stream));
-  if (n > 0 && b.capc == 0) {
-    hipLaunchKernelGGL(k32_dense_scatter, dim3(cdiv(n, TPB)), dim3(TPB), 0, c->stream, c->g,
-                       b.cell, c->rank, n, c->cell_start, c->sorted);
-  } else if (n > 0) {
-    const int64_t nb = std::min<int64_t>(cdiv(n, TPB), 1024);
+  if (ln > 0 && b.capc == 0) {
+    hipLaunchKernelGGL(k32_dense_scatter, dim3(cdiv(ln, TPB)), dim3(TPB), 0, c->stream, c->g,
+                       b.cell, c->rank, c->dsc, c->cell_start, c->sorted);
+  } else if (ln > 0) {
+    const int64_t nb = std::min<int64_t>(cdiv(ln, TPB), 1024);
     hipLaunchKernelGGL(k32_ovf_scatter, dim3(nb), dim3(TPB), 0, c->stream, c->g, b.cell, b.ovf,
                        c->dsc, c->cell_start, b.slot, b.capc, c->sorted);
   }
@@ -1129,6 +1132,29 @@ float *partial32(Ctx *c) { return reinterpret_cast<float *>(c->partial); }
 float *partial32b(Ctx *c) { return reinterpret_cast<float *>(c->partial2); }
 
 }  // namespace
+
+__global__ void k32_set_n(DevScalars *sc, int64_t n) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) sc->n = n;
+}
+
+int b32_publish_n(Ctx *c) {
+  hipLaunchKernelGGL(k32_set_n, dim3(1), dim3(64), 0, c->stream, c->dsc, c->n);
+  PMPIC_CHECK(hipGetLastError());
+  return PMPIC_OK;
+}
+
+// 16 entries while the mean is at most LOW_PPC parcels per cell (fewer empty entries to read),
+// else 32; a capacity fixed from the environment or picked by the tuning wins; never more than
+// the table holds
+int b32_choose_capc(const Ctx *c, int64_t n) {
+  const Bulk32 &b = c->b32;
+  int capc;
+  if (b.capmax == 0) capc = 0;
+  else if (b.capfix) capc = b.capfix;
+  else if (b.captuned) capc = b.captuned;
+  else capc = (n <= (int64_t)LOW_PPC * c->ncell) ? CAPC_LOW : CAPC;
+  return capc > b.capmax ? b.capmax : capc;
+}
 
 int b32_alloc(Ctx *c) {
   Bulk32 &b = c->b32;
@@ -1308,7 +1334,7 @@ int b32_rk_stage(Ctx *c, int stage, int stretch) {
   Grid32Set gs = {{c->b32.grid[0], c->b32.grid[1], c->b32.grid[2], c->b32.grid[3],
                    c->b32.grid[4], c->b32.grid[5]}};
   const Scale32 s3 = scale_of(c->g);
-  const dim3 gb(cdiv(n, TPB)), tb(TPB);
+  const dim3 gb(cdiv(c->graph_n ? c->graph_n : n, TPB)), tb(TPB);
   switch (stage) {
     case 1:
       PMPIC_TRY(launch_timed(c, T_RK, k32_rk_stage<1>, gb, tb, 0, c->g, c->b32, c->cap, n, gs,
@@ -1355,7 +1381,7 @@ int b32_split(Ctx *c) {
     PMPIC_CHECK(hipGetLastError());
   }
   c->n = n + ns;
-  return PMPIC_OK;
+  return b32_publish_n(c);
 }
 
 int b32_mix(Ctx *c) {
@@ -1450,7 +1476,7 @@ int b32_mix(Ctx *c) {
     }
   }
   c->n = np;
-  return PMPIC_OK;
+  return b32_publish_n(c);
 }
 
 // writebinary's grids (writebinary.F90:217-228): b, hg, p, q, r by par2grid of the FP32

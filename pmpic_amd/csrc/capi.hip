@@ -188,7 +188,16 @@ int set_dev_scalars(Ctx *c) {
   return PMPIC_OK;
 }
 
+void graph_drop(Ctx *c) {
+  if (c->g123x) hipGraphExecDestroy(c->g123x);
+  if (c->g123) hipGraphDestroy(c->g123);
+  c->g123x = nullptr;
+  c->g123 = nullptr;
+  c->g123_n = 0;
+}
+
 int free_all(Ctx *c) {
+  graph_drop(c);
   b32_free(c);
   hipFree(c->par); hipFree(c->tmp);
   hipFree(c->cell); hipFree(c->rank); hipFree(c->sorted); hipFree(c->flag);
@@ -306,6 +315,9 @@ int create(const pmpic_config *cf, Ctx *c) {
   PMPIC_TRY(fft_setup(c));
   PMPIC_TRY(spectral_setup(c));
   PMPIC_TRY(set_dev_scalars(c));
+  // PMPIC_GRAPH=1: the FP32-bulk form launches stages 1-3 of a step as a HIP graph
+  const char *genv = getenv("PMPIC_GRAPH");
+  c->graph_on = (genv && genv[0] == '1') ? 1 : 0;
   // The FP64 form's residual partials of mixing get their own buffer when the device still has
   // room for it (and a margin for the haloswap staging and scan storage allocated later);
   // otherwise, or with PMPIC_LEAN_MIXING=1, mixing sums the kept and the residual parcels in two
@@ -380,6 +392,7 @@ int pmpic_set_parcels(pmpic_ctx *ctx, int64_t n, const double *const *fields) {
   if (c->precision == 1) {
     PMPIC_TRY(b32_set_parcels(c, n, fields));
     c->n = n;
+    PMPIC_TRY(b32_publish_n(c));
     return take_status(c);
   }
   for (int f = 0; f < 17; f++) {
@@ -524,20 +537,105 @@ int pmpic_parcel_mixing(pmpic_ctx *ctx) {
   return st;
 }
 
+// one RK stage of a timestep: the components in the order of the reference's model loop
+static int one_stage(pmpic_ctx *ctx) {
+  Ctx *c = &ctx->c;
+  PMPIC_TRY(pmpic_vort2vel(ctx));
+  PMPIC_TRY(pmpic_vorticity_tendency(ctx));
+  PMPIC_TRY(pmpic_rk4_integrator(ctx));
+  if (c->splitting_enabled) PMPIC_TRY(pmpic_parcel_splitting(ctx));
+  else c->it_split++;
+  if (c->mixing_enabled) PMPIC_TRY(pmpic_parcel_mixing(ctx));
+  else c->it_mix++;
+  return PMPIC_OK;
+}
+
+// The FP32-bulk form's stages 1-3 as a HIP graph.  Their launches depend on the host only
+// through the slot capacity of the binning and the grid sizes of the per-parcel launches; the
+// kernels read the parcel count from device memory, so one capture serves every step whose
+// count lies within the captured bound (and not far below it) and whose capacity is the
+// captured one.  The graph
+// holds kernel launches only.  Stage 4 (splitting and mixing, whose launches depend on counts
+// read back) runs eagerly.
+static bool graph_eligible(const Ctx *c) {
+  return c->precision == 1 && c->graph_on && c->timing == 0 && !c->b32.captune &&
+         c->eager_steps > 0 && c->n > 0 && c->rkstep == 1 && c->it_v2v % 4 == 0 &&
+         c->it_vt % 4 == 0 && c->it_split % 4 == 0 && c->it_mix % 4 == 0;
+}
+
+// the parcel count a capture covers: the count plus 1/16 and 4096 (rounded to 4096), so that
+// the growth by splitting of a few steps stays inside it; at most the capacity
+static int64_t graph_bound(const Ctx *c) {
+  int64_t bound = c->n + c->n / 16 + 4096;
+  bound = (bound + 4095) / 4096 * 4096;
+  return bound > c->cap ? c->cap : bound;
+}
+
+static int graph_capture(pmpic_ctx *ctx, int capc) {
+  Ctx *c = &ctx->c;
+  const int64_t bound = graph_bound(c);
+  PMPIC_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  c->graph_n = bound;
+  int st = PMPIC_OK;
+  for (int r = 0; r < 3 && st == PMPIC_OK; r++) st = one_stage(ctx);
+  c->graph_n = 0;
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+  if (st == PMPIC_OK && ec != hipSuccess) st = PMPIC_ERR_HIP;
+  hipGraphExec_t x = nullptr;
+  if (st == PMPIC_OK && hipGraphInstantiate(&x, g, nullptr, nullptr, 0) != hipSuccess)
+    st = PMPIC_ERR_HIP;
+  if (st != PMPIC_OK) {
+    if (g) hipGraphDestroy(g);
+    return st;
+  }
+  c->g123 = g;
+  c->g123x = x;
+  c->g123_n = bound;
+  c->g123_capc = capc;
+  c->g123_captures++;
+  return PMPIC_OK;
+}
+
+// stages 1-3 of a step by the graph (captured first where none fits); the host counters and
+// state advance as the eager calls advance them
+static int graph_stages123(pmpic_ctx *ctx) {
+  Ctx *c = &ctx->c;
+  const int capc = b32_choose_capc(c, c->n);
+  // a new capture for another capacity, a count past the bound, or a bound fallen below 7/8
+  // of the captured one (the launches' idle groups kept few)
+  if (c->g123x && (capc != c->g123_capc || c->n > c->g123_n || 8 * graph_bound(c) < 7 * c->g123_n))
+    graph_drop(c);
+  if (!c->g123x) {
+    // the capture itself advanced the counters; the launch below does the work
+    PMPIC_TRY(graph_capture(ctx, capc));
+  } else {
+    c->it_v2v += 3;
+    c->it_vt += 3;
+    c->it_split += 3;
+    c->it_mix += 3;
+    c->rkstep = 4;
+    c->split_flags_fresh = 0;
+    c->b32.capc = capc;
+    c->b32.kept_pending[1] = c->b32.kept_pending[3] = false;
+    c->b32.kept_pending[4] = c->b32.kept_pending[5] = false;
+  }
+  PMPIC_CHECK(hipGraphLaunch(c->g123x, c->stream));
+  c->g123_replays++;
+  return PMPIC_OK;
+}
+
 // one timestep with the kernels' status checked once at its end (the components called one
 // by one check after each call); no kernel indexes memory from data a failed kernel left, so
 // the later launches of a failing step are harmless
 static int step_once(pmpic_ctx *ctx) {
   Ctx *c = &ctx->c;
-  for (int r = 0; r < 4; r++) {
-    PMPIC_TRY(pmpic_vort2vel(ctx));
-    PMPIC_TRY(pmpic_vorticity_tendency(ctx));
-    PMPIC_TRY(pmpic_rk4_integrator(ctx));
-    if (c->splitting_enabled) PMPIC_TRY(pmpic_parcel_splitting(ctx));
-    else c->it_split++;
-    if (c->mixing_enabled) PMPIC_TRY(pmpic_parcel_mixing(ctx));
-    else c->it_mix++;
+  if (graph_eligible(c)) {
+    PMPIC_TRY(graph_stages123(ctx));
+    return one_stage(ctx);
   }
+  for (int r = 0; r < 4; r++) PMPIC_TRY(one_stage(ctx));
+  c->eager_steps++;
   return PMPIC_OK;
 }
 
@@ -842,6 +940,13 @@ int pmpic_timing_stats(pmpic_ctx *ctx, int32_t which, double *total_ms, int64_t 
   if (calls) *calls = c->t_calls[which];
   if (max_ms) *max_ms = c->t_max[which];
   if (sumsq_ms2) *sumsq_ms2 = c->t_sq[which];
+  return PMPIC_OK;
+}
+
+int pmpic_graph_stats(pmpic_ctx *ctx, int64_t *captures, int64_t *launches) {
+  if (!ctx) return PMPIC_ERR_ARG;
+  if (captures) *captures = ctx->c.g123_captures;
+  if (launches) *launches = ctx->c.g123_replays;
   return PMPIC_OK;
 }
 

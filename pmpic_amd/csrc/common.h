@@ -146,6 +146,18 @@ struct Ctx {
   DevScalars hsc;                     // host mirror
   void *pin;                          // pinned host staging for small read-backs (256 B)
   hipEvent_t sync_ev;                 // read-back completion (spin wait)
+  // FP32-bulk form with PMPIC_GRAPH=1: RK stages 1-3 of a step (vort2vel, vorticity_tendency,
+  // rk4_integrator; the splitting and mixing callbacks do nothing there) captured once as a HIP
+  // graph and replayed by later steps; the stage-4 work, whose launches depend on data read
+  // back, stays eager
+  hipGraph_t g123;
+  hipGraphExec_t g123x;
+  int g123_capc;        // slot capacity the captured binning used
+  int64_t g123_n;       // parcel count the captured launches cover (their grids' size)
+  int64_t graph_n;      // while capturing: g123_n, the size of the per-parcel launches; else 0
+  int graph_on;         // PMPIC_GRAPH=1 in the environment at creation
+  int64_t eager_steps;  // steps run without the graph (the first sets up the scan storage)
+  int64_t g123_captures, g123_replays;  // graphs captured and launched
   // HIP-event timing: 0 = par2grid cell accumulation kernel, 1 = gather + RK stage kernel,
   // 2..6 = the component callbacks (timer regions named as the reference's timers)
   int timing;
@@ -342,6 +354,8 @@ int b32_rk_stage(Ctx *c, int stage, int stretch);
 int b32_split(Ctx *c);
 int b32_mix(Ctx *c);
 int b32_conservation_sums(Ctx *c, double out6[6]);
+int b32_choose_capc(const Ctx *c, int64_t n);  // the slot capacity a binning of n parcels takes
+int b32_publish_n(Ctx *c);                     // the parcel count into device memory
 int b32_export_grid(Ctx *c, int id);  // brings grid[id] up to date for a grid export
 int b32_dump_grids(Ctx *c);             // writebinary's b, hg, p, q, r grids
 int scan_exclusive_i32(Ctx *c, const int32_t *in, int32_t *out, int64_t n);

@@ -42,6 +42,7 @@ EXPORTS = [
     "pmpic_laplinv", "pmpic_conservation_sums", "pmpic_last_counts", "pmpic_timing_enable",
     "pmpic_timing_read", "pmpic_step_concurrent", "pmpic_timing_name", "pmpic_hbm_triad",
     "pmpic_writebinary_grids", "pmpic_par2grid_noalloc", "pmpic_timing_stats",
+    "pmpic_graph_stats",
 ]
 
 
@@ -113,6 +114,7 @@ def load():
         L.pmpic_timing_read.argtypes = [vp, i32, ctypes.POINTER(d), ctypes.POINTER(i64)]
         L.pmpic_timing_stats.argtypes = [vp, i32, ctypes.POINTER(d), ctypes.POINTER(i64),
                                          ctypes.POINTER(d), ctypes.POINTER(d)]
+        L.pmpic_graph_stats.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]
         L.pmpic_timing_name.restype = ctypes.c_char_p
         L.pmpic_timing_name.argtypes = [i32]
         L.pmpic_hbm_triad.argtypes = [i32, i64, i32, ctypes.POINTER(d), ctypes.POINTER(d)]
@@ -387,6 +389,13 @@ class Context:
                                             ctypes.byref(mx), ctypes.byref(sq)),
                   "pmpic_timing_stats")
         return t.value, n.value, mx.value, sq.value
+
+    def graph_stats(self):
+        """(graphs captured, graph launches) of pmpic_step in the FP32-bulk form."""
+        cap, run = ctypes.c_int64(), ctypes.c_int64()
+        self._chk(self.L.pmpic_graph_stats(self.h, ctypes.byref(cap), ctypes.byref(run)),
+                  "pmpic_graph_stats")
+        return cap.value, run.value
 
     def timer_stats(self):
         """{region name: (total ms, intervals, longest ms, sum of squared ms)} for every timer."""
