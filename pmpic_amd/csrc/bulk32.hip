@@ -1170,6 +1170,10 @@ int b32_alloc(Ctx *c) {
   PMPIC_CHECK(hipMemset(b.tag, 0, sizeof(double) * (size_t)c->cap));
   for (int i = 0; i < 6; i++) PMPIC_TRY(dalloc32(&b.grid[i], c->gsz));
   for (int i = 0; i < 12; i++) PMPIC_TRY(dalloc32(&b.mgrid[i], c->gsz));
+  for (int i = 0; i < 4; i++) {
+    PMPIC_TRY(dalloc32(&b.pgrid[i], c->gsz));
+    b.p2g_pending[i] = false;
+  }
   // 32-bit byte offsets into the slot table.  PMPIC_SLOT_CAPACITY in the environment: 0 drops
   // the table (every cell then binned through the dense region, as grids beyond that range
   // are); 16 or 32 fixes the capacity instead of choosing it by the mean parcels per cell;
@@ -1204,6 +1208,7 @@ void b32_free(Ctx *c) {
   hipFree(b.tag);
   for (int i = 0; i < 6; i++) hipFree(b.grid[i]);
   for (int i = 0; i < 12; i++) hipFree(b.mgrid[i]);
+  for (int i = 0; i < 4; i++) hipFree(b.pgrid[i]);
   hipFree(b.slot);
   hipFree(b.ovf);
 }
@@ -1311,18 +1316,21 @@ static int tune_capacity(Ctx *c, int64_t n) {
 
 int b32_par2grid(Ctx *c) {
   const int64_t n = c->n;
-  // this par2grid writes grid[GP..GR] and grid[GB]: the kept p, q, r, b are superseded
-  c->b32.kept_pending[1] = c->b32.kept_pending[3] = false;
-  c->b32.kept_pending[4] = c->b32.kept_pending[5] = false;
+  // this par2grid writes the FP32 p, q, r and total-buoyancy grids, current for the exports of
+  // GP..GR and GB: the kept p, q, r, b of the last mixing are superseded
+  Bulk32 &b = c->b32;
+  b.kept_pending[1] = b.kept_pending[3] = false;
+  b.kept_pending[4] = b.kept_pending[5] = false;
+  for (int i = 0; i < 4; i++) b.p2g_pending[i] = true;
   if (c->b32.captune && n > 0) {
     PMPIC_TRY(tune_capacity(c, n));
   } else {
     PMPIC_TRY(bin32(c, n));
     PMPIC_TRY(p2g_accum(c));
   }
-  Out32<double> out = {{c->grid[GP], c->grid[GQ], c->grid[GR], c->grid[GB], nullptr, nullptr}};
+  Out32<float> out = {{b.pgrid[0], b.pgrid[1], b.pgrid[2], b.pgrid[3], nullptr, nullptr}};
   const int64_t nnode = (int64_t)c->g.nzn * c->g.ny * c->g.nx;
-  hipLaunchKernelGGL((k32_node_sum<4, true, true, double>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0,
+  hipLaunchKernelGGL((k32_node_sum<4, true, true, float>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0,
                      c->stream, c->g, partial32(c), c->ncell, out, nullptr);
   PMPIC_CHECK(hipGetLastError());
   return PMPIC_OK;
@@ -1413,6 +1421,7 @@ int b32_mix(Ctx *c) {
   // (parcel_mixing.F90:131), which is what a grid export shows after mixing; nothing in the
   // step reads them there, so b32_export_grid widens them when one is exported
   for (int f = 0; f < 6; f++) b.kept_pending[f] = true;
+  for (int f = 0; f < 4; f++) b.p2g_pending[f] = false;
   if (norig > 0)
     hipLaunchKernelGGL(k32_keep_flags, dim3(cdiv(norig, TPB * KPT)), dim3(TPB), 0, c->stream, b.f,
                        c->cap, norig, vmin, c->flag);
@@ -1500,15 +1509,18 @@ int b32_dump_grids(Ctx *c) {
   hipLaunchKernelGGL((k32_node_sum<5, true, true, double>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0,
                      c->stream, g, partial32(c), c->ncell, out, nullptr);
   PMPIC_CHECK(hipGetLastError());
-  // these grids now hold writebinary's values, not the kept grids of the last mixing
+  // these grids now hold writebinary's values, not the kept grids of the last mixing or the
+  // step's FP32 grids
   for (int f = 1; f < 6; f++) b.kept_pending[f] = false;
+  for (int f = 0; f < 4; f++) b.p2g_pending[f] = false;
   return PMPIC_OK;
 }
 
 // the FP64 grid `id` as a grid export sees it in the FP32-bulk form: a kept grid of the last
-// mixing still in mgrid is widened into it first
+// mixing still in mgrid, or the step's FP32 p, q, r or total buoyancy, is widened into it first
 int b32_export_grid(Ctx *c, int id) {
   const GridId kept_ids[6] = {GVOL, GB, GHG, GP, GQ, GR};
+  const GridId p2g_ids[4] = {GP, GQ, GR, GB};
   Bulk32 &b = c->b32;
   const int64_t nnode = (int64_t)c->g.nzn * c->g.ny * c->g.nx;
   for (int f = 0; f < 6; f++)
@@ -1517,6 +1529,13 @@ int b32_export_grid(Ctx *c, int id) {
                          b.mgrid[f], c->grid[id], nnode);
       PMPIC_CHECK(hipGetLastError());
       b.kept_pending[f] = false;
+    }
+  for (int f = 0; f < 4; f++)
+    if (p2g_ids[f] == id && b.p2g_pending[f]) {
+      hipLaunchKernelGGL(k32_grid_to_f64, dim3(cdiv(nnode, TPB)), dim3(TPB), 0, c->stream,
+                         b.pgrid[f], c->grid[id], nnode);
+      PMPIC_CHECK(hipGetLastError());
+      b.p2g_pending[f] = false;
     }
   return PMPIC_OK;
 }

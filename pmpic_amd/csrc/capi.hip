@@ -619,6 +619,7 @@ static int graph_stages123(pmpic_ctx *ctx) {
     c->b32.capc = capc;
     c->b32.kept_pending[1] = c->b32.kept_pending[3] = false;
     c->b32.kept_pending[4] = c->b32.kept_pending[5] = false;
+    for (int i = 0; i < 4; i++) c->b32.p2g_pending[i] = true;
   }
   PMPIC_CHECK(hipGraphLaunch(c->g123x, c->stream));
   c->g123_replays++;

@@ -56,6 +56,11 @@ struct Bulk32 {
   // kept grid k (vol, b, hg, p, q, r) of the last mixing is current in mgrid[k] and not yet
   // widened into the FP64 grid a grid export reads
   bool kept_pending[6];
+  // the step's p, q, r and total-buoyancy grids in FP32 (par2grid's node sums, then vort2vel's
+  // filtered p, q, r), read by the spectral passes; pgrid[k] is current and not yet widened
+  // into FP64 grid GP, GQ, GR, GB (k = 0..3) while p2g_pending[k]
+  float *pgrid[4];
+  bool p2g_pending[4];
 };
 
 // grid ids exported through the C-ABI (writebinary.F90:287-296 order, then the tendencies)
@@ -294,9 +299,11 @@ int fft_forward(Ctx *c, int nf, const double *const *src, double *const *dst);
 enum SpecOp { SPEC_PLAIN = 0, SPEC_DX = 1, SPEC_DY = 2 };
 int fft_backward(Ctx *c, int nf, const double *const *src, double *const *dst,
                  float *const *dst32 = nullptr, const int *deriv = nullptr);
-// the same transforms with spectral arrays (and the transforms' arithmetic) of type S
+// the same transforms with spectral arrays (and the transforms' arithmetic) of type S; the
+// forward transform reads physical grids of type S, the inverse writes dst (FP64) and dst32
+// (FP32), either of which may be NULL
 template <typename S>
-int fft_forward_t(Ctx *c, int nf, const double *const *src, S *const *dst);
+int fft_forward_t(Ctx *c, int nf, const S *const *src, S *const *dst);
 template <typename S>
 int fft_backward_t(Ctx *c, int nf, const S *const *src, double *const *dst, float *const *dst32,
                    const int *deriv);

@@ -37,8 +37,9 @@ constexpr int MAXF = 8;
 
 // The transforms run in the precision S of the spectral arrays: FP64 (double2 complex
 // elements), or FP32 (float2) in the FP32-bulk form, whose LDS buffers then hold twice as many
-// elements, so a column pass covers twice as many adjacent columns.  Physical grids are FP64
-// in both.
+// elements, so a column pass covers twice as many adjacent columns.  The forward transform
+// reads physical grids of the same precision; the inverse writes FP64 grids, FP32 grids or
+// both.
 template <typename S> struct Cx;
 template <> struct Cx<double> { using T = double2; };
 template <> struct Cx<float> { using T = float2; };
@@ -58,7 +59,7 @@ __device__ __forceinline__ C mkc(S a, S b) {
 }
 
 struct FieldPtrs {
-  const void *in[MAXF];  // forward: FP64 physical grids; inverse: spectral arrays of type S
+  const void *in[MAXF];  // forward: physical grids of type S; inverse: spectral arrays of type S
   void *out[MAXF];       // forward: spectral arrays of type S; inverse: FP64 physical grids
   float *out32[MAXF];    // optional FP32 copy of a real output (the FP32-bulk form's gathers)
   int op[MAXF];          // SpecOp applied to the input of the inverse (k_rows_inv)
@@ -211,7 +212,7 @@ __global__ void __launch_bounds__(FFT_TPB) k_rows_r2c(Plan p, const cx_t<S> *__r
   constexpr int EPT = ept<S>();
   __shared__ C A[lds_cplx<S>()], B[lds_cplx<S>()], T[512];
   const int f = blockIdx.y;
-  const double *src = static_cast<const double *>(fp.in[f]);
+  const S *src = static_cast<const S *>(fp.in[f]);
   C *dst = c1 + (int64_t)f * c1_field;
   const int n = p.n, nl = n / 2 + 1;
   const int r0 = blockIdx.x * 2 * nb;                  // first row of this workgroup
@@ -224,8 +225,8 @@ __global__ void __launch_bounds__(FFT_TPB) k_rows_r2c(Plan p, const cx_t<S> *__r
     if (e < nbb * n) {
       const int t = e / n, x = e - t * n;
       const int ra = r0 + 2 * t, rb = ra + 1;
-      v[i].x = (S)src[(int64_t)ra * n + x];
-      v[i].y = (rb < nrows) ? (S)src[(int64_t)rb * n + x] : (S)0;
+      v[i].x = src[(int64_t)ra * n + x];
+      v[i].y = (rb < nrows) ? src[(int64_t)rb * n + x] : (S)0;
     }
   }
   put_twiddles(T, w, n, false);
@@ -507,7 +508,7 @@ static int64_t c1_field_size(const Geometry &g) { return (int64_t)g.nzn * g.ny *
 static int64_t c2_field_size(const Geometry &g) { return (int64_t)g.nzn * (g.ny / 2 + 1) * g.nx; }
 
 template <typename S>
-int fft_forward_t(Ctx *c, int nf, const double *const *src, S *const *dst) {
+int fft_forward_t(Ctx *c, int nf, const S *const *src, S *const *dst) {
   const Geometry &g = c->g;
   if (nf < 1 || nf > MAXF) return PMPIC_ERR_ARG;
   FieldPtrs fp{};
@@ -560,7 +561,7 @@ int fft_backward_t(Ctx *c, int nf, const S *const *src, double *const *dst, floa
 }
 
 template int fft_forward_t<double>(Ctx *, int, const double *const *, double *const *);
-template int fft_forward_t<float>(Ctx *, int, const double *const *, float *const *);
+template int fft_forward_t<float>(Ctx *, int, const float *const *, float *const *);
 template int fft_backward_t<double>(Ctx *, int, const double *const *, double *const *,
                                     float *const *, const int *);
 template int fft_backward_t<float>(Ctx *, int, const float *const *, double *const *,

@@ -839,30 +839,33 @@ __global__ void k_velocity_spectra(Geometry g, const S *__restrict__ a,
 }
 
 // vorticity_tendency.F90:169-265, physical interior, accumulation order as written
-__global__ void k_tendency(int64_t nnode, const double *__restrict__ P,
-                           const double *__restrict__ Q, const double *__restrict__ R,
-                           const double *__restrict__ dbdy, const double *__restrict__ dudx,
-                           const double *__restrict__ dudy, const double *__restrict__ dwdx,
-                           const double *__restrict__ dbdx, const double *__restrict__ dvdx,
-                           const double *__restrict__ dvdy, const double *__restrict__ dwdy,
-                           double *__restrict__ DP, double *__restrict__ DQ,
-                           double *__restrict__ DR, float *__restrict__ DP32,
-                           float *__restrict__ DQ32, float *__restrict__ DR32) {
+// T: the type of the input grids (FP64, or FP32 in the FP32-bulk form, widened exactly); the
+// products are formed in FP64 in both forms
+template <typename T>
+__global__ void k_tendency(int64_t nnode, const T *__restrict__ P, const T *__restrict__ Q,
+                           const T *__restrict__ R, const T *__restrict__ dbdy,
+                           const T *__restrict__ dudx, const T *__restrict__ dudy,
+                           const T *__restrict__ dwdx, const T *__restrict__ dbdx,
+                           const T *__restrict__ dvdx, const T *__restrict__ dvdy,
+                           const T *__restrict__ dwdy, double *__restrict__ DP,
+                           double *__restrict__ DQ, double *__restrict__ DR,
+                           float *__restrict__ DP32, float *__restrict__ DQ32,
+                           float *__restrict__ DR32) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nnode) return;
   const double p = P[q], qq = Q[q], r = R[q];
   double dp = dbdy[q];
-  dp = dp + p * dudx[q];
-  dp = dp + qq * dudy[q];
-  dp = dp + r * (qq + dwdx[q]);
+  dp = dp + p * (double)dudx[q];
+  dp = dp + qq * (double)dudy[q];
+  dp = dp + r * (qq + (double)dwdx[q]);
   double dq = dbdx[q];
-  dq = -dq + p * dvdx[q];
-  dq = dq + qq * dvdy[q];
-  dq = dq + r * (-p + dwdy[q]);
-  double dr = p * dwdx[q];
-  dr = dr + qq * dwdy[q];
-  dr = dr + r * (-dudx[q]);
-  dr = dr + r * (-dvdy[q]);
+  dq = -dq + p * (double)dvdx[q];
+  dq = dq + qq * (double)dvdy[q];
+  dq = dq + r * (-p + (double)dwdy[q]);
+  double dr = p * (double)dwdx[q];
+  dr = dr + qq * (double)dwdy[q];
+  dr = dr + r * (-(double)dudx[q]);
+  dr = dr + r * (-(double)dvdy[q]);
   if (DP) {
     DP[q] = dp;
     DQ[q] = dq;
@@ -878,11 +881,11 @@ __global__ void k_tendency(int64_t nnode, const double *__restrict__ P,
 // maxima of u^2, v^2, w^2 and p^2+q^2+r^2 over the interior (vort2vel.F90:331-337,
 // vorticity_tendency.F90:279-282), per block; a maximum is exact in any order, so the result
 // is deterministic.
-// TU: the velocity grids' type (double, or float for the FP32-bulk form's velocity grids)
+// TU: the grids' type (double, or float for the FP32-bulk form's grids)
 template <typename TU>
 __global__ void k_max4(int64_t nnode, const TU *__restrict__ U, const TU *__restrict__ V,
-                       const TU *__restrict__ W, const double *__restrict__ P,
-                       const double *__restrict__ Q, const double *__restrict__ R,
+                       const TU *__restrict__ W, const TU *__restrict__ P,
+                       const TU *__restrict__ Q, const TU *__restrict__ R,
                        double *__restrict__ blkmax) {
   __shared__ double sm[4][TPB];
   double m[4] = {0, 0, 0, 0};
@@ -1144,6 +1147,13 @@ template <typename S> S *sp(double *a) { return reinterpret_cast<S *>(a); }
 // filtered velocity spectra into su, sv, sw.  S: the spectral arrays' type (FP32 in the
 // FP32-bulk form: its transforms and spectral arrays are FP32, the mean flow, the boundary
 // planes and the column solvers' arithmetic FP64).
+// the physical p, q, r and total-buoyancy grids (k = 0..3) in the transforms' precision: the
+// FP64 grids GP, GQ, GR, GB, or the FP32-bulk form's FP32 grids
+static_assert(GQ == GP + 1 && GR == GP + 2 && GB == GP + 3, "grid ids p, q, r, b in sequence");
+template <typename S> S *pqrb(Ctx *c, int k);
+template <> double *pqrb<double>(Ctx *c, int k) { return c->grid[GP + k]; }
+template <> float *pqrb<float>(Ctx *c, int k) { return c->b32.pgrid[k]; }
+
 template <typename S>
 static int vort2vel_spectral_t(Ctx *c) {
   const Geometry &g = c->g;
@@ -1154,7 +1164,7 @@ static int vort2vel_spectral_t(Ctx *c) {
   S *sder[5] = {sp<S>(c->sder[0]), sp<S>(c->sder[1]), sp<S>(c->sder[2]), sp<S>(c->sder[3]),
                 sp<S>(c->sder[4])};
   {
-    const double *src[3] = {c->grid[GP], c->grid[GQ], c->grid[GR]};
+    const S *src[3] = {pqrb<S>(c, 0), pqrb<S>(c, 1), pqrb<S>(c, 2)};
     S *dst[3] = {sa, sb, sc};
     PMPIC_TRY(fft_forward_t<S>(c, 3, src, dst));
   }
@@ -1192,14 +1202,18 @@ static int vort2vel_spectral_t(Ctx *c) {
                      sp<S>(c->sv), sp<S>(c->sw), c->kx, c->ky, c->filt);
   {
     const S *src[6] = {sd, se, sf, sder[2], sder[3], sder[4]};
-    // the FP32-bulk form keeps u, v, w only as the FP32 grids its gathers read
+    // the FP32-bulk form keeps p, q, r in its FP32 grids and u, v, w only as the FP32 grids its
+    // gathers read
     const bool b32 = c->precision == 1;
-    double *dst[6] = {c->grid[GP], c->grid[GQ], c->grid[GR], b32 ? nullptr : c->grid[GU],
-                      b32 ? nullptr : c->grid[GV], b32 ? nullptr : c->grid[GW]};
-    // the FP32-bulk form gathers u, v, w from FP32 copies written by the same pass
+    double *dst[6] = {c->grid[GP], c->grid[GQ], c->grid[GR], c->grid[GU], c->grid[GV],
+                      c->grid[GW]};
     float *dst32[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     if (b32)
-      for (int a = 0; a < 3; a++) dst32[3 + a] = c->b32.grid[a];
+      for (int a = 0; a < 3; a++) {
+        dst[a] = dst[3 + a] = nullptr;
+        dst32[a] = c->b32.pgrid[a];
+        dst32[3 + a] = c->b32.grid[a];
+      }
     PMPIC_TRY(fft_backward_t<S>(c, 6, src, dst, dst32, nullptr));
   }
   PMPIC_CHECK(hipGetLastError());
@@ -1217,9 +1231,10 @@ int vort2vel_spectral(Ctx *c, int stage) {
 template <typename S>
 static int vorticity_tendency_spectral_t(Ctx *c) {
   const Geometry &g = c->g;
+  const bool b32 = c->precision == 1;  // the FP32-bulk form: FP32 grids, FP32 tendencies only
   S *sbs = sp<S>(c->sbs);
   {
-    const double *src[1] = {c->grid[GB]};
+    const S *src[1] = {pqrb<S>(c, 3)};
     S *dst[1] = {sbs};
     PMPIC_TRY(fft_forward_t<S>(c, 1, src, dst));
   }
@@ -1229,16 +1244,22 @@ static int vorticity_tendency_spectral_t(Ctx *c) {
     S *su = sp<S>(c->su), *sv = sp<S>(c->sv), *sw = sp<S>(c->sw);
     const S *src[8] = {sbs, su, su, sw, sbs, sv, sv, sw};
     const int op[8] = {SPEC_DY, SPEC_DX, SPEC_DY, SPEC_DX, SPEC_DX, SPEC_DX, SPEC_DY, SPEC_DY};
+    // the derivative grids in the transforms' precision (the FP32-bulk form's in the first
+    // half of each FP64 scratch grid)
     double *dst[8];
-    for (int f = 0; f < 8; f++) dst[f] = c->gtmp[f];
-    PMPIC_TRY(fft_backward_t<S>(c, 8, src, dst, nullptr, op));
+    float *dst32[8];
+    for (int f = 0; f < 8; f++) {
+      dst[f] = b32 ? nullptr : c->gtmp[f];
+      dst32[f] = b32 ? reinterpret_cast<float *>(c->gtmp[f]) : nullptr;
+    }
+    PMPIC_TRY(fft_backward_t<S>(c, 8, src, dst, dst32, op));
   }
   const int64_t nnode = (int64_t)g.nzn * g.ny * g.nx;
-  const bool b32 = c->precision == 1;  // the FP32-bulk form: FP32 tendencies only
-  hipLaunchKernelGGL(k_tendency, dim3(cdiv(nnode, TPB)), dim3(TPB), 0, c->stream, nnode,
-                     c->grid[GP], c->grid[GQ], c->grid[GR], c->gtmp[0], c->gtmp[1], c->gtmp[2],
-                     c->gtmp[3], c->gtmp[4], c->gtmp[5], c->gtmp[6], c->gtmp[7],
-                     b32 ? nullptr : c->grid[GDP], b32 ? nullptr : c->grid[GDQ],
+  const S *d[8];
+  for (int f = 0; f < 8; f++) d[f] = reinterpret_cast<const S *>(c->gtmp[f]);
+  hipLaunchKernelGGL(k_tendency<S>, dim3(cdiv(nnode, TPB)), dim3(TPB), 0, c->stream, nnode,
+                     pqrb<S>(c, 0), pqrb<S>(c, 1), pqrb<S>(c, 2), d[0], d[1], d[2], d[3], d[4],
+                     d[5], d[6], d[7], b32 ? nullptr : c->grid[GDP], b32 ? nullptr : c->grid[GDQ],
                      b32 ? nullptr : c->grid[GDR], b32 ? c->b32.grid[3] : nullptr,
                      b32 ? c->b32.grid[4] : nullptr, b32 ? c->b32.grid[5] : nullptr);
   PMPIC_CHECK(hipGetLastError());
@@ -1256,10 +1277,10 @@ int grid_max_reduce(Ctx *c) {
   const int64_t nnode = (int64_t)g.nzn * g.ny * g.nx;
   int64_t nb = cdiv(nnode, TPB);
   if (nb > 1024) nb = 1024;  // c->maxblk holds 4 * 1024 block maxima
-  if (c->precision == 1)  // the FP32-bulk form's velocity grids are FP32 only
+  if (c->precision == 1)  // the FP32-bulk form's grids are FP32
     hipLaunchKernelGGL(k_max4<float>, dim3(nb), dim3(TPB), 0, c->stream, nnode, c->b32.grid[0],
-                       c->b32.grid[1], c->b32.grid[2], c->grid[GP], c->grid[GQ], c->grid[GR],
-                       c->maxblk);
+                       c->b32.grid[1], c->b32.grid[2], c->b32.pgrid[0], c->b32.pgrid[1],
+                       c->b32.pgrid[2], c->maxblk);
   else
     hipLaunchKernelGGL(k_max4<double>, dim3(nb), dim3(TPB), 0, c->stream, nnode, c->grid[GU],
                        c->grid[GV], c->grid[GW], c->grid[GP], c->grid[GQ], c->grid[GR],
