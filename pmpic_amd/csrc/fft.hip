@@ -18,6 +18,7 @@
 #include "common.h"
 
 #include <math.h>
+#include <type_traits>
 #include <vector>
 
 namespace pmpic {
@@ -31,6 +32,25 @@ struct Plan {
   int nrad;
   uint64_t code;
 };
+
+// radix 4 stages first, then 2, 3, 5; nrad = -1 when n has another prime factor
+__host__ __device__ constexpr Plan make_plan(int n) {
+  Plan p{n, 0, 0};
+  int m = n;
+  const int radix[4] = {4, 2, 3, 5};
+  for (int a = 0; a < 4; a++)
+    while (m % radix[a] == 0 && p.nrad < 16) {
+      p.code |= (uint64_t)radix[a] << (4 * p.nrad);
+      p.nrad++;
+      m /= radix[a];
+    }
+  if (m != 1) p.nrad = -1;
+  return p;
+}
+
+// Transform lengths with kernels compiled for them (every index computation then divides by
+// constants); other lengths run the same kernels with the length read at run time (NT = 0).
+constexpr int FIXED_N[] = {16, 32, 48, 64, 96, 128, 256, 512};
 
 constexpr int FFT_TPB = 256;
 constexpr int MAXF = 8;
@@ -159,9 +179,26 @@ __device__ __forceinline__ void stockham_stage(const C *x, C *y, const C *tw, in
   }
 }
 
-// nb transforms of length p.n stored t-major in `a`; result in the returned buffer.
-template <bool INV, typename C>
+// the stages from stage ST of a fixed length N (NS = product of the earlier radices)
+template <bool INV, int N, int ST, int NS, typename C>
+__device__ __forceinline__ C *fixed_stages(C *a, C *b, const C *tw, int nb) {
+  constexpr Plan P = make_plan(N);
+  static_assert(P.nrad > 0, "a fixed length has a radix schedule");
+  if constexpr (ST == P.nrad) {
+    return a;
+  } else {
+    constexpr int R = (int)((P.code >> (4 * ST)) & 15u);
+    stockham_stage<R, INV>(a, b, tw, N, NS, nb);
+    __syncthreads();
+    return fixed_stages<INV, N, ST + 1, NS * R>(b, a, tw, nb);
+  }
+}
+
+// nb transforms of length p.n (NT when NT > 0) stored t-major in `a`; result in the returned
+// buffer.
+template <bool INV, int NT, typename C>
 __device__ C *lds_fft(C *a, C *b, const C *tw, const Plan &p, int nb) {
+  if constexpr (NT > 0) return fixed_stages<INV, NT, 0, 1>(a, b, tw, nb);
   int Ns = 1;
   for (int s = 0; s < p.nrad; s++) {
     const int r = (int)((p.code >> (4 * s)) & 15u);
@@ -184,6 +221,27 @@ template <typename S>
 __host__ __device__ constexpr int ept() { return lds_cplx<S>() / FFT_TPB; }  // per thread
 constexpr int TPT = 512 / FFT_TPB;  // twiddles per thread (n <= 512)
 
+// adjacent columns (column pairs for the inverse) a column pass takes per workgroup
+template <typename S>
+__host__ __device__ constexpr int col_batch(int n) {
+  return lds_cplx<S>() / n > 32 ? 32 : lds_cplx<S>() / n;
+}
+
+// q = a / d and r = a - q d for 0 <= a < 2^22, 0 < d, with rd = 1/d rounded to FP32: the
+// product's relative error is below 2^-23, so the truncated estimate is off by at most one
+__device__ __forceinline__ int div_small(int a, int d, float rd, int &r) {
+  int q = (int)((float)a * rd);
+  r = a - q * d;
+  if (r < 0) {
+    q--;
+    r += d;
+  } else if (r >= d) {
+    q++;
+    r -= d;
+  }
+  return q;
+}
+
 template <typename C>
 __device__ __forceinline__ void load_twiddles(C (&w)[TPT], const C *tw, int n) {
 #pragma unroll
@@ -204,7 +262,7 @@ __device__ __forceinline__ void put_twiddles(C *tws, const C (&w)[TPT], int n, b
 // forward pass 1: r2c along x of every row (k, y) -> C1[k][y][l], l <= nx/2.  Two real rows
 // a, b share one complex transform Z = FFT(a + i b): A[l] = (Z[l] + conj Z[n-l]) / 2 and
 // B[l] = (Z[l] - conj Z[n-l]) / (2i).
-template <typename S>
+template <typename S, int NT>
 __global__ void __launch_bounds__(FFT_TPB) k_rows_r2c(Plan p, const cx_t<S> *__restrict__ tw,
                                                       FieldPtrs fp, cx_t<S> *__restrict__ c1,
                                                       int64_t c1_field, int nrows, int nb) {
@@ -214,7 +272,7 @@ __global__ void __launch_bounds__(FFT_TPB) k_rows_r2c(Plan p, const cx_t<S> *__r
   const int f = blockIdx.y;
   const S *src = static_cast<const S *>(fp.in[f]);
   C *dst = c1 + (int64_t)f * c1_field;
-  const int n = p.n, nl = n / 2 + 1;
+  const int n = NT ? NT : p.n, nl = n / 2 + 1;
   const int r0 = blockIdx.x * 2 * nb;                  // first row of this workgroup
   const int nbb = min(nb, (nrows - r0 + 1) / 2);       // transforms (row pairs)
   C w[TPT], v[EPT];
@@ -236,7 +294,7 @@ __global__ void __launch_bounds__(FFT_TPB) k_rows_r2c(Plan p, const cx_t<S> *__r
     if (e < nbb * n) A[e] = v[i];
   }
   __syncthreads();
-  C *R = lds_fft<false>(A, B, T, p, nbb);
+  C *R = lds_fft<false, NT>(A, B, T, p, nbb);
   for (int e = threadIdx.x; e < nbb * nl; e += FFT_TPB) {
     const int t = e / nl, l = e - t * nl;
     const int ra = r0 + 2 * t, rb = ra + 1;
@@ -249,7 +307,7 @@ __global__ void __launch_bounds__(FFT_TPB) k_rows_r2c(Plan p, const cx_t<S> *__r
 
 // forward pass 2: complex FFT along y of nb adjacent columns l of one z level, then the
 // (Re,Im)-pair split into the semi-spectral layout
-template <typename S>
+template <typename S, int NT>
 __global__ void __launch_bounds__(FFT_TPB) k_cols_fwd(Plan p, const cx_t<S> *__restrict__ tw,
                                                       FieldPtrs fp, const cx_t<S> *__restrict__ c1,
                                                       int64_t c1_field, int nl, int nb, int sx,
@@ -260,7 +318,8 @@ __global__ void __launch_bounds__(FFT_TPB) k_cols_fwd(Plan p, const cx_t<S> *__r
   const int f = blockIdx.z;
   const C *src = c1 + (int64_t)f * c1_field;
   S *dst = static_cast<S *>(fp.out[f]);
-  const int ny = p.n;
+  const int ny = NT ? NT : p.n;
+  const int nbd = NT ? col_batch<S>(NT) : nb;  // the batch every workgroup but the last takes
   const int k = blockIdx.y;
   const int l0 = blockIdx.x * nb;
   const int nbb = min(nb, nl - l0);
@@ -272,8 +331,8 @@ __global__ void __launch_bounds__(FFT_TPB) k_cols_fwd(Plan p, const cx_t<S> *__r
 #pragma unroll
   for (int i = 0; i < EPT; i++) {
     const int e = threadIdx.x + i * FFT_TPB;
-    const int y = e / nbb, t = e - y * nbb;
-    at[i] = (e < nbb * ny) ? t * ny + y : -1;
+    const int y = e / nbd, t = e - y * nbd;
+    at[i] = (y < ny && t < nbb) ? t * ny + y : -1;
     vx[i] = 0;
     vy[i] = 0;
     if (at[i] >= 0) {
@@ -287,12 +346,13 @@ __global__ void __launch_bounds__(FFT_TPB) k_cols_fwd(Plan p, const cx_t<S> *__r
   for (int i = 0; i < EPT; i++)
     if (at[i] >= 0) A[at[i]] = mkc<C, S>(vx[i], vy[i]);
   __syncthreads();
-  C *R = lds_fft<false>(A, B, T, p, nbb);
+  C *R = lds_fft<false, NT>(A, B, T, p, nbb);
   const int nm = ny / 2 + 1;
   S *out = dst + (int64_t)k * sy * sx;
-  for (int e = threadIdx.x; e < nbb * nm; e += FFT_TPB) {
-    int m = e / nbb, t = e - m * nbb;
-    int l = l0 + t;
+  for (int e = threadIdx.x; e < nbd * nm; e += FFT_TPB) {
+    const int m = e / nbd, t = e - m * nbd;
+    if (t >= nbb) continue;
+    const int l = l0 + t;
     C g = R[t * ny + m];
     C h = R[t * ny + ((ny - m) % ny)];
     C s = mkc<C, S>(g.x + h.x, g.y + h.y);
@@ -307,11 +367,11 @@ __global__ void __launch_bounds__(FFT_TPB) k_cols_fwd(Plan p, const cx_t<S> *__r
 // inverse pass 1: per (k, m) build H = A_m + i B_m over l in [0, nx) from the two real
 // x-spectra (imaginary parts of the DC and Nyquist bins dropped as c2r drops them), inverse
 // FFT along x, scale 1/nx -> C2[k][m][x]
-template <typename S>
+template <typename S, int NT>
 __global__ void __launch_bounds__(FFT_TPB) k_rows_inv(Plan p, const cx_t<S> *__restrict__ tw,
                                                       FieldPtrs fp, cx_t<S> *__restrict__ c2,
-                                                      int64_t c2_field, int nrows, int nm, int nb,
-                                                      int sx) {
+                                                      int64_t c2_field, int nrows, int nm,
+                                                      float rnm, int nb, int sx) {
   using C = cx_t<S>;
   constexpr int EPT = ept<S>();
   __shared__ C A[lds_cplx<S>()], B[lds_cplx<S>()], T[512];
@@ -319,7 +379,7 @@ __global__ void __launch_bounds__(FFT_TPB) k_rows_inv(Plan p, const cx_t<S> *__r
   const S *src = static_cast<const S *>(fp.in[f]);
   const int op = fp.op[f];
   C *dst = c2 + (int64_t)f * c2_field;
-  const int nx = p.n;
+  const int nx = NT ? NT : p.n;
   const int r0 = blockIdx.x * nb;
   const int nbb = min(nb, nrows - r0);
   const int half = nx / 2;
@@ -332,7 +392,8 @@ __global__ void __launch_bounds__(FFT_TPB) k_rows_inv(Plan p, const cx_t<S> *__r
     if (e < nbb * nx) {
       const int t = e / nx, l = e - t * nx;
       const int row = r0 + t;  // row = k * nm + m
-      const int k = row / nm, m = row - k * nm;
+      int m;
+      const int k = div_small(row, nm, rnm, m);
       const S *sa = src + ((int64_t)k * (2 * nm) + 2 * m) * sx;
       const int ll = (l <= half) ? l : nx - l;
       va[i] = *reinterpret_cast<const C *>(sa + 2 * ll);       // (ar, ai)
@@ -373,7 +434,7 @@ __global__ void __launch_bounds__(FFT_TPB) k_rows_inv(Plan p, const cx_t<S> *__r
     }
   }
   __syncthreads();
-  C *R = lds_fft<true>(A, B, T, p, nbb);
+  C *R = lds_fft<true, NT>(A, B, T, p, nbb);
   const S dn = (S)1 / (S)nx;
   for (int e = threadIdx.x; e < nbb * nx; e += FFT_TPB) {
     int t = e / nx, x = e - t * nx;
@@ -385,7 +446,7 @@ __global__ void __launch_bounds__(FFT_TPB) k_rows_inv(Plan p, const cx_t<S> *__r
 // inverse pass 2: c2r along y for 2 nb adjacent x columns of one z level.  The Hermitian
 // spectra of two columns a, b share one complex inverse transform of H_a + i H_b, whose real
 // and imaginary parts are the two real columns.
-template <typename S>
+template <typename S, int NT>
 __global__ void __launch_bounds__(FFT_TPB) k_cols_c2r(Plan p, const cx_t<S> *__restrict__ tw,
                                                       FieldPtrs fp, const cx_t<S> *__restrict__ c2,
                                                       int64_t c2_field, int nx, int nb) {
@@ -395,7 +456,8 @@ __global__ void __launch_bounds__(FFT_TPB) k_cols_c2r(Plan p, const cx_t<S> *__r
   const int f = blockIdx.z;
   const C *src = c2 + (int64_t)f * c2_field;
   double *dst = static_cast<double *>(fp.out[f]);
-  const int ny = p.n, nm = ny / 2 + 1;
+  const int ny = NT ? NT : p.n, nm = ny / 2 + 1;
+  const int nbd = NT ? col_batch<S>(NT) : nb;  // the batch every workgroup but the last takes
   const int k = blockIdx.y;
   const int x0 = blockIdx.x * 2 * nb;
   const int nbb = min(nb, (nx - x0 + 1) / 2);
@@ -406,8 +468,8 @@ __global__ void __launch_bounds__(FFT_TPB) k_cols_c2r(Plan p, const cx_t<S> *__r
 #pragma unroll
   for (int i = 0; i < EPT; i++) {
     const int e = threadIdx.x + i * FFT_TPB;
-    if (e < nbb * ny) {
-      const int y = e / nbb, t = e - y * nbb;
+    const int y = e / nbd, t = e - y * nbd;
+    if (y < ny && t < nbb) {
       const int xa = x0 + 2 * t, xb = xa + 1;
       const int m = (y < nm) ? y : ny - y;
       va[i] = lev[(int64_t)m * nx + xa];
@@ -418,8 +480,8 @@ __global__ void __launch_bounds__(FFT_TPB) k_cols_c2r(Plan p, const cx_t<S> *__r
 #pragma unroll
   for (int i = 0; i < EPT; i++) {
     const int e = threadIdx.x + i * FFT_TPB;
-    if (e < nbb * ny) {
-      const int y = e / nbb, t = e - y * nbb;
+    const int y = e / nbd, t = e - y * nbd;
+    if (y < ny && t < nbb) {
       const int m = (y < nm) ? y : ny - y;
       C wa = va[i], wb = vb[i];
       if (m == 0 || (even && m == ny / 2)) { wa.y = 0; wb.y = 0; }
@@ -428,13 +490,14 @@ __global__ void __launch_bounds__(FFT_TPB) k_cols_c2r(Plan p, const cx_t<S> *__r
     }
   }
   __syncthreads();
-  C *R = lds_fft<true>(A, B, T, p, nbb);
+  C *R = lds_fft<true, NT>(A, B, T, p, nbb);
   const S dn = (S)1 / (S)ny;
   // either output may be absent: the FP32-bulk form keeps some fields only in FP32
   double *out = dst ? dst + (int64_t)k * ny * nx : nullptr;
   float *o32 = fp.out32[f] ? fp.out32[f] + (int64_t)k * ny * nx : nullptr;
-  for (int e = threadIdx.x; e < nbb * ny; e += FFT_TPB) {
-    const int y = e / nbb, t = e - y * nbb;
+  for (int e = threadIdx.x; e < nbd * ny; e += FFT_TPB) {
+    const int y = e / nbd, t = e - y * nbd;
+    if (t >= nbb) continue;
     const int xa = x0 + 2 * t, xb = xa + 1;
     const C v = R[t * ny + y];
     if (out) {
@@ -446,22 +509,6 @@ __global__ void __launch_bounds__(FFT_TPB) k_cols_c2r(Plan p, const cx_t<S> *__r
       if (xb < nx) o32[(int64_t)y * nx + xb] = (float)(v.y * dn);
     }
   }
-}
-
-Plan make_plan(int n) {
-  Plan p{};
-  p.n = n;
-  int m = n;
-  auto add = [&](int r) {
-    p.code |= (uint64_t)r << (4 * p.nrad);
-    p.nrad++;
-  };
-  while (m % 4 == 0) { add(4); m /= 4; }
-  while (m % 2 == 0) { add(2); m /= 2; }
-  while (m % 3 == 0) { add(3); m /= 3; }
-  while (m % 5 == 0) { add(5); m /= 5; }
-  if (m != 1 || p.nrad > 16) p.nrad = -1;
-  return p;
 }
 
 // W_n^e for e in [0, n), in FP64 and rounded to FP32
@@ -507,24 +554,47 @@ int fft_setup(Ctx *c) {
 static int64_t c1_field_size(const Geometry &g) { return (int64_t)g.nzn * g.ny * (g.nx / 2 + 1); }
 static int64_t c2_field_size(const Geometry &g) { return (int64_t)g.nzn * (g.ny / 2 + 1) * g.nx; }
 
+// f(std::integral_constant<int, N>) with N = n for a length in FIXED_N, else N = 0
+template <class F>
+int with_length(int n, F &&f) {
+  switch (n) {
+    case 16: return f(std::integral_constant<int, 16>{});
+    case 32: return f(std::integral_constant<int, 32>{});
+    case 48: return f(std::integral_constant<int, 48>{});
+    case 64: return f(std::integral_constant<int, 64>{});
+    case 96: return f(std::integral_constant<int, 96>{});
+    case 128: return f(std::integral_constant<int, 128>{});
+    case 256: return f(std::integral_constant<int, 256>{});
+    case 512: return f(std::integral_constant<int, 512>{});
+    default: return f(std::integral_constant<int, 0>{});
+  }
+}
+static_assert(sizeof(FIXED_N) / sizeof(int) == 8, "with_length lists every fixed length");
+
 template <typename S>
 int fft_forward_t(Ctx *c, int nf, const S *const *src, S *const *dst) {
   const Geometry &g = c->g;
   if (nf < 1 || nf > MAXF) return PMPIC_ERR_ARG;
   FieldPtrs fp{};
   for (int f = 0; f < nf; f++) { fp.in[f] = src[f]; fp.out[f] = dst[f]; }
-  Plan px = make_plan(g.nx), py = make_plan(g.ny);
+  const Plan px = make_plan(g.nx), py = make_plan(g.ny);
   const int64_t c1f = c1_field_size(g);
   const int nrows = g.nzn * g.ny;
   cx_t<S> *cb = reinterpret_cast<cx_t<S> *>(c->cbuf);
   const int nbx = lds_cplx<S>() / g.nx;  // transforms per workgroup, two rows each
-  hipLaunchKernelGGL(k_rows_r2c<S>, dim3(cdiv(nrows, 2 * nbx), nf), dim3(FFT_TPB), 0, c->stream,
-                     px, twiddles_x<S>(c), fp, cb, c1f, nrows, nbx);
+  with_length(g.nx, [&](auto N) {
+    hipLaunchKernelGGL((k_rows_r2c<S, decltype(N)::value>), dim3(cdiv(nrows, 2 * nbx), nf),
+                       dim3(FFT_TPB), 0, c->stream, px, twiddles_x<S>(c), fp, cb, c1f, nrows, nbx);
+    return 0;
+  });
   const int nl = g.nx / 2 + 1;
-  int nby = lds_cplx<S>() / g.ny;
-  if (nby > 32) nby = 32;
-  hipLaunchKernelGGL(k_cols_fwd<S>, dim3(cdiv(nl, nby), g.nzn, nf), dim3(FFT_TPB), 0, c->stream,
-                     py, twiddles_y<S>(c), fp, cb, c1f, nl, nby, g.sx, g.sy);
+  const int nby = col_batch<S>(g.ny);
+  with_length(g.ny, [&](auto N) {
+    hipLaunchKernelGGL((k_cols_fwd<S, decltype(N)::value>), dim3(cdiv(nl, nby), g.nzn, nf),
+                       dim3(FFT_TPB), 0, c->stream, py, twiddles_y<S>(c), fp, cb, c1f, nl, nby,
+                       g.sx, g.sy);
+    return 0;
+  });
   PMPIC_CHECK(hipGetLastError());
   return PMPIC_OK;
 }
@@ -544,18 +614,26 @@ int fft_backward_t(Ctx *c, int nf, const S *const *src, double *const *dst, floa
     fp.op[f] = deriv ? deriv[f] : SPEC_PLAIN;
     if (fp.op[f] < SPEC_PLAIN || fp.op[f] > SPEC_DY) return PMPIC_ERR_ARG;
   }
-  Plan px = make_plan(g.nx), py = make_plan(g.ny);
+  const Plan px = make_plan(g.nx), py = make_plan(g.ny);
   const int64_t c2f = c2_field_size(g);
   const int nm = g.ny / 2 + 1;
   const int nrows = g.nzn * nm;
+  if (nrows >= (1 << 22)) return PMPIC_ERR_ARG;  // the row split's reciprocal division
+  const float rnm = (float)(1.0 / nm);
   cx_t<S> *cb = reinterpret_cast<cx_t<S> *>(c->cbuf);
   const int nbx = lds_cplx<S>() / g.nx;
-  hipLaunchKernelGGL(k_rows_inv<S>, dim3(cdiv(nrows, nbx), nf), dim3(FFT_TPB), 0, c->stream, px,
-                     twiddles_x<S>(c), fp, cb, c2f, nrows, nm, nbx, g.sx);
-  int nby = lds_cplx<S>() / g.ny;  // transforms per workgroup, two x columns each
-  if (nby > 32) nby = 32;
-  hipLaunchKernelGGL(k_cols_c2r<S>, dim3(cdiv(g.nx, 2 * nby), g.nzn, nf), dim3(FFT_TPB), 0,
-                     c->stream, py, twiddles_y<S>(c), fp, cb, c2f, g.nx, nby);
+  with_length(g.nx, [&](auto N) {
+    hipLaunchKernelGGL((k_rows_inv<S, decltype(N)::value>), dim3(cdiv(nrows, nbx), nf),
+                       dim3(FFT_TPB), 0, c->stream, px, twiddles_x<S>(c), fp, cb, c2f, nrows, nm,
+                       rnm, nbx, g.sx);
+    return 0;
+  });
+  const int nby = col_batch<S>(g.ny);  // transforms per workgroup, two x columns each
+  with_length(g.ny, [&](auto N) {
+    hipLaunchKernelGGL((k_cols_c2r<S, decltype(N)::value>), dim3(cdiv(g.nx, 2 * nby), g.nzn, nf),
+                       dim3(FFT_TPB), 0, c->stream, py, twiddles_y<S>(c), fp, cb, c2f, g.nx, nby);
+    return 0;
+  });
   PMPIC_CHECK(hipGetLastError());
   return PMPIC_OK;
 }
