@@ -39,10 +39,13 @@ HBM_PEAK_GBS = 8000.0
 RK_STAGE_BYTES = {1: 24 + 24 + 18 * 8 + 48 + 1, 2: 24 + 12 * 8 + 12 * 8 + 48 + 1,
                   3: 24 + 12 * 8 + 12 * 8 + 48 + 1, 4: 24 + 7 * 8 + 13 * 8 + 48 + 1}
 # FP32-bulk form: cell accumulation reads per parcel its slot entry, offsets, vol, p,q,r,b,h
-# (40 B; the z level comes from the cell) and per cell its count and overflow offset and
-# writes 40 FP32 partials (168 B); the RK stages move
-# cell (4 B) + offsets (12 B) for the position and the FP32 RK state (see bulk32.hip)
+# (40 B; the z level comes from the cell) and per cell its count and overflow offset (8 B);
+# it writes 40 FP32 corner partials per cell (160 B), or, when nx is a multiple of 16 (tiles
+# of 16 cells in one x row), the partials combined along x: 4 corner pairs x 17 node positions
+# x 5 values per 16 cells (85 B per cell).  The RK stages move cell (4 B) + offsets (12 B) for
+# the position and the FP32 RK state (see bulk32.hip)
 ACC_BYTES = {0: (76, 328), 1: (40, 168)}
+ACC_BYTES_XROWS = (40, 8 + 85)
 RK32_STAGE_BYTES = {1: 28 + 80, 2: 68 + 52, 3: 68 + 52, 4: 60 + 56}
 GATHER_NODE_BYTES = {0: 48, 1: 24}
 
@@ -210,11 +213,16 @@ def time_form(precision, ic, args, n, nz):
     nnode = n * n * (nz + 1)
     avg_np = total / max(args.steps, 1)
     form = "fp32" if precision else "fp64"
+    xrows = precision == 1 and n % 16 == 0  # the FP32 accumulation's x-combined partials
+    acc_bytes = ACC_BYTES_XROWS if xrows else ACC_BYTES[precision]
     acc = {
-        "kernel": ("k32_cell_accum (FP32 par2grid per-cell corner sums)" if precision else
+        "kernel": ("k32_cell_accum (FP32 par2grid cell sums, corner partials combined along x)"
+                   if xrows else
+                   "k32_cell_accum (FP32 par2grid per-cell corner sums)" if precision else
                    "k_cell_accum (par2grid per-cell corner sums)"),
-        "family": "k32_cell_accum<0>" if precision else "k_cell_accum<0>",
-        "bytes": avg_np * ACC_BYTES[precision][0] + ncell * ACC_BYTES[precision][1],
+        "family": ("k_cell_accum<0>" if not precision else
+                   "k32_cell_accum<3>" if xrows else "k32_cell_accum<0>"),
+        "bytes": avg_np * acc_bytes[0] + ncell * acc_bytes[1],
         "ms": p2g_ms, "calls": p2g_calls,
     }
     table = RK32_STAGE_BYTES if precision else RK_STAGE_BYTES

@@ -364,7 +364,9 @@ __global__ void k32_sort_big(const int32_t *__restrict__ start, int64_t ncell, i
 // corner q over the cell's parcels in ascending parcel order, so the sums are deterministic.
 // P2G: p, q, r and the total buoyancy (vort2vel + vorticity_tendency); MIX: kept / residual
 // vol, b, h, p, q, r (parcel_mixing); DUMP: b, h, p, q, r (writebinary's grids)
-enum Acc32 { A32_P2G = 0, A32_MIX = 1, A32_DUMP = 2 };
+// P2GX: P2G whose tiles are 16 cells of one x row (nx a multiple of 16), each tile writing
+// its corner sums combined along x into the 17 node positions it touches (see k32_node_sum_x)
+enum Acc32 { A32_P2G = 0, A32_MIX = 1, A32_DUMP = 2, A32_P2GX = 3 };
 constexpr int TILE32 = 16;
 constexpr int ROWS32 = 512;     // LDS rows with the full capacity
 constexpr int ROWS_LOW = 256;   // LDS rows with CAPC_LOW
@@ -412,7 +414,8 @@ __attribute__((amdgpu_waves_per_eu(CAP == CAPC_LOW ? 7 : 1))) k32_cell_accum(Bul
                                                        int32_t *__restrict__ npercell,
                                                        uint8_t *__restrict__ tileres) {
   constexpr bool MIX = (MODE == A32_MIX);
-  constexpr bool FIVE = (MODE != A32_P2G);  // five values b, h, p, q, r
+  constexpr bool XC = (MODE == A32_P2GX);
+  constexpr bool FIVE = (MODE == A32_MIX || MODE == A32_DUMP);  // five values b, h, p, q, r
   constexpr int NV = FIVE ? 5 : 4;
   constexpr int NF = NV + 1;
   constexpr int SPT = TILE32 * CAP / ATPB;  // slot entries per thread
@@ -633,6 +636,32 @@ __attribute__((amdgpu_waves_per_eu(CAP == CAPC_LOW ? 7 : 1))) k32_cell_accum(Bul
       }
     }
   }
+  if constexpr (XC) {
+    // the tile's 16 cells lie in one x row: node position pos (0..16) of corner pair
+    // bb = (by, bz) sums the bx = 1 corner of cell pos - 1 and the bx = 0 corner of cell pos
+    // (left + right); positions 0 and 16 hold one side, completed by k32_node_sum_x
+    static_assert(TILE32 == 16 && NF == 5, "x-combined partials of 16-cell tiles, 5 values");
+    float *cor = reinterpret_cast<float *>(sgeo);  // free after the passes' last barrier
+#pragma unroll
+    for (int f = 0; f < NF; f++) cor[tid * NF + f] = acc[f];  // tid = cell * 8 + corner
+    __syncthreads();
+    if (tid < 4 * 17) {
+      const int bb = tid / 17, pos = tid - bb * 17;
+      const int ntx = g.nx / TILE32;
+      const int64_t row = c0 / g.nx;
+      const int tx = (int)((c0 - row * g.nx) / TILE32);
+      const int64_t nrows = (int64_t)g.ny * g.ncz;
+      float *dst = partial + (((int64_t)bb * nrows + row) * (ntx * 17) + tx * 17 + pos) * NF;
+      // cell pos - 1, bx = 1 and cell pos, bx = 0 (indices clamped into the tile; the ends use
+      // one side only)
+      const float *L = cor + ((pos > 0 ? pos - 1 : 0) * 8 + 1 + 2 * bb) * NF;
+      const float *R = cor + ((pos < 16 ? pos : 15) * 8 + 2 * bb) * NF;
+#pragma unroll
+      for (int f = 0; f < NF; f++)
+        dst[f] = pos == 0 ? R[f] : pos == 16 ? L[f] : L[f] + R[f];
+    }
+    return;
+  }
   // mixing: a tile without residual parcels writes no residual partials; tileres tells the
   // residual node sums which tiles have them
   bool anyres = false;
@@ -713,6 +742,50 @@ __global__ void k32_node_sum(Geometry g, const float *__restrict__ partial, int6
 #pragma unroll
     for (int f = 0; f <= NV; f++) out.p[f][t] = (T)(s[f] * m);
   }
+}
+
+// node sums of the x-combined partials of A32_P2GX, normalised by the summed weight into the
+// FP32 grids: node (i, j, k) adds, for each corner pair (by, bz) in a fixed order, the entry of
+// cell row (j - by, k - bz) at its node position, and at a tile boundary the left tile's last
+// position before the right tile's first (the same left + right order as inside a tile)
+__global__ void k32_node_sum_x(Geometry g, const float *__restrict__ partial, Out32<float> out) {
+  const int64_t nnode = (int64_t)g.nzn * g.ny * g.nx;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nnode) return;
+  const int i = (int)(t % g.nx);
+  const int j = (int)((t / g.nx) % g.ny);
+  const int k = (int)(t / ((int64_t)g.nx * g.ny));
+  const int ntx = g.nx / TILE32, tx = i / TILE32, pos = i - tx * TILE32;
+  const int txl = tx == 0 ? ntx - 1 : tx - 1;  // the tile left of a boundary (periodic in x)
+  const int64_t nrows = (int64_t)g.ny * g.ncz, w = (int64_t)ntx * 17;
+  float v[4][5], vl[4][5];
+  bool ok[4];
+#pragma unroll
+  for (int bb = 0; bb < 4; bb++) {
+    const int by = bb & 1, bz = bb >> 1;
+    int cj = j - by;
+    if (cj < 0) cj += g.ny;
+    const int ck = k - bz;
+    ok[bb] = ck >= 0 && ck < g.ncz;
+    const int64_t base = ok[bb] ? ((int64_t)bb * nrows + (int64_t)ck * g.ny + cj) * w : 0;
+    const float *e = partial + (base + tx * 17 + pos) * 5;
+    const float *el = partial + (base + txl * 17 + 16) * 5;
+#pragma unroll
+    for (int f = 0; f < 5; f++) {
+      v[bb][f] = ok[bb] ? e[f] : 0.f;
+      vl[bb][f] = (ok[bb] && pos == 0) ? el[f] : 0.f;
+    }
+  }
+  float s[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int bb = 0; bb < 4; bb++)
+    if (ok[bb])
+#pragma unroll
+      for (int f = 0; f < 5; f++) s[f] = s[f] + (pos == 0 ? vl[bb][f] + v[bb][f] : v[bb][f]);
+  float wsum = s[0];
+  if (wsum == 0.f) wsum = 1.f;
+#pragma unroll
+  for (int f = 1; f < 5; f++) out.p[f - 1][t] = s[f] / wsum;
 }
 
 __global__ void k32_grid_to_f64(const float *__restrict__ a, double *__restrict__ b, int64_t n) {
@@ -1267,7 +1340,21 @@ int b32_get_parcels(Ctx *c, double *const *fields) {
   return PMPIC_OK;
 }
 
+// the x-combined partials need tiles within one x row
+static bool p2g_xrows(const Ctx *c) { return c->g.nx % TILE32 == 0; }
+
 static int p2g_accum(Ctx *c) {
+  if (p2g_xrows(c)) {
+    if (c->b32.capc == CAPC_LOW)
+      return launch_timed(c, T_ACCUM, k32_cell_accum<A32_P2GX, CAPC_LOW, ROWS_LOW>,
+                          dim3(cdiv(c->ncell, TILE32)), dim3(ATPB), 0, c->b32, c->cap, c->g,
+                          zmap_of(c->g), (float)c->g.vmin, c->cell_count, c->cell_start,
+                          c->sorted, c->ncell, partial32(c), nullptr, nullptr, nullptr);
+    return launch_timed(c, T_ACCUM, k32_cell_accum<A32_P2GX, CAPC, ROWS32>,
+                        dim3(cdiv(c->ncell, TILE32)), dim3(ATPB), 0, c->b32, c->cap, c->g,
+                        zmap_of(c->g), (float)c->g.vmin, c->cell_count, c->cell_start, c->sorted,
+                        c->ncell, partial32(c), nullptr, nullptr, nullptr);
+  }
   if (c->b32.capc == CAPC_LOW)
     return launch_timed(c, T_ACCUM, k32_cell_accum<A32_P2G, CAPC_LOW, ROWS_LOW>,
                         dim3(cdiv(c->ncell, TILE32)), dim3(ATPB), 0, c->b32, c->cap, c->g,
@@ -1330,8 +1417,12 @@ int b32_par2grid(Ctx *c) {
   }
   Out32<float> out = {{b.pgrid[0], b.pgrid[1], b.pgrid[2], b.pgrid[3], nullptr, nullptr}};
   const int64_t nnode = (int64_t)c->g.nzn * c->g.ny * c->g.nx;
-  hipLaunchKernelGGL((k32_node_sum<4, true, true, float>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0,
-                     c->stream, c->g, partial32(c), c->ncell, out, nullptr);
+  if (p2g_xrows(c))
+    hipLaunchKernelGGL(k32_node_sum_x, dim3(cdiv(nnode, TPB)), dim3(TPB), 0, c->stream, c->g,
+                       partial32(c), out);
+  else
+    hipLaunchKernelGGL((k32_node_sum<4, true, true, float>), dim3(cdiv(nnode, TPB)), dim3(TPB),
+                       0, c->stream, c->g, partial32(c), c->ncell, out, nullptr);
   PMPIC_CHECK(hipGetLastError());
   return PMPIC_OK;
 }
