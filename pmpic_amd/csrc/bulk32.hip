@@ -651,14 +651,17 @@ __attribute__((amdgpu_waves_per_eu(CAP == CAPC_LOW ? 7 : 1))) k32_cell_accum(Bul
       const int64_t row = c0 / g.nx;
       const int tx = (int)((c0 - row * g.nx) / TILE32);
       const int64_t nrows = (int64_t)g.ny * g.ncz;
-      float *dst = partial + (((int64_t)bb * nrows + row) * (ntx * 17) + tx * 17 + pos) * NF;
+      // value f of entry (bb, row, tx, pos) at ((f 4 + bb) nrows + row) W + 17 tx + pos,
+      // W = 17 ntx: each value's plane is read coalesced by the node sums
+      const int64_t fstride = 4 * nrows * (int64_t)(ntx * 17);
+      float *dst = partial + ((int64_t)bb * nrows + row) * (ntx * 17) + tx * 17 + pos;
       // cell pos - 1, bx = 1 and cell pos, bx = 0 (indices clamped into the tile; the ends use
       // one side only)
       const float *L = cor + ((pos > 0 ? pos - 1 : 0) * 8 + 1 + 2 * bb) * NF;
       const float *R = cor + ((pos < 16 ? pos : 15) * 8 + 2 * bb) * NF;
 #pragma unroll
       for (int f = 0; f < NF; f++)
-        dst[f] = pos == 0 ? R[f] : pos == 16 ? L[f] : L[f] + R[f];
+        dst[f * fstride] = pos == 0 ? R[f] : pos == 16 ? L[f] : L[f] + R[f];
     }
     return;
   }
@@ -758,6 +761,7 @@ __global__ void k32_node_sum_x(Geometry g, const float *__restrict__ partial, Ou
   const int ntx = g.nx / TILE32, tx = i / TILE32, pos = i - tx * TILE32;
   const int txl = tx == 0 ? ntx - 1 : tx - 1;  // the tile left of a boundary (periodic in x)
   const int64_t nrows = (int64_t)g.ny * g.ncz, w = (int64_t)ntx * 17;
+  const int64_t fstride = 4 * nrows * w;  // value planes (the layout k32_cell_accum writes)
   float v[4][5], vl[4][5];
   bool ok[4];
 #pragma unroll
@@ -768,12 +772,12 @@ __global__ void k32_node_sum_x(Geometry g, const float *__restrict__ partial, Ou
     const int ck = k - bz;
     ok[bb] = ck >= 0 && ck < g.ncz;
     const int64_t base = ok[bb] ? ((int64_t)bb * nrows + (int64_t)ck * g.ny + cj) * w : 0;
-    const float *e = partial + (base + tx * 17 + pos) * 5;
-    const float *el = partial + (base + txl * 17 + 16) * 5;
+    const float *e = partial + base + tx * 17 + pos;
+    const float *el = partial + base + txl * 17 + 16;
 #pragma unroll
     for (int f = 0; f < 5; f++) {
-      v[bb][f] = ok[bb] ? e[f] : 0.f;
-      vl[bb][f] = (ok[bb] && pos == 0) ? el[f] : 0.f;
+      v[bb][f] = ok[bb] ? e[f * fstride] : 0.f;
+      vl[bb][f] = (ok[bb] && pos == 0) ? el[f * fstride] : 0.f;
     }
   }
   float s[5] = {0.f, 0.f, 0.f, 0.f, 0.f};

@@ -14,6 +14,7 @@
 #include "common.h"
 
 #include <math.h>
+#include <type_traits>
 #include <vector>
 
 namespace pmpic {
@@ -457,7 +458,11 @@ __global__ void __launch_bounds__(64) k_laplinv_q4(Geometry g, ColFields<S> cf,
   if (k2 == 0) return;
   const int df0 = cf.df0[blockIdx.y];
   S *f = cf.out[blockIdx.y] + col;
-  double F[L], CPl[L];
+  // the FP32-bulk form (S float) forms one reciprocal per pivot and multiplies by it, and
+  // holds the column and the c' values in FP32; the FP64 form divides as the reference does
+  constexpr bool F32 = std::is_same<S, float>::value;
+  using FT = typename std::conditional<F32, float, double>::type;
+  FT F[L], CPl[L];
 #pragma unroll
   for (int i = 0; i < L; i++) {
     F[i] = 0.0;
@@ -474,12 +479,13 @@ __global__ void __launch_bounds__(64) k_laplinv_q4(Geometry g, ColFields<S> cf,
   // the original of level q L + L, the next quarter's first
   const double nxt0 = __shfl(F[0], lane + 16);
   constexpr int PTOP = (NZ - 1) / L;  // quarter holding level NZ - 1
-  double denominator = 0.0, fprev = 0.0, cprev = 0.0, f0 = 0.0, fp = 0.0;
+  double denominator = 0.0, rden = 0.0, fprev = 0.0, cprev = 0.0, f0 = 0.0, fp = 0.0;
   bool bad = false;
 #pragma unroll
   for (int p = 0; p <= PTOP; p++) {
     if (p > 0) {
       denominator = __shfl(denominator, lane - 16);
+      if (F32) rden = __shfl(rden, lane - 16);
       fprev = __shfl(fprev, lane - 16);
       cprev = __shfl(cprev, lane - 16);
       f0 = __shfl(f0, lane - 16);
@@ -498,8 +504,13 @@ __global__ void __launch_bounds__(64) k_laplinv_q4(Geometry g, ColFields<S> cf,
           if (!df0) { b1 = 1.; c1 = 0.; d1 = 0.; }
           else { b1 = -idz2 - f13 * k2; c1 = idz2 - f16 * k2; d1 = f13 * f0 + f16 * fp; }
           denominator = b1;
-          fprev = d1 / denominator;
-          F[0] = fprev;
+          if constexpr (F32) {
+            rden = 1.0 / denominator;
+            fprev = d1 * rden;
+          } else {
+            fprev = d1 / denominator;
+          }
+          F[0] = (FT)fprev;
           cprev = c1;
           continue;
         }
@@ -516,18 +527,23 @@ __global__ void __launch_bounds__(64) k_laplinv_q4(Geometry g, ColFields<S> cf,
           a = idz2 - f16 * k2; b = -idz2 - f13 * k2; c = 0.;
           d = f13 * f0 + f16 * fm;
         }
-        const double cpk = cprev / denominator;
-        CPl[i] = cpk;
+        const double cpk = F32 ? cprev * rden : cprev / denominator;
+        CPl[i] = (FT)cpk;
         denominator = b - a * cpk;
         bad |= (denominator == 0);
-        fprev = (d - a * fprev) / (denominator);
-        F[i] = fprev;
+        if constexpr (F32) {
+          rden = 1.0 / denominator;
+          fprev = (d - a * fprev) * rden;
+        } else {
+          fprev = (d - a * fprev) / (denominator);
+        }
+        F[i] = (FT)fprev;
         cprev = c;
       }
     }
   }
   // backward sweep from the top quarter down: level j takes c' of level j + 1
-  const double cpn = __shfl(CPl[0], lane + 16);
+  const double cpn = __shfl((double)CPl[0], lane + 16);
   double fnext = fprev;
 #pragma unroll
   for (int p = PTOP; p >= 0; p--) {
@@ -537,9 +553,9 @@ __global__ void __launch_bounds__(64) k_laplinv_q4(Geometry g, ColFields<S> cf,
       for (int i = L - 1; i >= 0; i--) {
         const int j = p * L + i;
         if (j >= NZ - 1) continue;
-        const double cpj1 = (i + 1 < L) ? CPl[i + 1 < L ? i + 1 : 0] : cpn;
-        const double v = F[i] - cpj1 * fnext;
-        F[i] = v;
+        const double cpj1 = (i + 1 < L) ? (double)CPl[i + 1 < L ? i + 1 : 0] : cpn;
+        const double v = (double)F[i] - cpj1 * fnext;
+        F[i] = (FT)v;
         fnext = v;
       }
     }
