@@ -1,5 +1,5 @@
 """Build the in-tree libraries: libpmpic_hip.so (hipcc, gfx950) and, when amdflang is present,
-the Fortran host shim library libpmpic_fortran.so and its driver."""
+the Fortran host side (component modules and the model-core stand-in driver pmpic_driver)."""
 import os
 import shutil
 import subprocess

@@ -13,7 +13,7 @@ module pmpic_hip_mod
   public :: pmpic_create, pmpic_destroy, pmpic_set_parcels, pmpic_get_parcels, pmpic_get_state, &
        pmpic_set_time, pmpic_vort2vel, pmpic_vorticity_tendency, pmpic_rk4_integrator, &
        pmpic_parcel_splitting, pmpic_parcel_mixing, pmpic_step, pmpic_par2grid, pmpic_grid2par, &
-       pmpic_get_grid, pmpic_conservation_sums, pmpic_last_counts
+       pmpic_get_grid, pmpic_conservation_sums, pmpic_last_counts, pmpic_writebinary_grids
 
   !> pmpic_config of include/pmpic_hip.h
   type, bind(C) :: pmpic_config_t
@@ -103,6 +103,10 @@ module pmpic_hip_mod
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: id
       real(c_double), intent(out) :: grid(*)
+    end function
+    integer(c_int) function pmpic_writebinary_grids(ctx) bind(C, name="pmpic_writebinary_grids")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
     end function
     integer(c_int) function pmpic_conservation_sums(ctx, out6) bind(C, name="pmpic_conservation_sums")
       import :: c_int, c_ptr, c_double

@@ -1,5 +1,8 @@
-"""The Fortran host side (pmpic_amd/fortran): component callbacks driving libpmpic_hip.so through
-iso_c_binding give the same state as the same steps driven through ctypes."""
+"""The Fortran host side (pmpic_amd/fortran): the model-core stand-in driver registers the five
+components' descriptors, whose hooks take `type(model_state_type)` as the reference's do and
+reach libpmpic_hip.so through the context module (pmpic_gpu_mod) over iso_c_binding.  After the
+host-mirror sync writebinary needs, its parcels and writebinary's ten grids are the same, bit for
+bit, as the same steps driven through ctypes."""
 import os
 import subprocess
 
@@ -37,3 +40,12 @@ def test_fortran_driver_matches_ctypes(tmp_path):
     assert abs(t - g.time) < 1e-15
     for k in want:
         np.testing.assert_array_equal(got[k], want[k])
+    # writebinary's grids from the Fortran state's u .. vol, each (nz+1, ny+4, nx+4) z fastest
+    from pmpic_amd.dumpio import GRID_FIELDS
+    g.writebinary_grids()
+    raw = np.fromfile(fout + ".grids", dtype="<f8")
+    shp = (shape[0] + 4, shape[1] + 4, shape[2] + 1)
+    assert raw.size == len(GRID_FIELDS) * np.prod(shp)
+    for i, name in enumerate(GRID_FIELDS):
+        got_g = raw[i * np.prod(shp):(i + 1) * np.prod(shp)].reshape(shp)
+        np.testing.assert_array_equal(got_g, g.grid(name), err_msg=name)
