@@ -364,9 +364,10 @@ __global__ void k32_sort_big(const int32_t *__restrict__ start, int64_t ncell, i
 // corner q over the cell's parcels in ascending parcel order, so the sums are deterministic.
 // P2G: p, q, r and the total buoyancy (vort2vel + vorticity_tendency); MIX: kept / residual
 // vol, b, h, p, q, r (parcel_mixing); DUMP: b, h, p, q, r (writebinary's grids)
-// P2GX: P2G whose tiles are 16 cells of one x row (nx a multiple of 16), each tile writing
-// its corner sums combined along x into the 17 node positions it touches (see k32_node_sum_x)
-enum Acc32 { A32_P2G = 0, A32_MIX = 1, A32_DUMP = 2, A32_P2GX = 3 };
+// P2GX, MIXX: P2G and MIX whose tiles are 16 cells of one x row (nx a multiple of 16), each
+// tile writing its corner sums combined along x into the 17 node positions it touches (see
+// k32_node_sum_x)
+enum Acc32 { A32_P2G = 0, A32_MIX = 1, A32_DUMP = 2, A32_P2GX = 3, A32_MIXX = 4 };
 constexpr int TILE32 = 16;
 constexpr int ROWS32 = 512;     // LDS rows with the full capacity
 constexpr int ROWS_LOW = 256;   // LDS rows with CAPC_LOW
@@ -413,9 +414,9 @@ __attribute__((amdgpu_waves_per_eu(CAP == CAPC_LOW ? 7 : 1))) k32_cell_accum(Bul
                                                        float *__restrict__ partial2,
                                                        int32_t *__restrict__ npercell,
                                                        uint8_t *__restrict__ tileres) {
-  constexpr bool MIX = (MODE == A32_MIX);
-  constexpr bool XC = (MODE == A32_P2GX);
-  constexpr bool FIVE = (MODE == A32_MIX || MODE == A32_DUMP);  // five values b, h, p, q, r
+  constexpr bool MIX = (MODE == A32_MIX || MODE == A32_MIXX);
+  constexpr bool XC = (MODE == A32_P2GX || MODE == A32_MIXX);
+  constexpr bool FIVE = (MIX || MODE == A32_DUMP);  // five values b, h, p, q, r
   constexpr int NV = FIVE ? 5 : 4;
   constexpr int NF = NV + 1;
   constexpr int SPT = TILE32 * CAP / ATPB;  // slot entries per thread
@@ -639,29 +640,41 @@ __attribute__((amdgpu_waves_per_eu(CAP == CAPC_LOW ? 7 : 1))) k32_cell_accum(Bul
   if constexpr (XC) {
     // the tile's 16 cells lie in one x row: node position pos (0..16) of corner pair
     // bb = (by, bz) sums the bx = 1 corner of cell pos - 1 and the bx = 0 corner of cell pos
-    // (left + right); positions 0 and 16 hold one side, completed by k32_node_sum_x
-    static_assert(TILE32 == 16 && NF == 5, "x-combined partials of 16-cell tiles, 5 values");
+    // (left + right); positions 0 and 16 hold one side, completed by k32_node_sum_x.  Value f
+    // of entry (bb, row, tx, pos) sits at ((f 4 + bb) nrows + row) W + 17 tx + pos, W = 17 ntx:
+    // each value's plane is read coalesced by the node sums.
+    static_assert(TILE32 == 16, "x-combined partials of 16-cell tiles");
     float *cor = reinterpret_cast<float *>(sgeo);  // free after the passes' last barrier
+    static_assert(ATPB * NF <= ROWS * 4, "the corner sums fit the staging rows");
+    const int ntx = g.nx / TILE32;
+    const int64_t row = c0 / g.nx;
+    const int tx = (int)((c0 - row * g.nx) / TILE32);
+    const int64_t nrows = (int64_t)g.ny * g.ncz;
+    const int64_t fstride = 4 * nrows * (int64_t)(ntx * 17);
+    auto write_x = [&](const float *v, float *out) {
+      __syncthreads();  // cor free (the passes' staging, or the previous write_x's reads)
 #pragma unroll
-    for (int f = 0; f < NF; f++) cor[tid * NF + f] = acc[f];  // tid = cell * 8 + corner
-    __syncthreads();
-    if (tid < 4 * 17) {
-      const int bb = tid / 17, pos = tid - bb * 17;
-      const int ntx = g.nx / TILE32;
-      const int64_t row = c0 / g.nx;
-      const int tx = (int)((c0 - row * g.nx) / TILE32);
-      const int64_t nrows = (int64_t)g.ny * g.ncz;
-      // value f of entry (bb, row, tx, pos) at ((f 4 + bb) nrows + row) W + 17 tx + pos,
-      // W = 17 ntx: each value's plane is read coalesced by the node sums
-      const int64_t fstride = 4 * nrows * (int64_t)(ntx * 17);
-      float *dst = partial + ((int64_t)bb * nrows + row) * (ntx * 17) + tx * 17 + pos;
-      // cell pos - 1, bx = 1 and cell pos, bx = 0 (indices clamped into the tile; the ends use
-      // one side only)
-      const float *L = cor + ((pos > 0 ? pos - 1 : 0) * 8 + 1 + 2 * bb) * NF;
-      const float *R = cor + ((pos < 16 ? pos : 15) * 8 + 2 * bb) * NF;
+      for (int f = 0; f < NF; f++) cor[tid * NF + f] = v[f];  // tid = cell * 8 + corner
+      __syncthreads();
+      if (tid < 4 * 17) {
+        const int bb = tid / 17, pos = tid - bb * 17;
+        float *dst = out + ((int64_t)bb * nrows + row) * (ntx * 17) + tx * 17 + pos;
+        // cell pos - 1, bx = 1 and cell pos, bx = 0 (indices clamped into the tile; the ends
+        // use one side only)
+        const float *L = cor + ((pos > 0 ? pos - 1 : 0) * 8 + 1 + 2 * bb) * NF;
+        const float *R = cor + ((pos < 16 ? pos : 15) * 8 + 2 * bb) * NF;
 #pragma unroll
-      for (int f = 0; f < NF; f++)
-        dst[f * fstride] = pos == 0 ? R[f] : pos == 16 ? L[f] : L[f] + R[f];
+        for (int f = 0; f < NF; f++)
+          dst[f * fstride] = pos == 0 ? R[f] : pos == 16 ? L[f] : L[f] + R[f];
+      }
+    };
+    write_x(acc, partial);
+    if constexpr (MIX) {
+      // residual partials only from tiles that have residual parcels (tileres)
+      const bool anyres = __syncthreads_or(nres > 0);
+      if (tid == 0) tileres[tile] = anyres ? 1 : 0;
+      if (q == 0) npercell[c0 + cl] = kept;
+      if (anyres) write_x(res, partial2);
     }
     return;
   }
@@ -747,11 +760,16 @@ __global__ void k32_node_sum(Geometry g, const float *__restrict__ partial, int6
   }
 }
 
-// node sums of the x-combined partials of A32_P2GX, normalised by the summed weight into the
-// FP32 grids: node (i, j, k) adds, for each corner pair (by, bz) in a fixed order, the entry of
-// cell row (j - by, k - bz) at its node position, and at a tile boundary the left tile's last
-// position before the right tile's first (the same left + right order as inside a tile)
-__global__ void k32_node_sum_x(Geometry g, const float *__restrict__ partial, Out32<float> out) {
+// node sums of the x-combined partials of A32_P2GX / A32_MIXX (NF values: the weight and NF - 1
+// weighted sums): node (i, j, k) adds, for each corner pair (by, bz) in a fixed order, the
+// entry of cell row (j - by, k - bz) at its node position, and at a tile boundary the left
+// tile's last position before the right tile's first (the same left + right order as inside a
+// tile).  NORM divides by the summed weight into NF - 1 grids; otherwise the NF raw sums,
+// doubled on the z boundaries when DBL.  tileres (may be null): per tile, whether its entries
+// were written; the entries of a tile without them contribute zero.
+template <int NF, bool NORM, bool DBL>
+__global__ void k32_node_sum_x(Geometry g, const float *__restrict__ partial, Out32<float> out,
+                               const uint8_t *__restrict__ tileres) {
   const int64_t nnode = (int64_t)g.nzn * g.ny * g.nx;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nnode) return;
@@ -762,34 +780,50 @@ __global__ void k32_node_sum_x(Geometry g, const float *__restrict__ partial, Ou
   const int txl = tx == 0 ? ntx - 1 : tx - 1;  // the tile left of a boundary (periodic in x)
   const int64_t nrows = (int64_t)g.ny * g.ncz, w = (int64_t)ntx * 17;
   const int64_t fstride = 4 * nrows * w;  // value planes (the layout k32_cell_accum writes)
-  float v[4][5], vl[4][5];
-  bool ok[4];
+  float v[4][NF], vl[4][NF];
+  bool ok[4], okl[4];
 #pragma unroll
   for (int bb = 0; bb < 4; bb++) {
     const int by = bb & 1, bz = bb >> 1;
     int cj = j - by;
     if (cj < 0) cj += g.ny;
     const int ck = k - bz;
-    ok[bb] = ck >= 0 && ck < g.ncz;
-    const int64_t base = ok[bb] ? ((int64_t)bb * nrows + (int64_t)ck * g.ny + cj) * w : 0;
+    const bool in = ck >= 0 && ck < g.ncz;
+    const int64_t crow = in ? (int64_t)ck * g.ny + cj : 0;
+    ok[bb] = in && (!tileres || tileres[crow * ntx + tx]);
+    okl[bb] = in && pos == 0 && (!tileres || tileres[crow * ntx + txl]);
+    const int64_t base = ((int64_t)bb * nrows + crow) * w;
     const float *e = partial + base + tx * 17 + pos;
     const float *el = partial + base + txl * 17 + 16;
 #pragma unroll
-    for (int f = 0; f < 5; f++) {
+    for (int f = 0; f < NF; f++) {
       v[bb][f] = ok[bb] ? e[f * fstride] : 0.f;
-      vl[bb][f] = (ok[bb] && pos == 0) ? el[f * fstride] : 0.f;
+      vl[bb][f] = okl[bb] ? el[f * fstride] : 0.f;
     }
   }
-  float s[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+  float s[NF];
+#pragma unroll
+  for (int f = 0; f < NF; f++) s[f] = 0.f;
 #pragma unroll
   for (int bb = 0; bb < 4; bb++)
-    if (ok[bb])
 #pragma unroll
-      for (int f = 0; f < 5; f++) s[f] = s[f] + (pos == 0 ? vl[bb][f] + v[bb][f] : v[bb][f]);
-  float wsum = s[0];
-  if (wsum == 0.f) wsum = 1.f;
+    for (int f = 0; f < NF; f++) {
+      if (pos == 0) {
+        if (okl[bb] || ok[bb]) s[f] = s[f] + (vl[bb][f] + v[bb][f]);
+      } else if (ok[bb]) {
+        s[f] = s[f] + v[bb][f];
+      }
+    }
+  if constexpr (NORM) {
+    float wsum = s[0];
+    if (wsum == 0.f) wsum = 1.f;
 #pragma unroll
-  for (int f = 1; f < 5; f++) out.p[f - 1][t] = s[f] / wsum;
+    for (int f = 1; f < NF; f++) out.p[f - 1][t] = s[f] / wsum;
+  } else {
+    const float m = (DBL && (k == 0 || k == g.nzn - 1)) ? 2.f : 1.f;
+#pragma unroll
+    for (int f = 0; f < NF; f++) out.p[f][t] = s[f] * m;
+  }
 }
 
 __global__ void k32_grid_to_f64(const float *__restrict__ a, double *__restrict__ b, int64_t n) {
@@ -1422,8 +1456,8 @@ int b32_par2grid(Ctx *c) {
   Out32<float> out = {{b.pgrid[0], b.pgrid[1], b.pgrid[2], b.pgrid[3], nullptr, nullptr}};
   const int64_t nnode = (int64_t)c->g.nzn * c->g.ny * c->g.nx;
   if (p2g_xrows(c))
-    hipLaunchKernelGGL(k32_node_sum_x, dim3(cdiv(nnode, TPB)), dim3(TPB), 0, c->stream, c->g,
-                       partial32(c), out);
+    hipLaunchKernelGGL((k32_node_sum_x<5, true, false>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0,
+                       c->stream, c->g, partial32(c), out, nullptr);
   else
     hipLaunchKernelGGL((k32_node_sum<4, true, true, float>), dim3(cdiv(nnode, TPB)), dim3(TPB),
                        0, c->stream, c->g, partial32(c), c->ncell, out, nullptr);
@@ -1494,7 +1528,18 @@ int b32_mix(Ctx *c) {
   const float vmin = (float)g.vmin;
   // kept and residual grids (construct_grids, parcel_mixing.F90:440-624)
   PMPIC_TRY(bin32(c, norig));
-  if (b.capc == CAPC_LOW)
+  const bool xr = p2g_xrows(c);
+  if (xr && b.capc == CAPC_LOW)
+    hipLaunchKernelGGL((k32_cell_accum<A32_MIXX, CAPC_LOW, ROWS_LOW>), dim3(cdiv(c->ncell, TILE32)),
+                       dim3(ATPB), 0, c->stream, b, c->cap, g, zmap_of(g), vmin, c->cell_count,
+                       c->cell_start, c->sorted, c->ncell, partial32(c), partial32b(c),
+                       c->npercell, c->cellflag);
+  else if (xr)
+    hipLaunchKernelGGL((k32_cell_accum<A32_MIXX, CAPC, ROWS32>), dim3(cdiv(c->ncell, TILE32)),
+                       dim3(ATPB), 0, c->stream, b, c->cap, g, zmap_of(g), vmin, c->cell_count,
+                       c->cell_start, c->sorted, c->ncell, partial32(c), partial32b(c),
+                       c->npercell, c->cellflag);
+  else if (b.capc == CAPC_LOW)
     hipLaunchKernelGGL((k32_cell_accum<A32_MIX, CAPC_LOW, ROWS_LOW>), dim3(cdiv(c->ncell, TILE32)),
                        dim3(ATPB), 0, c->stream, b, c->cap, g, zmap_of(g), vmin, c->cell_count,
                        c->cell_start, c->sorted, c->ncell, partial32(c), partial32b(c),
@@ -1507,11 +1552,18 @@ int b32_mix(Ctx *c) {
   const int64_t nnode = (int64_t)g.nzn * g.ny * g.nx;
   Out32<float> kept = {{b.mgrid[0], b.mgrid[1], b.mgrid[2], b.mgrid[3], b.mgrid[4], b.mgrid[5]}};
   Out32<float> res = {{b.mgrid[6], b.mgrid[7], b.mgrid[8], b.mgrid[9], b.mgrid[10], b.mgrid[11]}};
-  hipLaunchKernelGGL((k32_node_sum<5, false, true, float>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0,
-                     c->stream, g, partial32(c), c->ncell, kept, nullptr);
   // cellflag holds the per-tile residual flags until the new-parcel flags reuse it below
-  hipLaunchKernelGGL((k32_node_sum<5, false, false, float>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0,
-                     c->stream, g, partial32b(c), c->ncell, res, c->cellflag);
+  if (xr) {
+    hipLaunchKernelGGL((k32_node_sum_x<6, false, true>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0,
+                       c->stream, g, partial32(c), kept, nullptr);
+    hipLaunchKernelGGL((k32_node_sum_x<6, false, false>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0,
+                       c->stream, g, partial32b(c), res, c->cellflag);
+  } else {
+    hipLaunchKernelGGL((k32_node_sum<5, false, true, float>), dim3(cdiv(nnode, TPB)), dim3(TPB),
+                       0, c->stream, g, partial32(c), c->ncell, kept, nullptr);
+    hipLaunchKernelGGL((k32_node_sum<5, false, false, float>), dim3(cdiv(nnode, TPB)), dim3(TPB),
+                       0, c->stream, g, partial32b(c), c->ncell, res, c->cellflag);
+  }
   // the reference builds the kept grids in the model's vol, b, hg, p, q, r grids
   // (parcel_mixing.F90:131), which is what a grid export shows after mixing; nothing in the
   // step reads them there, so b32_export_grid widens them when one is exported
