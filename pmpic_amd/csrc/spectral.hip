@@ -1044,6 +1044,9 @@ static int lds_cols(const Geometry &g) {
 #define PMPIC_REG_COLUMNS(X) X(9) X(17) X(33) X(65) X(129)
 // longer columns, up to LONG_Q * LONG_L z nodes: LONG_Q lanes per column, LONG_L levels each
 constexpr int LONG_Q = 8, LONG_L = 33;
+// other lengths up to 129 z nodes (48^3 and 96^3 grids, say) in the same lane-group form: four
+// lanes per column with 13 levels each up to 52 nodes, 33 each up to 129
+constexpr int MID_Q = 4, MID_L_SHORT = 13, MID_L = 33;
 
 template <typename S>
 int op_filter_t(Ctx *c, const S *in, S *out) {
@@ -1079,7 +1082,19 @@ int op_diffz_n(Ctx *c, int nf, const S *const *in, S *const *out, const double *
     default:
       break;
   }
-  if (c->g.nzn > 129 && c->g.nzn <= LONG_Q * LONG_L) {
+  if (c->g.nzn <= MID_Q * MID_L_SHORT) {
+    hipLaunchKernelGGL((k_diffz_qn<MID_Q, MID_L_SHORT, S>), dim3(cdiv(plane, 64 / MID_Q), nf),
+                       dim3(64), 0, c->stream, c->g, rf, c->dz_den, c->dz_cp, c->dsc);
+    PMPIC_CHECK(hipGetLastError());
+    return PMPIC_OK;
+  }
+  if (c->g.nzn <= 129) {
+    hipLaunchKernelGGL((k_diffz_qn<MID_Q, MID_L, S>), dim3(cdiv(plane, 64 / MID_Q), nf),
+                       dim3(64), 0, c->stream, c->g, rf, c->dz_den, c->dz_cp, c->dsc);
+    PMPIC_CHECK(hipGetLastError());
+    return PMPIC_OK;
+  }
+  if (c->g.nzn <= LONG_Q * LONG_L) {
     hipLaunchKernelGGL((k_diffz_qn<LONG_Q, LONG_L, S>), dim3(cdiv(plane, 64 / LONG_Q), nf),
                        dim3(64), 0, c->stream, c->g, rf, c->dz_den, c->dz_cp, c->dsc);
     PMPIC_CHECK(hipGetLastError());
@@ -1128,7 +1143,19 @@ int op_laplinv_n(Ctx *c, int nf, S *const *f, const int *df0) {
     default:
       break;
   }
-  if (c->g.nzn > 129 && c->g.nzn <= LONG_Q * LONG_L) {
+  if (c->g.nzn <= MID_Q * MID_L_SHORT) {
+    hipLaunchKernelGGL((k_laplinv_qn<MID_Q, MID_L_SHORT, S>), dim3(cdiv(plane, 64 / MID_Q), nf),
+                       dim3(64), 0, c->stream, c->g, rf, c->k2, c->dsc);
+    PMPIC_CHECK(hipGetLastError());
+    return PMPIC_OK;
+  }
+  if (c->g.nzn <= 129) {
+    hipLaunchKernelGGL((k_laplinv_qn<MID_Q, MID_L, S>), dim3(cdiv(plane, 64 / MID_Q), nf),
+                       dim3(64), 0, c->stream, c->g, rf, c->k2, c->dsc);
+    PMPIC_CHECK(hipGetLastError());
+    return PMPIC_OK;
+  }
+  if (c->g.nzn <= LONG_Q * LONG_L) {
     hipLaunchKernelGGL((k_laplinv_qn<LONG_Q, LONG_L, S>), dim3(cdiv(plane, 64 / LONG_Q), nf),
                        dim3(64), 0, c->stream, c->g, rf, c->k2, c->dsc);
     PMPIC_CHECK(hipGetLastError());

@@ -212,12 +212,15 @@ def test_twenty_steps_counts_match_oracle(oracle_mod, bar):
         bar("grid_" + name, rel_err(interior(g.grid(name)), interior(o.grid(name))), 1e-12)
 
 
-@pytest.mark.parametrize("shape", [(8, 6, 16), (8, 6, 32), (6, 8, 64), (6, 4, 128), (4, 6, 40),
-                                   (4, 4, 200), (4, 6, 256), (6, 4, 263)])
+@pytest.mark.parametrize("shape", [(8, 6, 16), (8, 6, 32), (6, 8, 64), (6, 4, 128), (4, 4, 6),
+                                   (4, 6, 40), (8, 6, 48), (4, 4, 51), (6, 4, 96), (4, 4, 127),
+                                   (4, 4, 200), (4, 6, 256), (6, 4, 263), (4, 4, 300)])
 def test_column_solvers_bitwise_all_forms(oracle_mod, shape):
     """diffz and laplinv bit-identical to the oracle for column lengths served by the
-    register-resident solvers (17, 33, 65, 129 z nodes), by the LDS form (41) and by the
-    eight-lanes-per-column form of long columns (201, 257 and its longest, 264)."""
+    register-resident solvers (17, 33, 65, 129 z nodes), by the four-lanes-per-column form of
+    other lengths up to 129 (7, 41, 49 and its longest, 52, with 13 levels a lane; 97 and 128
+    with 33), by the eight-lanes-per-column form of long columns (201, 257 and its longest,
+    264) and by the LDS form beyond (301)."""
     from pmpic_amd.lib import Context
     nx, ny, nz = shape
     b = box(nx, ny, nz)
