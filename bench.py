@@ -47,6 +47,9 @@ RK_STAGE_BYTES = {1: 24 + 24 + 18 * 8 + 48 + 1, 2: 24 + 12 * 8 + 12 * 8 + 48 + 1
 ACC_BYTES = {0: (76, 328), 1: (40, 168)}
 ACC_BYTES_XROWS = (40, 8 + 85)
 RK32_STAGE_BYTES = {1: 28 + 80, 2: 68 + 52, 3: 68 + 52, 4: 60 + 56}
+# on grids of 2^21 cells or more, RK stages 1-3 also bin the cells they write: a 4-byte slot
+# entry per parcel (the count atomics, one per run of a cell's parcels, left out)
+RK32_BIN_BYTES = 4
 GATHER_NODE_BYTES = {0: 48, 1: 24}
 
 
@@ -225,9 +228,15 @@ def time_form(precision, ic, args, n, nz):
         "bytes": avg_np * acc_bytes[0] + ncell * acc_bytes[1],
         "ms": p2g_ms, "calls": p2g_calls,
     }
-    table = RK32_STAGE_BYTES if precision else RK_STAGE_BYTES
+    table = dict(RK32_STAGE_BYTES if precision else RK_STAGE_BYTES)
+    fenv = os.environ.get("PMPIC_FUSED_BINNING")
+    fused = precision == 1 and (fenv == "1" if fenv else ncell >= (1 << 21))
+    if fused:
+        for s_ in (1, 2, 3):
+            table[s_] += RK32_BIN_BYTES
     rks = {
-        "kernel": ("k32_rk_stage (FP32 grid2par of 6 fields + RK4 stage update)" if precision else
+        "kernel": ("k32_rk_stage (FP32 grid2par of 6 fields + RK4 stage update"
+                   + (", cell binning in stages 1-3)" if fused else ")") if precision else
                    "k_rk_stage (grid2par of 6 fields + RK4 stage update)"),
         "family": "k32_rk_stage" if precision else "k_rk_stage",
         "bytes": avg_np * sum(table.values()) / 4 + nnode * GATHER_NODE_BYTES[precision],

@@ -840,15 +840,54 @@ struct Scale32 {
   float idx, idy, idz;  // cells per unit length
 };
 
+// binning fused into an RK stage: the cell lists k32_bin_slot would build from the cells the
+// stage writes (on = 0: no binning)
+struct BinOut {
+  int on;
+  int capc;              // slot capacity of the binning, 0 for the dense ranks
+  int32_t *count;        // per cell, zeroed before the stage
+  int32_t *slot;         // [cell * capc + rank]
+  int2 *ovf;             // (parcel, rank) past the capacity; their number in DevScalars::novf
+  int32_t *rank;         // per parcel, when capc = 0
+};
+
+// k32_bin_slot's binning of one parcel per lane (c: its cell as stored, or -1 for none): the
+// lanes of a run of equal cells share one count atomic, issued by the run's first lane
+__device__ __forceinline__ void bin_lane(const Geometry &g, int32_t cell, uint32_t p,
+                                         const BinOut &bo, DevScalars *dsc) {
+  const int lane = threadIdx.x & 63;
+  const int32_t c = cell >= 0 ? linear_cell(g, cell) : -1;
+  const int32_t cprev = __shfl_up(c, 1);
+  const bool head = (lane == 0) || (c != cprev);
+  const unsigned long long hm = __ballot(head);
+  const unsigned long long upto = hm & ((2ull << lane) - 1ull);
+  const int hl = 63 - __clzll(upto);
+  const unsigned long long above = hm & ~((2ull << hl) - 1ull);
+  const int next = above ? (__ffsll(above) - 1) : 64;
+  const int base = (head && c >= 0) ? atomicAdd(&bo.count[c], next - hl) : 0;
+  const int32_t rk = __shfl(base, hl) + (lane - hl);
+  if (bo.capc == 0) {
+    if (c >= 0) st(bo.rank, p, rk);
+    return;
+  }
+  const bool over = c >= 0 && rk >= bo.capc;
+  if (c >= 0 && !over) st(bo.slot, (uint32_t)(c * bo.capc + rk), (int32_t)p);
+  const unsigned long long om = __ballot(over);
+  if (om) {
+    const int lead = __ffsll(om) - 1;
+    int32_t ob = 0;
+    if (lane == lead) ob = atomicAdd(&dsc->novf, (int32_t)__popcll(om));
+    ob = __shfl(ob, lead);
+    if (over) bo.ovf[ob + __popcll(om & ((1ull << lane) - 1ull))] = make_int2((int32_t)p, rk);
+  }
+}
+
+// the RK stage of parcel p; returns its new cell, or -1 when it left the box (status raised)
 template <int STAGE>
-__global__ void __launch_bounds__(TPB) k32_rk_stage(Geometry g, Bulk32 b, int64_t cap, int64_t n,
-                                                    Grid32Set gs, Scale32 sc3, int stretch,
-                                                    uint8_t *__restrict__ split_flag,
-                                                    DevScalars *sc) {
-  const int64_t p64 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  (void)n;  // launch size only: the count comes from device memory (see k32_bin_slot)
-  if (p64 >= sc->n) return;
-  const uint32_t p = (uint32_t)p64;
+__device__ __forceinline__ int32_t rk_update(const Geometry &g, const Bulk32 &b, int64_t cap,
+                                             const Grid32Set &gs, const Scale32 &sc3,
+                                             int stretch, uint8_t *__restrict__ split_flag,
+                                             DevScalars *sc, uint32_t p) {
   float *F = b.f;
   auto fld = [&](int f) { return F + (size_t)f * cap; };
   const int32_t cell = ldnt(b.cell, p);
@@ -915,7 +954,7 @@ __global__ void __launch_bounds__(TPB) k32_rk_stage(Geometry g, Bulk32 b, int64_
   float nox, noy, noz;
   if (!place(g, io, jo, ko, tpos[0], tpos[1], tpos[2], ncell_, nox, noy, noz)) {
     raise_status(sc, PMPIC_ERR_OUT_OF_BOX_Z);
-    return;
+    return -1;
   }
   stnt(b.cell, p, ncell_);
   stnt(fld(B_OX), p, nox);
@@ -949,6 +988,28 @@ __global__ void __launch_bounds__(TPB) k32_rk_stage(Geometry g, Bulk32 b, int64_
       split_flag[p] = (snew > 4.f) ? 1 : 0;  // the test of parcel_splitting.F90:76
     }
   }
+  return ncell_;
+}
+
+template <int STAGE>
+__global__ void __launch_bounds__(TPB) k32_rk_stage(Geometry g, Bulk32 b, int64_t cap, int64_t n,
+                                                    Grid32Set gs, Scale32 sc3, int stretch,
+                                                    uint8_t *__restrict__ split_flag,
+                                                    DevScalars *sc, BinOut bo) {
+  const int64_t p64 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  (void)n;  // launch size only: the count comes from device memory (see k32_bin_slot)
+  const bool valid = p64 < sc->n;
+  const uint32_t p = (uint32_t)p64;
+  // with bo.on, the parcel's new cell is binned for the next stage's par2grid as k32_bin_slot
+  // bins it; every lane of the wavefront then takes part, parcels past the count as empty
+  if (bo.on) {
+    const int32_t nc = valid ? rk_update<STAGE>(g, b, cap, gs, sc3, stretch, split_flag, sc, p)
+                             : -1;
+    bin_lane(g, nc, p, bo, sc);
+    return;
+  }
+  if (!valid) return;
+  rk_update<STAGE>(g, b, cap, gs, sc3, stretch, split_flag, sc, p);
 }
 
 // ---- splitting (parcel_splitting.F90:76-131, then the haloswap's wrap and reflection) ----
@@ -1199,15 +1260,22 @@ int bin32(Ctx *c, int64_t n) {
   Bulk32 &b = c->b32;
   // the capacity of this binning: 16 entries while the mean is at most LOW_PPC parcels per
   // cell (fewer empty entries to read), else 32
-  b.capc = b32_choose_capc(c, n);
+  const int capc = b32_choose_capc(c, n);
   // the launches of a captured graph cover a parcel bound (the kernels read the count)
   const int64_t ln = c->graph_n ? c->graph_n : n;
-  hipLaunchKernelGGL(k32_bin_reset, dim3(cdiv(nc1, TPB)), dim3(TPB), 0, c->stream, c->cell_count,
-                     nc1, c->dsc);
-  if (ln > 0)
-    hipLaunchKernelGGL(k32_bin_slot, dim3(cdiv(ln, (int64_t)TPB * BIN_SEQ)), dim3(TPB), 0,
-                       c->stream, c->g, b.cell, ln, b.capc, b.slot, b.ovf, c->rank,
-                       c->cell_count, c->dsc);
+  if (b.prebinned && b.capc == capc) {
+    // the last RK stage binned the cells it wrote (same count, same capacity)
+    b.prebinned = false;
+  } else {
+    b.prebinned = false;
+    b.capc = capc;
+    hipLaunchKernelGGL(k32_bin_reset, dim3(cdiv(nc1, TPB)), dim3(TPB), 0, c->stream,
+                       c->cell_count, nc1, c->dsc);
+    if (ln > 0)
+      hipLaunchKernelGGL(k32_bin_slot, dim3(cdiv(ln, (int64_t)TPB * BIN_SEQ)), dim3(TPB), 0,
+                         c->stream, c->g, b.cell, ln, b.capc, b.slot, b.ovf, c->rank,
+                         c->cell_count, c->dsc);
+  }
   // dense-region offsets of the cells past the slot capacity
   auto it = rocprim::make_transform_iterator(static_cast<const int32_t *>(c->cell_count),
                                              OvfCount{b.capc});
@@ -1296,6 +1364,12 @@ int b32_alloc(Ctx *c) {
   b.capfix = 0;
   b.captune = false;
   b.captuned = 0;
+  // binning fused into RK stages 1-3: it costs the RK kernel more than the separate binning
+  // pass on small grids (the 32^3..96^3 ensemble members: 1% slower) and saves on large ones
+  // (256 x 256 x 128: 1.6% faster), so from 2^21 cells on; PMPIC_FUSED_BINNING=0 or 1 forces it
+  const char *fenv = getenv("PMPIC_FUSED_BINNING");
+  b.fused_bin = fenv ? fenv[0] == '1' : c->ncell >= ((int64_t)1 << 21);
+  b.prebinned = false;
   const char *env = getenv("PMPIC_SLOT_CAPACITY");
   if (env && strcmp(env, "tune") == 0) {
     b.captune = b.capmax == CAPC;
@@ -1327,6 +1401,7 @@ void b32_free(Ctx *c) {
 int b32_set_parcels(Ctx *c, int64_t n, const double *const *fields) {
   PMPIC_TRY(ensure_fbuf(c));
   Bulk32 &b = c->b32;
+  b.prebinned = false;
   const dim3 gb(cdiv(n > 0 ? n : 1, TPB)), tb(TPB);
   for (int a = 0; a < 3; a++) {
     if (n > 0) {
@@ -1410,6 +1485,7 @@ static int p2g_accum(Ctx *c) {
 // changes no result; the partials of the last run are this par2grid's.
 static int tune_capacity(Ctx *c, int64_t n) {
   Bulk32 &b = c->b32;
+  b.prebinned = false;
   const int caps[2] = {CAPC_LOW, CAPC};
   float ms[2] = {0.f, 0.f};
   hipEvent_t e0, e1;
@@ -1472,31 +1548,48 @@ int b32_rk_stage(Ctx *c, int stage, int stretch) {
                    c->b32.grid[4], c->b32.grid[5]}};
   const Scale32 s3 = scale_of(c->g);
   const dim3 gb(cdiv(c->graph_n ? c->graph_n : n, TPB)), tb(TPB);
+  Bulk32 &b = c->b32;
+  // stages 1-3 bin the cells they write for the next stage's par2grid (stage 4 is followed by
+  // splitting and mixing, which change the parcels); not while the capacity is being tuned
+  BinOut bo{0, 0, c->cell_count, b.slot, b.ovf, c->rank};
+  b.prebinned = false;
+  if (stage < 4 && b.fused_bin && !b.captune) {
+    bo.on = 1;
+    bo.capc = b32_choose_capc(c, n);
+    const int64_t nc1 = c->ncell + 1;
+    hipLaunchKernelGGL(k32_bin_reset, dim3(cdiv(nc1, TPB)), dim3(TPB), 0, c->stream,
+                       c->cell_count, nc1, c->dsc);
+  }
   switch (stage) {
     case 1:
       PMPIC_TRY(launch_timed(c, T_RK, k32_rk_stage<1>, gb, tb, 0, c->g, c->b32, c->cap, n, gs,
-                             s3, stretch, c->flag, c->dsc));
+                             s3, stretch, c->flag, c->dsc, bo));
       break;
     case 2:
       PMPIC_TRY(launch_timed(c, T_RK, k32_rk_stage<2>, gb, tb, 0, c->g, c->b32, c->cap, n, gs,
-                             s3, stretch, c->flag, c->dsc));
+                             s3, stretch, c->flag, c->dsc, bo));
       break;
     case 3:
       PMPIC_TRY(launch_timed(c, T_RK, k32_rk_stage<3>, gb, tb, 0, c->g, c->b32, c->cap, n, gs,
-                             s3, stretch, c->flag, c->dsc));
+                             s3, stretch, c->flag, c->dsc, bo));
       break;
     default:
       PMPIC_TRY(launch_timed(c, T_RK, k32_rk_stage<4>, gb, tb, 0, c->g, c->b32, c->cap, n, gs,
-                             s3, stretch, c->flag, c->dsc));
+                             s3, stretch, c->flag, c->dsc, bo));
       break;
   }
   PMPIC_CHECK(hipGetLastError());
+  if (bo.on) {
+    b.prebinned = true;
+    b.capc = bo.capc;
+  }
   c->split_flags_fresh = (stage == 4 && stretch) ? 1 : 0;
   return PMPIC_OK;
 }
 
 int b32_split(Ctx *c) {
   const int64_t n = c->n;
+  c->b32.prebinned = false;
   if (n > 0) {
     // the stage-4 RK kernel already flagged the parcels to split
     if (!c->split_flags_fresh)
@@ -1524,6 +1617,7 @@ int b32_split(Ctx *c) {
 int b32_mix(Ctx *c) {
   const Geometry &g = c->g;
   Bulk32 &b = c->b32;
+  b.prebinned = false;
   const int64_t norig = c->n;
   const float vmin = (float)g.vmin;
   // kept and residual grids (construct_grids, parcel_mixing.F90:440-624)

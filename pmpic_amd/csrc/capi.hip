@@ -593,6 +593,7 @@ static int graph_capture(pmpic_ctx *ctx, int capc) {
   c->g123x = x;
   c->g123_n = bound;
   c->g123_capc = capc;
+  c->g123_prebinned = c->b32.prebinned;
   c->g123_captures++;
   return PMPIC_OK;
 }
@@ -620,6 +621,8 @@ static int graph_stages123(pmpic_ctx *ctx) {
     c->b32.kept_pending[1] = c->b32.kept_pending[3] = false;
     c->b32.kept_pending[4] = c->b32.kept_pending[5] = false;
     for (int i = 0; i < 4; i++) c->b32.p2g_pending[i] = true;
+    c->b32.prebinned = c->g123_prebinned;
+    if (c->b32.prebinned) c->b32.capc = capc;
   }
   PMPIC_CHECK(hipGraphLaunch(c->g123x, c->stream));
   c->g123_replays++;

@@ -61,6 +61,11 @@ struct Bulk32 {
   // into FP64 grid GP, GQ, GR, GB (k = 0..3) while p2g_pending[k]
   float *pgrid[4];
   bool p2g_pending[4];
+  // RK stages 1-3 bin the cells they write for the next par2grid (fused_bin: grids of 2^21
+  // cells or more, or as PMPIC_FUSED_BINNING says); prebinned: the cell lists of the current
+  // cells are built
+  bool fused_bin;
+  bool prebinned;
 };
 
 // grid ids exported through the C-ABI (writebinary.F90:287-296 order, then the tendencies)
@@ -163,6 +168,7 @@ struct Ctx {
   int graph_on;         // PMPIC_GRAPH=1 in the environment at creation
   int64_t eager_steps;  // steps run without the graph (the first sets up the scan storage)
   int64_t g123_captures, g123_replays;  // graphs captured and launched
+  bool g123_prebinned;                  // the binning state the captured stages leave
   // HIP-event timing: 0 = par2grid cell accumulation kernel, 1 = gather + RK stage kernel,
   // 2..6 = the component callbacks (timer regions named as the reference's timers)
   int timing;
