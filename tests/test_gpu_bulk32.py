@@ -199,6 +199,41 @@ def test_slot_capacity_changes_no_result(monkeypatch, case):
             assert np.array_equal(gr[k], g16[k]), (mode, k)
 
 
+@pytest.mark.parametrize("case", ["8ppc", "27ppc", "crowded", "dense"])
+def test_fused_binning_changes_no_result(monkeypatch, case):
+    """RK stages 1-3 binning the cells they write (PMPIC_FUSED_BINNING=1, the default on grids
+    of 2^21 cells or more) against the separate binning pass (=0): the same cell lists, summed
+    in ascending parcel order, so three steps of splitting and mixing end bit-identical in
+    counts, parcels and grids; at 8 and 27 parcels per cell, with crowded cells (the overflow
+    list) and without the slot table (the dense ranks)."""
+    shape = (16, 12, 8)
+    if case == "crowded":
+        ic = crowded_case(*shape)
+    else:
+        ic = thermal_case(*shape, 3 if case == "27ppc" else 2, vort=2.0, shrink=0.03, thin=0.05,
+                          near_split=0.03)
+    if case == "dense":
+        monkeypatch.setenv("PMPIC_SLOT_CAPACITY", "0")
+    names = ("u", "v", "w", "p", "q", "r", "b", "vol", "hg")
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("PMPIC_FUSED_BINNING", mode)
+        g = make(shape, 1, ic)
+        counts = []
+        for _ in range(3):
+            g.step()
+            counts.append((g.n, g.last_counts()))
+        out[mode] = (counts, g.get_parcels(), {k: g.grid(k) for k in names})
+        g.close()
+    c0, p0, g0 = out["0"]
+    c1, p1, g1 = out["1"]
+    assert c1 == c0
+    for k in p0:
+        np.testing.assert_array_equal(p1[k], p0[k], err_msg=k)
+    for k in names:
+        np.testing.assert_array_equal(g1[k], g0[k], err_msg=k)
+
+
 def test_writebinary_grids_close_to_fp64():
     """writebinary's grids (p, q, r, b + 12.5 hgliq, hg, hgliq) from the FP32 parcels after two
     steps, within 1e-5 of the FP64 form's (which test_gpu_parity.py holds to the oracle)."""

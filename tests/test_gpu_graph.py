@@ -5,7 +5,8 @@ every step bit-identical to a context launching every kernel eagerly (the defaul
 parcel field and the grids, compared after each step, with the eager stage-4 work (splitting,
 mixing) and host transfers between the replays.  Cases: splitting and mixing changing the count
 from step to step, a re-capture when the count drops out of the captured range, the slot
-capacities 16 and 32, the tuned capacity, the dense binning, and members stepped concurrently.
+capacities 16 and 32, the tuned capacity, the dense binning, the binning done by the RK stages,
+and members stepped concurrently.
 The graph is shown in use by its capture and launch counts."""
 import numpy as np
 import pytest
@@ -81,9 +82,12 @@ def test_graph_recaptured_when_count_drops(monkeypatch):
     assert cap1 == cap0 + 1 and launches == 5
 
 
-@pytest.mark.parametrize("slots", ["16", "32", "tune", "dense"])
+@pytest.mark.parametrize("slots", ["16", "32", "tune", "dense", "fused"])
 def test_graph_matches_eager_at_each_binning(monkeypatch, slots):
-    monkeypatch.setenv("PMPIC_SLOT_CAPACITY", "0" if slots == "dense" else slots)
+    if slots == "fused":  # RK stages 1-3 bin for the next par2grid (as on large grids)
+        monkeypatch.setenv("PMPIC_FUSED_BINNING", "1")
+    else:
+        monkeypatch.setenv("PMPIC_SLOT_CAPACITY", "0" if slots == "dense" else slots)
     shape = (12, 10, 8)
     ic = crowded_case(*shape)
     eager, graph = _pair(monkeypatch, shape, ic)
