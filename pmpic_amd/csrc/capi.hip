@@ -77,9 +77,11 @@ int dalloc(T **p, int64_t n) {
   return PMPIC_OK;
 }
 
+// a transform length of the semi-spectral passes: factors 2, 3 and 5 only, and at most 512
+// (the twiddle table a workgroup stages)
 int check_factors(int n) {
   int m = n;
-  if (m < 1) return 0;
+  if (m < 1 || m > 512) return 0;
   while (m % 5 == 0) m /= 5;
   while (m % 3 == 0) m /= 3;
   while (m % 2 == 0) m /= 2;

@@ -219,6 +219,14 @@ __device__ C *lds_fft(C *a, C *b, const C *tw, const Plan &p, int nb) {
 // stored), so a workgroup waits on memory once per pass rather than once per element.
 template <typename S>
 __host__ __device__ constexpr int ept() { return lds_cplx<S>() / FFT_TPB; }  // per thread
+// The row passes read contiguous rows, so a smaller batch costs them no coalescing: in FP32
+// they stage half as many elements (8 KiB per buffer), so twice the workgroups fit a CU's LDS
+template <typename S>
+__host__ __device__ constexpr int lds_rows() {
+  return sizeof(S) == 4 ? lds_cplx<S>() / 2 : lds_cplx<S>();
+}
+template <typename S>
+__host__ __device__ constexpr int ept_rows() { return lds_rows<S>() / FFT_TPB; }
 constexpr int TPT = 512 / FFT_TPB;  // twiddles per thread (n <= 512)
 
 // adjacent columns (column pairs for the inverse) a column pass takes per workgroup
@@ -267,8 +275,8 @@ __global__ void __launch_bounds__(FFT_TPB) k_rows_r2c(Plan p, const cx_t<S> *__r
                                                       FieldPtrs fp, cx_t<S> *__restrict__ c1,
                                                       int64_t c1_field, int nrows, int nb) {
   using C = cx_t<S>;
-  constexpr int EPT = ept<S>();
-  __shared__ C A[lds_cplx<S>()], B[lds_cplx<S>()], T[512];
+  constexpr int EPT = ept_rows<S>();
+  __shared__ C A[lds_rows<S>()], B[lds_rows<S>()], T[512];
   const int f = blockIdx.y;
   const S *src = static_cast<const S *>(fp.in[f]);
   C *dst = c1 + (int64_t)f * c1_field;
@@ -373,8 +381,8 @@ __global__ void __launch_bounds__(FFT_TPB) k_rows_inv(Plan p, const cx_t<S> *__r
                                                       int64_t c2_field, int nrows, int nm,
                                                       float rnm, int nb, int sx) {
   using C = cx_t<S>;
-  constexpr int EPT = ept<S>();
-  __shared__ C A[lds_cplx<S>()], B[lds_cplx<S>()], T[512];
+  constexpr int EPT = ept_rows<S>();
+  __shared__ C A[lds_rows<S>()], B[lds_rows<S>()], T[512];
   const int f = blockIdx.y;
   const S *src = static_cast<const S *>(fp.in[f]);
   const int op = fp.op[f];
@@ -581,7 +589,7 @@ int fft_forward_t(Ctx *c, int nf, const S *const *src, S *const *dst) {
   const int64_t c1f = c1_field_size(g);
   const int nrows = g.nzn * g.ny;
   cx_t<S> *cb = reinterpret_cast<cx_t<S> *>(c->cbuf);
-  const int nbx = lds_cplx<S>() / g.nx;  // transforms per workgroup, two rows each
+  const int nbx = lds_rows<S>() / g.nx;  // transforms per workgroup, two rows each
   with_length(g.nx, [&](auto N) {
     hipLaunchKernelGGL((k_rows_r2c<S, decltype(N)::value>), dim3(cdiv(nrows, 2 * nbx), nf),
                        dim3(FFT_TPB), 0, c->stream, px, twiddles_x<S>(c), fp, cb, c1f, nrows, nbx);
@@ -621,7 +629,7 @@ int fft_backward_t(Ctx *c, int nf, const S *const *src, double *const *dst, floa
   if (nrows >= (1 << 22)) return PMPIC_ERR_ARG;  // the row split's reciprocal division
   const float rnm = (float)(1.0 / nm);
   cx_t<S> *cb = reinterpret_cast<cx_t<S> *>(c->cbuf);
-  const int nbx = lds_cplx<S>() / g.nx;
+  const int nbx = lds_rows<S>() / g.nx;
   with_length(g.nx, [&](auto N) {
     hipLaunchKernelGGL((k_rows_inv<S, decltype(N)::value>), dim3(cdiv(nrows, nbx), nf),
                        dim3(FFT_TPB), 0, c->stream, px, twiddles_x<S>(c), fp, cb, c2f, nrows, nm,

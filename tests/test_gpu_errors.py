@@ -65,3 +65,14 @@ def test_empty_parcel_set_steps(precision):
     g.step(1)
     assert g.n == ncell
     assert g.last_counts() == (0, 0, ncell)
+
+
+@pytest.mark.parametrize("nx,ny", [(14, 8), (8, 1024)])
+def test_unsupported_transform_length_is_refused(nx, ny):
+    """A horizontal size with a prime factor other than 2, 3, 5 (the reference's FFTE check,
+    pencilfft.F90:114-117) or longer than the 512 points a transform workgroup stages is
+    refused at creation with PMPIC_ERR_FFT_SIZE, before anything is allocated."""
+    from pmpic_amd.lib import PmpicError
+    with pytest.raises(PmpicError) as e:
+        ctx((nx, ny, 4), 64, 0)
+    assert e.value.status == 7  # PMPIC_ERR_FFT_SIZE
