@@ -223,7 +223,7 @@ def time_form(precision, ic, args, n, nz):
                    if xrows else
                    "k32_cell_accum (FP32 par2grid per-cell corner sums)" if precision else
                    "k_cell_accum (par2grid per-cell corner sums)"),
-        "family": ("k_cell_accum<0>" if not precision else
+        "family": ("k_cell_accum_lds<0>" if not precision else
                    "k32_cell_accum<3>" if xrows else "k32_cell_accum<0>"),
         "bytes": avg_np * acc_bytes[0] + ncell * acc_bytes[1],
         "ms": p2g_ms, "calls": p2g_calls,
