@@ -402,11 +402,12 @@ __device__ __forceinline__ void add_parcel(float *acc, float w, const float *v) 
 // CAP: the binning's slot capacity (any value when capc = 0); ROWS: LDS rows of a pass
 // The kernel is bound by memory latency, so waves in flight set its speed: with the 16-entry
 // table the LDS rows allow 16 workgroups (32 waves) per CU, and holding the registers to 72
-// allows 7 waves per SIMD (28 per CU) instead of 5 (4-5 % faster at 8 parcels per cell).  The
-// 32-entry form is bound by its LDS rows (8 workgroups per CU) instead.
+// allows 7 waves per SIMD (28 per CU) instead of 5 (4-5 % faster at 8 parcels per cell); the
+// x-row form fits 64 registers without spilling, so all 8 (0.95 -> 0.90 ms at 256x256x128).
+// The 32-entry form is bound by its LDS rows (8 workgroups per CU) instead.
 template <int MODE, int CAP, int ROWS>
 __global__ void __launch_bounds__(ATPB)
-__attribute__((amdgpu_waves_per_eu(CAP == CAPC_LOW ? 7 : 1))) k32_cell_accum(Bulk32 b, int64_t cap, Geometry g, Zmap zm,
+__attribute__((amdgpu_waves_per_eu(CAP == CAPC_LOW ? (MODE == A32_P2GX ? 8 : 7) : 1))) k32_cell_accum(Bulk32 b, int64_t cap, Geometry g, Zmap zm,
                                                        float vmin, const int32_t *__restrict__ count,
                                                        const int32_t *__restrict__ ostart,
                                                        const int32_t *__restrict__ sorted,
