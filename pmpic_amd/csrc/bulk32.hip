@@ -442,6 +442,9 @@ __attribute__((amdgpu_waves_per_eu(CAP == CAPC_LOW ? (MODE == A32_P2GX ? 8 : 7) 
   const float sz = bz ? 1.f : -1.f, tz = bz ? 0.f : 1.f;
   const uint32_t tile = xcd_block(blockIdx.x, gridDim.x);
   const int64_t c0 = (int64_t)tile * TILE32;
+  // cell, tile and row numbers are below 2^31, so their divisions are 32-bit; an x-row tile
+  // has ntx tiles per row
+  const uint32_t ntx = XC ? (uint32_t)g.nx / TILE32 : 1u;
   const int ntile = (int)min((int64_t)TILE32, ncell - c0);
   const int capc = b.capc;
   int32_t sl[SPT];
@@ -464,7 +467,9 @@ __attribute__((amdgpu_waves_per_eu(CAP == CAPC_LOW ? (MODE == A32_P2GX ? 8 : 7) 
       scnt[tid] = v;
       sost[tid] = o;
       soff[tid + 1] = incl;
-      skz[tid] = (tid < ntile) ? (float)((c0 + tid) / ((int64_t)g.nx * g.ny)) : 0.f;
+      const uint32_t kc = XC ? tile / ntx / (uint32_t)g.ny
+                             : (uint32_t)(c0 + tid) / ((uint32_t)g.nx * (uint32_t)g.ny);
+      skz[tid] = (tid < ntile) ? (float)kc : 0.f;
     }
     if (tid == 0) {
       soff[0] = 0;
@@ -473,7 +478,7 @@ __attribute__((amdgpu_waves_per_eu(CAP == CAPC_LOW ? (MODE == A32_P2GX ? 8 : 7) 
   }
   __syncthreads();
   const float *F = b.f;
-  auto fld = [&](int f) { return F + (size_t)f * cap; };
+  auto fld = [&](int f) { return b.fp[f]; };
   // one parcel's staged values
   auto fetch = [&](uint32_t pp, float4 &geo, float4 &va, float &vb) {
     geo = make_float4(ldnt(fld(B_OX), pp), ldnt(fld(B_OY), pp), ldnt(fld(B_OZ), pp),
@@ -647,9 +652,9 @@ __attribute__((amdgpu_waves_per_eu(CAP == CAPC_LOW ? (MODE == A32_P2GX ? 8 : 7) 
     static_assert(TILE32 == 16, "x-combined partials of 16-cell tiles");
     float *cor = reinterpret_cast<float *>(sgeo);  // free after the passes' last barrier
     static_assert(ATPB * NF <= ROWS * 4, "the corner sums fit the staging rows");
-    const int ntx = g.nx / TILE32;
-    const int64_t row = c0 / g.nx;
-    const int tx = (int)((c0 - row * g.nx) / TILE32);
+    const uint32_t xrow = tile / ntx;
+    const int64_t row = xrow;
+    const int tx = (int)(tile - xrow * ntx);
     const int64_t nrows = (int64_t)g.ny * g.ncz;
     const int64_t fstride = 4 * nrows * (int64_t)(ntx * 17);
     auto write_x = [&](const float *v, float *out) {
@@ -1340,6 +1345,7 @@ int b32_alloc(Ctx *c) {
   Bulk32 &b = c->b32;
   for (int f = 0; f < 6; f++) b.kept_pending[f] = false;
   PMPIC_TRY(dalloc32(&b.f, (int64_t)NF32 * c->cap));
+  for (int k = 0; k < NF32; k++) b.fp[k] = b.f + (int64_t)k * c->cap;
   PMPIC_TRY(dalloc32(&b.cell, c->cap));
   PMPIC_TRY(dalloc32(&b.cello, c->cap));
   PMPIC_TRY(dalloc32(&b.tag, c->cap));

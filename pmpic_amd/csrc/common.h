@@ -37,6 +37,8 @@ enum Field32 {
 
 struct Bulk32 {
   float *f;          // NF32 * cap
+  float *fp[NF32];   // fp[k] = f + k * cap: field k (kernels take these from their arguments
+                     // instead of forming them)
   int32_t *cell;     // cap: cell holding the parcel
   int32_t *cello;    // cap: cell of the RK4 origin
   double *tag;       // cap
