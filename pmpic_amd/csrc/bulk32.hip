@@ -378,17 +378,16 @@ struct Zmap {
   float zbot, dz;  // z of a parcel = zbot + (k + oz) dz
 };
 
-// One parcel's weights at the two x-corners (bx = 0, 1) of the corner pair (by, bz), in the
-// form's one order of operations (every accumulation kernel and path uses it, so a cell's sums
-// do not depend on which of them ran): the y and z factors are s * offset + t, the offset for an
-// upper corner and 1 - offset for a lower one.
-__device__ __forceinline__ void pair_weights(float ox, float oy, float oz, float vol, float sy,
-                                             float ty, float sz, float tz, float &w0, float &w1) {
-  const float fy = fmaf(sy, oy, ty);
-  const float fz = fmaf(sz, oz, tz);
+// A parcel's weight at corner (bx, by, bz) of its cell, in the form's one order of operations
+// (every accumulation kernel and path uses it, so a cell's sums do not depend on which of them
+// ran): the factor of an axis is |t - offset|, t = 1 for the lower corner (1 - offset) and 0 for
+// the upper (the offset, which lies in [0, 1]); the weight is ((fz fy) vol) fx.
+__device__ __forceinline__ float corner_weight(float ox, float oy, float oz, float vol, float tx,
+                                               float ty, float tz) {
+  const float fy = fabsf(ty - oy);
+  const float fz = fabsf(tz - oz);
   const float wv = (fz * fy) * vol;
-  w1 = wv * ox;
-  w0 = wv * (1.f - ox);
+  return wv * fabsf(tx - ox);
 }
 
 // acc[0] += w, acc[1 + f] += w * v[f], fused
@@ -426,7 +425,7 @@ __attribute__((amdgpu_waves_per_eu(CAP == CAPC_LOW ? (MODE == A32_P2GX ? 8 : 7) 
   __shared__ float4 sva[ROWS];   // values 0..3
   __shared__ float svb[FIVE ? ROWS : 1];
   __shared__ int32_t sidx[ROWS];
-  __shared__ int16_t sperm[ROWS];
+  __shared__ int16_t sperm[ROWS];  // the rows of a cell in parcel order, as byte offsets 16 r
   __shared__ uint8_t skeep[MIX ? ROWS : 1];
   __shared__ int32_t soff[TILE32 + 1];  // first dense row of each cell of the tile
   __shared__ int32_t scnt[TILE32], sost[TILE32];
@@ -438,8 +437,8 @@ __attribute__((amdgpu_waves_per_eu(CAP == CAPC_LOW ? (MODE == A32_P2GX ? 8 : 7) 
   const int q = tid & 7;
   const int bx = q & 1, by = (q >> 1) & 1, bz = (q >> 2) & 1;
   // corner factor f = s * offset + t: offset for the upper corner, 1 - offset for the lower
-  const float sy = by ? 1.f : -1.f, ty = by ? 0.f : 1.f;
-  const float sz = bz ? 1.f : -1.f, tz = bz ? 0.f : 1.f;
+  // corner factors |t - offset|: t = 1 at the lower corner, 0 at the upper
+  const float tx = bx ? 0.f : 1.f, ty = by ? 0.f : 1.f, tz = bz ? 0.f : 1.f;
   const uint32_t tile = xcd_block(blockIdx.x, gridDim.x);
   const int64_t c0 = (int64_t)tile * TILE32;
   // cell, tile and row numbers are below 2^31, so their divisions are 32-bit; an x-row tile
@@ -578,23 +577,21 @@ __attribute__((amdgpu_waves_per_eu(CAP == CAPC_LOW ? (MODE == A32_P2GX ? 8 : 7) 
         const int32_t key = sidx[a];
         int rk = 0;
         for (int bb = lo; bb < hi; bb++) rk += (sidx[bb] < key) ? 1 : 0;
-        sperm[lo + rk] = (int16_t)a;
+        sperm[lo + rk] = (int16_t)(a * 16);
       }
     } else {
-      for (int a = lo + q; a < hi; a += 8) sperm[a] = (int16_t)a;
+      for (int a = lo + q; a < hi; a += 8) sperm[a] = (int16_t)(a * 16);
     }
     __syncthreads();
 #pragma unroll 2
     for (int a = lo; a < hi; a++) {
-      const int r = sperm[a];
-      const float4 gr = sgeo[r];
-      const float4 v = sva[r];
-      float w0, w1;
-      pair_weights(gr.x, gr.y, gr.z, gr.w, sy, ty, sz, tz, w0, w1);
-      const float w = bx ? w1 : w0;
+      const int off = sperm[a];
+      const float4 gr = *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(sgeo) + off);
+      const float4 v = *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(sva) + off);
+      const float w = corner_weight(gr.x, gr.y, gr.z, gr.w, tx, ty, tz);
       float vals[5] = {v.x, v.y, v.z, v.w, 0.f};
-      if constexpr (FIVE) vals[4] = svb[r];
-      if (!MIX || skeep[r]) {
+      if constexpr (FIVE) vals[4] = svb[off >> 4];
+      if (!MIX || skeep[off >> 4]) {
         add_parcel<NV>(acc, w, vals);
         kept++;
       } else {
@@ -631,9 +628,7 @@ __attribute__((amdgpu_waves_per_eu(CAP == CAPC_LOW ? (MODE == A32_P2GX ? 8 : 7) 
         vals[3] = F[B_Q * cap + pp];
         vals[4] = F[B_R * cap + pp];
       }
-      float w0, w1;
-      pair_weights(ox, oy, oz, vol, sy, ty, sz, tz, w0, w1);
-      const float w = bx ? w1 : w0;
+      const float w = corner_weight(ox, oy, oz, vol, tx, ty, tz);
       if (!MIX || !(vol < vmin)) {
         add_parcel<NV>(acc, w, vals);
         kept++;
