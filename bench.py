@@ -73,6 +73,10 @@ def parse():
     ap.add_argument("--err-steps", type=int, default=100,
                     help="steps of the FP32-bulk vs FP64 velocity-error run on the workload "
                          "(0 = skip)")
+    ap.add_argument("--autotune", type=int, default=0, metavar="K",
+                    help="with --ensemble: pick the members' launch configurations first, "
+                         "timing the ensemble stepped together (ensemble.tune_together, K timed "
+                         "steps per candidate; untimed model steps)")
     ap.add_argument("--ensemble", type=int, default=0, metavar="M",
                     help="instead of the single thermal: M FP32-bulk thermals on grids cycling "
                          "32^3, 48^3, 64^3, 96^3 (8 or 27 parcels/cell, radius U(0.5,1), "
@@ -329,8 +333,22 @@ def ensemble(args):
     total = sum(len(ic["x"]) for ic in ics)
     print(f"bench: ensemble of {len(ics)} members, {total} parcels", file=sys.stderr, flush=True)
     out = {}
+    tuned = {}
+    if args.autotune:
+        # tuned on copies of the members (the tuning steps are model steps), so the timed
+        # members start from the same states as untuned ones
+        ctxs = [ens.member_context(s, ic) for s, ic in zip(specs, ics)]
+        tuned = ens.tune_together(ctxs, args.autotune)
+        for c in ctxs:
+            c.close()
+        print(f"bench: ensemble tuned: {tuned}", file=sys.stderr, flush=True)
     for mode in ("concurrent", "sequential"):
         ctxs = [ens.member_context(s, ic) for s, ic in zip(specs, ics)]
+        if args.autotune:
+            for c in ctxs:
+                for cls, cfg in tuned["by_slot_capacity"].items():
+                    if c.launch_config()["slot_capacity"] == cls:
+                        c.set_launch_config(0, **cfg)
         if mode == "concurrent":
             step_concurrent(ctxs, args.warmup)
         else:
@@ -358,6 +376,7 @@ def ensemble(args):
                                f"parcels/cell",
                    "members": [[s[0], s[1] ** 3] for s in specs], "parcels": total},
         "sequential_value": out["sequential"],
+        "autotuned": tuned or None,
         "concurrency_speedup": out["concurrent"] / out["sequential"]}))
     return 0
 

@@ -172,6 +172,27 @@ const char *pmpic_timing_name(int32_t which);
  * leaves the range a capture covers (the results are the same bits as eager launches; off by
  * default: it measured no faster).  `captures` graphs captured, `launches` graph launches. */
 int pmpic_graph_stats(pmpic_ctx *ctx, int64_t *captures, int64_t *launches);
+/* Launch-configuration autotuning of an FP32-bulk context (no reference counterpart; for the
+ * batched ensemble, whose members differ in grid size).  A configuration is four numbers:
+ * config[0] the binning's slot capacity (16 or 32; 0 where the grid has no slot table),
+ * config[1] the binning fused into RK stages 1-3 (1) or a pass of its own (0), config[2] RK
+ * stages 1-3 launched as one HIP graph (1, only with timing off) or eagerly (0), config[3] the
+ * RK stage's workgroup size (128 or 256).  Every configuration gives the same bits.  The first
+ * context of a grid size (nx, ny, nz) and occupancy class (the slot capacity the mean rule
+ * picks) runs each configuration twice (the list forward, then backward, so that a drift of
+ * the step time with the model state cancels) for one settling step and `steps` timed steps,
+ * all of them ordinary model steps, and keeps the fastest, for itself and for every later context of that
+ * size and class in the process (`measured` 0: applied from that table without stepping).
+ * Tune members one after another, before stepping them concurrently.  Errors: PMPIC_ERR_ARG
+ * for an FP64 context, no parcels or steps < 1; a step's error as pmpic_step returns it. */
+int pmpic_autotune(pmpic_ctx *ctx, int32_t steps, int32_t *config, float *ms_per_step,
+                   int32_t *measured);
+/* The launch configuration an FP32-bulk context uses now, as pmpic_autotune reports it. */
+int pmpic_launch_config(pmpic_ctx *ctx, int32_t *config);
+/* Sets an FP32-bulk context's launch configuration (for tuners that time members stepped
+ * together); config[0] 0 leaves the slot capacity to the mean-occupancy rule.  PMPIC_ERR_ARG
+ * for a value outside those pmpic_autotune lists or a capacity the slot table cannot hold. */
+int pmpic_set_launch_config(pmpic_ctx *ctx, const int32_t *config);
 
 #ifdef __cplusplus
 }

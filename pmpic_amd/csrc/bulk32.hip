@@ -1326,6 +1326,11 @@ int b32_publish_n(Ctx *c) {
 // 16 entries while the mean is at most LOW_PPC parcels per cell (fewer empty entries to read),
 // else 32; a capacity fixed from the environment or picked by the tuning wins; never more than
 // the table holds
+int b32_mean_capc(const Ctx *c, int64_t n) {
+  const int capc = (n <= (int64_t)LOW_PPC * c->ncell) ? CAPC_LOW : CAPC;
+  return capc > c->b32.capmax ? c->b32.capmax : capc;
+}
+
 int b32_choose_capc(const Ctx *c, int64_t n) {
   const Bulk32 &b = c->b32;
   int capc;
@@ -1366,6 +1371,7 @@ int b32_alloc(Ctx *c) {
   b.capfix = 0;
   b.captune = false;
   b.captuned = 0;
+  b.rk_tpb = TPB;
   // binning fused into RK stages 1-3: it costs the RK kernel more than the separate binning
   // pass on small grids (the 32^3..96^3 ensemble members: 1% slower) and saves on large ones
   // (256 x 256 x 128: 1.6% faster), so from 2^21 cells on; PMPIC_FUSED_BINNING=0 or 1 forces it
@@ -1549,8 +1555,8 @@ int b32_rk_stage(Ctx *c, int stage, int stretch) {
   Grid32Set gs = {{c->b32.grid[0], c->b32.grid[1], c->b32.grid[2], c->b32.grid[3],
                    c->b32.grid[4], c->b32.grid[5]}};
   const Scale32 s3 = scale_of(c->g);
-  const dim3 gb(cdiv(c->graph_n ? c->graph_n : n, TPB)), tb(TPB);
   Bulk32 &b = c->b32;
+  const dim3 gb(cdiv(c->graph_n ? c->graph_n : n, b.rk_tpb)), tb(b.rk_tpb);
   // stages 1-3 bin the cells they write for the next stage's par2grid (stage 4 is followed by
   // splitting and mixing, which change the parcels); not while the capacity is being tuned
   BinOut bo{0, 0, c->cell_count, b.slot, b.ovf, c->rank};

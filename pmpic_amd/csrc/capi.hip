@@ -5,6 +5,8 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <array>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -338,6 +340,38 @@ int create(const pmpic_config *cf, Ctx *c) {
 // the stage number (1..4) a component acting now belongs to
 inline int stage_of(int it) { return (it % 4) + 1; }
 
+}  // namespace
+
+// ---- launch-configuration autotuning (FP32-bulk form) ----
+// A configuration: the binning's slot capacity (16 or 32; 0 without a slot table), the binning
+// fused into RK stages 1-3 or run as a pass of its own, RK stages 1-3 as a graph or eagerly,
+// and the RK stage's workgroup size.  Every configuration gives the same bits, so the tuning
+// steps are ordinary steps of the model.  The choice is kept per grid size and occupancy class
+// for the life of the process, so later contexts of that size take it without measuring.
+namespace {
+struct TuneEntry {
+  int nx, ny, ncz, capmax, capmean;
+  int32_t cfg[4];
+  float ms;
+};
+std::mutex tune_mu;
+std::vector<TuneEntry> tune_table;
+
+void tune_apply(Ctx *c, const int32_t cfg[4]) {
+  c->b32.capfix = cfg[0];
+  c->b32.fused_bin = cfg[1] != 0;
+  c->graph_on = cfg[2];
+  c->b32.rk_tpb = cfg[3];
+  c->b32.captune = false;
+  graph_drop(c);
+}
+
+void tune_current(const Ctx *c, int32_t cfg[4]) {
+  cfg[0] = b32_choose_capc(c, c->n);
+  cfg[1] = c->b32.fused_bin ? 1 : 0;
+  cfg[2] = c->graph_on;
+  cfg[3] = c->b32.rk_tpb;
+}
 }  // namespace
 
 extern "C" {
@@ -946,6 +980,109 @@ int pmpic_timing_stats(pmpic_ctx *ctx, int32_t which, double *total_ms, int64_t 
   if (calls) *calls = c->t_calls[which];
   if (max_ms) *max_ms = c->t_max[which];
   if (sumsq_ms2) *sumsq_ms2 = c->t_sq[which];
+  return PMPIC_OK;
+}
+
+
+// every launch configuration timed over `steps` steps (after one step to settle), the fastest
+// kept for this context and for later ones of its grid size
+int pmpic_autotune(pmpic_ctx *ctx, int32_t steps, int32_t *config, float *ms_per_step,
+                   int32_t *measured) {
+  if (!ctx || steps < 1 || ctx->c.precision != 1 || ctx->c.n == 0) return PMPIC_ERR_ARG;
+  Ctx *c = &ctx->c;
+  const int capmean = b32_mean_capc(c, c->n);
+  auto same = [&](const TuneEntry &e) {
+    return e.nx == c->g.nx && e.ny == c->g.ny && e.ncz == c->g.ncz && e.capmax == c->b32.capmax &&
+           e.capmean == capmean;
+  };
+  {
+    std::lock_guard<std::mutex> lk(tune_mu);
+    for (const TuneEntry &e : tune_table)
+      if (same(e)) {
+        tune_apply(c, e.cfg);
+        if (config) memcpy(config, e.cfg, sizeof(e.cfg));
+        if (ms_per_step) *ms_per_step = e.ms;
+        if (measured) *measured = 0;
+        return PMPIC_OK;
+      }
+  }
+  std::vector<int> caps;
+  if (c->b32.capmax == 0) caps.push_back(0);
+  if (c->b32.capmax >= 16) caps.push_back(16);
+  if (c->b32.capmax >= 32) caps.push_back(32);
+  const int graphs = c->timing == 0 ? 2 : 1;  // a graph needs the timers off
+  hipEvent_t e0, e1;
+  PMPIC_CHECK(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    hipEventDestroy(e0);
+    return PMPIC_ERR_HIP;
+  }
+  std::vector<std::array<int32_t, 4>> cand;
+  for (int cap : caps)
+    for (int fused = 0; fused < 2; fused++)
+      for (int graph = 0; graph < graphs; graph++)
+        for (int tpb = 128; tpb <= 256; tpb *= 2) cand.push_back({cap, fused, graph, tpb});
+  // The candidates run in order and then in reverse, and each is scored by the mean of its two
+  // timings: the model state (and with it the speed of a step) drifts over the tuning steps,
+  // and a linear drift cancels in that mean.
+  const int nc = (int)cand.size();
+  std::vector<float> total(nc, 0.f);
+  int st = PMPIC_OK;
+  for (int r = 0; r < 2 * nc && st == PMPIC_OK; r++) {
+    const int i = r < nc ? r : 2 * nc - 1 - r;
+    tune_apply(c, cand[i].data());
+    // one step to settle (a capture, a new binning capacity), then the timed steps
+    st = pmpic_step(ctx, 1);
+    float ms = 0.f;
+    if (st == PMPIC_OK && hipEventRecord(e0, c->stream) != hipSuccess) st = PMPIC_ERR_HIP;
+    if (st == PMPIC_OK) st = pmpic_step(ctx, steps);
+    if (st == PMPIC_OK && (hipEventRecord(e1, c->stream) != hipSuccess ||
+                           hipEventSynchronize(e1) != hipSuccess ||
+                           hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+      st = PMPIC_ERR_HIP;
+    total[i] += ms;
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (st != PMPIC_OK) return st;
+  int bi = 0;
+  for (int i = 1; i < nc; i++)
+    if (total[i] < total[bi]) bi = i;
+  int32_t best[4];
+  memcpy(best, cand[bi].data(), sizeof(best));
+  const float best_ms = total[bi] / (2.f * steps);
+  tune_apply(c, best);
+  {
+    std::lock_guard<std::mutex> lk(tune_mu);
+    bool have = false;
+    for (const TuneEntry &e : tune_table) have = have || same(e);
+    if (!have) {
+      TuneEntry e{c->g.nx, c->g.ny, c->g.ncz, c->b32.capmax, capmean, {0, 0, 0, 0}, best_ms};
+      memcpy(e.cfg, best, sizeof(best));
+      tune_table.push_back(e);
+    }
+  }
+  if (config) memcpy(config, best, sizeof(best));
+  if (ms_per_step) *ms_per_step = best_ms;
+  if (measured) *measured = 1;
+  return PMPIC_OK;
+}
+
+int pmpic_set_launch_config(pmpic_ctx *ctx, const int32_t *config) {
+  if (!ctx || !config || ctx->c.precision != 1) return PMPIC_ERR_ARG;
+  Ctx *c = &ctx->c;
+  const int32_t cap = config[0];
+  if (!(cap == 0 || cap == 16 || cap == 32) || cap > c->b32.capmax ||
+      (config[1] != 0 && config[1] != 1) || (config[2] != 0 && config[2] != 1) ||
+      (config[3] != 128 && config[3] != 256))
+    return PMPIC_ERR_ARG;
+  tune_apply(c, config);
+  return PMPIC_OK;
+}
+
+int pmpic_launch_config(pmpic_ctx *ctx, int32_t *config) {
+  if (!ctx || !config || ctx->c.precision != 1) return PMPIC_ERR_ARG;
+  tune_current(&ctx->c, config);
   return PMPIC_OK;
 }
 

@@ -55,6 +55,7 @@ struct Bulk32 {
   int capfix;        // capacity forced by PMPIC_SLOT_CAPACITY (16 or 32), or 0 for the mean rule
   bool captune;      // PMPIC_SLOT_CAPACITY=tune and the first par2grid has not timed both yet
   int captuned;      // the capacity that timing picked (16 or 32), or 0 while none has
+  int rk_tpb;        // workgroup size of the RK stage launches (128 or 256)
   // kept grid k (vol, b, hg, p, q, r) of the last mixing is current in mgrid[k] and not yet
   // widened into the FP64 grid a grid export reads
   bool kept_pending[6];
@@ -370,6 +371,7 @@ int b32_split(Ctx *c);
 int b32_mix(Ctx *c);
 int b32_conservation_sums(Ctx *c, double out6[6]);
 int b32_choose_capc(const Ctx *c, int64_t n);  // the slot capacity a binning of n parcels takes
+int b32_mean_capc(const Ctx *c, int64_t n);    // ... by the mean-occupancy rule alone
 int b32_publish_n(Ctx *c);                     // the parcel count into device memory
 int b32_export_grid(Ctx *c, int id);  // brings grid[id] up to date for a grid export
 int b32_dump_grids(Ctx *c);             // writebinary's b, hg, p, q, r grids

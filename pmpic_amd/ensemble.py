@@ -50,3 +50,58 @@ def member_context(spec, ic=None, cap_factor=2.0):
                 precision=1)
     c.set_parcels(ic)
     return c
+
+
+def tune_together(ctxs, steps=2):
+    """Pick launch configurations for members stepped together (pmpic_step_concurrent).
+
+    pmpic_autotune times one context alone, which does not predict the members' speed when
+    they share the GPU; this tuner times the whole ensemble instead.  The candidates are the
+    binning fused into the RK stages or not, RK stages 1-3 as a graph or not, and RK workgroups
+    of 128 or 256, with each member's slot capacity left to the mean-occupancy rule.  First one
+    configuration for every member, then, from that start, one per occupancy class (the
+    capacity the rule gives) while the other classes keep theirs.  Each trial runs one settling
+    step and `steps` timed steps of the whole ensemble, and each list of trials runs forward and
+    then backward, a trial scored by the mean of its two timings, so that the drift of the step
+    time with the model state cancels.  The steps are model steps: tune copies of the members
+    where their states matter.  Returns {class: configuration} and the ensemble ms per step of
+    the choice."""
+    import itertools
+    import time
+
+    from .lib import step_concurrent
+    cands = [dict(fused_binning=f, graph=g, rk_workgroup=w)
+             for f, g, w in itertools.product((False, True), (False, True), (128, 256))]
+    classes = {}
+    for c in ctxs:
+        classes.setdefault(c.launch_config()["slot_capacity"], []).append(c)
+
+    def run(assign):
+        for cls, members in classes.items():
+            for c in members:
+                c.set_launch_config(0, **assign[cls])
+        step_concurrent(ctxs, 1)
+        t0 = time.perf_counter()
+        step_concurrent(ctxs, steps)
+        return (time.perf_counter() - t0) * 1e3 / steps
+
+    def best_of(trials):
+        ms = [0.0] * len(trials)
+        for i in list(range(len(trials))) + list(reversed(range(len(trials)))):
+            ms[i] += run(trials[i]) / 2
+        i = min(range(len(trials)), key=ms.__getitem__)
+        return trials[i], ms[i]
+
+    choice, best_ms = best_of([{cls: cand for cls in classes} for cand in cands])
+    if len(classes) > 1:
+        for cls in classes:
+            trials = []
+            for cand in cands:
+                trial = dict(choice)
+                trial[cls] = cand
+                trials.append(trial)
+            choice, best_ms = best_of(trials)
+    for cls, members in classes.items():
+        for c in members:
+            c.set_launch_config(0, **choice[cls])
+    return {"by_slot_capacity": choice, "ms_per_step": best_ms}

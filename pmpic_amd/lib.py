@@ -42,7 +42,7 @@ EXPORTS = [
     "pmpic_laplinv", "pmpic_conservation_sums", "pmpic_last_counts", "pmpic_timing_enable",
     "pmpic_timing_read", "pmpic_step_concurrent", "pmpic_timing_name", "pmpic_hbm_triad",
     "pmpic_writebinary_grids", "pmpic_par2grid_noalloc", "pmpic_timing_stats",
-    "pmpic_graph_stats",
+    "pmpic_graph_stats", "pmpic_autotune", "pmpic_launch_config", "pmpic_set_launch_config",
 ]
 
 
@@ -115,6 +115,10 @@ def load():
         L.pmpic_timing_stats.argtypes = [vp, i32, ctypes.POINTER(d), ctypes.POINTER(i64),
                                          ctypes.POINTER(d), ctypes.POINTER(d)]
         L.pmpic_graph_stats.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]
+        L.pmpic_autotune.argtypes = [vp, i32, vp, ctypes.POINTER(ctypes.c_float),
+                                     ctypes.POINTER(i32)]
+        L.pmpic_launch_config.argtypes = [vp, vp]
+        L.pmpic_set_launch_config.argtypes = [vp, vp]
         L.pmpic_timing_name.restype = ctypes.c_char_p
         L.pmpic_timing_name.argtypes = [i32]
         L.pmpic_hbm_triad.argtypes = [i32, i64, i32, ctypes.POINTER(d), ctypes.POINTER(d)]
@@ -389,6 +393,38 @@ class Context:
                                             ctypes.byref(mx), ctypes.byref(sq)),
                   "pmpic_timing_stats")
         return t.value, n.value, mx.value, sq.value
+
+    def autotune(self, steps=2):
+        """Pick this FP32-bulk context's launch configuration by timing each over `steps` model
+        steps (or take the one picked for its grid size earlier in the process).  Returns the
+        configuration (see launch_config), the chosen ms per step and whether it was measured
+        here."""
+        cfg = (ctypes.c_int32 * 4)()
+        ms, meas = ctypes.c_float(), ctypes.c_int32()
+        self._chk(self.L.pmpic_autotune(self.h, int(steps), cfg, ctypes.byref(ms),
+                                        ctypes.byref(meas)), "pmpic_autotune")
+        out = self._config(cfg)
+        out.update(ms_per_step=ms.value, measured=bool(meas.value))
+        return out
+
+    @staticmethod
+    def _config(cfg):
+        return {"slot_capacity": cfg[0], "fused_binning": bool(cfg[1]), "graph": bool(cfg[2]),
+                "rk_workgroup": cfg[3]}
+
+    def launch_config(self):
+        """The FP32-bulk launch configuration in use: slot capacity, binning fused into the RK
+        stages, stages 1-3 as a HIP graph, RK workgroup size."""
+        cfg = (ctypes.c_int32 * 4)()
+        self._chk(self.L.pmpic_launch_config(self.h, cfg), "pmpic_launch_config")
+        return self._config(cfg)
+
+    def set_launch_config(self, slot_capacity=0, fused_binning=False, graph=False,
+                          rk_workgroup=256):
+        """Set the FP32-bulk launch configuration (slot_capacity 0: the mean-occupancy rule)."""
+        cfg = (ctypes.c_int32 * 4)(int(slot_capacity), int(bool(fused_binning)), int(bool(graph)),
+                                   int(rk_workgroup))
+        self._chk(self.L.pmpic_set_launch_config(self.h, cfg), "pmpic_set_launch_config")
 
     def graph_stats(self):
         """(graphs captured, graph launches) of pmpic_step in the FP32-bulk form."""
