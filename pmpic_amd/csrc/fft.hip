@@ -229,10 +229,20 @@ template <typename S>
 __host__ __device__ constexpr int ept_rows() { return lds_rows<S>() / FFT_TPB; }
 constexpr int TPT = 512 / FFT_TPB;  // twiddles per thread (n <= 512)
 
+// The column passes of FP32 columns of at most 128 points stage half the batch (8 KiB per
+// buffer), so twice the workgroups fit a CU; that still takes at least 8 adjacent columns, 64
+// bytes of each row.  Longer columns keep the full batch (fewer columns would cut the rows' runs
+// to 32 bytes: 256-point columns measured 160 -> 264 us per inverse pass).  n = 0: a length
+// known only at run time.
+template <typename S>
+__host__ __device__ constexpr int lds_cols(int n) {
+  return (sizeof(S) == 4 && n > 0 && n <= 128) ? lds_cplx<S>() / 2 : lds_cplx<S>();
+}
+
 // adjacent columns (column pairs for the inverse) a column pass takes per workgroup
 template <typename S>
 __host__ __device__ constexpr int col_batch(int n) {
-  return lds_cplx<S>() / n > 32 ? 32 : lds_cplx<S>() / n;
+  return lds_cols<S>(n) / n > 32 ? 32 : lds_cols<S>(n) / n;
 }
 
 // q = a / d and r = a - q d for 0 <= a < 2^22, 0 < d, with rd = 1/d rounded to FP32: the
@@ -321,8 +331,8 @@ __global__ void __launch_bounds__(FFT_TPB) k_cols_fwd(Plan p, const cx_t<S> *__r
                                                       int64_t c1_field, int nl, int nb, int sx,
                                                       int sy) {
   using C = cx_t<S>;
-  constexpr int EPT = ept<S>();
-  __shared__ C A[lds_cplx<S>()], B[lds_cplx<S>()], T[512];
+  constexpr int EPT = lds_cols<S>(NT) / FFT_TPB;
+  __shared__ C A[lds_cols<S>(NT)], B[lds_cols<S>(NT)], T[512];
   const int f = blockIdx.z;
   const C *src = c1 + (int64_t)f * c1_field;
   S *dst = static_cast<S *>(fp.out[f]);
@@ -459,8 +469,8 @@ __global__ void __launch_bounds__(FFT_TPB) k_cols_c2r(Plan p, const cx_t<S> *__r
                                                       FieldPtrs fp, const cx_t<S> *__restrict__ c2,
                                                       int64_t c2_field, int nx, int nb) {
   using C = cx_t<S>;
-  constexpr int EPT = ept<S>();
-  __shared__ C A[lds_cplx<S>()], B[lds_cplx<S>()], T[512];
+  constexpr int EPT = lds_cols<S>(NT) / FFT_TPB;
+  __shared__ C A[lds_cols<S>(NT)], B[lds_cols<S>(NT)], T[512];
   const int f = blockIdx.z;
   const C *src = c2 + (int64_t)f * c2_field;
   double *dst = static_cast<double *>(fp.out[f]);
