@@ -1151,7 +1151,8 @@ __global__ void k32_mix_new(Geometry g, Bulk32 b, int64_t cap, int64_t norig,
 }
 
 __global__ void k32_mix_residual(Geometry g, Mix32 mg, const int32_t *__restrict__ npercell,
-                                 const float *__restrict__ portion_cell) {
+                                 const float *__restrict__ portion_cell,
+                                 uint8_t *__restrict__ touch) {
   const int64_t nnode = (int64_t)g.nzn * g.ny * g.nx;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nnode) return;
@@ -1179,28 +1180,43 @@ __global__ void k32_mix_residual(Geometry g, Mix32 mg, const int32_t *__restrict
   const float mlt = (k == 0 || k == g.nzn - 1) ? 2.f : 1.f;
   // stored divided by the node's kept volume: the absorption weights each node's residual by
   // w / vol_g (parcel_mixing.F90:247-340), so the division is done once per node here
+  bool any = false;
 #pragma unroll
-  for (int f = 0; f < 6; f++) mg.R[f][t] = (r[f] * mlt) / gv[0];
+  for (int f = 0; f < 6; f++) {
+    const float v = (r[f] * mlt) / gv[0];
+    mg.R[f][t] = v;
+    any = any || v != 0.f;  // a NaN counts as not zero
+  }
+  touch[t] = any ? 1 : 0;
 }
 
+// A parcel none of whose corner nodes carries a residual (almost all of them: residuals arise
+// only around removed parcels and the cells that received new ones) keeps its values as they
+// are, without the reads, the arithmetic (which would return them to within rounding) and the
+// writes; `touch` marks the nodes with a residual.
 __global__ void k32_mix_absorb(Geometry g, Bulk32 b, int64_t cap, int64_t norig,
-                               const uint8_t *__restrict__ keep, Mix32 mg) {
+                               const uint8_t *__restrict__ keep, Mix32 mg,
+                               const uint8_t *__restrict__ touch) {
   const int64_t p64 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p64 >= norig || keep[p64] != 1) return;
   const uint32_t p = (uint32_t)p64;
+  const int32_t cell = ldnt(b.cell, p);
+  int i, j, k;
+  cell_ijk(g, cell, i, j, k);
+  int32_t o[8];
+  corner_nodes(g, i, j, k, o);
+  int any = 0;
+#pragma unroll
+  for (int c = 0; c < 8; c++) any |= touch[o[c]];
+  if (!any) return;
   float *F = b.f;
   auto fp = [&](int f) { return F + (size_t)f * cap; };
   const int fld[5] = {B_B, B_H, B_P, B_Q, B_R};
-  const int32_t cell = ldnt(b.cell, p);
   const float ox = ldnt(fp(B_OX), p), oy = ldnt(fp(B_OY), p), oz = ldnt(fp(B_OZ), p);
   const float v = ldnt(fp(B_VOL), p);
   float fv[5];
 #pragma unroll
   for (int f = 0; f < 5; f++) fv[f] = ldnt(fp(fld[f]), p);
-  int i, j, k;
-  cell_ijk(g, cell, i, j, k);
-  int32_t o[8];
-  corner_nodes(g, i, j, k, o);
   float w[8];
   w[0] = (1 - oz) * (1 - oy) * (1 - ox) * v;
   w[1] = (1 - oz) * (1 - oy) * (ox) * v;
@@ -1356,6 +1372,7 @@ int b32_alloc(Ctx *c) {
   PMPIC_CHECK(hipMemset(b.tag, 0, sizeof(double) * (size_t)c->cap));
   for (int i = 0; i < 6; i++) PMPIC_TRY(dalloc32(&b.grid[i], c->gsz));
   for (int i = 0; i < 12; i++) PMPIC_TRY(dalloc32(&b.mgrid[i], c->gsz));
+  PMPIC_TRY(dalloc32(&b.mtouch, c->gsz));
   for (int i = 0; i < 4; i++) {
     PMPIC_TRY(dalloc32(&b.pgrid[i], c->gsz));
     b.p2g_pending[i] = false;
@@ -1401,6 +1418,7 @@ void b32_free(Ctx *c) {
   hipFree(b.tag);
   for (int i = 0; i < 6; i++) hipFree(b.grid[i]);
   for (int i = 0; i < 12; i++) hipFree(b.mgrid[i]);
+  hipFree(b.mtouch);
   for (int i = 0; i < 4; i++) hipFree(b.pgrid[i]);
   hipFree(b.slot);
   hipFree(b.ovf);
@@ -1687,10 +1705,10 @@ int b32_mix(Ctx *c) {
     hipLaunchKernelGGL(k32_mix_new, dim3(cdiv(nadd, TPB)), dim3(TPB), 0, c->stream, g, b, c->cap,
                        norig, c->sorted, nadd, c->list_a, Z, c->flag, mg, vmin, portion, c->dsc);
   hipLaunchKernelGGL(k32_mix_residual, dim3(cdiv(nnode, TPB)), dim3(TPB), 0, c->stream, g, mg,
-                     c->npercell, portion);
+                     c->npercell, portion, b.mtouch);
   if (norig > 0)
     hipLaunchKernelGGL(k32_mix_absorb, dim3(cdiv(norig, TPB)), dim3(TPB), 0, c->stream, g, b,
-                       c->cap, norig, c->flag, mg);
+                       c->cap, norig, c->flag, mg, b.mtouch);
   PMPIC_CHECK(hipGetLastError());
   // new count (parcel_mixing.F90:217), then the backfill of removed slots (:347-389)
   int64_t np = norig;

@@ -44,6 +44,7 @@ struct Bulk32 {
   double *tag;       // cap
   float *grid[6];    // gathered grids U, V, W, DP, DQ, DR [nzn][ny][nx]
   float *mgrid[12];  // mixing: kept vol, b, h, p, q, r (0..5), residuals (6..11)
+  uint8_t *mtouch;   // mixing, per node: a residual grid is not zero there
   // cell binning: slot[c * capc + rank] lists the parcels of cell c while rank < capc; a cell
   // with more parcels lists all of them in the dense overflow region.  The table holds 32
   // entries per cell (none, and capc = 0, when its byte offsets would exceed 32 bits); each
