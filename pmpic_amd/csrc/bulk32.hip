@@ -384,15 +384,20 @@ struct Zmap {
 // the upper (the offset, which lies in [0, 1]); the weight is ((fz fy) vol) fx.
 __device__ __forceinline__ float corner_weight(float ox, float oy, float oz, float vol, float tx,
                                                float ty, float tz) {
+#pragma clang fp contract(off)
   const float fy = fabsf(ty - oy);
   const float fz = fabsf(tz - oz);
   const float wv = (fz * fy) * vol;
   return wv * fabsf(tx - ox);
 }
 
-// acc[0] += w, acc[1 + f] += w * v[f], fused
+// acc[0] += w, acc[1 + f] += w * v[f], fused.  No contraction here or in corner_weight: the
+// compiler would otherwise fuse the weight's last product into the first sum in some of the
+// kernels that inline these and not in others, and every accumulation path must give the same
+// bits.
 template <int NV>
 __device__ __forceinline__ void add_parcel(float *acc, float w, const float *v) {
+#pragma clang fp contract(off)
   acc[0] += w;
 #pragma unroll
   for (int f = 0; f < NV; f++) acc[f + 1] = fmaf(w, v[f], acc[f + 1]);
@@ -403,27 +408,35 @@ __device__ __forceinline__ void add_parcel(float *acc, float w, const float *v) 
 // table the LDS rows allow 16 workgroups (32 waves) per CU, and holding the registers to 72
 // allows 7 waves per SIMD (28 per CU) instead of 5 (4-5 % faster at 8 parcels per cell); the
 // x-row form fits 64 registers without spilling, so all 8 (0.95 -> 0.90 ms at 256x256x128).
+// The x-row mixing form's seven sums and three staged values per row allow 12 workgroups (6
+// waves per SIMD) by LDS, so it takes the registers of 6 waves.
 // The 32-entry form is bound by its LDS rows (8 workgroups per CU) instead.
 template <int MODE, int CAP, int ROWS>
 __global__ void __launch_bounds__(ATPB)
-__attribute__((amdgpu_waves_per_eu(CAP == CAPC_LOW ? (MODE == A32_P2GX ? 8 : 7) : 1))) k32_cell_accum(Bulk32 b, int64_t cap, Geometry g, Zmap zm,
+__attribute__((amdgpu_waves_per_eu(CAP == CAPC_LOW ? (MODE == A32_P2GX ? 8 : MODE == A32_MIXX ? 6 : 7) : 1))) k32_cell_accum(Bulk32 b, int64_t cap, Geometry g, Zmap zm,
                                                        float vmin, const int32_t *__restrict__ count,
                                                        const int32_t *__restrict__ ostart,
                                                        const int32_t *__restrict__ sorted,
                                                        int64_t ncell, float *__restrict__ partial,
                                                        float *__restrict__ partial2,
                                                        int32_t *__restrict__ npercell,
-                                                       uint8_t *__restrict__ tileres) {
+                                                       uint8_t *__restrict__ tileres,
+                                                       const uint8_t *__restrict__ dirty) {
   constexpr bool MIX = (MODE == A32_MIX || MODE == A32_MIXX);
   constexpr bool XC = (MODE == A32_P2GX || MODE == A32_MIXX);
-  constexpr bool FIVE = (MIX || MODE == A32_DUMP);  // five values b, h, p, q, r
-  constexpr int NV = FIVE ? 5 : 4;
+  // BT: the values start p, q, r and the total buoyancy (par2grid's four; the x-row mixing adds
+  // b and h after them, so its first five planes are the ones the x-row par2grid writes);
+  // FIVE: b, h, p, q, r (the other mixing form and the dump)
+  constexpr bool BT = (MODE == A32_P2G || MODE == A32_P2GX || MODE == A32_MIXX);
+  constexpr bool FIVE = !BT;
+  constexpr int NV = MODE == A32_MIXX ? 6 : (FIVE ? 5 : 4);
   constexpr int NF = NV + 1;
   constexpr int SPT = TILE32 * CAP / ATPB;  // slot entries per thread
   constexpr int RPT = ROWS / ATPB;         // dense rows per thread
   __shared__ float4 sgeo[ROWS];  // ox, oy, oz, vol
   __shared__ float4 sva[ROWS];   // values 0..3
-  __shared__ float svb[FIVE ? ROWS : 1];
+  __shared__ float svb[NV >= 5 ? ROWS : 1];
+  __shared__ float svc[NV >= 6 ? ROWS : 1];
   __shared__ int32_t sidx[ROWS];
   __shared__ int16_t sperm[ROWS];  // the rows of a cell in parcel order, as byte offsets 16 r
   __shared__ uint8_t skeep[MIX ? ROWS : 1];
@@ -440,6 +453,9 @@ __attribute__((amdgpu_waves_per_eu(CAP == CAPC_LOW ? (MODE == A32_P2GX ? 8 : 7) 
   // corner factors |t - offset|: t = 1 at the lower corner, 0 at the upper
   const float tx = bx ? 0.f : 1.f, ty = by ? 0.f : 1.f, tz = bz ? 0.f : 1.f;
   const uint32_t tile = xcd_block(blockIdx.x, gridDim.x);
+  // a par2grid whose partials the last x-row mixing sums left current: only the tiles mixing
+  // changed (dirty) are summed again
+  if (dirty && !dirty[tile]) return;
   const int64_t c0 = (int64_t)tile * TILE32;
   // cell, tile and row numbers are below 2^31, so their divisions are 32-bit; an x-row tile
   // has ntx tiles per row
@@ -482,7 +498,7 @@ __attribute__((amdgpu_waves_per_eu(CAP == CAPC_LOW ? (MODE == A32_P2GX ? 8 : 7) 
   auto fetch = [&](uint32_t pp, float4 &geo, float4 &va, float &vb) {
     geo = make_float4(ldnt(fld(B_OX), pp), ldnt(fld(B_OY), pp), ldnt(fld(B_OZ), pp),
                       ldnt(fld(B_VOL), pp));
-    if constexpr (!FIVE) {
+    if constexpr (BT) {
       va = make_float4(ldnt(fld(B_P), pp), ldnt(fld(B_Q), pp), ldnt(fld(B_R), pp),
                        ldnt(fld(B_B), pp));
       vb = ldnt(fld(B_H), pp);  // humidity, for the total buoyancy below
@@ -494,15 +510,20 @@ __attribute__((amdgpu_waves_per_eu(CAP == CAPC_LOW ? (MODE == A32_P2GX ? 8 : 7) 
   };
   // kz: the z level of the parcel's cell (known from the tile, so the cell is not read)
   auto put = [&](int r, int32_t pp, float4 geo, float4 v, float vb, float kz) {
-    if constexpr (!FIVE) {
+    if constexpr (BT) {
       // total buoyancy (vorticity_tendency.F90:121-126)
-      const float z = zm.zbot + (kz + geo.z) * zm.dz;
+      const float z = fmaf(kz + geo.z, zm.dz, zm.zbot);
       const float hl = vb - expf(-z);
-      v.w = v.w + 12.5f * (hl > 0.f ? hl : 0.f);
+      const float bv = v.w;
+      v.w = fmaf(12.5f, hl > 0.f ? hl : 0.f, bv);
+      if constexpr (NV >= 6) {
+        svb[r] = bv;
+        svc[r] = vb;
+      }
     } else {
       svb[r] = vb;
-      if constexpr (MIX) skeep[r] = (geo.w < vmin) ? 0 : 1;
     }
+    if constexpr (MIX) skeep[r] = (geo.w < vmin) ? 0 : 1;
     sgeo[r] = geo;
     sva[r] = v;
     sidx[r] = pp;
@@ -589,8 +610,9 @@ __attribute__((amdgpu_waves_per_eu(CAP == CAPC_LOW ? (MODE == A32_P2GX ? 8 : 7) 
       const float4 gr = *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(sgeo) + off);
       const float4 v = *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(sva) + off);
       const float w = corner_weight(gr.x, gr.y, gr.z, gr.w, tx, ty, tz);
-      float vals[5] = {v.x, v.y, v.z, v.w, 0.f};
-      if constexpr (FIVE) vals[4] = svb[off >> 4];
+      float vals[6] = {v.x, v.y, v.z, v.w, 0.f, 0.f};
+      if constexpr (NV >= 5) vals[4] = svb[off >> 4];
+      if constexpr (NV >= 6) vals[5] = svc[off >> 4];
       if (!MIX || skeep[off >> 4]) {
         add_parcel<NV>(acc, w, vals);
         kept++;
@@ -612,15 +634,17 @@ __attribute__((amdgpu_waves_per_eu(CAP == CAPC_LOW ? (MODE == A32_P2GX ? 8 : 7) 
       const int64_t pp = sorted[s];
       const float ox = F[B_OX * cap + pp], oy = F[B_OY * cap + pp], oz = F[B_OZ * cap + pp];
       const float vol = F[B_VOL * cap + pp];
-      float vals[5];
-      if constexpr (!FIVE) {
-        const float z = zm.zbot + (skz[cl] + oz) * zm.dz;
-        const float hl = F[B_H * cap + pp] - expf(-z);
+      float vals[6];
+      if constexpr (BT) {
+        const float z = fmaf(skz[cl] + oz, zm.dz, zm.zbot);
+        const float hv = F[B_H * cap + pp], bv = F[B_B * cap + pp];
+        const float hl = hv - expf(-z);
         vals[0] = F[B_P * cap + pp];
         vals[1] = F[B_Q * cap + pp];
         vals[2] = F[B_R * cap + pp];
-        vals[3] = F[B_B * cap + pp] + 12.5f * (hl > 0.f ? hl : 0.f);
-        vals[4] = 0.f;
+        vals[3] = fmaf(12.5f, hl > 0.f ? hl : 0.f, bv);
+        vals[4] = bv;
+        vals[5] = hv;
       } else {
         vals[0] = F[B_B * cap + pp];
         vals[1] = F[B_H * cap + pp];
@@ -703,7 +727,7 @@ __attribute__((amdgpu_waves_per_eu(CAP == CAPC_LOW ? (MODE == A32_P2GX ? 8 : 7) 
 // boundaries when DBL (mixing kept grids)
 template <typename T>
 struct Out32 {
-  T *p[6];
+  T *p[7];  // null: a plane not written
 };
 
 // tileres (may be null): per accumulation tile, whether its cells' partials were written; the
@@ -819,11 +843,13 @@ __global__ void k32_node_sum_x(Geometry g, const float *__restrict__ partial, Ou
     float wsum = s[0];
     if (wsum == 0.f) wsum = 1.f;
 #pragma unroll
-    for (int f = 1; f < NF; f++) out.p[f - 1][t] = s[f] / wsum;
+    for (int f = 1; f < NF; f++)
+      if (out.p[f - 1]) out.p[f - 1][t] = s[f] / wsum;
   } else {
     const float m = (DBL && (k == 0 || k == g.nzn - 1)) ? 2.f : 1.f;
 #pragma unroll
-    for (int f = 0; f < NF; f++) out.p[f][t] = s[f] * m;
+    for (int f = 0; f < NF; f++)
+      if (out.p[f]) out.p[f][t] = s[f] * m;
   }
 }
 
@@ -1152,7 +1178,7 @@ __global__ void k32_mix_new(Geometry g, Bulk32 b, int64_t cap, int64_t norig,
 
 __global__ void k32_mix_residual(Geometry g, Mix32 mg, const int32_t *__restrict__ npercell,
                                  const float *__restrict__ portion_cell,
-                                 uint8_t *__restrict__ touch) {
+                                 uint8_t *__restrict__ touch, uint8_t *__restrict__ dirty) {
   const int64_t nnode = (int64_t)g.nzn * g.ny * g.nx;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nnode) return;
@@ -1188,6 +1214,41 @@ __global__ void k32_mix_residual(Geometry g, Mix32 mg, const int32_t *__restrict
     any = any || v != 0.f;  // a NaN counts as not zero
   }
   touch[t] = any ? 1 : 0;
+  if (any && dirty) {
+    // the parcels of the eight cells around the node absorb: their tiles are summed again
+    const int ntx = g.nx / TILE32;
+    for (int kc = k - 1; kc <= k; kc++) {
+      if (kc < 0 || kc >= g.ncz) continue;
+      for (int bb = 0; bb < 2; bb++)
+        for (int a = 0; a < 2; a++)
+          dirty[((int64_t)kc * g.ny + cj[bb]) * ntx + ci[a] / TILE32] = 1;
+    }
+  }
+}
+
+// dirty tiles (x-row grids): the tile of each listed parcel's cell (parcels != 0) or of each
+// listed new cell, given in the new-cell flags' order (z fastest, then y, then x)
+__global__ void k32_mark_tiles(Geometry g, const int32_t *__restrict__ cellv,
+                               const int32_t *__restrict__ list, int64_t n, int parcels,
+                               uint8_t *__restrict__ dirty) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int32_t e = list[t];
+  int64_t lc;
+  if (parcels) {
+    lc = linear_cell(g, cellv[e]);
+  } else {
+    const int k = (int)(e % g.ncz);
+    const int j = (int)((e / g.ncz) % g.ny);
+    const int i = (int)(e / ((int64_t)g.ncz * g.ny));
+    lc = ((int64_t)k * g.ny + j) * g.nx + i;
+  }
+  dirty[lc / TILE32] = 1;
+}
+
+__global__ void k32_fill_u8(uint8_t *__restrict__ a, int64_t n, uint8_t v) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = v;
 }
 
 // A parcel none of whose corner nodes carries a residual (almost all of them: residuals arise
@@ -1246,11 +1307,15 @@ __global__ void k32_mix_absorb(Geometry g, Bulk32 b, int64_t cap, int64_t norig,
 
 // backfill: the state fields of the j-th kept slot from the end into the j-th removed slot
 // (the reference copies x, y, z, p, q, r, b, h, vol, tag and stretch)
-__global__ void k32_mix_backfill(Bulk32 b, int64_t cap, const int32_t *__restrict__ dst, int64_t R,
-                                 const int32_t *__restrict__ k1, int64_t K1) {
+__global__ void k32_mix_backfill(Geometry g, Bulk32 b, int64_t cap,
+                                 const int32_t *__restrict__ dst, int64_t R,
+                                 const int32_t *__restrict__ k1, int64_t K1,
+                                 uint8_t *__restrict__ dirty) {
   const int64_t jx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (jx >= R) return;
   const int64_t s = k1[K1 - 1 - jx], d = dst[jx];
+  // a moved parcel changes the parcel order of its cell's sums
+  if (dirty) dirty[linear_cell(g, b.cell[s]) / TILE32] = 1;
   float *F = b.f;
   const int fl[10] = {B_OX, B_OY, B_OZ, B_P, B_Q, B_R, B_B, B_H, B_VOL, B_STRETCH};
 #pragma unroll
@@ -1373,6 +1438,10 @@ int b32_alloc(Ctx *c) {
   for (int i = 0; i < 6; i++) PMPIC_TRY(dalloc32(&b.grid[i], c->gsz));
   for (int i = 0; i < 12; i++) PMPIC_TRY(dalloc32(&b.mgrid[i], c->gsz));
   PMPIC_TRY(dalloc32(&b.mtouch, c->gsz));
+  PMPIC_TRY(dalloc32(&b.dirty, c->ncell / TILE32 + 1));
+  b.p2g_reuse = false;
+  const char *renv = getenv("PMPIC_P2G_REUSE");
+  b.reuse_off = renv && renv[0] == '0';
   for (int i = 0; i < 4; i++) {
     PMPIC_TRY(dalloc32(&b.pgrid[i], c->gsz));
     b.p2g_pending[i] = false;
@@ -1419,6 +1488,7 @@ void b32_free(Ctx *c) {
   for (int i = 0; i < 6; i++) hipFree(b.grid[i]);
   for (int i = 0; i < 12; i++) hipFree(b.mgrid[i]);
   hipFree(b.mtouch);
+  hipFree(b.dirty);
   for (int i = 0; i < 4; i++) hipFree(b.pgrid[i]);
   hipFree(b.slot);
   hipFree(b.ovf);
@@ -1428,6 +1498,7 @@ int b32_set_parcels(Ctx *c, int64_t n, const double *const *fields) {
   PMPIC_TRY(ensure_fbuf(c));
   Bulk32 &b = c->b32;
   b.prebinned = false;
+  b.p2g_reuse = false;
   const dim3 gb(cdiv(n > 0 ? n : 1, TPB)), tb(TPB);
   for (int a = 0; a < 3; a++) {
     if (n > 0) {
@@ -1482,27 +1553,44 @@ int b32_get_parcels(Ctx *c, double *const *fields) {
 // the x-combined partials need tiles within one x row
 static bool p2g_xrows(const Ctx *c) { return c->g.nx % TILE32 == 0; }
 
-static int p2g_accum(Ctx *c) {
-  if (p2g_xrows(c)) {
-    if (c->b32.capc == CAPC_LOW)
-      return launch_timed(c, T_ACCUM, k32_cell_accum<A32_P2GX, CAPC_LOW, ROWS_LOW>,
-                          dim3(cdiv(c->ncell, TILE32)), dim3(ATPB), 0, c->b32, c->cap, c->g,
-                          zmap_of(c->g), (float)c->g.vmin, c->cell_count, c->cell_start,
-                          c->sorted, c->ncell, partial32(c), nullptr, nullptr, nullptr);
-    return launch_timed(c, T_ACCUM, k32_cell_accum<A32_P2GX, CAPC, ROWS32>,
-                        dim3(cdiv(c->ncell, TILE32)), dim3(ATPB), 0, c->b32, c->cap, c->g,
-                        zmap_of(c->g), (float)c->g.vmin, c->cell_count, c->cell_start, c->sorted,
-                        c->ncell, partial32(c), nullptr, nullptr, nullptr);
+// A graph of stages 1-3 always captures stage 1's par2grid reading the dirty-tile flags (on
+// x-row grids with the reuse on); where the last mixing's sums are not current, every tile is
+// marked dirty first, so the captured par2grid sums them all.
+void b32_graph_reuse(Ctx *c) {
+  Bulk32 &b = c->b32;
+  if (!p2g_xrows(c) || b.reuse_off) return;
+  if (!b.p2g_reuse) {
+    const int64_t nt = c->ncell / TILE32 + 1;
+    hipLaunchKernelGGL(k32_fill_u8, dim3(cdiv(nt, TPB)), dim3(TPB), 0, c->stream, b.dirty, nt,
+                       (uint8_t)1);
   }
-  if (c->b32.capc == CAPC_LOW)
-    return launch_timed(c, T_ACCUM, k32_cell_accum<A32_P2G, CAPC_LOW, ROWS_LOW>,
-                        dim3(cdiv(c->ncell, TILE32)), dim3(ATPB), 0, c->b32, c->cap, c->g,
-                        zmap_of(c->g), (float)c->g.vmin, c->cell_count, c->cell_start, c->sorted,
-                        c->ncell, partial32(c), nullptr, nullptr, nullptr);
-  return launch_timed(c, T_ACCUM, k32_cell_accum<A32_P2G, CAPC, ROWS32>,
-                      dim3(cdiv(c->ncell, TILE32)), dim3(ATPB), 0, c->b32, c->cap, c->g,
-                      zmap_of(c->g), (float)c->g.vmin, c->cell_count, c->cell_start, c->sorted,
-                      c->ncell, partial32(c), nullptr, nullptr, nullptr);
+  b.p2g_reuse = true;
+}
+
+// the par2grid accumulation; with `dirty`, only the tiles it marks (the others keep the
+// partials the last mixing summed): that launch is left out of the kernel timer, whose
+// per-launch figures are those of full launches
+static int p2g_accum(Ctx *c, const uint8_t *dirty = nullptr) {
+  const dim3 grid(cdiv(c->ncell, TILE32)), blk(ATPB);
+  const Bulk32 &b = c->b32;
+  if (p2g_xrows(c)) {
+    auto k = b.capc == CAPC_LOW ? k32_cell_accum<A32_P2GX, CAPC_LOW, ROWS_LOW>
+                                : k32_cell_accum<A32_P2GX, CAPC, ROWS32>;
+    if (dirty) {
+      hipLaunchKernelGGL(k, grid, blk, 0, c->stream, b, c->cap, c->g, zmap_of(c->g),
+                         (float)c->g.vmin, c->cell_count, c->cell_start, c->sorted, c->ncell,
+                         partial32(c), nullptr, nullptr, nullptr, dirty);
+      return PMPIC_OK;
+    }
+    return launch_timed(c, T_ACCUM, k, grid, blk, 0, b, c->cap, c->g, zmap_of(c->g),
+                        (float)c->g.vmin, c->cell_count, c->cell_start, c->sorted, c->ncell,
+                        partial32(c), nullptr, nullptr, nullptr, nullptr);
+  }
+  auto k = b.capc == CAPC_LOW ? k32_cell_accum<A32_P2G, CAPC_LOW, ROWS_LOW>
+                              : k32_cell_accum<A32_P2G, CAPC, ROWS32>;
+  return launch_timed(c, T_ACCUM, k, grid, blk, 0, b, c->cap, c->g, zmap_of(c->g),
+                      (float)c->g.vmin, c->cell_count, c->cell_start, c->sorted, c->ncell,
+                      partial32(c), nullptr, nullptr, nullptr, nullptr);
 }
 
 // The capacity tuning: bin at 16 and at 32 slots, time the accumulation of each (its second
@@ -1549,11 +1637,15 @@ int b32_par2grid(Ctx *c) {
   b.kept_pending[1] = b.kept_pending[3] = false;
   b.kept_pending[4] = b.kept_pending[5] = false;
   for (int i = 0; i < 4; i++) b.p2g_pending[i] = true;
+  // the last mixing's kept sums, where it left them current, stand for this accumulation
+  // except on the tiles it changed
+  const uint8_t *dirty = (b.p2g_reuse && p2g_xrows(c) && !b.captune) ? b.dirty : nullptr;
+  b.p2g_reuse = false;
   if (c->b32.captune && n > 0) {
     PMPIC_TRY(tune_capacity(c, n));
   } else {
     PMPIC_TRY(bin32(c, n));
-    PMPIC_TRY(p2g_accum(c));
+    PMPIC_TRY(p2g_accum(c, dirty));
   }
   Out32<float> out = {{b.pgrid[0], b.pgrid[1], b.pgrid[2], b.pgrid[3], nullptr, nullptr}};
   const int64_t nnode = (int64_t)c->g.nzn * c->g.ny * c->g.nx;
@@ -1574,6 +1666,7 @@ int b32_rk_stage(Ctx *c, int stage, int stretch) {
                    c->b32.grid[4], c->b32.grid[5]}};
   const Scale32 s3 = scale_of(c->g);
   Bulk32 &b = c->b32;
+  b.p2g_reuse = false;
   const dim3 gb(cdiv(c->graph_n ? c->graph_n : n, b.rk_tpb)), tb(b.rk_tpb);
   // stages 1-3 bin the cells they write for the next stage's par2grid (stage 4 is followed by
   // splitting and mixing, which change the parcels); not while the capacity is being tuned
@@ -1616,6 +1709,7 @@ int b32_rk_stage(Ctx *c, int stage, int stretch) {
 int b32_split(Ctx *c) {
   const int64_t n = c->n;
   c->b32.prebinned = false;
+  c->b32.p2g_reuse = false;
   if (n > 0) {
     // the stage-4 RK kernel already flagged the parcels to split
     if (!c->split_flags_fresh)
@@ -1649,35 +1743,51 @@ int b32_mix(Ctx *c) {
   // kept and residual grids (construct_grids, parcel_mixing.F90:440-624)
   PMPIC_TRY(bin32(c, norig));
   const bool xr = p2g_xrows(c);
+  // x-row grids: the kept sums' first five planes (weight, p, q, r, total buoyancy) are what
+  // the next par2grid would sum on every tile this mixing leaves unchanged; the tiles it changes
+  // are marked below (removed, new, absorbing and moved parcels)
+  b.p2g_reuse = false;
+  const bool reuse = xr && !b.reuse_off;
+  uint8_t *dirty = reuse ? b.dirty : nullptr;
+  if (reuse) {
+    const int64_t nt = c->ncell / TILE32 + 1;
+    hipLaunchKernelGGL(k32_fill_u8, dim3(cdiv(nt, TPB)), dim3(TPB), 0, c->stream, b.dirty, nt,
+                       (uint8_t)0);
+  }
   if (xr && b.capc == CAPC_LOW)
     hipLaunchKernelGGL((k32_cell_accum<A32_MIXX, CAPC_LOW, ROWS_LOW>), dim3(cdiv(c->ncell, TILE32)),
                        dim3(ATPB), 0, c->stream, b, c->cap, g, zmap_of(g), vmin, c->cell_count,
                        c->cell_start, c->sorted, c->ncell, partial32(c), partial32b(c),
-                       c->npercell, c->cellflag);
+                       c->npercell, c->cellflag, nullptr);
   else if (xr)
     hipLaunchKernelGGL((k32_cell_accum<A32_MIXX, CAPC, ROWS32>), dim3(cdiv(c->ncell, TILE32)),
                        dim3(ATPB), 0, c->stream, b, c->cap, g, zmap_of(g), vmin, c->cell_count,
                        c->cell_start, c->sorted, c->ncell, partial32(c), partial32b(c),
-                       c->npercell, c->cellflag);
+                       c->npercell, c->cellflag, nullptr);
   else if (b.capc == CAPC_LOW)
     hipLaunchKernelGGL((k32_cell_accum<A32_MIX, CAPC_LOW, ROWS_LOW>), dim3(cdiv(c->ncell, TILE32)),
                        dim3(ATPB), 0, c->stream, b, c->cap, g, zmap_of(g), vmin, c->cell_count,
                        c->cell_start, c->sorted, c->ncell, partial32(c), partial32b(c),
-                       c->npercell, c->cellflag);
+                       c->npercell, c->cellflag, nullptr);
   else
     hipLaunchKernelGGL((k32_cell_accum<A32_MIX, CAPC, ROWS32>), dim3(cdiv(c->ncell, TILE32)),
                        dim3(ATPB), 0, c->stream, b, c->cap, g, zmap_of(g), vmin, c->cell_count,
                        c->cell_start, c->sorted, c->ncell, partial32(c), partial32b(c),
-                       c->npercell, c->cellflag);
+                       c->npercell, c->cellflag, nullptr);
   const int64_t nnode = (int64_t)g.nzn * g.ny * g.nx;
   Out32<float> kept = {{b.mgrid[0], b.mgrid[1], b.mgrid[2], b.mgrid[3], b.mgrid[4], b.mgrid[5]}};
   Out32<float> res = {{b.mgrid[6], b.mgrid[7], b.mgrid[8], b.mgrid[9], b.mgrid[10], b.mgrid[11]}};
   // cellflag holds the per-tile residual flags until the new-parcel flags reuse it below
   if (xr) {
-    hipLaunchKernelGGL((k32_node_sum_x<6, false, true>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0,
-                       c->stream, g, partial32(c), kept, nullptr);
-    hipLaunchKernelGGL((k32_node_sum_x<6, false, false>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0,
-                       c->stream, g, partial32b(c), res, c->cellflag);
+    // planes weight, p, q, r, total buoyancy (not kept), b, h
+    const Out32<float> keptx = {{b.mgrid[0], b.mgrid[3], b.mgrid[4], b.mgrid[5], nullptr,
+                                 b.mgrid[1], b.mgrid[2]}};
+    const Out32<float> resx = {{b.mgrid[6], b.mgrid[9], b.mgrid[10], b.mgrid[11], nullptr,
+                                b.mgrid[7], b.mgrid[8]}};
+    hipLaunchKernelGGL((k32_node_sum_x<7, false, true>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0,
+                       c->stream, g, partial32(c), keptx, nullptr);
+    hipLaunchKernelGGL((k32_node_sum_x<7, false, false>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0,
+                       c->stream, g, partial32b(c), resx, c->cellflag);
   } else {
     hipLaunchKernelGGL((k32_node_sum<5, false, true, float>), dim3(cdiv(nnode, TPB)), dim3(TPB),
                        0, c->stream, g, partial32(c), c->ncell, kept, nullptr);
@@ -1700,12 +1810,21 @@ int b32_mix(Ctx *c) {
   const int64_t Z = c->hsc.nzero, nadd = c->hsc.nadd;
   Mix32 mg = {{b.mgrid[0], b.mgrid[1], b.mgrid[2], b.mgrid[3], b.mgrid[4], b.mgrid[5]},
               {b.mgrid[6], b.mgrid[7], b.mgrid[8], b.mgrid[9], b.mgrid[10], b.mgrid[11]}};
-  float *portion = partial32(c);  // the kept partials are consumed by now
+  float *portion = partial32b(c);  // the residual partials are consumed by now
+  if (reuse) {
+    // the removed parcels' cells (read before new parcels take their slots) and the new cells
+    if (Z > 0)
+      hipLaunchKernelGGL(k32_mark_tiles, dim3(cdiv(Z, TPB)), dim3(TPB), 0, c->stream, g, b.cell,
+                         c->list_a, Z, 1, b.dirty);
+    if (nadd > 0)
+      hipLaunchKernelGGL(k32_mark_tiles, dim3(cdiv(nadd, TPB)), dim3(TPB), 0, c->stream, g,
+                         b.cell, c->sorted, nadd, 0, b.dirty);
+  }
   if (nadd > 0)
     hipLaunchKernelGGL(k32_mix_new, dim3(cdiv(nadd, TPB)), dim3(TPB), 0, c->stream, g, b, c->cap,
                        norig, c->sorted, nadd, c->list_a, Z, c->flag, mg, vmin, portion, c->dsc);
   hipLaunchKernelGGL(k32_mix_residual, dim3(cdiv(nnode, TPB)), dim3(TPB), 0, c->stream, g, mg,
-                     c->npercell, portion, b.mtouch);
+                     c->npercell, portion, b.mtouch, dirty);
   if (norig > 0)
     hipLaunchKernelGGL(k32_mix_absorb, dim3(cdiv(norig, TPB)), dim3(TPB), 0, c->stream, g, b,
                        c->cap, norig, c->flag, mg, b.mtouch);
@@ -1745,13 +1864,14 @@ int b32_mix(Ctx *c) {
       PMPIC_TRY(sync_scalars(c));
       const int64_t K1 = c->hsc.nkeep1;
       if (R > K1) return PMPIC_ERR_BACKFILL;
-      hipLaunchKernelGGL(k32_mix_backfill, dim3(cdiv(R, TPB)), dim3(TPB), 0, c->stream, b, c->cap,
-                         dst, R, c->list_b, K1);
+      hipLaunchKernelGGL(k32_mix_backfill, dim3(cdiv(R, TPB)), dim3(TPB), 0, c->stream, g, b,
+                         c->cap, dst, R, c->list_b, K1, dirty);
       PMPIC_CHECK(hipGetLastError());
       np -= R;
     }
   }
   c->n = np;
+  b.p2g_reuse = reuse;
   return b32_publish_n(c);
 }
 
@@ -1760,17 +1880,18 @@ int b32_mix(Ctx *c) {
 int b32_dump_grids(Ctx *c) {
   Bulk32 &b = c->b32;
   const Geometry &g = c->g;
+  b.p2g_reuse = false;  // the dump's sums overwrite the partials
   PMPIC_TRY(bin32(c, c->n));
   if (b.capc == CAPC_LOW)
     hipLaunchKernelGGL((k32_cell_accum<A32_DUMP, CAPC_LOW, ROWS_LOW>), dim3(cdiv(c->ncell, TILE32)),
                        dim3(ATPB), 0, c->stream, b, c->cap, g, zmap_of(g), (float)g.vmin,
                        c->cell_count, c->cell_start, c->sorted, c->ncell, partial32(c), nullptr,
-                       nullptr, nullptr);
+                       nullptr, nullptr, nullptr);
   else
     hipLaunchKernelGGL((k32_cell_accum<A32_DUMP, CAPC, ROWS32>), dim3(cdiv(c->ncell, TILE32)),
                        dim3(ATPB), 0, c->stream, b, c->cap, g, zmap_of(g), (float)g.vmin,
                        c->cell_count, c->cell_start, c->sorted, c->ncell, partial32(c), nullptr,
-                       nullptr, nullptr);
+                       nullptr, nullptr, nullptr);
   Out32<double> out = {{c->grid[GB], c->grid[GHG], c->grid[GP], c->grid[GQ], c->grid[GR], nullptr}};
   const int64_t nnode = (int64_t)g.nzn * g.ny * g.nx;
   hipLaunchKernelGGL((k32_node_sum<5, true, true, double>), dim3(cdiv(nnode, TPB)), dim3(TPB), 0,

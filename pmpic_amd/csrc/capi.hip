@@ -643,6 +643,7 @@ static int graph_stages123(pmpic_ctx *ctx) {
   // of the captured one (the launches' idle groups kept few)
   if (c->g123x && (capc != c->g123_capc || c->n > c->g123_n || 8 * graph_bound(c) < 7 * c->g123_n))
     graph_drop(c);
+  b32_graph_reuse(c);
   if (!c->g123x) {
     // the capture itself advanced the counters; the launch below does the work
     PMPIC_TRY(graph_capture(ctx, capc));
@@ -658,6 +659,7 @@ static int graph_stages123(pmpic_ctx *ctx) {
     c->b32.kept_pending[4] = c->b32.kept_pending[5] = false;
     for (int i = 0; i < 4; i++) c->b32.p2g_pending[i] = true;
     c->b32.prebinned = c->g123_prebinned;
+    c->b32.p2g_reuse = false;  // consumed by stage 1's par2grid, as the eager call does
     if (c->b32.prebinned) c->b32.capc = capc;
   }
   PMPIC_CHECK(hipGraphLaunch(c->g123x, c->stream));

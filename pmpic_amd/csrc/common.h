@@ -70,6 +70,13 @@ struct Bulk32 {
   // cells are built
   bool fused_bin;
   bool prebinned;
+  // x-row grids: the last mixing's kept sums are the next par2grid's partials except on the
+  // tiles mixing changed (dirty: one byte per 16-cell tile); p2g_reuse while that holds (set by
+  // mixing, cleared by the par2grid that uses it and by anything else that changes the
+  // parcels or the partials); reuse_off: PMPIC_P2G_REUSE=0
+  uint8_t *dirty;
+  bool p2g_reuse;
+  bool reuse_off;
 };
 
 // grid ids exported through the C-ABI (writebinary.F90:287-296 order, then the tendencies)
@@ -373,6 +380,7 @@ int b32_mix(Ctx *c);
 int b32_conservation_sums(Ctx *c, double out6[6]);
 int b32_choose_capc(const Ctx *c, int64_t n);  // the slot capacity a binning of n parcels takes
 int b32_mean_capc(const Ctx *c, int64_t n);    // ... by the mean-occupancy rule alone
+void b32_graph_reuse(Ctx *c);  // stage 1's par2grid reading the dirty-tile flags, all set if stale
 int b32_publish_n(Ctx *c);                     // the parcel count into device memory
 int b32_export_grid(Ctx *c, int id);  // brings grid[id] up to date for a grid export
 int b32_dump_grids(Ctx *c);             // writebinary's b, hg, p, q, r grids

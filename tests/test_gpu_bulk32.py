@@ -234,6 +234,43 @@ def test_fused_binning_changes_no_result(monkeypatch, case):
         np.testing.assert_array_equal(g1[k], g0[k], err_msg=k)
 
 
+@pytest.mark.parametrize("case,shape", [("8ppc", (32, 16, 8)), ("27ppc", (16, 12, 8)),
+                                        ("crowded", (32, 12, 8)), ("dense", (16, 12, 8))])
+def test_mixing_sums_reused_by_next_par2grid_change_no_result(monkeypatch, case, shape):
+    """The next step's first par2grid taking the mixing's kept sums on every tile mixing left
+    unchanged and summing only the changed ones (the default on x-row grids) against summing
+    every tile (PMPIC_P2G_REUSE=0): four steps of splitting and mixing, removed, new, absorbing
+    and moved parcels, end bit-identical in counts, parcels and grids after every step; several
+    tiles per x row (the periodic boundary between the last tile and the first), 8 and 27
+    parcels per cell, crowded cells and the dense ranks."""
+    if case == "crowded":
+        ic = crowded_case(*shape)
+    else:
+        ic = thermal_case(*shape, 3 if case == "27ppc" else 2, vort=3.0, shrink=0.05, thin=0.1,
+                          near_split=0.05)
+    if case == "dense":
+        monkeypatch.setenv("PMPIC_SLOT_CAPACITY", "0")
+    names = ("u", "v", "w", "p", "q", "r", "b", "vol", "hg")
+    ctx = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("PMPIC_P2G_REUSE", mode)
+        ctx[mode] = make(shape, 1, ic)
+    removed = added = 0
+    for s in range(4):
+        for mode in ("0", "1"):
+            ctx[mode].step()
+        a, b = ctx["0"], ctx["1"]
+        assert (a.n, a.last_counts()) == (b.n, b.last_counts()), s
+        removed += a.last_counts()[1]
+        added += a.last_counts()[2]
+        pa, pb = a.get_parcels(), b.get_parcels()
+        for k in pa:
+            np.testing.assert_array_equal(pb[k], pa[k], err_msg=f"step {s}: {k}")
+        for k in names:
+            np.testing.assert_array_equal(b.grid(k), a.grid(k), err_msg=f"step {s}: {k}")
+    assert removed > 0 or case == "crowded"
+
+
 def test_writebinary_grids_close_to_fp64():
     """writebinary's grids (p, q, r, b + 12.5 hgliq, hg, hgliq) from the FP32 parcels after two
     steps, within 1e-5 of the FP64 form's (which test_gpu_parity.py holds to the oracle)."""
