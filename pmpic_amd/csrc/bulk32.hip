@@ -1246,6 +1246,23 @@ __global__ void k32_mark_tiles(Geometry g, const int32_t *__restrict__ cellv,
   dirty[lc / TILE32] = 1;
 }
 
+// *any = 1 if a byte of a[0, n) is set (a is read a 32-bit word at a time; n a multiple of 4
+// or the tail read as bytes)
+__global__ void k32_any_u8(const uint8_t *__restrict__ a, int64_t n, int64_t *any) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool set = false;
+  if (4 * t + 3 < n) {
+    set = *reinterpret_cast<const uint32_t *>(a + 4 * t) != 0u;
+  } else {
+    for (int64_t e = 4 * t; e < n; e++) set = set || a[e] != 0;
+  }
+  if (__ballot(set) != 0ull && (threadIdx.x & 63) == 0) *any = 1;
+}
+
+__global__ void k32_zero_i64(int64_t *a) {
+  if (threadIdx.x == 0) *a = 0;
+}
+
 __global__ void k32_fill_u8(uint8_t *__restrict__ a, int64_t n, uint8_t v) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) a[t] = v;
@@ -1440,6 +1457,7 @@ int b32_alloc(Ctx *c) {
   PMPIC_TRY(dalloc32(&b.mtouch, c->gsz));
   PMPIC_TRY(dalloc32(&b.dirty, c->ncell / TILE32 + 1));
   b.p2g_reuse = false;
+  b.dirty_seq = 0;
   const char *renv = getenv("PMPIC_P2G_REUSE");
   b.reuse_off = renv && renv[0] == '0';
   for (int i = 0; i < 4; i++) {
@@ -1641,9 +1659,12 @@ int b32_par2grid(Ctx *c) {
   // except on the tiles it changed
   const uint8_t *dirty = (b.p2g_reuse && p2g_xrows(c) && !b.captune) ? b.dirty : nullptr;
   b.p2g_reuse = false;
+  // a mixing that changed no tile (known to the host once it has read the device scalars back,
+  // and not inside a graph capture): the partials are all current, so no binning or sums
+  const bool none = dirty && c->graph_n == 0 && b.dirty_seq < c->sync_seq && c->hsc.anydirty == 0;
   if (c->b32.captune && n > 0) {
     PMPIC_TRY(tune_capacity(c, n));
-  } else {
+  } else if (!none) {
     PMPIC_TRY(bin32(c, n));
     PMPIC_TRY(p2g_accum(c, dirty));
   }
@@ -1872,6 +1893,15 @@ int b32_mix(Ctx *c) {
   }
   c->n = np;
   b.p2g_reuse = reuse;
+  if (reuse) {
+    // whether any tile changed; the host learns it with the next read-back of the device
+    // scalars (the status check that ends the component or the step)
+    const int64_t nt = c->ncell / TILE32;
+    hipLaunchKernelGGL(k32_zero_i64, dim3(1), dim3(64), 0, c->stream, &c->dsc->anydirty);
+    hipLaunchKernelGGL(k32_any_u8, dim3(cdiv(cdiv(nt, 4), TPB)), dim3(TPB), 0, c->stream, b.dirty,
+                       nt, &c->dsc->anydirty);
+    b.dirty_seq = c->sync_seq;
+  }
   return b32_publish_n(c);
 }
 

@@ -75,6 +75,8 @@ struct Bulk32 {
   // mixing, cleared by the par2grid that uses it and by anything else that changes the
   // parcels or the partials); reuse_off: PMPIC_P2G_REUSE=0
   uint8_t *dirty;
+  int64_t dirty_seq;  // Ctx::sync_seq when the last mixing left its flags: the host knows
+                      // whether any tile changed once a later read-back has happened
   bool p2g_reuse;
   bool reuse_off;
 };
@@ -104,6 +106,7 @@ struct DevScalars {
   int64_t cross;     // parcels crossing x/y this haloswap
   int64_t nsplit;    // parcels split this step
   int64_t nremove, nadd, nzero, nkeep1, nbackfill;
+  int64_t anydirty;  // the last x-row mixing changed some tile (Bulk32::dirty)
   int status;        // first error code raised by a kernel
   int32_t novf;      // parcels past their cell's slot capacity in the last FP32 binning
 };
@@ -122,6 +125,7 @@ struct Ctx {
   int split_flags_fresh; // flag[] holds the split test of the last stage-4 RK update
   int splitting_enabled, mixing_enabled;
   int64_t last_split, last_removed, last_added;
+  int64_t sync_seq;      // device scalars read back so far (sync_scalars)
   hipStream_t stream;
   Bulk32 b32;            // FP32-bulk form state (precision == 1)
   // parcels

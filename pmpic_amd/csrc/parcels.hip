@@ -576,6 +576,7 @@ int read_back(Ctx *c, void *host, const void *dev, size_t bytes) {
 
 int sync_scalars(Ctx *c) {
   PMPIC_TRY(read_back(c, &c->hsc, c->dsc, sizeof(DevScalars)));
+  c->sync_seq++;
   return c->hsc.status;
 }
 
