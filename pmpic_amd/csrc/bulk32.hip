@@ -366,7 +366,8 @@ __global__ void k32_sort_big(const int32_t *__restrict__ start, int64_t ncell, i
 // vol, b, h, p, q, r (parcel_mixing); DUMP: b, h, p, q, r (writebinary's grids)
 // P2GX, MIXX: P2G and MIX whose tiles are 16 cells of one x row (nx a multiple of 16), each
 // tile writing its corner sums combined along x into the 17 node positions it touches (see
-// k32_node_sum_x)
+// k32_node_sum_x); MIXX sums vol, p, q, r, the total buoyancy, b and h, so that its first five
+// planes are the ones P2GX writes (the next par2grid keeps them where mixing changed nothing)
 enum Acc32 { A32_P2G = 0, A32_MIX = 1, A32_DUMP = 2, A32_P2GX = 3, A32_MIXX = 4 };
 constexpr int TILE32 = 16;
 constexpr int ROWS32 = 512;     // LDS rows with the full capacity
