@@ -232,17 +232,22 @@ constexpr int TPT = 512 / FFT_TPB;  // twiddles per thread (n <= 512)
 // The column passes of FP32 columns of at most 128 points stage half the batch (8 KiB per
 // buffer), so twice the workgroups fit a CU; that still takes at least 8 adjacent columns, 64
 // bytes of each row.  Longer columns keep the full batch (fewer columns would cut the rows' runs
-// to 32 bytes: 256-point columns measured 160 -> 264 us per inverse pass).  n = 0: a length
-// known only at run time.
+// to 32 bytes: 256-point columns measured 160 -> 264 us per inverse pass), and the inverse pass
+// of 512-point FP32 columns takes twice the batch (eight column pairs, 64-byte runs: 1.97 ->
+// 1.64 ms at 512 x 512 x 257 nodes, with half the workgroups per CU; the forward pass, whose
+// registers then hold twice the elements, measured slower that way).  n = 0: a length known
+// only at run time; inv: the inverse (complex-to-real) pass.
 template <typename S>
-__host__ __device__ constexpr int lds_cols(int n) {
-  return (sizeof(S) == 4 && n > 0 && n <= 128) ? lds_cplx<S>() / 2 : lds_cplx<S>();
+__host__ __device__ constexpr int lds_cols(int n, bool inv) {
+  return (sizeof(S) == 4 && n > 0 && n <= 128)   ? lds_cplx<S>() / 2
+         : (sizeof(S) == 4 && inv && n == 512)   ? lds_cplx<S>() * 2
+                                                 : lds_cplx<S>();
 }
 
 // adjacent columns (column pairs for the inverse) a column pass takes per workgroup
 template <typename S>
-__host__ __device__ constexpr int col_batch(int n) {
-  return lds_cols<S>(n) / n > 32 ? 32 : lds_cols<S>(n) / n;
+__host__ __device__ constexpr int col_batch(int n, bool inv) {
+  return lds_cols<S>(n, inv) / n > 32 ? 32 : lds_cols<S>(n, inv) / n;
 }
 
 // q = a / d and r = a - q d for 0 <= a < 2^22, 0 < d, with rd = 1/d rounded to FP32: the
@@ -331,13 +336,13 @@ __global__ void __launch_bounds__(FFT_TPB) k_cols_fwd(Plan p, const cx_t<S> *__r
                                                       int64_t c1_field, int nl, int nb, int sx,
                                                       int sy) {
   using C = cx_t<S>;
-  constexpr int EPT = lds_cols<S>(NT) / FFT_TPB;
-  __shared__ C A[lds_cols<S>(NT)], B[lds_cols<S>(NT)], T[512];
+  constexpr int EPT = lds_cols<S>(NT, false) / FFT_TPB;
+  __shared__ C A[lds_cols<S>(NT, false)], B[lds_cols<S>(NT, false)], T[512];
   const int f = blockIdx.z;
   const C *src = c1 + (int64_t)f * c1_field;
   S *dst = static_cast<S *>(fp.out[f]);
   const int ny = NT ? NT : p.n;
-  const int nbd = NT ? col_batch<S>(NT) : nb;  // the batch every workgroup but the last takes
+  const int nbd = NT ? col_batch<S>(NT, false) : nb;  // the batch every workgroup but the last takes
   const int k = blockIdx.y;
   const int l0 = blockIdx.x * nb;
   const int nbb = min(nb, nl - l0);
@@ -469,13 +474,13 @@ __global__ void __launch_bounds__(FFT_TPB) k_cols_c2r(Plan p, const cx_t<S> *__r
                                                       FieldPtrs fp, const cx_t<S> *__restrict__ c2,
                                                       int64_t c2_field, int nx, int nb) {
   using C = cx_t<S>;
-  constexpr int EPT = lds_cols<S>(NT) / FFT_TPB;
-  __shared__ C A[lds_cols<S>(NT)], B[lds_cols<S>(NT)], T[512];
+  constexpr int EPT = lds_cols<S>(NT, true) / FFT_TPB;
+  __shared__ C A[lds_cols<S>(NT, true)], B[lds_cols<S>(NT, true)], T[512];
   const int f = blockIdx.z;
   const C *src = c2 + (int64_t)f * c2_field;
   double *dst = static_cast<double *>(fp.out[f]);
   const int ny = NT ? NT : p.n, nm = ny / 2 + 1;
-  const int nbd = NT ? col_batch<S>(NT) : nb;  // the batch every workgroup but the last takes
+  const int nbd = NT ? col_batch<S>(NT, true) : nb;  // the batch every workgroup but the last takes
   const int k = blockIdx.y;
   const int x0 = blockIdx.x * 2 * nb;
   const int nbb = min(nb, (nx - x0 + 1) / 2);
@@ -606,7 +611,7 @@ int fft_forward_t(Ctx *c, int nf, const S *const *src, S *const *dst) {
     return 0;
   });
   const int nl = g.nx / 2 + 1;
-  const int nby = col_batch<S>(g.ny);
+  const int nby = col_batch<S>(g.ny, false);
   with_length(g.ny, [&](auto N) {
     hipLaunchKernelGGL((k_cols_fwd<S, decltype(N)::value>), dim3(cdiv(nl, nby), g.nzn, nf),
                        dim3(FFT_TPB), 0, c->stream, py, twiddles_y<S>(c), fp, cb, c1f, nl, nby,
@@ -646,7 +651,7 @@ int fft_backward_t(Ctx *c, int nf, const S *const *src, double *const *dst, floa
                        rnm, nbx, g.sx);
     return 0;
   });
-  const int nby = col_batch<S>(g.ny);  // transforms per workgroup, two x columns each
+  const int nby = col_batch<S>(g.ny, true);  // transforms per workgroup, two x columns each
   with_length(g.ny, [&](auto N) {
     hipLaunchKernelGGL((k_cols_c2r<S, decltype(N)::value>), dim3(cdiv(g.nx, 2 * nby), g.nzn, nf),
                        dim3(FFT_TPB), 0, c->stream, py, twiddles_y<S>(c), fp, cb, c2f, g.nx, nby);
