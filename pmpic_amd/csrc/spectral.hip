@@ -572,53 +572,64 @@ __global__ void __launch_bounds__(64) k_diffz_reg(Geometry g, ColFields<S> cf,
                                                   const double *__restrict__ tcp,
                                                   DevScalars *sc) {
   // the Thomas tables are the same for every column: staged once per block in LDS, so the
-  // recurrence reads them without a memory round trip per level
+  // recurrence reads them without a memory round trip per level.  The FP32-bulk form (S float)
+  // stages the pivots' reciprocals and multiplies by them, takes 1/dz and 1/(2 dz) as factors
+  // and holds the column in FP32 between the sweeps (the sweeps themselves run in FP64); the
+  // FP64 form divides as the reference does
+  constexpr bool F32 = std::is_same<S, float>::value;
+  using FT = typename std::conditional<F32, float, double>::type;
   __shared__ double den[NZ], cpt[NZ];
   const int64_t plane = (int64_t)g.sy * g.sx;
   const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int fi = blockIdx.y;
   const bool bt = cf.bot[fi] != nullptr && cf.top[fi] != nullptr;
   for (int k = threadIdx.x; k < NZ; k += blockDim.x) {
-    den[k] = tden[(bt ? NZ : 0) + k];
+    const double dn = tden[(bt ? NZ : 0) + k];
+    if (F32 && dn == 0 && blockIdx.x == 0) raise_status(sc, PMPIC_ERR_ZERO_PIVOT);
+    den[k] = F32 ? 1.0 / dn : dn;
     cpt[k] = tcp[(bt ? NZ : 0) + k];
   }
   __syncthreads();
   if (col >= plane) return;
   const S *fin = cf.in[fi] + col;
   S *o = cf.out[fi] + col;
-  double F[NZ];
+  FT F[NZ];
 #pragma unroll
   for (int k = 0; k < NZ; k++) F[k] = fin[(int64_t)k * plane];
   const double bot = bt ? cf.bot[fi][col] : 0.0, top = bt ? cf.top[fi][col] : 0.0;
   const double a1 = (double)(1.f / 6.f), al = (double)(1.f / 3.f);
-  const double dz = g.dz;
+  const double dz = g.dz, idz = 1.0 / dz, ih2dz = 0.5 / dz;
   double fkm2 = F[0], fkm1 = F[1];  // originals f(k-2), f(k-1) of the next row
-  const double d0 = bt ? fkm1 / dz - a1 * bot * dz : 0.0;
-  double fprev = d0 / den[0];
-  F[0] = fprev;
+  const double d0 = bt ? (F32 ? fkm1 * idz : fkm1 / dz) - a1 * bot * dz : 0.0;
+  double fprev = F32 ? d0 * den[0] : d0 / den[0];
+  F[0] = (FT)fprev;
 #pragma unroll
   for (int k = 2; k <= NZ; k++) {
     double a, d, fk = 0.0;
     if (k < NZ) {
       a = a1;
       fk = F[k < NZ ? k : 0];
-      d = (fk - fkm2) / 2. / dz;
+      d = F32 ? (fk - fkm2) * ih2dz : (fk - fkm2) / 2. / dz;
     } else {
       a = bt ? al : 0.;
-      d = bt ? -fkm2 / dz + a1 * top * dz : 0.0;
+      d = bt ? (F32 ? -fkm2 * idz : -fkm2 / dz) + a1 * top * dz : 0.0;
     }
     const double dn = den[k - 1];
-    if (dn == 0) { raise_status(sc, PMPIC_ERR_ZERO_PIVOT); return; }
-    fprev = (d - a * fprev) / (dn);
-    F[k - 1] = fprev;
+    if constexpr (F32) {
+      fprev = (d - a * fprev) * dn;
+    } else {
+      if (dn == 0) { raise_status(sc, PMPIC_ERR_ZERO_PIVOT); return; }
+      fprev = (d - a * fprev) / (dn);
+    }
+    F[k - 1] = (FT)fprev;
     fkm2 = fkm1;
     fkm1 = fk;
   }
   double fnext = fprev;
 #pragma unroll
   for (int k = NZ - 1; k >= 1; k--) {
-    const double v = F[k - 1] - cpt[k] * fnext;
-    F[k - 1] = v;
+    const double v = (double)F[k - 1] - cpt[k] * fnext;
+    F[k - 1] = (FT)v;
     fnext = v;
   }
 #pragma unroll
@@ -645,7 +656,10 @@ __global__ void __launch_bounds__(64) k_laplinv_qn(Geometry g, ColFields<S> cf,
   if (k2 == 0) return;
   const int df0 = cf.df0[blockIdx.y];
   S *f = cf.out[blockIdx.y] + col;
-  double F[L], CPl[L];
+  // the FP32-bulk form as in k_laplinv_q4: one reciprocal per pivot, FP32 column and c'
+  constexpr bool F32 = std::is_same<S, float>::value;
+  using FT = typename std::conditional<F32, float, double>::type;
+  FT F[L], CPl[L];
 #pragma unroll
   for (int i = 0; i < L; i++) {
     F[i] = 0.0;
@@ -659,13 +673,14 @@ __global__ void __launch_bounds__(64) k_laplinv_qn(Geometry g, ColFields<S> cf,
   const double dz = g.dz;
   const double i2dz2 = 2. / dz / dz, idz2 = 1. / dz / dz;
   const double ai = idz2 - f112 * k2, bi = -i2dz2 - f56 * k2, ci = idz2 - f112 * k2;
-  const double nxt0 = __shfl(F[0], lane + CPW);  // the original of the next group's first level
-  const int PTOP = (NZ - 1) / L;                 // group holding level NZ - 1
-  double denominator = 0.0, fprev = 0.0, cprev = 0.0, f0 = 0.0, fp = 0.0;
+  const double nxt0 = __shfl((double)F[0], lane + CPW);  // the next group's first original
+  const int PTOP = (NZ - 1) / L;                         // group holding level NZ - 1
+  double denominator = 0.0, rden = 0.0, fprev = 0.0, cprev = 0.0, f0 = 0.0, fp = 0.0;
   bool bad = false;
   for (int p = 0; p <= PTOP; p++) {
     if (p > 0) {
       denominator = __shfl(denominator, lane - CPW);
+      if (F32) rden = __shfl(rden, lane - CPW);
       fprev = __shfl(fprev, lane - CPW);
       cprev = __shfl(cprev, lane - CPW);
       f0 = __shfl(f0, lane - CPW);
@@ -684,8 +699,13 @@ __global__ void __launch_bounds__(64) k_laplinv_qn(Geometry g, ColFields<S> cf,
           if (!df0) { b1 = 1.; c1 = 0.; d1 = 0.; }
           else { b1 = -idz2 - f13 * k2; c1 = idz2 - f16 * k2; d1 = f13 * f0 + f16 * fp; }
           denominator = b1;
-          fprev = d1 / denominator;
-          F[0] = fprev;
+          if constexpr (F32) {
+            rden = 1.0 / denominator;
+            fprev = d1 * rden;
+          } else {
+            fprev = d1 / denominator;
+          }
+          F[0] = (FT)fprev;
           cprev = c1;
           continue;
         }
@@ -702,17 +722,22 @@ __global__ void __launch_bounds__(64) k_laplinv_qn(Geometry g, ColFields<S> cf,
           a = idz2 - f16 * k2; b = -idz2 - f13 * k2; c = 0.;
           d = f13 * f0 + f16 * fm;
         }
-        const double cpk = cprev / denominator;
-        CPl[i] = cpk;
+        const double cpk = F32 ? cprev * rden : cprev / denominator;
+        CPl[i] = (FT)cpk;
         denominator = b - a * cpk;
         bad |= (denominator == 0);
-        fprev = (d - a * fprev) / (denominator);
-        F[i] = fprev;
+        if constexpr (F32) {
+          rden = 1.0 / denominator;
+          fprev = (d - a * fprev) * rden;
+        } else {
+          fprev = (d - a * fprev) / (denominator);
+        }
+        F[i] = (FT)fprev;
         cprev = c;
       }
     }
   }
-  const double cpn = __shfl(CPl[0], lane + CPW);
+  const double cpn = __shfl((double)CPl[0], lane + CPW);
   double fnext = fprev;
   for (int p = PTOP; p >= 0; p--) {
     if (p < PTOP) fnext = __shfl(fnext, lane + CPW);
@@ -721,9 +746,9 @@ __global__ void __launch_bounds__(64) k_laplinv_qn(Geometry g, ColFields<S> cf,
       for (int i = L - 1; i >= 0; i--) {
         const int j = p * L + i;
         if (j >= NZ - 1) continue;
-        const double cpj1 = (i + 1 < L) ? CPl[i + 1 < L ? i + 1 : 0] : cpn;
-        const double v = F[i] - cpj1 * fnext;
-        F[i] = v;
+        const double cpj1 = (i + 1 < L) ? (double)CPl[i + 1 < L ? i + 1 : 0] : cpn;
+        const double v = (double)F[i] - cpj1 * fnext;
+        F[i] = (FT)v;
         fnext = v;
       }
     }
@@ -740,6 +765,9 @@ __global__ void __launch_bounds__(64) k_diffz_qn(Geometry g, ColFields<S> cf,
                                                  const double *__restrict__ tcp,
                                                  DevScalars *sc) {
   constexpr int CPW = 64 / Q;
+  // the FP32-bulk form as in k_diffz_reg: pivot reciprocals, 1/dz factors, FP32 column
+  constexpr bool F32 = std::is_same<S, float>::value;
+  using FT = typename std::conditional<F32, float, double>::type;
   __shared__ double den[Q * L], cpt[Q * L];
   const int NZ = g.nzn;
   const int lane = threadIdx.x;
@@ -749,20 +777,21 @@ __global__ void __launch_bounds__(64) k_diffz_qn(Geometry g, ColFields<S> cf,
   const int fi = blockIdx.y;
   const bool bt = cf.bot[fi] != nullptr && cf.top[fi] != nullptr;
   for (int k = threadIdx.x; k < NZ; k += blockDim.x) {
-    den[k] = tden[(bt ? NZ : 0) + k];
+    const double dn = tden[(bt ? NZ : 0) + k];
+    den[k] = F32 ? 1.0 / dn : dn;
     cpt[k] = tcp[(bt ? NZ : 0) + k];
   }
   __syncthreads();
   if (col >= plane) return;  // all Q lanes of a column leave together
   const S *fin = cf.in[fi] + col;
   S *o = cf.out[fi] + col;
-  double F[L];
+  FT F[L];
 #pragma unroll
-  for (int i = 0; i < L; i++) F[i] = (q * L + i < NZ) ? fin[(int64_t)(q * L + i) * plane] : 0.0;
+  for (int i = 0; i < L; i++) F[i] = (q * L + i < NZ) ? fin[(int64_t)(q * L + i) * plane] : (S)0;
   const double bot = bt ? cf.bot[fi][col] : 0.0, top = bt ? cf.top[fi][col] : 0.0;
   const double a1 = (double)(1.f / 6.f), al = (double)(1.f / 3.f);
-  const double dz = g.dz;
-  const double nxt0 = __shfl(F[0], lane + CPW);  // the original of the next group's first level
+  const double dz = g.dz, idz = 1.0 / dz, ih2dz = 0.5 / dz;
+  const double nxt0 = __shfl((double)F[0], lane + CPW);  // the original of the next group's first level
   const int PTOP = (NZ - 1) / L;
   double fprev = 0.0, fkm2 = 0.0, fkm1 = 0.0;  // originals f(k-2), f(k-1) of the next row
   bool bad = false;
@@ -781,25 +810,35 @@ __global__ void __launch_bounds__(64) k_diffz_qn(Geometry g, ColFields<S> cf,
         if (j == 0) {
           fkm2 = F[0];
           fkm1 = onext;
-          const double d0 = bt ? fkm1 / dz - a1 * bot * dz : 0.0;
-          bad |= (den[0] == 0);
-          fprev = d0 / den[0];
-          F[0] = fprev;
+          const double d0 = bt ? (F32 ? fkm1 * idz : fkm1 / dz) - a1 * bot * dz : 0.0;
+          if constexpr (F32) {
+            bad |= !__builtin_isfinite(den[0]);
+            fprev = d0 * den[0];
+          } else {
+            bad |= (den[0] == 0);
+            fprev = d0 / den[0];
+          }
+          F[0] = (FT)fprev;
           continue;
         }
         double a, d, fk = 0.0;
         if (j + 1 < NZ) {
           a = a1;
           fk = onext;
-          d = (fk - fkm2) / 2. / dz;
+          d = F32 ? (fk - fkm2) * ih2dz : (fk - fkm2) / 2. / dz;
         } else {
           a = bt ? al : 0.;
-          d = bt ? -fkm2 / dz + a1 * top * dz : 0.0;
+          d = bt ? (F32 ? -fkm2 * idz : -fkm2 / dz) + a1 * top * dz : 0.0;
         }
         const double dn = den[j];
-        bad |= (dn == 0);
-        fprev = (d - a * fprev) / (dn);
-        F[i] = fprev;
+        if constexpr (F32) {
+          bad |= !__builtin_isfinite(dn);
+          fprev = (d - a * fprev) * dn;
+        } else {
+          bad |= (dn == 0);
+          fprev = (d - a * fprev) / (dn);
+        }
+        F[i] = (FT)fprev;
         fkm2 = fkm1;
         fkm1 = fk;
       }
@@ -813,8 +852,8 @@ __global__ void __launch_bounds__(64) k_diffz_qn(Geometry g, ColFields<S> cf,
       for (int i = L - 1; i >= 0; i--) {
         const int j = p * L + i;
         if (j >= NZ - 1) continue;
-        const double v = F[i] - cpt[j + 1] * fnext;
-        F[i] = v;
+        const double v = (double)F[i] - cpt[j + 1] * fnext;
+        F[i] = (FT)v;
         fnext = v;
       }
     }
@@ -1047,6 +1086,10 @@ constexpr int LONG_Q = 8, LONG_L = 33;
 // other lengths up to 129 z nodes (48^3 and 96^3 grids, say) in the same lane-group form: four
 // lanes per column with 13 levels each up to 52 nodes, 33 each up to 129
 constexpr int MID_Q = 4, MID_L_SHORT = 13, MID_L = 33;
+// the FP32-bulk form, whose columns sit in registers as FP32, takes long columns up to
+// F32_Q * F32_L z nodes with four lanes per column (sixteen columns a wavefront, twice the FP64
+// form's), and the 257-node columns of 256-cell-deep grids in the register-resident solvers
+constexpr int F32_Q = 4, F32_L = 65, F32_REG_LONG = 257;
 
 template <typename S>
 int op_filter_t(Ctx *c, const S *in, S *out) {
@@ -1093,6 +1136,20 @@ int op_diffz_n(Ctx *c, int nf, const S *const *in, S *const *out, const double *
                        dim3(64), 0, c->stream, c->g, rf, c->dz_den, c->dz_cp, c->dsc);
     PMPIC_CHECK(hipGetLastError());
     return PMPIC_OK;
+  }
+  if constexpr (std::is_same<S, float>::value) {
+    if (c->g.nzn == F32_REG_LONG) {
+      hipLaunchKernelGGL((k_diffz_reg<F32_REG_LONG, S>), dim3(cdiv(plane, 64), nf), dim3(64), 0,
+                         c->stream, c->g, rf, c->dz_den, c->dz_cp, c->dsc);
+      PMPIC_CHECK(hipGetLastError());
+      return PMPIC_OK;
+    }
+    if (c->g.nzn <= F32_Q * F32_L) {
+      hipLaunchKernelGGL((k_diffz_qn<F32_Q, F32_L, S>), dim3(cdiv(plane, 64 / F32_Q), nf),
+                         dim3(64), 0, c->stream, c->g, rf, c->dz_den, c->dz_cp, c->dsc);
+      PMPIC_CHECK(hipGetLastError());
+      return PMPIC_OK;
+    }
   }
   if (c->g.nzn <= LONG_Q * LONG_L) {
     hipLaunchKernelGGL((k_diffz_qn<LONG_Q, LONG_L, S>), dim3(cdiv(plane, 64 / LONG_Q), nf),
@@ -1154,6 +1211,20 @@ int op_laplinv_n(Ctx *c, int nf, S *const *f, const int *df0) {
                        dim3(64), 0, c->stream, c->g, rf, c->k2, c->dsc);
     PMPIC_CHECK(hipGetLastError());
     return PMPIC_OK;
+  }
+  if constexpr (std::is_same<S, float>::value) {
+    if (c->g.nzn == F32_REG_LONG) {
+      hipLaunchKernelGGL((k_laplinv_q4<F32_REG_LONG, S>), dim3(cdiv(plane, 16), nf), dim3(64), 0,
+                         c->stream, c->g, rf, c->k2, c->dsc);
+      PMPIC_CHECK(hipGetLastError());
+      return PMPIC_OK;
+    }
+    if (c->g.nzn <= F32_Q * F32_L) {
+      hipLaunchKernelGGL((k_laplinv_qn<F32_Q, F32_L, S>), dim3(cdiv(plane, 64 / F32_Q), nf),
+                         dim3(64), 0, c->stream, c->g, rf, c->k2, c->dsc);
+      PMPIC_CHECK(hipGetLastError());
+      return PMPIC_OK;
+    }
   }
   if (c->g.nzn <= LONG_Q * LONG_L) {
     hipLaunchKernelGGL((k_laplinv_qn<LONG_Q, LONG_L, S>), dim3(cdiv(plane, 64 / LONG_Q), nf),

@@ -57,6 +57,10 @@ def test_fp64_only_entry_points_refuse():
     ((12, 10, 8), dict(vort=1.0), 1),
     ((16, 16, 8), dict(vort=2.0, shrink=0.05, thin=0.1), 6),
     ((10, 12, 6), dict(vort=0.5, thin=0.2), 10),
+    # tall columns: the FP32 column solvers of 129, 201 and 257 z nodes
+    ((8, 6, 128), dict(vort=1.0), 2),
+    ((6, 8, 200), dict(vort=1.0), 2),
+    ((8, 8, 256), dict(vort=1.0), 2),
 ])
 def test_steps_close_to_fp64(shape, kw, steps):
     ic = thermal_case(*shape, 2, **kw)
