@@ -438,21 +438,24 @@ __global__ void k_dz_tables(Geometry g, double *__restrict__ den, double *__rest
   }
 }
 
-// laplinv with four lanes per column: lane q * 16 + c holds levels [q L, q L + L) of column c
-// of the wavefront's 16, both the column values and the forward sweep's c', in registers.  The
-// sweeps run quarter after quarter (the others masked), the recurrence state crossing to the
-// next quarter by a lane shuffle, so no level waits on memory and every operation is the one of
-// laplinv_column in the same order: results are bit-identical.  A zero pivot is flagged and
-// raised after the sweeps (the reference stops there; the column's values are then moot).
-template <int NZ, typename S>
-__global__ void __launch_bounds__(64) k_laplinv_q4(Geometry g, ColFields<S> cf,
-                                                   const double *__restrict__ k2a,
-                                                   DevScalars *sc) {
-  constexpr int L = (NZ + 3) / 4;
+// laplinv with Q lanes per column (1, 2 or 4): lane q * CPW + c holds levels [q L, q L + L) of
+// column c of the wavefront's CPW = 64 / Q, both the column values and the forward sweep's c',
+// in registers.  The sweeps run lane group after lane group (the others masked), the recurrence
+// state crossing to the next group by a lane shuffle, so no level waits on memory and every
+// operation is the one of laplinv_column in the same order: results are bit-identical.  A zero
+// pivot is flagged and raised after the sweeps (the reference stops there; the column's values
+// are then moot).
+template <int Q, int NZ, typename S>
+__global__ void __launch_bounds__(64) k_laplinv_reg(Geometry g, ColFields<S> cf,
+                                                    const double *__restrict__ k2a,
+                                                    DevScalars *sc) {
+  static_assert(Q == 1 || Q == 2 || Q == 4, "one, two or four lanes per column");
+  constexpr int CPW = 64 / Q;  // columns per wavefront
+  constexpr int L = (NZ + Q - 1) / Q;
   const int lane = threadIdx.x;
-  const int q = lane >> 4;
+  const int q = lane / CPW;
   const int64_t plane = (int64_t)g.sy * g.sx;
-  const int64_t col = (int64_t)blockIdx.x * 16 + (lane & 15);
+  const int64_t col = (int64_t)blockIdx.x * CPW + (lane % CPW);
   if (col >= plane) return;
   const double k2 = k2a[col];
   if (k2 == 0) return;
@@ -476,20 +479,20 @@ __global__ void __launch_bounds__(64) k_laplinv_q4(Geometry g, ColFields<S> cf,
   const double dz = g.dz;
   const double i2dz2 = 2. / dz / dz, idz2 = 1. / dz / dz;
   const double ai = idz2 - f112 * k2, bi = -i2dz2 - f56 * k2, ci = idz2 - f112 * k2;
-  // the original of level q L + L, the next quarter's first
-  const double nxt0 = __shfl(F[0], lane + 16);
-  constexpr int PTOP = (NZ - 1) / L;  // quarter holding level NZ - 1
+  // the original of level q L + L, the next group's first (unused with one lane per column)
+  const double nxt0 = __shfl((double)F[0], (lane + CPW) & 63);
+  constexpr int PTOP = (NZ - 1) / L;  // group holding level NZ - 1
   double denominator = 0.0, rden = 0.0, fprev = 0.0, cprev = 0.0, f0 = 0.0, fp = 0.0;
   bool bad = false;
 #pragma unroll
   for (int p = 0; p <= PTOP; p++) {
     if (p > 0) {
-      denominator = __shfl(denominator, lane - 16);
-      if (F32) rden = __shfl(rden, lane - 16);
-      fprev = __shfl(fprev, lane - 16);
-      cprev = __shfl(cprev, lane - 16);
-      f0 = __shfl(f0, lane - 16);
-      fp = __shfl(fp, lane - 16);
+      denominator = __shfl(denominator, lane - CPW);
+      if (F32) rden = __shfl(rden, lane - CPW);
+      fprev = __shfl(fprev, lane - CPW);
+      cprev = __shfl(cprev, lane - CPW);
+      f0 = __shfl(f0, lane - CPW);
+      fp = __shfl(fp, lane - CPW);
     }
     if (q == p) {
 #pragma unroll
@@ -542,12 +545,12 @@ __global__ void __launch_bounds__(64) k_laplinv_q4(Geometry g, ColFields<S> cf,
       }
     }
   }
-  // backward sweep from the top quarter down: level j takes c' of level j + 1
-  const double cpn = __shfl((double)CPl[0], lane + 16);
+  // backward sweep from the top group down: level j takes c' of level j + 1
+  const double cpn = __shfl((double)CPl[0], (lane + CPW) & 63);
   double fnext = fprev;
 #pragma unroll
   for (int p = PTOP; p >= 0; p--) {
-    if (p < PTOP) fnext = __shfl(fnext, lane + 16);
+    if (p < PTOP) fnext = __shfl(fnext, lane + CPW);
     if (q == p) {
 #pragma unroll
       for (int i = L - 1; i >= 0; i--) {
@@ -638,7 +641,7 @@ __global__ void __launch_bounds__(64) k_diffz_reg(Geometry g, ColFields<S> cf,
 
 // Long columns (more z nodes than the instances above): Q lanes per column, lane q holding
 // levels [q L, q L + L) of the column in registers, for any column length nz <= Q L chosen at
-// run time.  The sweeps run lane group after lane group as in k_laplinv_q4, the recurrence
+// run time.  The sweeps run lane group after lane group as in k_laplinv_reg, the recurrence
 // state crossing by lane shuffles, every operation the one of laplinv_column / k_diffz_reg in
 // the same order, so results are bit-identical to them.
 template <int Q, int L, typename S>
@@ -656,7 +659,7 @@ __global__ void __launch_bounds__(64) k_laplinv_qn(Geometry g, ColFields<S> cf,
   if (k2 == 0) return;
   const int df0 = cf.df0[blockIdx.y];
   S *f = cf.out[blockIdx.y] + col;
-  // the FP32-bulk form as in k_laplinv_q4: one reciprocal per pivot, FP32 column and c'
+  // the FP32-bulk form as in k_laplinv_reg: one reciprocal per pivot, FP32 column and c'
   constexpr bool F32 = std::is_same<S, float>::value;
   using FT = typename std::conditional<F32, float, double>::type;
   FT F[L], CPl[L];
@@ -1090,6 +1093,16 @@ constexpr int MID_Q = 4, MID_L_SHORT = 13, MID_L = 33;
 // F32_Q * F32_L z nodes with four lanes per column (sixteen columns a wavefront, twice the FP64
 // form's), and the 257-node columns of 256-cell-deep grids in the register-resident solvers
 constexpr int F32_Q = 4, F32_L = 65, F32_REG_LONG = 257;
+// lanes per column of the register-resident laplinv: four, but two (32 columns a wavefront) for
+// the FP32-bulk form's columns of up to F32_LAP_Q2 nodes, whose column and c' then still fit in
+// registers at three wavefronts per SIMD (18.8 against 24.7 us a call at 65 nodes, 128^2
+// columns).  One lane per column, or two on longer columns, gains nothing (the sweeps are then
+// held at one or two wavefronts per SIMD by their registers)
+constexpr int F32_LAP_Q2 = 65;
+template <typename S>
+constexpr int lap_q(int nz) {
+  return std::is_same<S, float>::value && nz <= F32_LAP_Q2 ? 2 : 4;
+}
 
 template <typename S>
 int op_filter_t(Ctx *c, const S *in, S *out) {
@@ -1191,8 +1204,8 @@ int op_laplinv_n(Ctx *c, int nf, S *const *f, const int *df0) {
   switch (c->g.nzn) {
 #define PMPIC_LAP_CASE(NZ)                                                                    \
   case NZ:                                                                                     \
-    hipLaunchKernelGGL((k_laplinv_q4<NZ, S>), dim3(cdiv(plane, 16), nf), dim3(64), 0,         \
-                       c->stream, c->g, rf, c->k2, c->dsc);                                    \
+    hipLaunchKernelGGL((k_laplinv_reg<lap_q<S>(NZ), NZ, S>), dim3(cdiv(plane, 64 / lap_q<S>(NZ)), \
+                       nf), dim3(64), 0, c->stream, c->g, rf, c->k2, c->dsc);                  \
     PMPIC_CHECK(hipGetLastError());                                                            \
     return PMPIC_OK;
     PMPIC_REG_COLUMNS(PMPIC_LAP_CASE)
@@ -1214,8 +1227,9 @@ int op_laplinv_n(Ctx *c, int nf, S *const *f, const int *df0) {
   }
   if constexpr (std::is_same<S, float>::value) {
     if (c->g.nzn == F32_REG_LONG) {
-      hipLaunchKernelGGL((k_laplinv_q4<F32_REG_LONG, S>), dim3(cdiv(plane, 16), nf), dim3(64), 0,
-                         c->stream, c->g, rf, c->k2, c->dsc);
+      constexpr int Q = lap_q<S>(F32_REG_LONG);
+      hipLaunchKernelGGL((k_laplinv_reg<Q, F32_REG_LONG, S>), dim3(cdiv(plane, 64 / Q), nf),
+                         dim3(64), 0, c->stream, c->g, rf, c->k2, c->dsc);
       PMPIC_CHECK(hipGetLastError());
       return PMPIC_OK;
     }
