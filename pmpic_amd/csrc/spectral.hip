@@ -71,13 +71,28 @@ __global__ void k_diffy(Geometry g, const double *__restrict__ in, double *__res
   out[q] = dy_at(in, ky, q, jj, g.sx, col);
 }
 
+// The element-wise spectral kernels take one thread per element of a z level (blockIdx.y): the
+// column col = (jj, ii) of the level's sy x sx plane and the element q = k plane + col, with no
+// 64-bit division.  SPEC_ELEMENT declares plane, k, col, q, ii, jj and leaves threads past the
+// plane.
+#define SPEC_ELEMENT(g)                                                                      \
+  const int64_t plane = (int64_t)(g).sy * (g).sx;                                            \
+  const int col32 = (int)(blockIdx.x * blockDim.x + threadIdx.x);                            \
+  if (col32 >= plane) return;                                                                \
+  const int k = (int)blockIdx.y;                                                             \
+  const int64_t col = col32;                                                                 \
+  const int64_t q = (int64_t)k * plane + col;                                                \
+  const int ii = col32 % (g).sx, jj = col32 / (g).sx;                                        \
+  (void)k; (void)ii; (void)jj
+inline dim3 spec_grid(const Geometry &g) {
+  return dim3((unsigned)cdiv((int64_t)g.sy * g.sx, TPB), (unsigned)g.nzn);
+}
+
 template <typename S>
 __global__ void k_filter(Geometry g, const S *__restrict__ in, S *__restrict__ out,
                          const double *__restrict__ filt) {
-  const int64_t ssz = (int64_t)g.nzn * g.sy * g.sx;
-  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= ssz) return;
-  out[q] = (S)((double)in[q] * filt[q % ((int64_t)g.sy * g.sx)]);
+  SPEC_ELEMENT(g);
+  out[q] = (S)((double)in[q] * filt[col]);
 }
 
 // vort2vel.F90:158-174: f = (diffx(a) + diffy(b)) + dc/dz (explicit 2nd order, one-sided at
@@ -88,13 +103,7 @@ __global__ void k_divergence(Geometry g, const S *__restrict__ a, const S *__res
                              const double *__restrict__ kx, const double *__restrict__ ky,
                              double hdzi) {
   auto c = [&](int64_t i) { return (double)cs[i]; };
-  const int64_t plane = (int64_t)g.sy * g.sx;
-  const int64_t ssz = (int64_t)g.nzn * plane;
-  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= ssz) return;
-  const int64_t col = q % plane;
-  const int k = (int)(q / plane);
-  const int ii = (int)(q % g.sx), jj = (int)(col / g.sx);
+  SPEC_ELEMENT(g);
   const int nz = g.nzn;
   const double d = dx_at(a, kx, q, ii, col);
   const double e = dy_at(b, ky, q, jj, g.sx, col);
@@ -115,12 +124,7 @@ template <typename S>
 __global__ void k_correct_ab(Geometry g, S *__restrict__ a, S *__restrict__ b,
                              const S *__restrict__ f, const double *__restrict__ kx,
                              const double *__restrict__ ky) {
-  const int64_t plane = (int64_t)g.sy * g.sx;
-  const int64_t ssz = (int64_t)g.nzn * plane;
-  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= ssz) return;
-  const int64_t col = q % plane;
-  const int ii = (int)(q % g.sx), jj = (int)(col / g.sx);
+  SPEC_ELEMENT(g);
   a[q] = (S)((double)a[q] - dx_at(f, kx, q, ii, col));
   b[q] = (S)((double)b[q] - dy_at(f, ky, q, jj, g.sx, col));
 }
@@ -135,12 +139,7 @@ __global__ void k_correct_c_filter(Geometry g, const S *__restrict__ a,
                                    const double *__restrict__ filt, double *__restrict__ atop,
                                    double *__restrict__ abot, double *__restrict__ btop,
                                    double *__restrict__ bbot) {
-  const int64_t plane = (int64_t)g.sy * g.sx;
-  const int64_t ssz = (int64_t)g.nzn * plane;
-  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= ssz) return;
-  const int64_t col = q % plane;
-  const int k = (int)(q / plane);
+  SPEC_ELEMENT(g);
   double cv = (double)c[q] - (double)dzf[q];
   if (k == 0) cv = 0.;
   c[q] = (S)cv;
@@ -879,13 +878,7 @@ __global__ void k_velocity_spectra(Geometry g, const S *__restrict__ a,
                                    S *__restrict__ vs, S *__restrict__ ws,
                                    const double *__restrict__ kx, const double *__restrict__ ky,
                                    const double *__restrict__ filt) {
-  const int64_t plane = (int64_t)g.sy * g.sx;
-  const int64_t ssz = (int64_t)g.nzn * plane;
-  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= ssz) return;
-  const int64_t col = q % plane;
-  const int k = (int)(q / plane);
-  const int ii = (int)(q % g.sx), jj = (int)(col / g.sx);
+  SPEC_ELEMENT(g);
   const double fl = filt[col];
   double u = (double)dzb[q] - dy_at(c, ky, q, jj, g.sx, col);
   if (col == 0) u = ubar[k];
@@ -1106,7 +1099,7 @@ constexpr int lap_q(int nz) {
 
 template <typename S>
 int op_filter_t(Ctx *c, const S *in, S *out) {
-  hipLaunchKernelGGL(k_filter<S>, dim3(cdiv(ssz_of(c->g), TPB)), dim3(TPB), 0, c->stream, c->g,
+  hipLaunchKernelGGL(k_filter<S>, spec_grid(c->g), dim3(TPB), 0, c->stream, c->g,
                      in, out, c->filt);
   PMPIC_CHECK(hipGetLastError());
   return PMPIC_OK;
@@ -1285,8 +1278,7 @@ template <> float *pqrb<float>(Ctx *c, int k) { return c->b32.pgrid[k]; }
 template <typename S>
 static int vort2vel_spectral_t(Ctx *c) {
   const Geometry &g = c->g;
-  const int64_t ssz = ssz_of(g);
-  const dim3 eb(cdiv(ssz, TPB)), et(TPB);
+  const dim3 eb = spec_grid(g), et(TPB);
   S *sa = sp<S>(c->sa), *sb = sp<S>(c->sb), *sc = sp<S>(c->sc), *sd = sp<S>(c->sd);
   S *se = sp<S>(c->se), *sf = sp<S>(c->sf);
   S *sder[5] = {sp<S>(c->sder[0]), sp<S>(c->sder[1]), sp<S>(c->sder[2]), sp<S>(c->sder[3]),
