@@ -57,7 +57,10 @@ def test_fp64_only_entry_points_refuse():
     ((12, 10, 8), dict(vort=1.0), 1),
     ((16, 16, 8), dict(vort=2.0, shrink=0.05, thin=0.1), 6),
     ((10, 12, 6), dict(vort=0.5, thin=0.2), 10),
-    # tall columns: the FP32 column solvers of 129, 201 and 257 z nodes
+    # the FP32 column solvers of 17, 33 and 65 z nodes (two lanes a column), 129, 201 and 257
+    ((8, 6, 16), dict(vort=1.0), 2),
+    ((6, 8, 32), dict(vort=1.0), 2),
+    ((8, 8, 64), dict(vort=1.0), 2),
     ((8, 6, 128), dict(vort=1.0), 2),
     ((6, 8, 200), dict(vort=1.0), 2),
     ((8, 8, 256), dict(vort=1.0), 2),
